@@ -1,0 +1,152 @@
+/* C ABI of the MI355X-native batched gas-phase kinetics library.
+ *
+ * Every entry point replaces one piece of the reference's C++ surface; the
+ * reference interface is cited beside each declaration (paths relative to
+ * sandialabs/TChem-atm).  Conventions:
+ *   - plain pointers and sizes only; `stream` is a hipStream_t passed as
+ *     void* (NULL = the default stream);
+ *   - device pointers are marked [device]; all device arrays are row-major
+ *     FP64 unless stated (the reference's LayoutRight views);
+ *   - every function returns TCHEM_OK (0) or a negative TCHEM_ERR_* code,
+ *     never throws and never exits; tchem_last_error() gives the message;
+ *   - batch calls enqueue on `stream` and return without synchronising
+ *     (the reference's Kokkos::parallel_for is asynchronous too, and its
+ *     driver fences: src/examples/TChem_AtmosphericChemistry.cpp:317).
+ */
+#ifndef TCHEM_ATM_H
+#define TCHEM_ATM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TCHEM_OK 0
+#define TCHEM_ERR_INVALID -1  /* bad argument or size */
+#define TCHEM_ERR_PARSE -2    /* mechanism / scenario file could not be read */
+#define TCHEM_ERR_UNSUPPORTED -3
+#define TCHEM_ERR_HIP -4      /* a HIP runtime call failed */
+#define TCHEM_ERR_NO_DEVICE -5
+
+typedef struct tchem_kmd_s* tchem_kmd_t;   /* host kinetic model data */
+typedef struct tchem_kmcd_s* tchem_kmcd_t; /* device-resident constant data */
+
+/* Byte layout of TChem::TimeAdvance (src/core/TChem_Util.hpp:469-476). */
+typedef struct {
+  double _tbeg, _tend;
+  double _dt, _dtmin, _dtmax;
+  int _max_num_newton_iterations;
+  int _num_time_iterations_per_interval;
+  int _num_outer_time_iterations_per_interval;
+  int _jacobian_interval;
+} tchem_time_advance_t;
+
+/* Per-cell counters (no counterpart in the reference; the reference only
+ * prints a warning for failed cells, src/core/problems/TChem_Impl_AtmosphericChemistry.hpp:213-221). */
+typedef struct {
+  int accepted, rejected, newton_failures, newton_iterations, factorizations, failed;
+} tchem_cell_stats_t;
+
+/* Kernel form and launch shape of tchem_atmchem_run_device_batch. */
+#define TCHEM_PRECISION_FP64 0  /* all-FP64 ("plain") */
+#define TCHEM_PRECISION_MIXED 1 /* FP32 iteration matrix + LU, FP64 state/residual/error/step control */
+#define TCHEM_SCHEDULE_GRID 0       /* one workgroup per cell, grid = nBatch */
+#define TCHEM_SCHEDULE_PERSISTENT 1 /* resident workgroups pull cells from a device work queue */
+typedef struct {
+  int precision;         /* TCHEM_PRECISION_* */
+  int schedule;          /* TCHEM_SCHEDULE_* */
+  int blocks_per_cu;     /* persistent schedule: resident workgroups per CU (0 = from occupancy) */
+  int reserved[5];       /* must be zero */
+} tchem_atmchem_options_t;
+
+/* Last error message of the calling thread ("" when none). */
+const char* tchem_last_error(void);
+
+/* ---- kinetic model (KineticModelData) ----------------------------------- */
+
+/* Replaces KineticModelData::KineticModelData(mechfile[, echofile])
+ * (src/core/TChem_KineticModelData.cpp:53-78, initChemNCAR :80-1017).
+ * echo_path: nullable; when given, the loader echo ("kmod.echo") is written there. */
+int tchem_kmd_load_yaml(const char* path, const char* echo_path, tchem_kmd_t* out);
+int tchem_kmd_destroy(tchem_kmd_t kmd);
+
+/* Sizes: kmcd.nSpec, kmcd.nConstSpec, kmcd.nReac (src/core/TChem_KineticModelNCAR_ConstData.hpp:30-113)
+ * and the number of photolysis reactions (ARRHENIUS reactions whose note
+ * starts with "PHOTOLYSIS"). Any output pointer may be NULL. */
+int tchem_kmd_info(tchem_kmd_t kmd, int* nSpec, int* nConstSpec, int* nReac, int* nPhoto);
+
+/* Species name i (the reference's kmcd.speciesNames, 18 characters max). */
+int tchem_kmd_species_name(tchem_kmd_t kmd, int i, char* buf, size_t len);
+
+/* Reaction indices of the photolysis slots, in slot order (nPhoto entries). */
+int tchem_kmd_photolysis_reactions(tchem_kmd_t kmd, int* reactions, int capacity);
+
+/* Replaces AtmChemistry::setScenarioConditions (src/core/TChem_AtmosphericChemistry.cpp:429-504):
+ * reads a scenario YAML into host state rows [nBatch][nSpec+3].  Call with
+ * state_host == NULL to query nBatch. */
+int tchem_scenario_load(tchem_kmd_t kmd, const char* path, double* state_host, int capacity_rows, int* nBatch);
+
+/* ---- device constant data (KineticModelNCAR_ConstData) ------------------ */
+
+/* Replaces createNCAR_KineticModelConstData<device>(kmd)
+ * (src/core/TChem_KineticModelNCAR_ConstData.hpp:115-148): compiles the
+ * mechanism into device tables on the current HIP device. The kmcd is
+ * independent of the kmd afterwards. */
+int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out);
+int tchem_kmcd_destroy(tchem_kmcd_t kmcd);
+
+/* Replaces AtmosphericChemistry::getWorkSpaceSize (src/core/TChem_AtmosphericChemistry.hpp:47-81):
+ * LDS bytes one cell uses in the given precision. */
+int tchem_atmchem_workspace_bytes(tchem_kmcd_t kmcd, int precision, size_t* bytes);
+
+/* ---- batched kernels ------------------------------------------------------ */
+
+/* Replaces AtmosphericChemistry::runDeviceBatch(policy, tol_newton, tol_time,
+ * fac, tadv, state, t_out, dt_out, state_out, kmcd)
+ * (src/core/TChem_AtmosphericChemistry.hpp:83-97, .cpp:210-271) and its
+ * per-cell-kmcd overload (.hpp:141-155, .cpp:319-378) through photo_rates.
+ *   tol_newton [device] 2: (atol, rtol)
+ *   tol_time   [device] [m][2]: (atol, rtol) per species
+ *   fac        unused (FD-Jacobian scratch in the reference); may be NULL
+ *   tadv       [device] [nBatch] tchem_time_advance_t
+ *   state      [device] [nBatch][nSpec+3] in/out: active species integrated in place
+ *   photo_rates [device] [nBatch][nPhoto] or NULL: per-cell A of the photolysis reactions
+ *   t_out, dt_out [device] [nBatch] in/out: cells with t_out >= tend are skipped
+ *   state_out  [device] [nBatch][nSpec+3]: density, pressure, temperature copied;
+ *              species copied too when state_out != state (may alias state)
+ *   stats      [device] [nBatch] tchem_cell_stats_t or NULL (overwritten)
+ * Per-cell failure follows the reference: species zeroed, t_out = tend, dt_out = -1. */
+int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBatch, const double* tol_newton,
+                                   const double* tol_time, double* fac, const tchem_time_advance_t* tadv,
+                                   double* state, const double* photo_rates, double* t_out, double* dt_out,
+                                   double* state_out, tchem_cell_stats_t* stats, tchem_kmcd_t kmcd, void* stream);
+
+/* The driver's between-interval update (src/examples/TChem_AtmosphericChemistry.cpp:340-350):
+ * tadv[i]._tbeg = t_out[i], tadv[i]._dt = dt_out[i]. */
+int tchem_atmchem_update_time_advance(int nBatch, tchem_time_advance_t* tadv, const double* t_out,
+                                      const double* dt_out, void* stream);
+
+/* Source term omega[nBatch][m] at state rows [nBatch][nSpec+3]: the batched
+ * computeFunction of src/examples/TChem_GasChemistry_RHSs.cpp:184-236
+ * (AtmosphericChemistry_Problem::computeFunction,
+ * src/core/problems/TChem_Impl_AtmosphericChemistry_Problem.hpp:127-143),
+ * argument order as NetProductionRates::runDeviceBatch (src/core/TChem_NetProductionRates.hpp:61-69).
+ *   photo_rates [device] [nBatch][nPhoto] or NULL
+ *   external_sources [device] [nBatch][m] or NULL: added to omega */
+int tchem_sourceterm_run_device_batch(int nBatch, const double* state, const double* photo_rates,
+                                      const double* external_sources, double* omega, tchem_kmcd_t kmcd,
+                                      void* stream);
+
+/* Jacobian jac[nBatch][m][m] = d omega_i / d x_j at state rows: the batched
+ * computeJacobian of src/examples/TChem_GasChemistry_RHSs.cpp:254-309
+ * (analytic here; forward differences or Sacado AD in the reference,
+ * src/core/problems/TChem_Impl_AtmosphericChemistry_Problem.hpp:166-241). */
+int tchem_jacobian_run_device_batch(int nBatch, const double* state, const double* photo_rates, double* jac,
+                                    tchem_kmcd_t kmcd, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,173 @@
+"""FP64 CPU oracle for the batched gas-phase kinetics path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the CPU-baseline leg of ``bench.py`` as the checker.  The product library never
+imports, links or runs anything under ``oracle/``.
+
+``kmd.load`` restates the reference's YAML loader in Python; ``oracle.c``
+restates its numerics in C (see the citations there).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from . import kmd as kmd_mod
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+class _KMD(ctypes.Structure):
+    _fields_ = [
+        ("nSpec", ctypes.c_int), ("nConstSpec", ctypes.c_int), ("nReac", ctypes.c_int),
+        ("maxSpecInReac", ctypes.c_int),
+        ("reacNreac", ctypes.c_void_p), ("reacNprod", ctypes.c_void_p), ("reacSidx", ctypes.c_void_p),
+        ("reacNuki", ctypes.c_void_p), ("convExponent", ctypes.c_void_p),
+        ("nArr", ctypes.c_int), ("arrIdx", ctypes.c_void_p), ("arrPar", ctypes.c_void_p),
+        ("nTroe", ctypes.c_int), ("troeIdx", ctypes.c_void_p), ("troePar", ctypes.c_void_p),
+        ("nCmaq", ctypes.c_int), ("cmaqIdx", ctypes.c_void_p), ("cmaqPar", ctypes.c_void_p),
+        ("nEmis", ctypes.c_int), ("emisSpecies", ctypes.c_void_p), ("emisRate", ctypes.c_void_p),
+        ("nPhoto", ctypes.c_int), ("photoReac", ctypes.c_void_p),
+        ("CONV_PPM", ctypes.c_double),
+    ]
+
+
+class TimeAdvance(ctypes.Structure):
+    """Byte layout of TChem::TimeAdvance (src/core/TChem_Util.hpp:469-476)."""
+    _fields_ = [
+        ("tbeg", ctypes.c_double), ("tend", ctypes.c_double), ("dt", ctypes.c_double),
+        ("dtmin", ctypes.c_double), ("dtmax", ctypes.c_double),
+        ("max_num_newton_iterations", ctypes.c_int), ("num_time_iterations_per_interval", ctypes.c_int),
+        ("num_outer_time_iterations_per_interval", ctypes.c_int), ("jacobian_interval", ctypes.c_int),
+    ]
+
+
+TADV_DTYPE = np.dtype([
+    ("tbeg", "f8"), ("tend", "f8"), ("dt", "f8"), ("dtmin", "f8"), ("dtmax", "f8"),
+    ("max_num_newton_iterations", "i4"), ("num_time_iterations_per_interval", "i4"),
+    ("num_outer_time_iterations_per_interval", "i4"), ("jacobian_interval", "i4"),
+])
+STATS_DTYPE = np.dtype([("accepted", "i4"), ("rejected", "i4"), ("newton_failures", "i4"),
+                        ("newton_iterations", "i4"), ("factorizations", "i4"), ("failed", "i4")])
+
+
+def build(force: bool = False) -> str:
+    """Compile oracle.c with the recipe in oracle/Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        _lib.oracle_kforward.argtypes = [vp, d, d, vp, vp]
+        _lib.oracle_omega.argtypes = [vp, vp, vp, vp]
+        _lib.oracle_jacobian.argtypes = [vp, vp, vp, vp]
+        _lib.oracle_lu.argtypes = [i, vp, i, vp]
+        _lib.oracle_lu.restype = i
+        _lib.oracle_lu_solve.argtypes = [i, vp, i, vp, vp]
+        _lib.oracle_sourceterm_batch.argtypes = [vp, i, vp, vp, vp]
+        _lib.oracle_jacobian_batch.argtypes = [vp, i, vp, vp, vp]
+        _lib.oracle_run_batch.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp, vp, i, i]
+        _lib.oracle_trbdf2_constants.argtypes = [vp]
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class Oracle:
+    """Binds one mechanism's tables to the C oracle."""
+
+    def __init__(self, mech):
+        self.k = mech if isinstance(mech, kmd_mod.OracleKMD) else kmd_mod.load(mech)
+        k = self.k
+        self._keep = [k.reacNreac, k.reacNprod, k.reacSidx, k.reacNuki, k.convExponent, k.arrIdx, k.arrPar,
+                      k.troeIdx, k.troePar, k.cmaqIdx, k.cmaqPar, k.emisSpecies, k.emisRate, k.photoReac]
+        self.s = _KMD(k.nSpec, k.nConstSpec, k.nReac, k.maxSpecInReac,
+                      _p(k.reacNreac), _p(k.reacNprod), _p(k.reacSidx), _p(k.reacNuki), _p(k.convExponent),
+                      len(k.arrIdx), _p(k.arrIdx), _p(k.arrPar),
+                      len(k.troeIdx), _p(k.troeIdx), _p(k.troePar),
+                      len(k.cmaqIdx), _p(k.cmaqIdx), _p(k.cmaqPar),
+                      len(k.emisSpecies), _p(k.emisSpecies), _p(k.emisRate),
+                      len(k.photoReac), _p(k.photoReac), kmd_mod.CONV_PPM)
+        self.ptr = ctypes.addressof(self.s)
+
+    @property
+    def m(self):
+        return self.k.m
+
+    def kforward(self, T, p, photo=None):
+        kfor = np.zeros(self.k.nReac)
+        ph = None if photo is None else np.ascontiguousarray(photo, np.float64)
+        lib().oracle_kforward(self.ptr, float(T), float(p), _p(ph), _p(kfor))
+        return kfor
+
+    def omega(self, kfor, c):
+        c = np.ascontiguousarray(c, np.float64)
+        out = np.zeros(self.m)
+        lib().oracle_omega(self.ptr, _p(np.ascontiguousarray(kfor)), _p(c), _p(out))
+        return out
+
+    def jacobian(self, kfor, c):
+        c = np.ascontiguousarray(c, np.float64)
+        out = np.zeros((self.m, self.m))
+        lib().oracle_jacobian(self.ptr, _p(np.ascontiguousarray(kfor)), _p(c), _p(out))
+        return out
+
+    def sourceterm_batch(self, state, photo=None):
+        state = np.ascontiguousarray(state, np.float64)
+        ph = None if photo is None else np.ascontiguousarray(photo, np.float64)
+        out = np.zeros((state.shape[0], self.m))
+        lib().oracle_sourceterm_batch(self.ptr, state.shape[0], _p(state), _p(ph), _p(out))
+        return out
+
+    def jacobian_batch(self, state, photo=None):
+        state = np.ascontiguousarray(state, np.float64)
+        ph = None if photo is None else np.ascontiguousarray(photo, np.float64)
+        out = np.zeros((state.shape[0], self.m, self.m))
+        lib().oracle_jacobian_batch(self.ptr, state.shape[0], _p(state), _p(ph), _p(out))
+        return out
+
+    def run_batch(self, tol_newton, tol_time, tadv, state, t_out, dt_out, photo=None, stats=None,
+                  jacobian_mode=0, nthreads=0):
+        """One runDeviceBatch interval; state/t_out/dt_out updated in place."""
+        assert state.dtype == np.float64 and state.flags.c_contiguous
+        assert tadv.dtype == TADV_DTYPE
+        tn = np.ascontiguousarray(tol_newton, np.float64)
+        tt = np.ascontiguousarray(tol_time, np.float64)
+        ph = None if photo is None else np.ascontiguousarray(photo, np.float64)
+        lib().oracle_run_batch(self.ptr, state.shape[0], _p(tn), _p(tt), _p(tadv), _p(state), _p(ph),
+                               _p(t_out), _p(dt_out), _p(stats), int(jacobian_mode), int(nthreads))
+
+
+def lu(A):
+    A = np.array(A, np.float64, order="C")
+    n = A.shape[0]
+    piv = np.zeros(n, np.int32)
+    info = lib().oracle_lu(n, _p(A), n, _p(piv))
+    return A, piv, info
+
+
+def lu_solve(LU, piv, b):
+    b = np.array(b, np.float64)
+    lib().oracle_lu_solve(LU.shape[0], _p(np.ascontiguousarray(LU)), LU.shape[0], _p(piv), _p(b))
+    return b
+
+
+def trbdf2_constants():
+    out = np.zeros(6)
+    lib().oracle_trbdf2_constants(_p(out))
+    return out

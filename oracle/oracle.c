@@ -1,0 +1,441 @@
+/* FP64 CPU oracle — TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Each function cites the reference code it restates.  Parity status of each
+ * part is stated in the header comment of the function; pins live in
+ * tests/test_oracle_*.py.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* TR-BDF2 with gamma = 2 - sqrt(2) (Bank et al. 1985). */
+#define TRB_GAMMA (2.0 - 1.41421356237309504880)
+
+static double pow_int(double x, int n) { return pow(x, (double)n); }
+
+/* Rate constants: src/core/impl/TChem_Impl_KForwardAerosol.hpp:54-131.
+ * Same expression order as the reference; photolysis overrides replace A of
+ * the listed Arrhenius reactions (the per-cell kmcd "model variation" form,
+ * src/core/TChem_AtmosphericChemistry.cpp:79).
+ * Pinned by: closed forms at fixed T, p (tests/test_oracle_rates.py). */
+void oracle_kforward(const oracle_kmd* k, double t, double p, const double* photo, double* kfor) {
+  const double t_1 = 1.0 / t;
+  const double one = 1.0;
+  const double conv = k->CONV_PPM * p * t_1;
+  for (int i = 0; i < k->nArr; ++i) {
+    const double* P = k->arrPar + 5 * i;
+    const int r = k->arrIdx[i];
+    double A = P[0];
+    if (photo)
+      for (int q = 0; q < k->nPhoto; ++q)
+        if (k->photoReac[q] == r) A = photo[q];
+    const double B = P[1], C = P[2], D = P[3], E = P[4];
+    kfor[r] = A * exp(C * t_1) * (B == 0.0 ? one : pow(t / D, B)) * (E == 0.0 ? one : (one + E * p)) *
+              pow_int(conv, k->convExponent[r]);
+  }
+  for (int i = 0; i < k->nTroe; ++i) {
+    const double* rp = k->troePar + 8 * i;
+    const int r = k->troeIdx[i];
+    const double k0_A = rp[0], k0_B = rp[1], k0_C = rp[2];
+    const double kinf_A = rp[3], kinf_B = rp[4], kinf_C = rp[5], Fc = rp[6], N = rp[7];
+    const double k0 =
+        k0_A * (k0_C == 0.0 ? one : exp(k0_C * t_1)) * (k0_B == 0.0 ? one : pow(t / 300.0, k0_B)) * conv;
+    const double k0_kinf =
+        k0 / (kinf_A * (kinf_C == 0.0 ? one : exp(kinf_C * t_1)) * (kinf_B == 0.0 ? one : pow(t / 300.0, kinf_B)));
+    kfor[r] = k0 / (1.0 + k0_kinf) * pow(Fc, 1.0 / (1.0 + pow(log10(k0_kinf) / N, 2.0))) *
+              pow_int(conv, k->convExponent[r]);
+  }
+  for (int i = 0; i < k->nCmaq; ++i) {
+    const double* P = k->cmaqPar + 6 * i;
+    const int r = k->cmaqIdx[i];
+    kfor[r] = (P[0] * exp(P[2] * t_1) * (P[1] == 0.0 ? one : pow(t / 300.0, P[1])) +
+               P[3] * exp(P[5] * t_1) * (P[4] == 0.0 ? one : pow(t / 300.0, P[4])) * conv) *
+              pow_int(conv, k->convExponent[r]);
+  }
+}
+
+/* Source term: src/core/impl/TChem_Impl_ReactionRatesAerosol.hpp:78-138.
+ * omega := 0; omega[sp] := emission rate; rop_r = kfor_r * prod c^|int nu|;
+ * omega[k] += nu * rop for active k, reactants then products, in the
+ * reference's loop order (the serial order of its atomic adds).
+ * Pinned by: the CB05 golden trajectory and brute-force/invariant checks. */
+void oracle_omega(const oracle_kmd* k, const double* kfor, const double* c, double* omega) {
+  const int m = k->nSpec - k->nConstSpec;
+  const int ms = k->maxSpecInReac;
+  for (int i = 0; i < m; ++i) omega[i] = 0.0;
+  for (int i = 0; i < k->nEmis; ++i)
+    if (k->emisSpecies[i] < m) omega[k->emisSpecies[i]] = k->emisRate[i];
+  for (int r = 0; r < k->nReac; ++r) {
+    double rop = kfor[r];
+    for (int j = 0; j < k->reacNreac[r]; ++j) {
+      const int kspec = k->reacSidx[r * ms + j];
+      const int niup = abs((int)k->reacNuki[r * ms + j]);
+      rop *= pow_int(c[kspec], niup);
+    }
+    for (int j = 0; j < k->reacNreac[r]; ++j) {
+      const int kspec = k->reacSidx[r * ms + j];
+      if (kspec < m) omega[kspec] += k->reacNuki[r * ms + j] * rop;
+    }
+    const int joff = ms / 2;
+    for (int j = 0; j < k->reacNprod[r]; ++j) {
+      const int kspec = k->reacSidx[r * ms + j + joff];
+      if (kspec < m) omega[kspec] += k->reacNuki[r * ms + j + joff] * rop;
+    }
+  }
+}
+
+/* Analytic Jacobian of oracle_omega with respect to the active species.
+ * The reference differentiates the same function by forward differences or
+ * Sacado forward AD (src/core/problems/TChem_Impl_AtmosphericChemistry_Problem.hpp:166-241).
+ * Pinned by: central differences and complex-step-free brute force in
+ * tests/test_oracle_jacobian.py. */
+void oracle_jacobian(const oracle_kmd* k, const double* kfor, const double* c, double* J) {
+  const int m = k->nSpec - k->nConstSpec;
+  const int ms = k->maxSpecInReac;
+  memset(J, 0, sizeof(double) * (size_t)m * m);
+  for (int r = 0; r < k->nReac; ++r) {
+    const int nre = k->reacNreac[r];
+    for (int s = 0; s < nre; ++s) {
+      const int ks = k->reacSidx[r * ms + s];
+      if (ks >= m) continue; /* constant species: no derivative */
+      /* d rop / d c_ks = kfor * n_s c_ks^(n_s-1) * prod_{j != s} c_j^n_j */
+      const int ns = abs((int)k->reacNuki[r * ms + s]);
+      double d = kfor[r] * (double)ns * pow_int(c[ks], ns - 1);
+      for (int j = 0; j < nre; ++j) {
+        if (j == s) continue;
+        d *= pow_int(c[k->reacSidx[r * ms + j]], abs((int)k->reacNuki[r * ms + j]));
+      }
+      for (int j = 0; j < nre; ++j) {
+        const int kt = k->reacSidx[r * ms + j];
+        if (kt < m) J[(size_t)kt * m + ks] += k->reacNuki[r * ms + j] * d;
+      }
+      const int joff = ms / 2;
+      for (int j = 0; j < k->reacNprod[r]; ++j) {
+        const int kt = k->reacSidx[r * ms + j + joff];
+        if (kt < m) J[(size_t)kt * m + ks] += k->reacNuki[r * ms + j + joff] * d;
+      }
+    }
+  }
+}
+
+/* Dense LU with partial pivoting (row interchanges), Doolittle form.
+ * The reference factors with Tines' UTV; both solve the same Newton system.
+ * Pinned by: scipy.linalg.lu_factor and the backward error on random and
+ * CB05 iteration matrices. */
+int oracle_lu(int n, double* A, int lda, int* piv) {
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double amax = fabs(A[(size_t)k * lda + k]);
+    for (int i = k + 1; i < n; ++i) {
+      const double a = fabs(A[(size_t)i * lda + k]);
+      if (a > amax) { amax = a; p = i; }
+    }
+    piv[k] = p;
+    if (amax == 0.0) return k + 1;
+    if (p != k)
+      for (int j = 0; j < n; ++j) {
+        const double tmp = A[(size_t)k * lda + j];
+        A[(size_t)k * lda + j] = A[(size_t)p * lda + j];
+        A[(size_t)p * lda + j] = tmp;
+      }
+    const double inv = A[(size_t)k * lda + k];
+    for (int i = k + 1; i < n; ++i) {
+      const double l = A[(size_t)i * lda + k] / inv;
+      A[(size_t)i * lda + k] = l;
+      if (l != 0.0)
+        for (int j = k + 1; j < n; ++j) A[(size_t)i * lda + j] -= l * A[(size_t)k * lda + j];
+    }
+  }
+  return 0;
+}
+
+void oracle_lu_solve(int n, const double* A, int lda, const int* piv, double* b) {
+  for (int k = 0; k < n; ++k) {
+    const int p = piv[k];
+    if (p != k) { const double t = b[k]; b[k] = b[p]; b[p] = t; }
+  }
+  for (int k = 0; k < n; ++k) {
+    const double yk = b[k];
+    if (yk != 0.0)
+      for (int i = k + 1; i < n; ++i) b[i] -= A[(size_t)i * lda + k] * yk;
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    const double xk = b[k] / A[(size_t)k * lda + k];
+    b[k] = xk;
+    if (xk != 0.0)
+      for (int i = 0; i < k; ++i) b[i] -= A[(size_t)i * lda + k] * xk;
+  }
+}
+
+void oracle_trbdf2_constants(double* out6) {
+  const double g = TRB_GAMMA;
+  out6[0] = g;
+  out6[1] = 0.5 * g;                                /* TR stage: u - un - (g/2) h (f + fn) */
+  out6[2] = 1.0 / (g * (2.0 - g));                  /* BDF2: coefficient of u_gamma */
+  out6[3] = (1.0 - g) * (1.0 - g) / (g * (2.0 - g));/* BDF2: coefficient of u_n */
+  out6[4] = (1.0 - g) / (2.0 - g);                  /* BDF2: coefficient of h f(u) (== g/2) */
+  out6[5] = (-3.0 * g * g + 4.0 * g - 2.0) / (12.0 * (2.0 - g)); /* local error constant k */
+}
+
+/* ---------------------------------------------------------------------------
+ * Per-cell integrator state
+ */
+typedef struct {
+  const oracle_kmd* k;
+  int m, mode;
+  double T, p;
+  const double* photo;
+  double* kfor;  /* [nReac] */
+  double* c;     /* [nSpec]: active part is the evaluation point */
+  double* omega; /* [m] */
+  double* J;     /* [m*m] */
+  double* A;     /* [m*m] */
+  int* piv;      /* [m] */
+  double* fh;    /* [m] FD scratch */
+} cell_ctx;
+
+/* source term at u (active part); the constant part of c is fixed per cell */
+static void rhs(cell_ctx* x, const double* u, double* f) {
+  memcpy(x->c, u, sizeof(double) * x->m);
+  if (x->mode == 1) oracle_kforward(x->k, x->T, x->p, x->photo, x->kfor); /* reference: every call */
+  oracle_omega(x->k, x->kfor, x->c, f);
+}
+
+/* Forward-difference Jacobian with m+1 source-term evaluations — the
+ * reference's default (NumericalJacobianForwardDifference,
+ * src/core/problems/TChem_Impl_AtmosphericChemistry_Problem.hpp:217-241).
+ * The increment rule is Tines' and is not vendored; this uses
+ * h_j = sqrt(eps) * max(|u_j|, 1e-10). */
+static void jac_fd(cell_ctx* x, const double* u, double* J) {
+  const int m = x->m;
+  double* u2 = x->A; /* borrow: A is rebuilt right after */
+  double* f0 = x->omega;
+  rhs(x, u, f0);
+  for (int j = 0; j < m; ++j) u2[j] = u[j];
+  for (int j = 0; j < m; ++j) {
+    const double h = 1.4901161193847656e-08 * (fabs(u[j]) > 1e-10 ? fabs(u[j]) : 1e-10);
+    u2[j] = u[j] + h;
+    rhs(x, u2, x->fh);
+    u2[j] = u[j];
+    for (int i = 0; i < m; ++i) J[(size_t)i * m + j] = (x->fh[i] - f0[i]) / h;
+  }
+}
+
+static void jac(cell_ctx* x, const double* u, double* J) {
+  if (x->mode == 1) {
+    jac_fd(x, u, J);
+  } else {
+    memcpy(x->c, u, sizeof(double) * x->m);
+    oracle_jacobian(x->k, x->kfor, x->c, J);
+  }
+}
+
+/* Newton iteration for one TR-BDF2 stage (Tines NewtonSolver, restated):
+ * G(u) = u - b - ch * f(u), iteration matrix I - ch*J.  The Jacobian and its
+ * factorisation are refreshed when the iteration count within the stage is a
+ * multiple of jacobian_interval (reference default 1: every iteration).
+ * Converged when max_i |dx_i| / (atol + rtol |u_i|) <= 1. */
+static int newton(cell_ctx* x, double* u, const double* b, double ch, const double* tol_newton, int max_it,
+                  int jac_interval, double* f, double* dx, oracle_stats* st) {
+  const int m = x->m;
+  if (jac_interval < 1) jac_interval = 1;
+  for (int it = 0; it < max_it; ++it) {
+    if (it % jac_interval == 0) {
+      jac(x, u, x->J);
+      for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j)
+          x->A[(size_t)i * m + j] = (i == j ? 1.0 : 0.0) - ch * x->J[(size_t)i * m + j];
+      if (oracle_lu(m, x->A, m, x->piv) != 0) return 0;
+      if (st) st->factorizations++;
+    }
+    rhs(x, u, f);
+    for (int i = 0; i < m; ++i) dx[i] = -(u[i] - b[i] - ch * f[i]);
+    oracle_lu_solve(m, x->A, m, x->piv, dx);
+    double nrm = 0.0;
+    for (int i = 0; i < m; ++i) {
+      u[i] += dx[i];
+      const double w = fabs(dx[i]) / (tol_newton[0] + tol_newton[1] * fabs(u[i]));
+      if (!(w <= nrm)) nrm = w; /* NaN propagates as non-convergence */
+    }
+    if (st) st->newton_iterations++;
+    if (nrm <= 1.0) return 1;
+  }
+  return 0;
+}
+
+/* One cell of AtmosphericChemistry::runDeviceBatch
+ * (src/core/TChem_AtmosphericChemistry.cpp:78-163): integrate the active
+ * species of `row` in place from tbeg for at most
+ * num_time_iterations_per_interval TR-BDF2 iterations.
+ *
+ * Step control (Tines' TimeIntegratorTrBDF2 is not vendored; the rules below
+ * are the ones the reference goldens fix, thresholds are this project's):
+ *   h = min(max(dt, dtmin), tend - t)           clamp to t_end before the step
+ *   Newton failure in either stage -> dt = h/2, up to 4 attempts, then hard
+ *     failure (species zeroed, t_out = tend, dt_out = -1)
+ *   e = max_i |est_i| / (atol_i + rtol_i |u_i|), est = LTE estimate
+ *   e > 1 and h > dtmin -> reject, dt = h/4 (the iteration is consumed)
+ *   otherwise accept; next dt = 2h if e <= 1/8 else h, clamped to dtmax.
+ */
+static void integrate_cell(const oracle_kmd* k, const double* tol_newton, const double* tol_time,
+                           const oracle_time_advance* ta, double* row, const double* photo, double* t_out,
+                           double* dt_out, oracle_stats* st, int mode, double* work, int* iwork) {
+  const int nSpec = k->nSpec, m = nSpec - k->nConstSpec;
+  const double tend = ta->tend;
+  if (*t_out >= tend) return; /* .cpp:83 */
+  const double dt_in = ta->dt;
+  if (dt_in < 0.0) return; /* a failed cell stays failed */
+
+  double g6[6];
+  oracle_trbdf2_constants(g6);
+  const double g = g6[0], c_tr = g6[1], a_g = g6[2], a_n = g6[3], c_bdf = g6[4], k_err = g6[5];
+
+  cell_ctx x;
+  x.k = k;
+  x.m = m;
+  x.mode = mode;
+  x.p = row[1];
+  x.T = row[2];
+  x.photo = photo;
+  double* w = work;
+  x.kfor = w; w += k->nReac;
+  x.c = w; w += nSpec;
+  x.omega = w; w += m;
+  x.J = w; w += (size_t)m * m;
+  x.A = w; w += (size_t)m * m;
+  x.fh = w; w += m;
+  x.piv = iwork;
+  double* un = w; w += m;
+  double* u = w; w += m;
+  double* ug = w; w += m;
+  double* fn = w; w += m;
+  double* fg = w; w += m;
+  double* f1 = w; w += m;
+  double* f = w; w += m;
+  double* dx = w; w += m;
+  double* b = w; w += m;
+
+  for (int i = 0; i < nSpec; ++i) x.c[i] = row[3 + i];
+  for (int i = 0; i < m; ++i) un[i] = row[3 + i];
+  if (mode == 0) oracle_kforward(k, x.T, x.p, photo, x.kfor); /* constant per cell */
+
+  const int max_it = ta->num_time_iterations_per_interval;
+  const int max_newton = ta->max_num_newton_iterations;
+  const int jint = ta->jacobian_interval;
+  double t = ta->tbeg, dt = dt_in;
+  int fn_valid = 0, failed = 0;
+
+  for (int iter = 0; iter < max_it && t < tend; ++iter) {
+    int conv = 0;
+    double h = 0.0;
+    for (int attempt = 0; attempt < 4 && !conv; ++attempt) {
+      h = dt > ta->dtmin ? dt : ta->dtmin;
+      if (h > tend - t) h = tend - t;
+      if (!(t + h > t)) break; /* no representable progress: hard failure */
+      if (!fn_valid) { rhs(&x, un, fn); fn_valid = 1; }
+      /* TR stage */
+      for (int i = 0; i < m; ++i) { u[i] = un[i]; b[i] = un[i] + c_tr * h * fn[i]; }
+      conv = newton(&x, u, b, c_tr * h, tol_newton, max_newton, jint, f, dx, st);
+      if (conv) {
+        /* BDF2 stage */
+        for (int i = 0; i < m; ++i) { ug[i] = u[i]; b[i] = a_g * ug[i] - a_n * un[i]; }
+        conv = newton(&x, u, b, c_bdf * h, tol_newton, max_newton, jint, f, dx, st);
+      }
+      if (!conv) {
+        if (st) st->newton_failures++;
+        dt = 0.5 * h;
+      }
+    }
+    if (!conv) { failed = 1; break; }
+
+    rhs(&x, ug, fg);
+    rhs(&x, u, f1);
+    double e = 0.0;
+    for (int i = 0; i < m; ++i) {
+      const double est = 2.0 * k_err * h * (fn[i] / g - fg[i] / (g * (1.0 - g)) + f1[i] / (1.0 - g));
+      const double wi = fabs(est) / (tol_time[2 * i] + tol_time[2 * i + 1] * fabs(u[i]));
+      if (!(wi <= e)) e = wi;
+    }
+    if (e > 1.0 && h > ta->dtmin) {
+      if (st) st->rejected++;
+      dt = 0.25 * h;
+      continue;
+    }
+    t = (h == tend - t) ? tend : t + h;
+    for (int i = 0; i < m; ++i) { un[i] = u[i]; fn[i] = f1[i]; }
+    dt = (e <= 0.125) ? 2.0 * h : h;
+    if (dt > ta->dtmax) dt = ta->dtmax;
+    if (st) st->accepted++;
+  }
+
+  if (failed) {
+    for (int i = 0; i < m; ++i) row[3 + i] = 0.0;
+    *t_out = tend;
+    *dt_out = -1.0;
+    if (st) st->failed = 1;
+  } else {
+    for (int i = 0; i < m; ++i) row[3 + i] = un[i];
+    *t_out = t;
+    *dt_out = dt;
+  }
+}
+
+void oracle_run_batch(const oracle_kmd* k, int nBatch, const double* tol_newton, const double* tol_time,
+                      const oracle_time_advance* tadv, double* state, const double* photo, double* t_out,
+                      double* dt_out, oracle_stats* stats, int jacobian_mode, int nthreads) {
+  const int nSpec = k->nSpec, m = nSpec - k->nConstSpec;
+  const size_t wlen = (size_t)k->nReac + nSpec + 2 * (size_t)m * m + 12 * (size_t)m;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+  {
+    double* work = (double*)malloc(sizeof(double) * wlen);
+    int* iwork = (int*)malloc(sizeof(int) * (size_t)m);
+#pragma omp for schedule(static)
+    for (int c = 0; c < nBatch; ++c) {
+      integrate_cell(k, tol_newton, tol_time, &tadv[c], state + (size_t)c * (nSpec + 3),
+                     photo ? photo + (size_t)c * k->nPhoto : NULL, &t_out[c], &dt_out[c],
+                     stats ? &stats[c] : NULL, jacobian_mode, work, iwork);
+    }
+    free(work);
+    free(iwork);
+  }
+}
+
+void oracle_sourceterm_batch(const oracle_kmd* k, int nBatch, const double* state, const double* photo,
+                             double* omega) {
+  const int nSpec = k->nSpec, m = nSpec - k->nConstSpec;
+#pragma omp parallel
+  {
+    double* kfor = (double*)malloc(sizeof(double) * (size_t)k->nReac);
+#pragma omp for schedule(static)
+    for (int c = 0; c < nBatch; ++c) {
+      const double* row = state + (size_t)c * (nSpec + 3);
+      oracle_kforward(k, row[2], row[1], photo ? photo + (size_t)c * k->nPhoto : NULL, kfor);
+      oracle_omega(k, kfor, row + 3, omega + (size_t)c * m);
+    }
+    free(kfor);
+  }
+}
+
+void oracle_jacobian_batch(const oracle_kmd* k, int nBatch, const double* state, const double* photo,
+                           double* jacv) {
+  const int nSpec = k->nSpec, m = nSpec - k->nConstSpec;
+#pragma omp parallel
+  {
+    double* kfor = (double*)malloc(sizeof(double) * (size_t)k->nReac);
+#pragma omp for schedule(static)
+    for (int c = 0; c < nBatch; ++c) {
+      const double* row = state + (size_t)c * (nSpec + 3);
+      oracle_kforward(k, row[2], row[1], photo ? photo + (size_t)c * k->nPhoto : NULL, kfor);
+      oracle_jacobian(k, kfor, row + 3, jacv + (size_t)c * m * m);
+    }
+    free(kfor);
+  }
+}

@@ -1,0 +1,265 @@
+"""Host-side mirror of TChem-atm's batched gas-phase kinetics interface.
+
+Python classes named after the reference's C++ types, calling the C ABI of
+``libtchem_atm.so`` (declared in include/tchem_atm.h) through ctypes:
+
+  KineticModelData(mechfile)                   src/core/TChem_KineticModelData.cpp:53-62
+  KineticModelData.createConstData()           createNCAR_KineticModelConstData, ...ConstData.hpp:115-148
+  setScenarioConditions(file, kmd)             src/core/TChem_AtmosphericChemistry.cpp:429-504
+  AtmosphericChemistry.runDeviceBatch(...)     src/core/TChem_AtmosphericChemistry.hpp:83-97
+  AtmosphericChemistry.getWorkSpaceSize(kmcd)  src/core/TChem_AtmosphericChemistry.hpp:47-81
+  SourceTerm.runDeviceBatch(...)               src/examples/TChem_GasChemistry_RHSs.cpp:184-236
+  Jacobian.runDeviceBatch(...)                 src/examples/TChem_GasChemistry_RHSs.cpp:254-309
+
+Device arrays are torch tensors on a HIP device (float64, contiguous); torch
+provides device memory and streams only.  Every compute call goes to the HIP
+kernels: there is no CPU fallback, and a missing library raises at import.
+Errors raise TChemError with the library's message (the reference throws
+std::logic_error from TCHEM_CHECK_ERROR, src/core/TChem_Util.hpp:108-114).
+
+The package directory name contains a dash; import it with
+``importlib.import_module("tchem-atm_amd")``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtchem_atm.so")
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+SCHEDULE_GRID = 0
+SCHEDULE_PERSISTENT = 1
+
+TIME_ADVANCE_DTYPE = np.dtype([
+    ("_tbeg", "f8"), ("_tend", "f8"), ("_dt", "f8"), ("_dtmin", "f8"), ("_dtmax", "f8"),
+    ("_max_num_newton_iterations", "i4"), ("_num_time_iterations_per_interval", "i4"),
+    ("_num_outer_time_iterations_per_interval", "i4"), ("_jacobian_interval", "i4"),
+])
+assert TIME_ADVANCE_DTYPE.itemsize == 56
+CELL_STATS_DTYPE = np.dtype([("accepted", "i4"), ("rejected", "i4"), ("newton_failures", "i4"),
+                             ("newton_iterations", "i4"), ("factorizations", "i4"), ("failed", "i4")])
+
+# every symbol include/tchem_atm.h declares
+EXPORTED_SYMBOLS = (
+    "tchem_last_error", "tchem_kmd_load_yaml", "tchem_kmd_destroy", "tchem_kmd_info", "tchem_kmd_species_name",
+    "tchem_kmd_photolysis_reactions", "tchem_scenario_load", "tchem_kmcd_create", "tchem_kmcd_destroy",
+    "tchem_atmchem_workspace_bytes", "tchem_atmchem_run_device_batch", "tchem_atmchem_update_time_advance",
+    "tchem_sourceterm_run_device_batch", "tchem_jacobian_run_device_batch",
+)
+
+
+class TChemError(RuntimeError):
+    pass
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("schedule", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
+                ("reserved", ctypes.c_int * 5)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C tchem-atm_amd/csrc)")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, i, cp = ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p
+    lib.tchem_last_error.restype = cp
+    sig = {
+        "tchem_kmd_load_yaml": [cp, cp, ctypes.POINTER(vp)],
+        "tchem_kmd_destroy": [vp],
+        "tchem_kmd_info": [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)],
+        "tchem_kmd_species_name": [vp, i, cp, ctypes.c_size_t],
+        "tchem_kmd_photolysis_reactions": [vp, vp, i],
+        "tchem_scenario_load": [vp, cp, vp, i, ctypes.POINTER(i)],
+        "tchem_kmcd_create": [vp, ctypes.POINTER(vp)],
+        "tchem_kmcd_destroy": [vp],
+        "tchem_atmchem_workspace_bytes": [vp, i, ctypes.POINTER(ctypes.c_size_t)],
+        "tchem_atmchem_run_device_batch": [ctypes.POINTER(_Options), i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                           vp],
+        "tchem_atmchem_update_time_advance": [i, vp, vp, vp, vp],
+        "tchem_sourceterm_run_device_batch": [i, vp, vp, vp, vp, vp, vp],
+        "tchem_jacobian_run_device_batch": [i, vp, vp, vp, vp, vp],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = i
+    return lib
+
+
+_lib = _load()
+
+
+def lib():
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise TChemError(f"tchem error {rc}: {_lib.tchem_last_error().decode()}")
+
+
+def _ptr(t, dtype_name="float64", what="array"):
+    """Device pointer of a contiguous torch tensor (None passes NULL)."""
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    if not t.is_cuda:
+        raise TChemError(f"{what} must be a device tensor")
+    if not t.is_contiguous():
+        raise TChemError(f"{what} must be contiguous")
+    if dtype_name and str(t.dtype) != f"torch.{dtype_name}":
+        raise TChemError(f"{what} must be {dtype_name}, got {t.dtype}")
+    return t.data_ptr()
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    return stream if isinstance(stream, int) else stream.cuda_stream
+
+
+class KineticModelData:
+    """NCAR-format mechanism loaded on the host (KineticModelData)."""
+
+    def __init__(self, mechfile: str, echofile: str | None = None):
+        h = ctypes.c_void_p()
+        _check(_lib.tchem_kmd_load_yaml(mechfile.encode(), echofile.encode() if echofile else None,
+                                        ctypes.byref(h)))
+        self._h = h
+        n = [ctypes.c_int() for _ in range(4)]
+        _check(_lib.tchem_kmd_info(h, *[ctypes.byref(x) for x in n]))
+        self.nSpec, self.nConstSpec, self.nReac, self.nPhoto = (x.value for x in n)
+        buf = ctypes.create_string_buffer(64)
+        self.speciesNames = []
+        for k in range(self.nSpec):
+            _check(_lib.tchem_kmd_species_name(h, k, buf, 64))
+            self.speciesNames.append(buf.value.decode())
+        pr = np.zeros(max(self.nPhoto, 1), np.int32)
+        _check(_lib.tchem_kmd_photolysis_reactions(h, pr.ctypes.data, self.nPhoto))
+        self.photolysisReactions = pr[: self.nPhoto].copy()
+
+    @property
+    def nActive(self) -> int:
+        return self.nSpec - self.nConstSpec
+
+    def createConstData(self) -> "KineticModelConstData":
+        return KineticModelConstData(self)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.tchem_kmd_destroy(h)
+            self._h = None
+
+
+class KineticModelConstData:
+    """Device-resident compiled mechanism (KineticModelNCAR_ConstData)."""
+
+    def __init__(self, kmd: KineticModelData):
+        h = ctypes.c_void_p()
+        _check(_lib.tchem_kmcd_create(kmd._h, ctypes.byref(h)))
+        self._h = h
+        self.nSpec, self.nConstSpec, self.nReac, self.nPhoto = kmd.nSpec, kmd.nConstSpec, kmd.nReac, kmd.nPhoto
+
+    @property
+    def nActive(self) -> int:
+        return self.nSpec - self.nConstSpec
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.tchem_kmcd_destroy(h)
+            self._h = None
+
+
+def createNCAR_KineticModelConstData(kmd: KineticModelData) -> KineticModelConstData:
+    return KineticModelConstData(kmd)
+
+
+def setScenarioConditions(filename: str, kmd: KineticModelData) -> np.ndarray:
+    """Host state rows [nBatch][nSpec+3] from a scenario YAML."""
+    n = ctypes.c_int()
+    _check(_lib.tchem_scenario_load(kmd._h, filename.encode(), None, 0, ctypes.byref(n)))
+    state = np.zeros((n.value, kmd.nSpec + 3))
+    _check(_lib.tchem_scenario_load(kmd._h, filename.encode(), state.ctypes.data, n.value, ctypes.byref(n)))
+    return state
+
+
+class AtmosphericChemistry:
+    @staticmethod
+    def getWorkSpaceSize(kmcd: KineticModelConstData, precision: int = PRECISION_FP64) -> int:
+        """Bytes of LDS one cell uses."""
+        b = ctypes.c_size_t()
+        _check(_lib.tchem_atmchem_workspace_bytes(kmcd._h, precision, ctypes.byref(b)))
+        return b.value
+
+    @staticmethod
+    def runDeviceBatch(tol_newton, tol_time, fac, tadv, state, t_out, dt_out, state_out, kmcd, photo_rates=None,
+                       stats=None, precision=PRECISION_FP64, schedule=SCHEDULE_GRID, blocks_per_cu=0, stream=None):
+        """One integration interval over the batch (asynchronous on `stream`).
+
+        tadv is a uint8 device tensor holding TIME_ADVANCE_DTYPE records (or any
+        tensor with 56 bytes per cell); stats likewise CELL_STATS_DTYPE."""
+        nBatch = state.shape[0]
+        opts = _Options(precision, schedule, blocks_per_cu)
+        _check(_lib.tchem_atmchem_run_device_batch(
+            ctypes.byref(opts), nBatch, _ptr(tol_newton, what="tol_newton"), _ptr(tol_time, what="tol_time"),
+            _ptr(fac, None, "fac"), _ptr(tadv, None, "tadv"), _ptr(state, what="state"),
+            _ptr(photo_rates, what="photo_rates"), _ptr(t_out, what="t_out"), _ptr(dt_out, what="dt_out"),
+            _ptr(state_out, what="state_out"), _ptr(stats, None, "stats"), kmcd._h, _stream_ptr(stream)))
+
+    @staticmethod
+    def updateTimeAdvance(tadv, t_out, dt_out, stream=None):
+        """tadv._tbeg = t_out, tadv._dt = dt_out (the driver's between-interval step)."""
+        n = t_out.shape[0]
+        _check(_lib.tchem_atmchem_update_time_advance(n, _ptr(tadv, None, "tadv"), _ptr(t_out), _ptr(dt_out),
+                                                      _stream_ptr(stream)))
+
+
+class SourceTerm:
+    @staticmethod
+    def runDeviceBatch(state, photo_rates, external_sources, omega, kmcd, stream=None):
+        _check(_lib.tchem_sourceterm_run_device_batch(
+            state.shape[0], _ptr(state, what="state"), _ptr(photo_rates, what="photo_rates"),
+            _ptr(external_sources, what="external_sources"), _ptr(omega, what="omega"), kmcd._h,
+            _stream_ptr(stream)))
+
+
+class Jacobian:
+    @staticmethod
+    def runDeviceBatch(state, photo_rates, jacobian, kmcd, stream=None):
+        _check(_lib.tchem_jacobian_run_device_batch(
+            state.shape[0], _ptr(state, what="state"), _ptr(photo_rates, what="photo_rates"),
+            _ptr(jacobian, what="jacobian"), kmcd._h, _stream_ptr(stream)))
+
+
+def time_advance_tensor(n, tbeg, tend, dt, dtmin, dtmax, max_newton, iters_per_interval, jacobian_interval=1,
+                        device="cuda"):
+    """Device tensor (uint8, 56 bytes per cell) of TimeAdvance records."""
+    import torch
+    rec = np.zeros(n, TIME_ADVANCE_DTYPE)
+    rec["_tbeg"], rec["_tend"], rec["_dt"], rec["_dtmin"], rec["_dtmax"] = tbeg, tend, dt, dtmin, dtmax
+    rec["_max_num_newton_iterations"] = max_newton
+    rec["_num_time_iterations_per_interval"] = iters_per_interval
+    rec["_jacobian_interval"] = jacobian_interval
+    return torch.from_numpy(rec.view(np.uint8).copy()).to(device)
+
+
+def time_advance_from_tensor(t) -> np.ndarray:
+    return t.cpu().numpy().view(TIME_ADVANCE_DTYPE)
+
+
+def stats_tensor(n, device="cuda"):
+    import torch
+    return torch.zeros(n * CELL_STATS_DTYPE.itemsize, dtype=torch.uint8, device=device)
+
+
+def stats_from_tensor(t) -> np.ndarray:
+    return t.cpu().numpy().view(CELL_STATS_DTYPE)

@@ -1,0 +1,317 @@
+// C ABI (include/tchem_atm.h): handles, validation, device table upload and
+// kernel launches.  No exception crosses this boundary; every entry point
+// returns a status code and records a message for tchem_last_error().
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/tchem_atm.h"
+#include "device_mech.hpp"
+#include "kinetic_model.hpp"
+#include "mechanism.hpp"
+
+namespace tchem_amd {
+hipError_t launchIntegrator(const IntegratorArgs& a, int precision, int grid, size_t lds, hipStream_t stream);
+hipError_t integratorMaxBlocksPerCU(int precision, int m, size_t lds, int* blocks);
+hipError_t launchResetCounter(int* counter, hipStream_t stream);
+hipError_t launchUpdateTadv(int n, tchem_time_advance_t* tadv, const double* t_out, const double* dt_out,
+                            hipStream_t stream);
+hipError_t launchSourceTerm(const DevMech& M, int nBatch, const double* state, const double* photo, const double* ext,
+                            double* omega, hipStream_t stream);
+hipError_t launchJacobian(const DevMech& M, int nBatch, const double* state, const double* photo, double* jac,
+                          hipStream_t stream);
+}  // namespace tchem_amd
+
+using namespace tchem_amd;
+
+struct tchem_kmd_s {
+  KineticModel km;
+};
+
+struct tchem_kmcd_s {
+  MechTables tables;
+  DevMech dev{};
+  std::vector<void*> allocations;
+  int device = 0;
+  int numCUs = 0;
+  size_t maxLds = 0;
+  int* queue = nullptr;
+  ~tchem_kmcd_s() {
+    int cur = 0;
+    if (hipGetDevice(&cur) == hipSuccess && cur != device) (void)hipSetDevice(device);
+    for (void* p : allocations) (void)hipFree(p);
+    if (cur != device) (void)hipSetDevice(cur);
+  }
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+int setError(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+int hipFail(hipError_t e, const char* what) {
+  return setError(TCHEM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+int upload(tchem_kmcd_s* k, const std::vector<T>& v, const T** out) {
+  if (v.empty()) {
+    *out = nullptr;
+    return TCHEM_OK;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, sizeof(T) * v.size());
+  if (e != hipSuccess) return hipFail(e, "hipMalloc");
+  k->allocations.push_back(p);
+  e = hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hipFail(e, "hipMemcpy");
+  *out = static_cast<const T*>(p);
+  return TCHEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tchem_last_error(void) { return g_error.c_str(); }
+
+int tchem_kmd_load_yaml(const char* path, const char* echo_path, tchem_kmd_t* out) {
+  if (!path || !out) return setError(TCHEM_ERR_INVALID, "tchem_kmd_load_yaml: null argument");
+  *out = nullptr;
+  try {
+    std::unique_ptr<tchem_kmd_s> h(new tchem_kmd_s);
+    if (echo_path) {
+      std::ofstream echo(echo_path);
+      if (!echo) return setError(TCHEM_ERR_INVALID, std::string("cannot write echo file ") + echo_path);
+      h->km = loadKineticModel(path, &echo);
+    } else {
+      h->km = loadKineticModel(path, nullptr);
+    }
+    *out = h.release();
+    return TCHEM_OK;
+  } catch (const std::exception& e) {
+    return setError(TCHEM_ERR_PARSE, e.what());
+  }
+}
+
+int tchem_kmd_destroy(tchem_kmd_t kmd) {
+  delete kmd;
+  return TCHEM_OK;
+}
+
+int tchem_kmd_info(tchem_kmd_t kmd, int* nSpec, int* nConstSpec, int* nReac, int* nPhoto) {
+  if (!kmd) return setError(TCHEM_ERR_INVALID, "tchem_kmd_info: null handle");
+  if (nSpec) *nSpec = kmd->km.nSpec;
+  if (nConstSpec) *nConstSpec = kmd->km.nConstSpec;
+  if (nReac) *nReac = kmd->km.nReac;
+  if (nPhoto) *nPhoto = int(kmd->km.photolysisReactions.size());
+  return TCHEM_OK;
+}
+
+int tchem_kmd_species_name(tchem_kmd_t kmd, int i, char* buf, size_t len) {
+  if (!kmd || !buf || len == 0) return setError(TCHEM_ERR_INVALID, "tchem_kmd_species_name: bad argument");
+  if (i < 0 || i >= kmd->km.nSpec) return setError(TCHEM_ERR_INVALID, "tchem_kmd_species_name: index out of range");
+  std::strncpy(buf, kmd->km.speciesNames[i].c_str(), len - 1);
+  buf[len - 1] = '\0';
+  return TCHEM_OK;
+}
+
+int tchem_kmd_photolysis_reactions(tchem_kmd_t kmd, int* reactions, int capacity) {
+  if (!kmd || (!reactions && capacity > 0)) return setError(TCHEM_ERR_INVALID, "tchem_kmd_photolysis_reactions: bad argument");
+  const auto& v = kmd->km.photolysisReactions;
+  if (capacity < int(v.size())) return setError(TCHEM_ERR_INVALID, "tchem_kmd_photolysis_reactions: capacity too small");
+  for (size_t q = 0; q < v.size(); ++q) reactions[q] = v[q];
+  return TCHEM_OK;
+}
+
+int tchem_scenario_load(tchem_kmd_t kmd, const char* path, double* state_host, int capacity_rows, int* nBatch) {
+  if (!kmd || !path || !nBatch) return setError(TCHEM_ERR_INVALID, "tchem_scenario_load: null argument");
+  try {
+    int n = 0;
+    std::vector<double> s = loadScenarioState(path, kmd->km, &n);
+    *nBatch = n;
+    if (state_host) {
+      if (capacity_rows < n) return setError(TCHEM_ERR_INVALID, "tchem_scenario_load: capacity too small");
+      std::memcpy(state_host, s.data(), sizeof(double) * s.size());
+    }
+    return TCHEM_OK;
+  } catch (const std::exception& e) {
+    return setError(TCHEM_ERR_PARSE, e.what());
+  }
+}
+
+int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
+  if (!kmd || !out) return setError(TCHEM_ERR_INVALID, "tchem_kmcd_create: null argument");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return setError(TCHEM_ERR_NO_DEVICE, "tchem_kmcd_create: no HIP device");
+  std::unique_ptr<tchem_kmcd_s> k(new tchem_kmcd_s);
+  try {
+    k->tables = compileMechanism(kmd->km);
+  } catch (const std::exception& e) {
+    return setError(TCHEM_ERR_UNSUPPORTED, e.what());
+  }
+  const MechTables& t = k->tables;
+  if (t.m > 256) return setError(TCHEM_ERR_UNSUPPORTED, "more than 256 active species");
+  hipError_t e = hipGetDevice(&k->device);
+  if (e != hipSuccess) return hipFail(e, "hipGetDevice");
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, k->device);
+  if (e != hipSuccess) return hipFail(e, "hipGetDeviceProperties");
+  k->numCUs = prop.multiProcessorCount;
+  k->maxLds = prop.sharedMemPerBlock;
+
+  DevMech& d = k->dev;
+  d.nSpec = t.nSpec;
+  d.nConst = t.nConst;
+  d.m = t.m;
+  d.nReac = t.nReac;
+  d.nPhoto = t.nPhoto;
+  d.maxR = t.maxReactants;
+  d.nPairs = t.nPairs();
+  d.nnzJ = t.nnzJ();
+  d.convPPM = t.convPPM;
+  std::vector<int> rowcol(t.jnzRow.size());
+  for (size_t n = 0; n < rowcol.size(); ++n) rowcol[n] = (t.jnzRow[n] << 16) | t.jnzCol[n];
+  int rc = TCHEM_OK;
+  if ((rc = upload(k.get(), t.rateKind, &d.rateKind)) || (rc = upload(k.get(), t.ratePar, &d.ratePar)) ||
+      (rc = upload(k.get(), t.convExp, &d.convExp)) || (rc = upload(k.get(), t.photoSlot, &d.photoSlot)) ||
+      (rc = upload(k.get(), t.reacSpec, &d.reacSpec)) || (rc = upload(k.get(), t.reacPow, &d.reacPow)) ||
+      (rc = upload(k.get(), t.omegaPtr, &d.omegaPtr)) || (rc = upload(k.get(), t.omegaReac, &d.omegaReac)) ||
+      (rc = upload(k.get(), t.omegaCoef, &d.omegaCoef)) || (rc = upload(k.get(), t.emission, &d.emission)) ||
+      (rc = upload(k.get(), t.pairPtr, &d.pairPtr)) || (rc = upload(k.get(), t.pairSlot, &d.pairSlot)) ||
+      (rc = upload(k.get(), rowcol, &d.jnzRowCol)) || (rc = upload(k.get(), t.jnzPtr, &d.jnzPtr)) ||
+      (rc = upload(k.get(), t.jcPair, &d.jcPair)) || (rc = upload(k.get(), t.jcCoef, &d.jcCoef)))
+    return rc;
+  void* q = nullptr;
+  e = hipMalloc(&q, 256);
+  if (e != hipSuccess) return hipFail(e, "hipMalloc");
+  k->allocations.push_back(q);
+  k->queue = static_cast<int*>(q);
+  *out = k.release();
+  return TCHEM_OK;
+}
+
+int tchem_kmcd_destroy(tchem_kmcd_t kmcd) {
+  delete kmcd;
+  return TCHEM_OK;
+}
+
+int tchem_atmchem_workspace_bytes(tchem_kmcd_t kmcd, int precision, size_t* bytes) {
+  if (!kmcd || !bytes) return setError(TCHEM_ERR_INVALID, "tchem_atmchem_workspace_bytes: null argument");
+  if (precision != TCHEM_PRECISION_FP64 && precision != TCHEM_PRECISION_MIXED)
+    return setError(TCHEM_ERR_INVALID, "unknown precision");
+  *bytes = integratorLdsBytes(kmcd->dev, precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float));
+  return TCHEM_OK;
+}
+
+int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBatch, const double* tol_newton,
+                                   const double* tol_time, double* fac, const tchem_time_advance_t* tadv,
+                                   double* state, const double* photo_rates, double* t_out, double* dt_out,
+                                   double* state_out, tchem_cell_stats_t* stats, tchem_kmcd_t kmcd, void* stream) {
+  (void)fac;
+  tchem_atmchem_options_t o{};
+  if (opts) o = *opts;
+  if (!kmcd) return setError(TCHEM_ERR_INVALID, "run_device_batch: null kmcd");
+  if (nBatch < 0) return setError(TCHEM_ERR_INVALID, "run_device_batch: negative nBatch");
+  for (int r : o.reserved)
+    if (r != 0) return setError(TCHEM_ERR_INVALID, "run_device_batch: reserved option fields must be zero");
+  if (o.precision != TCHEM_PRECISION_FP64 && o.precision != TCHEM_PRECISION_MIXED)
+    return setError(TCHEM_ERR_INVALID, "run_device_batch: unknown precision");
+  if (o.schedule != TCHEM_SCHEDULE_GRID && o.schedule != TCHEM_SCHEDULE_PERSISTENT)
+    return setError(TCHEM_ERR_INVALID, "run_device_batch: unknown schedule");
+  if (o.blocks_per_cu < 0) return setError(TCHEM_ERR_INVALID, "run_device_batch: negative blocks_per_cu");
+  if (nBatch == 0) return TCHEM_OK;
+  if (!tol_newton || !tol_time || !tadv || !state || !t_out || !dt_out || !state_out)
+    return setError(TCHEM_ERR_INVALID, "run_device_batch: null device array");
+  if (kmcd->dev.nPhoto == 0) photo_rates = nullptr;
+
+  const size_t lds =
+      integratorLdsBytes(kmcd->dev, o.precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float));
+  if (lds > kmcd->maxLds)
+    return setError(TCHEM_ERR_UNSUPPORTED, "run_device_batch: mechanism needs " + std::to_string(lds) +
+                                               " bytes of LDS per cell, device limit " + std::to_string(kmcd->maxLds));
+  int cur = 0;
+  if (hipGetDevice(&cur) != hipSuccess || cur != kmcd->device)
+    return setError(TCHEM_ERR_INVALID, "run_device_batch: kmcd belongs to another device");
+
+  IntegratorArgs a{};
+  a.M = kmcd->dev;
+  a.nBatch = nBatch;
+  a.tol_newton = tol_newton;
+  a.tol_time = tol_time;
+  a.tadv = tadv;
+  a.state = state;
+  a.photo = photo_rates;
+  a.t_out = t_out;
+  a.dt_out = dt_out;
+  a.state_out = state_out;
+  a.stats = stats;
+  a.queue = kmcd->queue;
+  a.persistent = (o.schedule == TCHEM_SCHEDULE_PERSISTENT) ? 1 : 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+
+  int grid = nBatch;
+  if (a.persistent) {
+    int bpc = o.blocks_per_cu;
+    if (bpc == 0) {
+      hipError_t e = integratorMaxBlocksPerCU(o.precision, kmcd->dev.m, lds, &bpc);
+      if (e != hipSuccess) return hipFail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+      if (bpc < 1) bpc = 1;
+    }
+    const long long g = (long long)bpc * kmcd->numCUs;
+    grid = int(g < nBatch ? g : nBatch);
+    hipError_t e = launchResetCounter(kmcd->queue, s);
+    if (e != hipSuccess) return hipFail(e, "reset_counter_kernel");
+  }
+  hipError_t e = launchIntegrator(a, o.precision, grid, lds, s);
+  if (e != hipSuccess) return hipFail(e, "trbdf2_kernel launch");
+  return TCHEM_OK;
+}
+
+int tchem_atmchem_update_time_advance(int nBatch, tchem_time_advance_t* tadv, const double* t_out,
+                                      const double* dt_out, void* stream) {
+  if (nBatch < 0 || (nBatch > 0 && (!tadv || !t_out || !dt_out)))
+    return setError(TCHEM_ERR_INVALID, "update_time_advance: bad argument");
+  hipError_t e = launchUpdateTadv(nBatch, tadv, t_out, dt_out, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hipFail(e, "update_tadv_kernel");
+  return TCHEM_OK;
+}
+
+int tchem_sourceterm_run_device_batch(int nBatch, const double* state, const double* photo_rates,
+                                      const double* external_sources, double* omega, tchem_kmcd_t kmcd,
+                                      void* stream) {
+  if (!kmcd || nBatch < 0) return setError(TCHEM_ERR_INVALID, "sourceterm: bad argument");
+  if (nBatch == 0) return TCHEM_OK;
+  if (!state || !omega) return setError(TCHEM_ERR_INVALID, "sourceterm: null device array");
+  if (kmcd->dev.nPhoto == 0) photo_rates = nullptr;
+  hipError_t e = launchSourceTerm(kmcd->dev, nBatch, state, photo_rates, external_sources, omega,
+                                  static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hipFail(e, "sourceterm_kernel launch");
+  return TCHEM_OK;
+}
+
+int tchem_jacobian_run_device_batch(int nBatch, const double* state, const double* photo_rates, double* jac,
+                                    tchem_kmcd_t kmcd, void* stream) {
+  if (!kmcd || nBatch < 0) return setError(TCHEM_ERR_INVALID, "jacobian: bad argument");
+  if (nBatch == 0) return TCHEM_OK;
+  if (!state || !jac) return setError(TCHEM_ERR_INVALID, "jacobian: null device array");
+  if (kmcd->dev.nPhoto == 0) photo_rates = nullptr;
+  const size_t lds = integratorLdsBytes(kmcd->dev, sizeof(double));
+  if (lds > kmcd->maxLds) return setError(TCHEM_ERR_UNSUPPORTED, "jacobian: mechanism too large for LDS");
+  hipError_t e = launchJacobian(kmcd->dev, nBatch, state, photo_rates, jac, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hipFail(e, "jacobian_kernel launch");
+  return TCHEM_OK;
+}
+
+}  // extern "C"

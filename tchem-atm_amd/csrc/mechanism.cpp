@@ -1,0 +1,131 @@
+#include "mechanism.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <map>
+#include <stdexcept>
+
+namespace tchem_amd {
+
+MechTables compileMechanism(const KineticModel& km) {
+  MechTables t;
+  t.nSpec = km.nSpec;
+  t.nConst = km.nConstSpec;
+  t.m = km.nActive();
+  t.nReac = km.nReac;
+  t.convPPM = km.CONV_PPM;
+  const int nR = km.nReac, m = t.m;
+
+  for (int r = 0; r < nR; ++r) {
+    const int ty = km.reactionType[r];
+    if (ty != kArrhenius && ty != kTroe && ty != kCmaqH2O2)
+      throw std::runtime_error("reaction " + std::to_string(r) +
+                               " has a type the gas-phase box model does not evaluate (E3SM kinetics)");
+  }
+
+  // rate-constant records
+  t.rateKind.assign(nR, kRateArrhenius);
+  t.ratePar.assign(size_t(nR) * 8, 0.0);
+  t.convExp = km.convExponent;
+  t.photoSlot.assign(nR, -1);
+  for (const auto& a : km.arrhenius) {
+    t.rateKind[a.reaction] = kRateArrhenius;
+    double* p = &t.ratePar[size_t(a.reaction) * 8];
+    p[0] = a.A; p[1] = a.B; p[2] = a.C; p[3] = a.D; p[4] = a.E;
+  }
+  for (size_t i = 0; i < km.reacPfal.size(); ++i) {
+    const int r = km.reacPfal[i];
+    t.rateKind[r] = kRateTroe;
+    for (int j = 0; j < 8; ++j) t.ratePar[size_t(r) * 8 + j] = km.reacPpar[i][j];
+  }
+  for (const auto& c : km.cmaqH2O2) {
+    t.rateKind[c.reaction] = kRateCmaqH2O2;
+    double* p = &t.ratePar[size_t(c.reaction) * 8];
+    p[0] = c.A1; p[1] = c.B1; p[2] = c.C1; p[3] = c.A2; p[4] = c.B2; p[5] = c.C2;
+  }
+  t.photoReac = km.photolysisReactions;
+  t.nPhoto = int(t.photoReac.size());
+  for (int q = 0; q < t.nPhoto; ++q) t.photoSlot[t.photoReac[q]] = q;
+
+  // reactants (ordered by species index, as the reference's std::map)
+  int maxR = 0;
+  for (int r = 0; r < nR; ++r) maxR = std::max(maxR, km.reacNreac[r]);
+  t.maxReactants = std::max(maxR, 1);
+  t.reacSpec.assign(size_t(nR) * t.maxReactants, -1);
+  t.reacPow.assign(size_t(nR) * t.maxReactants, 0);
+  for (int r = 0; r < nR; ++r)
+    for (int j = 0; j < km.reacNreac[r]; ++j) {
+      t.reacSpec[size_t(r) * t.maxReactants + j] = km.sidx(r, j);
+      t.reacPow[size_t(r) * t.maxReactants + j] = std::abs(int(km.nuki(r, j)));
+    }
+
+  // net coefficient of species i in reaction r (active species only);
+  // reactant and product entries of the same species are combined
+  std::vector<std::map<int, double>> net(nR);  // r -> (species -> nu)
+  for (int r = 0; r < nR; ++r) {
+    for (int j = 0; j < km.reacNreac[r]; ++j) {
+      const int s = km.sidx(r, j);
+      if (s < m) net[r][s] += km.nuki(r, j);
+    }
+    const int joff = km.maxSpecInReac / 2;
+    for (int j = 0; j < km.reacNprod[r]; ++j) {
+      const int s = km.sidx(r, j + joff);
+      const double nu = km.nuki(r, j + joff);
+      if (s < m && nu != 0.0) net[r][s] += nu;  // unknown products keep (0, 0.0)
+    }
+  }
+
+  // source term CSR
+  std::vector<std::vector<std::pair<int, double>>> rows(m);
+  for (int r = 0; r < nR; ++r)
+    for (const auto& e : net[r])
+      if (e.second != 0.0) rows[e.first].push_back({r, e.second});
+  t.omegaPtr.assign(m + 1, 0);
+  for (int i = 0; i < m; ++i) {
+    t.omegaPtr[i + 1] = t.omegaPtr[i] + int(rows[i].size());
+    for (const auto& e : rows[i]) {
+      t.omegaReac.push_back(e.first);
+      t.omegaCoef.push_back(e.second);
+    }
+  }
+  t.emission.assign(m, 0.0);
+  for (const auto& e : km.emission) {
+    if (e.species < m) t.emission[e.species] = e.rate;  // assignment: the last entry wins
+    else std::fprintf(stderr, "tchem: emission for constant species %d ignored\n", e.species);
+  }
+
+  // Jacobian pairs and contributions
+  t.pairPtr.assign(nR + 1, 0);
+  std::vector<int> pairReac, pairSpec;
+  for (int r = 0; r < nR; ++r) {
+    for (int j = 0; j < km.reacNreac[r]; ++j) {
+      const int s = km.sidx(r, j);
+      if (s < m) {
+        t.pairSlot.push_back(j);
+        pairReac.push_back(r);
+        pairSpec.push_back(s);
+      }
+    }
+    t.pairPtr[r + 1] = int(t.pairSlot.size());
+  }
+  std::map<std::pair<int, int>, std::vector<std::pair<int, double>>> jmap;  // (row, col) -> contributions
+  for (int i = 0; i < m; ++i) jmap[{i, i}];                                // structural diagonal
+  for (int p = 0; p < t.nPairs(); ++p) {
+    const int r = pairReac[p], s = pairSpec[p];
+    for (const auto& e : net[r])
+      if (e.second != 0.0) jmap[{e.first, s}].push_back({p, e.second});
+  }
+  t.jnzPtr.push_back(0);
+  for (const auto& e : jmap) {
+    t.jnzRow.push_back(e.first.first);
+    t.jnzCol.push_back(e.first.second);
+    for (const auto& c : e.second) {
+      t.jcPair.push_back(c.first);
+      t.jcCoef.push_back(c.second);
+    }
+    t.jnzPtr.push_back(int(t.jcPair.size()));
+  }
+  return t;
+}
+
+}  // namespace tchem_amd

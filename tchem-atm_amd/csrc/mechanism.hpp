@@ -1,0 +1,68 @@
+// Mechanism compiler: turns a KineticModel into the flat, index-based tables
+// the HIP kernels read.  All tables are shared by every cell; per-cell data
+// (state rows, photolysis rates) never enter here.
+//
+// Layout choices (the reference scatters with atomics, one team per cell,
+// src/core/impl/TChem_Impl_ReactionRatesAerosol.hpp:116-138):
+//   * rate constants: one record per reaction (type, 8 parameters, conv
+//     exponent, photolysis slot) so a lane evaluates any reaction without
+//     branching on per-type index lists;
+//   * rate of progress: reactant species/power table [nReac][maxReactants];
+//   * source term: species-major CSR of (reaction, net coefficient) — a
+//     deterministic gather instead of atomic scatter;
+//   * Jacobian: per-reaction list of derivative "pairs" (reaction, active
+//     reactant) and a CSR over the structural nonzeros of J holding
+//     (pair, coefficient) contributions.  Every diagonal entry is a
+//     structural nonzero so the iteration matrix I - c h J is written in one
+//     pass.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "kinetic_model.hpp"
+
+namespace tchem_amd {
+
+enum RateKind : int { kRateArrhenius = 0, kRateTroe = 1, kRateCmaqH2O2 = 2 };
+
+struct MechTables {
+  int nSpec = 0, nConst = 0, m = 0, nReac = 0, nPhoto = 0;
+  int maxReactants = 0;
+  double convPPM = kConvPPM;
+
+  // rate constants, per reaction
+  std::vector<int> rateKind;     // RateKind
+  std::vector<double> ratePar;   // [nReac][8]
+  std::vector<int> convExp;      // [nReac]
+  std::vector<int> photoSlot;    // [nReac], -1 or index into the per-cell photolysis rates
+  std::vector<int> photoReac;    // [nPhoto] reaction index of each photolysis slot
+
+  // rate of progress
+  std::vector<int> reacSpec;  // [nReac][maxReactants], -1 padded
+  std::vector<int> reacPow;   // [nReac][maxReactants]
+
+  // source term gather (species-major), emission per active species
+  std::vector<int> omegaPtr;     // [m+1]
+  std::vector<int> omegaReac;    // [nnzOmega]
+  std::vector<double> omegaCoef; // [nnzOmega]
+  std::vector<double> emission;  // [m]
+
+  // Jacobian
+  std::vector<int> pairPtr;      // [nReac+1] pairs of reaction r
+  std::vector<int> pairSlot;     // [nPairs] reactant slot j within the reaction
+  std::vector<int> jnzRow, jnzCol;   // [nnzJ]
+  std::vector<int> jnzPtr;       // [nnzJ+1]
+  std::vector<int> jcPair;       // [nContrib]
+  std::vector<double> jcCoef;    // [nContrib]
+
+  int nPairs() const { return int(pairSlot.size()); }
+  int nnzJ() const { return int(jnzRow.size()); }
+};
+
+// Throws std::runtime_error for mechanisms the gas-phase box model cannot
+// evaluate (JPL / R_JPL_ARRHENIUS / PHOTO_RATE reactions belong to the E3SM
+// kinetics path).
+MechTables compileMechanism(const KineticModel& km);
+
+}  // namespace tchem_amd

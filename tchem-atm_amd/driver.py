@@ -1,0 +1,90 @@
+"""Box-model driver: the host interval loop of TChem_AtmosphericChemistry.x.
+
+Restates src/examples/TChem_AtmosphericChemistry.cpp:262-352 over the C ABI:
+relaunch runDeviceBatch until the mean cell time reaches 0.9999 * tend, and
+between launches carry t_out/dt_out into tadv._tbeg/_dt.  The persistent
+form integrates every cell to tend in one launch instead.
+"""
+from __future__ import annotations
+
+import importlib
+
+import numpy as np
+
+T = importlib.import_module("tchem-atm_amd")
+
+
+class DeviceRun:
+    """Device buffers of one workload, reusable across repeated runs."""
+
+    def __init__(self, kmcd, workload, device="cuda"):
+        import torch
+        self.torch = torch
+        self.kmcd = kmcd
+        self.w = workload
+        m = kmcd.nActive
+        n = workload.n
+        self.n = n
+        tn = np.array([workload.atol_newton, workload.rtol_newton])
+        tt = np.tile([workload.atol_time, workload.rtol_time], (m, 1))
+        self.tol_newton = torch.tensor(tn, dtype=torch.float64, device=device)
+        self.tol_time = torch.tensor(tt, dtype=torch.float64, device=device)
+        self.state0 = torch.tensor(workload.state, dtype=torch.float64, device=device)
+        self.photo = torch.tensor(workload.photo, dtype=torch.float64, device=device) if kmcd.nPhoto > 0 else None
+        self.state = torch.empty_like(self.state0)
+        self.t = torch.empty(n, dtype=torch.float64, device=device)
+        self.dt = torch.empty(n, dtype=torch.float64, device=device)
+        self.stats = T.stats_tensor(n, device)
+        self.device = device
+
+    def reset(self, iters_per_interval=None):
+        w = self.w
+        self.state.copy_(self.state0)
+        self.t.fill_(w.tbeg)
+        self.dt.fill_(w.dtmax)  # examples/...:227 (not read: tadv._dt is what the integrator uses)
+        ipi = w.iters_per_interval if iters_per_interval is None else iters_per_interval
+        self.tadv = T.time_advance_tensor(self.n, w.tbeg, w.tend, w.dt_in, w.dtmin, w.dtmax, w.max_newton, ipi,
+                                          w.jacobian_interval, self.device)
+
+    def run_intervals(self, precision=T.PRECISION_FP64, schedule=T.SCHEDULE_GRID, max_intervals=100000,
+                      blocks_per_cu=0, collect_stats=False, stream=None):
+        """Reference host loop; returns the number of launches."""
+        torch = self.torch
+        tend = self.w.tend
+        it = 0
+        total = None
+        while it < max_intervals:
+            T.AtmosphericChemistry.runDeviceBatch(self.tol_newton, self.tol_time, None, self.tadv, self.state,
+                                                  self.t, self.dt, self.state, self.kmcd, self.photo,
+                                                  self.stats if collect_stats else None, precision, schedule,
+                                                  blocks_per_cu, stream)
+            if collect_stats:
+                s = T.stats_from_tensor(self.stats)
+                arr = np.stack([s[f] for f in s.dtype.names], 1).astype(np.int64)
+                arr[:, 5] = s["failed"]
+                if total is None:
+                    total = arr.copy()
+                else:
+                    total[:, :5] += arr[:, :5]
+                    total[:, 5] = np.maximum(total[:, 5], arr[:, 5])
+            T.AtmosphericChemistry.updateTimeAdvance(self.tadv, self.t, self.dt, stream)
+            it += 1
+            if float(self.t.mean()) > tend * 0.9999:
+                break
+        self.total_stats = total
+        return it
+
+    def run_to_end(self, precision=T.PRECISION_MIXED, schedule=T.SCHEDULE_PERSISTENT, blocks_per_cu=0,
+                   collect_stats=False, stream=None):
+        """Every cell to tend in one launch (iteration cap 10^7 per cell)."""
+        w = self.w
+        self.tadv = T.time_advance_tensor(self.n, w.tbeg, w.tend, w.dt_in, w.dtmin, w.dtmax, w.max_newton,
+                                          10_000_000, w.jacobian_interval, self.device)
+        T.AtmosphericChemistry.runDeviceBatch(self.tol_newton, self.tol_time, None, self.tadv, self.state, self.t,
+                                              self.dt, self.state, self.kmcd, self.photo,
+                                              self.stats if collect_stats else None, precision, schedule,
+                                              blocks_per_cu, stream)
+        if collect_stats:
+            s = T.stats_from_tensor(self.stats)
+            self.total_stats = np.stack([s[f] for f in s.dtype.names], 1).astype(np.int64)
+        return 1

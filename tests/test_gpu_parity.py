@@ -1,0 +1,228 @@
+"""HIP path against the CPU oracle — parity tests proper (need an MI355X).
+
+All compute goes through the C ABI (libtchem_atm.so); the oracle is the
+checker.  Error metric per species: |gpu - oracle| / max(|oracle|, 1e-12 ppm),
+the floor being the integration's absolute tolerance.
+"""
+import importlib
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+FLOOR = 1e-12
+
+
+def relerr(a, b, floor=FLOOR):
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+@pytest.fixture(scope="module")
+def cb05(gpu, pkg, cb05_oracle):
+    kmd = pkg.KineticModelData(golden("cb05cl_ae5.yaml"))
+    kmcd = kmd.createConstData()
+    base = pkg.setScenarioConditions(golden("cb05cl_ae5.yaml"), kmd)
+    return kmd, kmcd, base
+
+
+@pytest.fixture(scope="module")
+def scen():
+    return importlib.import_module("tchem-atm_amd.scenarios")
+
+
+@pytest.fixture(scope="module")
+def drv():
+    return importlib.import_module("tchem-atm_amd.driver")
+
+
+def oracle_intervals(o, oracle_mod, w, stats=None, iters=None):
+    n, m = w.n, o.m
+    st = w.state.copy()
+    t = np.full(n, w.tbeg)
+    dt = np.full(n, w.dtmax)
+    ta = np.zeros(n, oracle_mod.TADV_DTYPE)
+    ta["tbeg"], ta["tend"], ta["dt"], ta["dtmin"], ta["dtmax"] = w.tbeg, w.tend, w.dt_in, w.dtmin, w.dtmax
+    ta["max_num_newton_iterations"] = w.max_newton
+    ta["num_time_iterations_per_interval"] = iters or w.iters_per_interval
+    ta["jacobian_interval"] = w.jacobian_interval
+    tn = np.array([w.atol_newton, w.rtol_newton])
+    tt = np.tile([w.atol_time, w.rtol_time], (m, 1))
+    total = np.zeros(n, oracle_mod.STATS_DTYPE)
+    for _ in range(100000):
+        s = np.zeros(n, oracle_mod.STATS_DTYPE)
+        o.run_batch(tn, tt, ta, st, t, dt, photo=w.photo, stats=s)
+        for f in ("accepted", "rejected", "newton_failures", "newton_iterations", "factorizations"):
+            total[f] += s[f]
+        total["failed"] |= s["failed"]
+        ta["tbeg"], ta["dt"] = t, dt
+        if t.mean() > w.tend * 0.9999:
+            break
+    return st, t, dt, total
+
+
+def test_source_term_matches_oracle(cb05, cb05_oracle, scen, gpu, pkg):
+    kmd, kmcd, base = cb05
+    torch = gpu
+    w = scen.config0(kmd, base[0], n=1024, seed=0)
+    st = torch.tensor(w.state, device="cuda")
+    ph = torch.tensor(w.photo, device="cuda")
+    om = torch.full((w.n, kmd.nActive), np.nan, dtype=torch.float64, device="cuda")
+    pkg.SourceTerm.runDeviceBatch(st, ph, None, om, kmcd)
+    ref = cb05_oracle.sourceterm_batch(w.state, w.photo)
+    got = om.cpu().numpy()
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    assert (np.abs(got - ref) / scale).max() < 1e-14
+    assert relerr(got, ref, 1e-30).max() < 1e-11
+    # external sources are added; photo_rates=None uses the mechanism's A values
+    ext = torch.ones_like(om)
+    pkg.SourceTerm.runDeviceBatch(st, None, ext, om, kmcd)
+    ref2 = cb05_oracle.sourceterm_batch(w.state, None) + 1.0
+    assert (np.abs(om.cpu().numpy() - ref2) / (np.abs(ref2).max(axis=1, keepdims=True))).max() < 1e-14
+
+
+def test_jacobian_matches_oracle(cb05, cb05_oracle, scen, gpu, pkg):
+    kmd, kmcd, base = cb05
+    torch = gpu
+    w = scen.config0(kmd, base[0], n=256, seed=3)
+    st = torch.tensor(w.state, device="cuda")
+    ph = torch.tensor(w.photo, device="cuda")
+    m = kmd.nActive
+    jac = torch.full((w.n, m, m), np.nan, dtype=torch.float64, device="cuda")
+    pkg.Jacobian.runDeviceBatch(st, ph, jac, kmcd)
+    ref = cb05_oracle.jacobian_batch(w.state, w.photo)
+    got = jac.cpu().numpy()
+    assert np.array_equal(got != 0, ref != 0)
+    scale = np.abs(ref).max(axis=(1, 2), keepdims=True)
+    assert (np.abs(got - ref) / scale).max() < 1e-14
+
+
+def test_cb05cl_ae5_golden_on_gpu(cb05, gpu, pkg):
+    """The reference's ctest case run through runDeviceBatch on the GPU, both precisions."""
+    kmd, kmcd, base = cb05
+    torch = gpu
+    m = kmd.nActive
+    gold = np.load(golden("cb05cl_ae5_full_gas.npy"))
+    for precision in (pkg.PRECISION_FP64, pkg.PRECISION_MIXED):
+        st = torch.tensor(base, device="cuda")
+        tn = torch.tensor([1e-18, 1e-8], dtype=torch.float64, device="cuda")
+        tt = torch.tensor(np.tile([1e-12, 1e-4], (m, 1)), device="cuda")
+        tadv = pkg.time_advance_tensor(1, 0.0, 600.0, 1e-20, 1e-20, 10.0, 20, 10, 1)
+        t = torch.zeros(1, dtype=torch.float64, device="cuda")
+        dt = torch.full((1,), 10.0, dtype=torch.float64, device="cuda")
+        rows = [np.concatenate([[-1, 0.0, 10.0], base[0]])]
+        for it in range(2):
+            pkg.AtmosphericChemistry.runDeviceBatch(tn, tt, None, tadv, st, t, dt, st, kmcd, precision=precision)
+            pkg.AtmosphericChemistry.updateTimeAdvance(tadv, t, dt)
+            torch.cuda.synchronize()
+            rows.append(np.concatenate([[it, t.item(), dt.item()], st.cpu().numpy()[0]]))
+        rows = np.array(rows)
+        l2 = np.linalg.norm(gold, 2, 0)
+        mx = np.where(l2 > 1e-23, l2, 1.0)
+        crit = max((np.linalg.norm(rows - gold, o, 0) / mx).max() for o in (1, 2, np.inf))
+        assert crit < 1e-8, (precision, crit)
+
+
+@pytest.fixture(scope="module")
+def fp64_vs_oracle(cb05, cb05_oracle, oracle_mod, scen, drv, gpu, pkg):
+    kmd, kmcd, base = cb05
+    w = scen.config0(kmd, base[0], n=64, seed=0)
+    run = drv.DeviceRun(kmcd, w)
+    run.reset()
+    run.run_intervals(precision=pkg.PRECISION_FP64, schedule=pkg.SCHEDULE_GRID, collect_stats=True)
+    gpu.cuda.synchronize()
+    g_state = run.state.cpu().numpy()
+    g_t = run.t.cpu().numpy()
+    g_stats = run.total_stats
+    o_state, o_t, o_dt, o_stats = oracle_intervals(cb05_oracle, oracle_mod, w)
+    return w, run, g_state, g_t, g_stats, o_state, o_t, o_stats
+
+
+def test_fp64_integration_matches_oracle(fp64_vs_oracle, cb05):
+    """config0 inputs (seed 0), 64 cells, 3600 s: the all-FP64 GPU form against
+    the FP64 oracle.  Cells whose step sequence is identical agree to 1e-9;
+    a cell whose accept/reject or Newton count differs because of last-bit
+    differences (GPU vs host libm, FMA contraction) stays within the
+    integration tolerance."""
+    w, run, g_state, g_t, g_stats, o_state, o_t, o_stats = fp64_vs_oracle
+    m = cb05[0].nActive
+    assert np.all(g_t == w.tend) and np.all(o_t == w.tend)
+    assert g_stats[:, 5].sum() == 0 and o_stats["failed"].sum() == 0
+    err = relerr(g_state[:, 3:3 + m], o_state[:, 3:3 + m]).max(axis=1)
+    same = (g_stats[:, 0] == o_stats["accepted"]) & (g_stats[:, 1] == o_stats["rejected"])
+    print("cells with identical step counts:", same.sum(), "of", w.n, "max err there:", err[same].max(),
+          "elsewhere:", err[~same].max() if (~same).any() else None)
+    assert same.sum() >= 0.9 * w.n
+    assert err[same].max() < 1e-9
+    assert err.max() < 1e-4
+    np.testing.assert_array_equal(g_state[:, 3 + m:], w.state[:, 3 + m:])  # constant species untouched
+    np.testing.assert_array_equal(g_state[:, :3], w.state[:, :3])
+
+
+def test_persistent_schedule_is_bitwise_identical(fp64_vs_oracle, gpu, pkg):
+    """The work-queue launch runs the same per-cell code: identical bits."""
+    w, run, g_state, g_t, g_stats, *_ = fp64_vs_oracle
+    run.reset()
+    run.run_intervals(precision=pkg.PRECISION_FP64, schedule=pkg.SCHEDULE_PERSISTENT)
+    gpu.cuda.synchronize()
+    assert np.array_equal(run.state.cpu().numpy(), g_state)
+    assert np.array_equal(run.t.cpu().numpy(), g_t)
+
+
+def test_mixed_precision_error_against_fp64_oracle(fp64_vs_oracle, cb05, gpu, pkg):
+    """FP32 iteration matrix and LU, FP64 state/residual/error/step control:
+    within 1e-4 of the FP64 oracle per species."""
+    w, run, g_state, g_t, g_stats, o_state, o_t, o_stats = fp64_vs_oracle
+    m = cb05[0].nActive
+    run.reset()
+    run.run_to_end(precision=pkg.PRECISION_MIXED, schedule=pkg.SCHEDULE_PERSISTENT, collect_stats=True)
+    gpu.cuda.synchronize()
+    assert np.all(run.t.cpu().numpy() == w.tend)
+    assert run.total_stats[:, 5].sum() == 0
+    err = relerr(run.state.cpu().numpy()[:, 3:3 + m], o_state[:, 3:3 + m])
+    print("mixed vs oracle: max", err.max(), "median", np.median(err))
+    assert err.max() < 1e-4
+
+
+def test_forced_newton_failure_on_gpu(cb05, gpu, pkg):
+    """One Newton iteration and a minimum step of 100 s: every cell fails the
+    reference way (species zeroed, t_out = tend, dt_out = -1) and the kernel exits."""
+    kmd, kmcd, base = cb05
+    torch = gpu
+    m = kmd.nActive
+    n = 4
+    st = torch.tensor(np.repeat(base, n, 0), device="cuda")
+    tn = torch.tensor([1e-30, 1e-14], dtype=torch.float64, device="cuda")
+    tt = torch.tensor(np.tile([1e-12, 1e-6], (m, 1)), device="cuda")
+    tadv = pkg.time_advance_tensor(n, 0.0, 3600.0, 100.0, 100.0, 100.0, 1, 5, 1)
+    t = torch.zeros(n, dtype=torch.float64, device="cuda")
+    dt = torch.zeros(n, dtype=torch.float64, device="cuda")
+    stats = pkg.stats_tensor(n)
+    pkg.AtmosphericChemistry.runDeviceBatch(tn, tt, None, tadv, st, t, dt, st, kmcd, stats=stats)
+    torch.cuda.synchronize()
+    s = pkg.stats_from_tensor(stats)
+    assert np.all(s["failed"] == 1) and np.all(s["newton_failures"] == 4)
+    assert np.all(t.cpu().numpy() == 3600.0) and np.all(dt.cpu().numpy() == -1.0)
+    out = st.cpu().numpy()
+    assert np.all(out[:, 3:3 + m] == 0.0)
+    # a second launch skips the failed cells
+    pkg.AtmosphericChemistry.runDeviceBatch(tn, tt, None, tadv, st, t, dt, st, kmcd)
+    torch.cuda.synchronize()
+    assert np.array_equal(st.cpu().numpy(), out)
+
+
+def test_invalid_arguments_are_rejected(cb05, gpu, pkg):
+    kmd, kmcd, base = cb05
+    torch = gpu
+    st = torch.tensor(base, device="cuda")
+    with pytest.raises(pkg.TChemError, match="device tensor"):
+        pkg.SourceTerm.runDeviceBatch(torch.tensor(base), None, None, torch.zeros(1, 74), kmcd)
+    with pytest.raises(pkg.TChemError, match="float64"):
+        pkg.SourceTerm.runDeviceBatch(st.float(), None, None, torch.zeros(1, 74, device="cuda"), kmcd)
+    lib = pkg.lib()
+    opts = pkg._Options(5, 0, 0)
+    assert lib.tchem_atmchem_run_device_batch(opts, 1, None, None, None, None, None, None, None, None, None, None,
+                                              kmcd._h, None) == -1
+    assert "precision" in lib.tchem_last_error().decode()
