@@ -123,7 +123,8 @@ void oracle_jacobian(const oracle_kmd* k, const double* kfor, const double* c, d
   }
 }
 
-/* Dense LU with partial pivoting (row interchanges), Doolittle form.
+/* Dense LU with partial pivoting (row interchanges), Doolittle form;
+ * multipliers use the reciprocal of the pivot, l = a_ik * (1 / a_kk).
  * The reference factors with Tines' UTV; both solve the same Newton system.
  * Pinned by: scipy.linalg.lu_factor and the backward error on random and
  * CB05 iteration matrices. */
@@ -143,9 +144,9 @@ int oracle_lu(int n, double* A, int lda, int* piv) {
         A[(size_t)k * lda + j] = A[(size_t)p * lda + j];
         A[(size_t)p * lda + j] = tmp;
       }
-    const double inv = A[(size_t)k * lda + k];
+    const double inv = 1.0 / A[(size_t)k * lda + k];
     for (int i = k + 1; i < n; ++i) {
-      const double l = A[(size_t)i * lda + k] / inv;
+      const double l = A[(size_t)i * lda + k] * inv;
       A[(size_t)i * lda + k] = l;
       if (l != 0.0)
         for (int j = k + 1; j < n; ++j) A[(size_t)i * lda + j] -= l * A[(size_t)k * lda + j];
@@ -165,7 +166,7 @@ void oracle_lu_solve(int n, const double* A, int lda, const int* piv, double* b)
       for (int i = k + 1; i < n; ++i) b[i] -= A[(size_t)i * lda + k] * yk;
   }
   for (int k = n - 1; k >= 0; --k) {
-    const double xk = b[k] / A[(size_t)k * lda + k];
+    const double xk = b[k] * (1.0 / A[(size_t)k * lda + k]);
     b[k] = xk;
     if (xk != 0.0)
       for (int i = 0; i < k; ++i) b[i] -= A[(size_t)i * lda + k] * xk;

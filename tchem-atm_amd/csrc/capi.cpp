@@ -180,19 +180,48 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
   d.maxR = t.maxReactants;
   d.nPairs = t.nPairs();
   d.nnzJ = t.nnzJ();
+  d.nSeg = t.nSeg();
   d.convPPM = t.convPPM;
   std::vector<int> rowcol(t.jnzRow.size());
   for (size_t n = 0; n < rowcol.size(); ++n) rowcol[n] = (t.jnzRow[n] << 16) | t.jnzCol[n];
+  // pack every table into one int and one double buffer
+  std::vector<int> ib;
+  std::vector<double> db;
+  auto addI = [&](const std::vector<int>& v) {
+    const int off = int(ib.size());
+    ib.insert(ib.end(), v.begin(), v.end());
+    while (ib.size() % 4) ib.push_back(0);
+    return off;
+  };
+  auto addD = [&](const std::vector<double>& v) {
+    const int off = int(db.size());
+    db.insert(db.end(), v.begin(), v.end());
+    while (db.size() % 2) db.push_back(0.0);
+    return off;
+  };
+  d.oPerm = addI(t.perm);
+  d.oRateKind = addI(t.rateKind);
+  d.oConvExp = addI(t.convExp);
+  d.oPhotoSlot = addI(t.photoSlot);
+  d.oReacSpec = addI(t.reacSpec);
+  d.oReacPow = addI(t.reacPow);
+  d.oOmegaReac = addI(t.omegaReac);
+  d.oSegBegin = addI(t.segBegin);
+  d.oSegEnd = addI(t.segEnd);
+  d.oRowSegPtr = addI(t.rowSegPtr);
+  d.oPairPtr = addI(t.pairPtr);
+  d.oPairSlot = addI(t.pairSlot);
+  d.oJnzRowCol = addI(rowcol);
+  d.oJnzPtr = addI(t.jnzPtr);
+  d.oJcPair = addI(t.jcPair);
+  d.oRatePar = addD(t.ratePar);
+  d.oOmegaCoef = addD(t.omegaCoef);
+  d.oEmission = addD(t.emission);
+  d.oJcCoef = addD(t.jcCoef);
+  ib.push_back(0);
+  db.push_back(0.0);
   int rc = TCHEM_OK;
-  if ((rc = upload(k.get(), t.rateKind, &d.rateKind)) || (rc = upload(k.get(), t.ratePar, &d.ratePar)) ||
-      (rc = upload(k.get(), t.convExp, &d.convExp)) || (rc = upload(k.get(), t.photoSlot, &d.photoSlot)) ||
-      (rc = upload(k.get(), t.reacSpec, &d.reacSpec)) || (rc = upload(k.get(), t.reacPow, &d.reacPow)) ||
-      (rc = upload(k.get(), t.omegaPtr, &d.omegaPtr)) || (rc = upload(k.get(), t.omegaReac, &d.omegaReac)) ||
-      (rc = upload(k.get(), t.omegaCoef, &d.omegaCoef)) || (rc = upload(k.get(), t.emission, &d.emission)) ||
-      (rc = upload(k.get(), t.pairPtr, &d.pairPtr)) || (rc = upload(k.get(), t.pairSlot, &d.pairSlot)) ||
-      (rc = upload(k.get(), rowcol, &d.jnzRowCol)) || (rc = upload(k.get(), t.jnzPtr, &d.jnzPtr)) ||
-      (rc = upload(k.get(), t.jcPair, &d.jcPair)) || (rc = upload(k.get(), t.jcCoef, &d.jcCoef)))
-    return rc;
+  if ((rc = upload(k.get(), ib, &d.ib)) || (rc = upload(k.get(), db, &d.db))) return rc;
   void* q = nullptr;
   e = hipMalloc(&q, 256);
   if (e != hipSuccess) return hipFail(e, "hipMalloc");

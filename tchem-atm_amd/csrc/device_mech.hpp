@@ -3,29 +3,43 @@
 // arguments; every pointer is a device pointer owned by the kmcd object.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include "../../include/tchem_atm.h"
 
 namespace tchem_amd {
 
 struct DevMech {
-  int nSpec, nConst, m, nReac, nPhoto, maxR, nPairs, nnzJ;
+  int nSpec, nConst, m, nReac, nPhoto, maxR, nPairs, nnzJ, nSeg;
   double convPPM;
-  const int* rateKind;      // [nReac]
-  const double* ratePar;    // [nReac][8]
-  const int* convExp;       // [nReac]
-  const int* photoSlot;     // [nReac]
-  const int* reacSpec;      // [nReac][maxR]
-  const int* reacPow;       // [nReac][maxR]
-  const int* omegaPtr;      // [m+1]
-  const int* omegaReac;
-  const double* omegaCoef;
-  const double* emission;   // [m]
-  const int* pairPtr;       // [nReac+1]
-  const int* pairSlot;      // [nPairs]
-  const int* jnzRowCol;     // [nnzJ] row * 65536 + col
-  const int* jnzPtr;        // [nnzJ+1]
-  const int* jcPair;
-  const double* jcCoef;
+  // Every table lives in one int buffer and one double buffer; the kernels
+  // address them by 32-bit offsets so a wavefront keeps two base pointers
+  // instead of twenty.
+  const int* ib;
+  const double* db;
+  int oPerm, oRateKind, oConvExp, oPhotoSlot, oReacSpec, oReacPow, oOmegaReac, oSegBegin, oSegEnd, oRowSegPtr,
+      oPairPtr, oPairSlot, oJnzRowCol, oJnzPtr, oJcPair;
+  int oRatePar, oOmegaCoef, oEmission, oJcCoef;
+
+  __host__ __device__ const int* perm() const { return ib + oPerm; }             // [m] internal -> reference species
+  __host__ __device__ const int* rateKind() const { return ib + oRateKind; }     // [nReac]
+  __host__ __device__ const int* convExp() const { return ib + oConvExp; }       // [nReac]
+  __host__ __device__ const int* photoSlot() const { return ib + oPhotoSlot; }   // [nReac]
+  __host__ __device__ const int* reacSpec() const { return ib + oReacSpec; }     // [nReac][maxR]
+  __host__ __device__ const int* reacPow() const { return ib + oReacPow; }       // [nReac][maxR]
+  __host__ __device__ const int* omegaReac() const { return ib + oOmegaReac; }   // [nnzOmega]
+  __host__ __device__ const int* segBegin() const { return ib + oSegBegin; }     // [nSeg]
+  __host__ __device__ const int* segEnd() const { return ib + oSegEnd; }         // [nSeg]
+  __host__ __device__ const int* rowSegPtr() const { return ib + oRowSegPtr; }   // [m+1]
+  __host__ __device__ const int* pairPtr() const { return ib + oPairPtr; }       // [nReac+1]
+  __host__ __device__ const int* pairSlot() const { return ib + oPairSlot; }     // [nPairs]
+  __host__ __device__ const int* jnzRowCol() const { return ib + oJnzRowCol; }   // [nnzJ] row * 65536 + col
+  __host__ __device__ const int* jnzPtr() const { return ib + oJnzPtr; }         // [nnzJ+1]
+  __host__ __device__ const int* jcPair() const { return ib + oJcPair; }         // [nContrib]
+  __host__ __device__ const double* ratePar() const { return db + oRatePar; }    // [nReac][8]
+  __host__ __device__ const double* omegaCoef() const { return db + oOmegaCoef; }// [nnzOmega]
+  __host__ __device__ const double* emission() const { return db + oEmission; }  // [m]
+  __host__ __device__ const double* jcCoef() const { return db + oJcCoef; }      // [nContrib]
 };
 
 struct IntegratorArgs {
@@ -44,15 +58,17 @@ struct IntegratorArgs {
   int persistent;
 };
 
-// LDS bytes of one cell in the TR-BDF2 kernel: iteration matrix in TA
-// precision, then doubles for concentrations, rate constants and the
-// rate-of-progress / derivative scratch, then pivots.
+// LDS bytes of one cell: the iteration matrix in TA precision (bytesA per
+// entry, 0 for kernels without a matrix), then doubles for concentrations,
+// rate constants, rate-of-progress / derivative scratch and source-term
+// segment sums and reciprocal pivots, then the row-interchange list of the
+// factorisation.
 inline size_t integratorLdsBytes(const DevMech& M, size_t bytesA) {
   const size_t nScr = size_t(M.nReac > M.nPairs ? M.nReac : M.nPairs);
   size_t a = bytesA * size_t(M.m) * size_t(M.m);
   a = (a + 15) & ~size_t(15);
-  size_t b = sizeof(double) * (size_t(M.nSpec) + size_t(M.nReac) + nScr);
-  return a + b + sizeof(int) * size_t(M.m);
+  size_t b = sizeof(double) * (size_t(M.nSpec) + size_t(M.nReac) + nScr + size_t(M.nSeg));
+  return a + b + sizeof(double) * size_t(M.m) + sizeof(int) * 2 * size_t(M.m);
 }
 
 }  // namespace tchem_amd

@@ -3,17 +3,20 @@
 //
 // Mapping: one cell per 64-lane wavefront (one wavefront per workgroup).
 //   * per-species vectors (state, stage values, residuals, Newton updates)
-//     live in registers, species i on lane i % 64, register slot i / 64;
-//   * data other lanes must read goes through LDS: concentrations, rate
-//     constants, rate-of-progress / derivative scratch, and the dense
-//     iteration matrix I - c h J with its LU factors;
-//   * the LU (partial pivoting) maps lanes to the columns of the trailing
-//     submatrix; the pivot search is a wavefront arg-max over shuffles; a
-//     row whose multiplier is exactly zero is skipped by the whole wavefront
-//     (ballot), so the structural sparsity of the chemistry Jacobian costs
-//     nothing without fixing a pivot order.
-// The iteration matrix and its factorisation are in TA precision (double for
-// the all-FP64 form, float for the mixed form); everything else is FP64.
+//     live in registers: internal species a on lane a % 64, slot a / 64;
+//   * data other lanes read goes through LDS: concentrations, rate
+//     constants, rate-of-progress / derivative / segment scratch, and the
+//     dense iteration matrix I - c h J with its LU factors;
+//   * species are numbered in a fill-reducing elimination order compiled
+//     from the mechanism, so the dense LU meets few nonzero multipliers.
+// LU with partial pivoting: the pivot row is found with one ballot when the
+// diagonal already has the largest magnitude in its column (the common
+// case) and with a wavefront arg-max otherwise; rows whose multiplier is
+// exactly zero are skipped by the whole wavefront; lanes own the columns of
+// the trailing update; the Newton right-hand side is eliminated alongside
+// the matrix so only back substitution remains.
+// The iteration matrix and its factors are in TA precision (double for the
+// all-FP64 form, float for the mixed form); everything else is FP64.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -23,11 +26,26 @@
 
 namespace tchem_amd {
 
+// Optional per-phase cycle counters for profiling builds (-DTCHEM_PHASE_TIMERS):
+// wavefront-summed shader-clock cycles of the source term, Jacobian assembly,
+// factorisation, back substitution and whole cells.
+#ifdef TCHEM_PHASE_TIMERS
+__device__ unsigned long long g_phase_cycles[8];
+#define TCHEM_PT_BEGIN(nm) const unsigned long long tchem_pt_##nm = __builtin_readcyclecounter();
+#define TCHEM_PT_END(nm, slot) \
+  if (threadIdx.x == 0) atomicAdd(&g_phase_cycles[slot], __builtin_readcyclecounter() - tchem_pt_##nm);
+#else
+#define TCHEM_PT_BEGIN(nm)
+#define TCHEM_PT_END(nm, slot)
+#endif
+
 namespace {
 
 constexpr int kWave = 64;
 
-__device__ __forceinline__ void wave_sync() { __syncthreads(); }  // workgroup == one wavefront
+// A workgroup is one wavefront: the barrier orders the wavefront's LDS
+// traffic and fences the compiler.
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
 __device__ __forceinline__ double readlane(double v, int l) {
   const long long x = __double_as_longlong(v);
@@ -39,10 +57,23 @@ __device__ __forceinline__ float readlane(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
+__device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
   return v;
+}
+
+// value of slot `idx` (lane idx % 64, register slot idx / 64) on every lane
+template <int QM, typename TA>
+__device__ __forceinline__ TA bcast(const TA (&v)[QM], int idx) {
+  const int q = idx >> 6, l = idx & 63;
+  TA out = TA(0);
+#pragma unroll
+  for (int s = 0; s < QM; ++s)
+    if (s == q) out = readlane(v[s], l);
+  return out;
 }
 
 __device__ __forceinline__ double pow_small(double x, int n) {
@@ -52,13 +83,13 @@ __device__ __forceinline__ double pow_small(double x, int n) {
 // Rate constant of reaction r: src/core/impl/TChem_Impl_KForwardAerosol.hpp:54-131.
 __device__ double rate_constant(const DevMech& M, int r, double t, double p, double t_1, double conv,
                                 const double* photo) {
-  const double* P = M.ratePar + 8 * r;
+  const double* P = M.ratePar() + 8 * r;
   const double one = 1.0;
-  const int kind = M.rateKind[r];
+  const int kind = M.rateKind()[r];
   double k;
   if (kind == 0) {  // Arrhenius
     double A = P[0];
-    const int slot = M.photoSlot[r];
+    const int slot = M.photoSlot()[r];
     if (photo != nullptr && slot >= 0) A = photo[slot];
     k = A * exp(P[2] * t_1) * (P[1] == 0.0 ? one : pow(t / P[3], P[1])) * (P[4] == 0.0 ? one : (one + P[4] * p));
   } else if (kind == 1) {  // Troe
@@ -71,16 +102,18 @@ __device__ double rate_constant(const DevMech& M, int r, double t, double p, dou
     k = P[0] * exp(P[2] * t_1) * (P[1] == 0.0 ? one : pow(t / 300.0, P[1])) +
         P[3] * exp(P[5] * t_1) * (P[4] == 0.0 ? one : pow(t / 300.0, P[4])) * conv;
   }
-  return k * pow(conv, double(M.convExp[r]));
+  return k * pow(conv, double(M.convExp()[r]));
 }
 
 template <typename TA>
 struct CellLds {
-  TA* A;       // [m][m] iteration matrix / LU factors (row-major, lda = m)
-  double* c;   // [nSpec] concentrations: active (evaluation point) then constant
-  double* kf;  // [nReac] rate constants
-  double* scr; // [max(nReac, nPairs)] rate of progress or derivative pairs
-  int* piv;    // [m]
+  TA* A;        // [m][m] iteration matrix / LU factors (row-major, lda = m)
+  double* c;    // [nSpec] concentrations: active (internal order) then constant
+  double* kf;   // [nReac] rate constants
+  double* scr;  // [max(nReac, nPairs)] rate of progress or derivative pairs
+  double* seg;  // [nSeg] source-term segment sums
+  TA* rdiag;    // [m] reciprocal pivots 1 / u_kk
+  int* swp;     // [2m] row interchanges (k, p) of the factorisation
 };
 
 template <typename TA>
@@ -93,11 +126,12 @@ __device__ CellLds<TA> carve(char* base, const DevMech& M, bool withMatrix = tru
   L.kf = L.c + M.nSpec;
   L.scr = L.kf + M.nReac;
   const int nScr = M.nReac > M.nPairs ? M.nReac : M.nPairs;
-  L.piv = reinterpret_cast<int*>(L.scr + nScr);
+  L.seg = L.scr + nScr;
+  L.rdiag = reinterpret_cast<TA*>(L.seg + M.nSeg);
+  L.swp = reinterpret_cast<int*>(L.seg + M.nSeg + M.m);
   return L;
 }
 
-// rate constants of one cell into LDS
 template <typename TA>
 __device__ void load_rate_constants(const DevMech& M, const CellLds<TA>& L, int lane, double T, double p,
                                     const double* photo) {
@@ -109,8 +143,8 @@ __device__ void load_rate_constants(const DevMech& M, const CellLds<TA>& L, int 
 // rate of progress of reaction r at the concentrations in LDS
 __device__ __forceinline__ double rate_of_progress(const DevMech& M, const double* c, const double* kf, int r) {
   double v = kf[r];
-  const int* sp = M.reacSpec + r * M.maxR;
-  const int* pw = M.reacPow + r * M.maxR;
+  const int* sp = M.reacSpec() + r * M.maxR;
+  const int* pw = M.reacPow() + r * M.maxR;
   for (int j = 0; j < M.maxR; ++j) {
     const int s = sp[j];
     if (s < 0) break;
@@ -119,20 +153,31 @@ __device__ __forceinline__ double rate_of_progress(const DevMech& M, const doubl
   return v;
 }
 
-// omega at the concentrations already in LDS (c); result in registers,
-// species i = lane + 64 q.  src/core/impl/TChem_Impl_ReactionRatesAerosol.hpp:78-138
+// omega (internal order) at the concentrations in LDS; result in registers.
+// src/core/impl/TChem_Impl_ReactionRatesAerosol.hpp:78-138, as a gather:
+// segment sums over (reaction, net coefficient) entries, then per species
+// the emission rate plus its segment sums in order.
 template <int QM, typename TA>
 __device__ void source_term_from_lds(const DevMech& M, const CellLds<TA>& L, int lane, double (&f)[QM]) {
+#pragma unroll 2
   for (int r = lane; r < M.nReac; r += kWave) L.scr[r] = rate_of_progress(M, L.c, L.kf, r);
+  wave_sync();
+  for (int s = lane; s < M.nSeg; s += kWave) {
+    const int e1 = M.segEnd()[s];
+    double acc = 0.0;
+#pragma unroll 4
+    for (int e = M.segBegin()[s]; e < e1; ++e) acc += M.omegaCoef()[e] * L.scr[M.omegaReac()[e]];
+    L.seg[s] = acc;
+  }
   wave_sync();
 #pragma unroll
   for (int q = 0; q < QM; ++q) {
-    const int i = lane + kWave * q;
+    const int a = lane + kWave * q;
     double s = 0.0;
-    if (i < M.m) {
-      s = M.emission[i];
-      const int e1 = M.omegaPtr[i + 1];
-      for (int e = M.omegaPtr[i]; e < e1; ++e) s += M.omegaCoef[e] * L.scr[M.omegaReac[e]];
+    if (a < M.m) {
+      s = M.emission()[a];
+      const int s1 = M.rowSegPtr()[a + 1];
+      for (int g = M.rowSegPtr()[a]; g < s1; ++g) s += L.seg[g];
     }
     f[q] = s;
   }
@@ -143,28 +188,34 @@ template <int QM, typename TA>
 __device__ void eval_rhs(const DevMech& M, const CellLds<TA>& L, int lane, const double (&u)[QM], double (&f)[QM]) {
 #pragma unroll
   for (int q = 0; q < QM; ++q) {
-    const int i = lane + kWave * q;
-    if (i < M.m) L.c[i] = u[q];
+    const int a = lane + kWave * q;
+    if (a < M.m) L.c[a] = u[q];
   }
   wave_sync();
   source_term_from_lds<QM>(M, L, lane, f);
 }
 
-// Assemble A = diag * I + scale * J at the concentrations in LDS.
+// A = diag * I + scale * J at the concentrations in LDS.  With external_order
+// the entries are placed at the reference's species indices.
 template <typename TA>
-__device__ void assemble_jacobian_from_lds(const DevMech& M, const CellLds<TA>& L, int lane, double diag,
-                                           double scale) {
+__device__ void assemble_jacobian_from_lds(const DevMech& M, const CellLds<TA>& L, int lane, double diag, double scale,
+                                           bool external_order) {
   const int m = M.m;
-  const int total = m * m;
-  for (int idx = lane; idx < total; idx += kWave) L.A[idx] = TA(0);
+  {
+    const size_t bytes = sizeof(TA) * size_t(m) * size_t(m);
+    const int n16 = int(bytes / 16);
+    uint4* A16 = reinterpret_cast<uint4*>(L.A);
+    for (int idx = lane; idx < n16; idx += kWave) A16[idx] = make_uint4(0, 0, 0, 0);
+    for (int idx = n16 * int(16 / sizeof(TA)) + lane; idx < m * m; idx += kWave) L.A[idx] = TA(0);
+  }
   // derivative pairs: d rop_r / d c_s = k * n_s c_s^(n_s-1) * prod_{j != s} c_j^n_j
   for (int r = lane; r < M.nReac; r += kWave) {
-    const int p0 = M.pairPtr[r], p1 = M.pairPtr[r + 1];
+    const int p0 = M.pairPtr()[r], p1 = M.pairPtr()[r + 1];
     if (p0 == p1) continue;
-    const int* sp = M.reacSpec + r * M.maxR;
-    const int* pw = M.reacPow + r * M.maxR;
+    const int* sp = M.reacSpec() + r * M.maxR;
+    const int* pw = M.reacPow() + r * M.maxR;
     for (int p = p0; p < p1; ++p) {
-      const int js = M.pairSlot[p];
+      const int js = M.pairSlot()[p];
       const int ns = pw[js];
       double d = L.kf[r] * double(ns) * pow_small(L.c[sp[js]], ns - 1);
       for (int j = 0; j < M.maxR; ++j) {
@@ -179,142 +230,226 @@ __device__ void assemble_jacobian_from_lds(const DevMech& M, const CellLds<TA>& 
   wave_sync();
   for (int n = lane; n < M.nnzJ; n += kWave) {
     double s = 0.0;
-    const int e1 = M.jnzPtr[n + 1];
-    for (int e = M.jnzPtr[n]; e < e1; ++e) s += M.jcCoef[e] * L.scr[M.jcPair[e]];
-    const int rc = M.jnzRowCol[n];
-    const int row = rc >> 16, col = rc & 0xffff;
-    L.A[row * m + col] = TA((row == col ? diag : 0.0) + scale * s);
+    const int e1 = M.jnzPtr()[n + 1];
+#pragma unroll 4
+    for (int e = M.jnzPtr()[n]; e < e1; ++e) s += M.jcCoef()[e] * L.scr[M.jcPair()[e]];
+    const int rc = M.jnzRowCol()[n];
+    int row = rc >> 16, col = rc & 0xffff;
+    const double v = (row == col ? diag : 0.0) + scale * s;
+    if (external_order) {
+      row = M.perm()[row];
+      col = M.perm()[col];
+    }
+    L.A[row * m + col] = TA(v);
   }
   wave_sync();
 }
 
+// In-place LU with partial pivoting of the m x m matrix in LDS (row
+// interchanges recorded in L.swp, count in nswp), eliminating the right-hand
+// side b (row a on lane a % 64, slot a / 64) alongside.  Same arithmetic as
+// oracle_lu + the forward half of oracle_lu_solve: pivot = first row of
+// largest |a_ik|, l = a_ik * (1 / a_kk), a_ij -= l a_kj, b_i -= l b_k.
+// The reciprocal pivots are kept for the back substitution.
+// Returns false for a zero or non-finite pivot.
 template <int QM, typename TA>
-__device__ void eval_iteration_matrix(const DevMech& M, const CellLds<TA>& L, int lane, const double (&u)[QM],
-                                      double ch) {
-#pragma unroll
-  for (int q = 0; q < QM; ++q) {
-    const int i = lane + kWave * q;
-    if (i < M.m) L.c[i] = u[q];
-  }
-  wave_sync();
-  assemble_jacobian_from_lds(M, L, lane, 1.0, -ch);
-}
-
-// In-place LU with partial pivoting of the m x m matrix in LDS.  Returns
-// false for a zero (or non-finite) pivot.  Mirrors oracle_lu's arithmetic:
-// multipliers l = a_ik / a_kk, trailing update a_ij -= l * a_kj.
-template <int QM, typename TA>
-__device__ bool lu_factor(const CellLds<TA>& L, int m, int lane) {
+__device__ bool lu_factor_fused(const CellLds<TA>& L, int m, int lane, TA (&b)[QM], int& nswp) {
   TA* A = L.A;
-  bool ok = true;
+  nswp = 0;
   for (int k = 0; k < m; ++k) {
-    // pivot search: smallest index among the largest |a_ik|, i >= k
-    TA best = TA(-1);
-    int bi = m;
-    for (int i = k + lane; i < m; i += kWave) {
-      const TA v = fabs(A[i * m + k]);
-      if (v > best) { best = v; bi = i; }
-    }
+    TA a[QM], uk[QM];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const TA ob = __shfl_xor(best, off, kWave);
-      const int oi = __shfl_xor(bi, off, kWave);
-      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    for (int q = 0; q < QM; ++q) {
+      const int i = lane + kWave * q;
+      a[q] = (i >= k && i < m) ? A[i * m + k] : TA(0);
+      const int j = k + 1 + lane + kWave * q;
+      uk[q] = (j < m) ? A[k * m + j] : TA(0);
     }
-    const int p = __builtin_amdgcn_readfirstlane(bi);
-    if (p >= m || !(best > TA(0))) { ok = false; break; }
-    if (lane == 0) L.piv[k] = p;
-    if (p != k)
+    const TA akk = bcast<QM>(a, k);
+    const TA dabs = fabs(akk);
+    unsigned long long gt = 0;
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const int i = lane + kWave * q;
+      gt |= ballot(i > k && i < m && fabs(a[q]) > dabs);
+    }
+    int p = k;
+    TA apiv = akk;
+    if (gt != 0 || !(dabs > TA(0))) {
+      // arg-max over rows i >= k; ties go to the smallest row index
+      TA best = TA(-1);
+      int bi = m;
+#pragma unroll
+      for (int q = 0; q < QM; ++q) {
+        const int i = lane + kWave * q;
+        if (i >= k && i < m) {
+          const TA v = fabs(a[q]);
+          if (v > best) { best = v; bi = i; }
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const TA ob = __shfl_xor(best, off, kWave);
+        const int oi = __shfl_xor(bi, off, kWave);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+      }
+      p = __builtin_amdgcn_readfirstlane(bi);
+      if (p >= m || !(best > TA(0))) return false;
+      apiv = bcast<QM>(a, p);
+    }
+    if (!isfinite(double(apiv))) return false;
+    if (p != k) {
       for (int j = lane; j < m; j += kWave) {
         const TA t = A[k * m + j];
         A[k * m + j] = A[p * m + j];
         A[p * m + j] = t;
       }
-    wave_sync();
-    const TA akk = A[k * m + k];
-    TA l[QM], uk[QM];
+      const TA bk = bcast<QM>(b, k), bp = bcast<QM>(b, p);
 #pragma unroll
-    for (int q = 0; q < QM; ++q) {
-      const int i = k + 1 + lane + kWave * q;
-      l[q] = TA(0);
-      uk[q] = TA(0);
-      if (i < m) {
-        l[q] = A[i * m + k] / akk;
-        A[i * m + k] = l[q];
-        uk[q] = A[k * m + i];  // pivot row, column i (same offset pattern)
+      for (int q = 0; q < QM; ++q) {
+        const int i = lane + kWave * q;
+        if (i == k) { a[q] = apiv; b[q] = bp; }
+        else if (i == p) { a[q] = akk; b[q] = bk; }
+      }
+      if (lane == 0) {
+        L.swp[2 * nswp] = k;
+        L.swp[2 * nswp + 1] = p;
+      }
+      ++nswp;
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < QM; ++q) {
+        const int j = k + 1 + lane + kWave * q;
+        uk[q] = (j < m) ? A[k * m + j] : TA(0);
       }
     }
-    // rank-1 update of the trailing submatrix, one row at a time for the
-    // rows with a nonzero multiplier; lanes own columns k+1+lane+64c
+    const TA bk = bcast<QM>(b, k);
+    const TA inv = TA(1) / apiv;
+    if (lane == 0) L.rdiag[k] = inv;
+    TA l[QM];
 #pragma unroll
     for (int q = 0; q < QM; ++q) {
-      unsigned long long mask = __ballot(l[q] != TA(0));
-      while (mask) {
-        const int b = __builtin_ctzll(mask);
-        mask &= mask - 1;
-        const TA li = readlane(l[q], b);
-        const int i = k + 1 + kWave * q + b;
+      const int i = lane + kWave * q;
+      l[q] = TA(0);
+      if (i > k && i < m && a[q] != TA(0)) {
+        l[q] = a[q] * inv;
+        A[i * m + k] = l[q];
+        b[q] -= l[q] * bk;
+      }
+    }
+    // trailing update: rows with a nonzero multiplier, two at a time
 #pragma unroll
-        for (int c = 0; c < QM; ++c) {
-          const int j = k + 1 + lane + kWave * c;
-          if (j < m) A[i * m + j] -= li * uk[c];
+    for (int q = 0; q < QM; ++q) {
+      unsigned long long mask = ballot(l[q] != TA(0));
+      while (mask) {
+        const int b0 = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const TA l0 = readlane(l[q], b0);
+        const int i0 = kWave * q + b0;
+        if (mask) {
+          const int b1 = __builtin_ctzll(mask);
+          mask &= mask - 1;
+          const TA l1 = readlane(l[q], b1);
+          const int i1 = kWave * q + b1;
+#pragma unroll
+          for (int c = 0; c < QM; ++c) {
+            const int j = k + 1 + lane + kWave * c;
+            if (j < m && uk[c] != TA(0)) {
+              const TA x0 = A[i0 * m + j];
+              const TA x1 = A[i1 * m + j];
+              A[i0 * m + j] = x0 - l0 * uk[c];
+              A[i1 * m + j] = x1 - l1 * uk[c];
+            }
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < QM; ++c) {
+            const int j = k + 1 + lane + kWave * c;
+            if (j < m && uk[c] != TA(0)) A[i0 * m + j] -= l0 * uk[c];
+          }
         }
       }
     }
     wave_sync();
   }
-  wave_sync();
-  return ok;
+  return true;
 }
 
+// Forward elimination with stored factors: interchanges, then unit-lower L.
 template <int QM, typename TA>
-__device__ __forceinline__ TA bcast(const TA (&v)[QM], int idx) {
-  const int q = idx / kWave, l = idx % kWave;
-  TA out = TA(0);
-#pragma unroll
-  for (int s = 0; s < QM; ++s)
-    if (s == q) out = readlane(v[s], l);
-  return out;
-}
-
-// Solve (LU) x = b with b distributed over lanes (row i = lane + 64 q).
-template <int QM, typename TA>
-__device__ void lu_solve(const CellLds<TA>& L, int m, int lane, TA (&b)[QM]) {
+__device__ void lu_forward(const CellLds<TA>& L, int m, int lane, TA (&b)[QM], int nswp) {
   const TA* A = L.A;
-  for (int k = 0; k < m; ++k) {
-    const int p = L.piv[k];
-    if (p != k) {
-      const TA bk = bcast<QM>(b, k), bp = bcast<QM>(b, p);
-#pragma unroll
-      for (int q = 0; q < QM; ++q) {
-        const int i = lane + kWave * q;
-        if (i == k) b[q] = bp;
-        else if (i == p) b[q] = bk;
-      }
-    }
-  }
-  for (int k = 0; k < m - 1; ++k) {
-    const TA yk = bcast<QM>(b, k);
-    if (yk == TA(0)) continue;
+  for (int s = 0; s < nswp; ++s) {
+    const int k = L.swp[2 * s], p = L.swp[2 * s + 1];
+    const TA bk = bcast<QM>(b, k), bp = bcast<QM>(b, p);
 #pragma unroll
     for (int q = 0; q < QM; ++q) {
       const int i = lane + kWave * q;
-      if (i > k && i < m) b[q] -= A[i * m + k] * yk;
+      if (i == k) b[q] = bp;
+      else if (i == p) b[q] = bk;
     }
   }
+  TA col[QM];
+#pragma unroll
+  for (int q = 0; q < QM; ++q) {
+    const int i = lane + kWave * q;
+    col[q] = (i > 0 && i < m) ? A[i * m] : TA(0);
+  }
+  for (int k = 0; k < m - 1; ++k) {
+    TA nxt[QM];
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const int i = lane + kWave * q;
+      nxt[q] = (i > k + 1 && i < m) ? A[i * m + k + 1] : TA(0);
+    }
+    const TA yk = bcast<QM>(b, k);
+    if (yk != TA(0)) {
+#pragma unroll
+      for (int q = 0; q < QM; ++q) b[q] -= col[q] * yk;
+    }
+#pragma unroll
+    for (int q = 0; q < QM; ++q) col[q] = nxt[q];
+  }
+}
+
+// Back substitution with the upper factor (column order, as oracle_lu_solve);
+// the next column is fetched from LDS while the current one is applied.
+template <int QM, typename TA>
+__device__ void lu_back(const CellLds<TA>& L, int m, int lane, TA (&b)[QM]) {
+  const TA* A = L.A;
+  TA col[QM];
+  TA ukk = L.rdiag[m - 1];
+#pragma unroll
+  for (int q = 0; q < QM; ++q) {
+    const int i = lane + kWave * q;
+    col[q] = (i < m - 1) ? A[i * m + m - 1] : TA(0);
+  }
   for (int k = m - 1; k >= 0; --k) {
-    const TA xk = bcast<QM>(b, k) / A[k * m + k];
+    TA nxt[QM];
+    TA nkk = TA(1);
+#pragma unroll
+    for (int q = 0; q < QM; ++q) nxt[q] = TA(0);
+    if (k > 0) {
+      nkk = L.rdiag[k - 1];
+#pragma unroll
+      for (int q = 0; q < QM; ++q) {
+        const int i = lane + kWave * q;
+        nxt[q] = (i < k - 1) ? A[i * m + k - 1] : TA(0);
+      }
+    }
+    const TA xk = bcast<QM>(b, k) * ukk;  // ukk holds 1 / u_kk
 #pragma unroll
     for (int q = 0; q < QM; ++q) {
       const int i = lane + kWave * q;
       if (i == k) b[q] = xk;
     }
-    if (xk == TA(0)) continue;
+    if (xk != TA(0)) {
 #pragma unroll
-    for (int q = 0; q < QM; ++q) {
-      const int i = lane + kWave * q;
-      if (i < k) b[q] -= A[i * m + k] * xk;
+      for (int q = 0; q < QM; ++q) b[q] -= col[q] * xk;
     }
+    ukk = nkk;
+#pragma unroll
+    for (int q = 0; q < QM; ++q) col[q] = nxt[q];
   }
 }
 
@@ -322,34 +457,48 @@ struct CellCounters {
   int accepted = 0, rejected = 0, newton_failures = 0, newton_iterations = 0, factorizations = 0, failed = 0;
 };
 
-// Newton iteration for one TR-BDF2 stage: G(u) = u - b - ch f(u), iteration
+// Newton iteration for one TR-BDF2 stage: G(u) = u - bc - ch f(u), iteration
 // matrix I - ch J refreshed every jacobian_interval iterations; converged when
-// max_i |dx_i| / (atol + rtol |u_i|) <= 1 (oracle/oracle.c: newton).
+// max_i |dx_i| / (atol + rtol |u_i|) <= 1 (oracle/oracle.c: newton).  The
+// residual and the Jacobian are evaluated at the same iterate, the residual
+// first so the factorisation eliminates it on the way.
 template <int QM, typename TA>
 __device__ bool newton(const DevMech& M, const CellLds<TA>& L, int lane, double (&u)[QM], const double (&bc)[QM],
                        double ch, double atol, double rtol, int max_it, int jint, CellCounters& cnt) {
   const int m = M.m;
   if (jint < 1) jint = 1;
+  int nswp = 0;
   for (int it = 0; it < max_it; ++it) {
-    if (it % jint == 0) {
-      eval_iteration_matrix<QM>(M, L, lane, u, ch);
-      if (!lu_factor<QM>(L, m, lane)) return false;
-      ++cnt.factorizations;
-    }
     double f[QM];
+    TCHEM_PT_BEGIN(rhs)
     eval_rhs<QM>(M, L, lane, u, f);
+    TCHEM_PT_END(rhs, 0)
     TA x[QM];
 #pragma unroll
     for (int q = 0; q < QM; ++q) {
-      const int i = lane + kWave * q;
-      x[q] = (i < m) ? TA(-(u[q] - bc[q] - ch * f[q])) : TA(0);
+      const int a = lane + kWave * q;
+      x[q] = (a < m) ? TA(-(u[q] - bc[q] - ch * f[q])) : TA(0);
     }
-    lu_solve<QM>(L, m, lane, x);
+    if (it % jint == 0) {
+      TCHEM_PT_BEGIN(jac)
+      assemble_jacobian_from_lds(M, L, lane, 1.0, -ch, false);
+      TCHEM_PT_END(jac, 1)
+      TCHEM_PT_BEGIN(lu)
+      const bool factored = lu_factor_fused<QM>(L, m, lane, x, nswp);
+      TCHEM_PT_END(lu, 2)
+      if (!factored) return false;
+      ++cnt.factorizations;
+    } else {
+      lu_forward<QM>(L, m, lane, x, nswp);
+    }
+    TCHEM_PT_BEGIN(back)
+    lu_back<QM>(L, m, lane, x);
+    TCHEM_PT_END(back, 3)
     double nrm = 0.0;
 #pragma unroll
     for (int q = 0; q < QM; ++q) {
-      const int i = lane + kWave * q;
-      if (i < m) {
+      const int a = lane + kWave * q;
+      if (a < m) {
         const double dx = double(x[q]);
         u[q] += dx;
         double w = fabs(dx) / (atol + rtol * fabs(u[q]));
@@ -389,18 +538,17 @@ __device__ void integrate_cell(const IntegratorArgs& a, const CellLds<TA>& L, in
   const double c_bdf = (1.0 - gam) / (2.0 - gam);
   const double k_err = (-3.0 * gam * gam + 4.0 * gam - 2.0) / (12.0 * (2.0 - gam));
 
+  TCHEM_PT_BEGIN(cell)
   const double p = row[1], T = row[2];
   load_rate_constants(M, L, lane, T, p, a.photo ? a.photo + size_t(cell) * M.nPhoto : nullptr);
   for (int i = m + lane; i < nSpec; i += kWave) L.c[i] = row[3 + i];
 
-  double un[QM], u[QM], ug[QM], fn[QM], fg[QM], f1[QM], bc[QM];
-  double atl[QM], rtl[QM];
+  // live across steps: un (state), fn (f at un); per step: u, bc, est, f1
+  double un[QM], u[QM], fn[QM], f1[QM], bc[QM], est[QM];
 #pragma unroll
   for (int q = 0; q < QM; ++q) {
-    const int i = lane + kWave * q;
-    un[q] = (i < m) ? row[3 + i] : 0.0;
-    atl[q] = (i < m) ? a.tol_time[2 * i] : 1.0;
-    rtl[q] = (i < m) ? a.tol_time[2 * i + 1] : 0.0;
+    const int ai = lane + kWave * q;
+    un[q] = (ai < m) ? row[3 + M.perm()[ai]] : 0.0;
     fn[q] = 0.0;
   }
   const double atol_n = a.tol_newton[0], rtol_n = a.tol_newton[1];
@@ -430,10 +578,13 @@ __device__ void integrate_cell(const IntegratorArgs& a, const CellLds<TA>& L, in
       }
       conv = newton<QM>(M, L, lane, u, bc, c_tr * h, atol_n, rtol_n, max_newton, jint, cnt);
       if (conv) {
+        // f at the TR-stage value enters the error estimate; the stage value
+        // itself only enters the BDF2 constant term
+        eval_rhs<QM>(M, L, lane, u, f1);
 #pragma unroll
         for (int q = 0; q < QM; ++q) {
-          ug[q] = u[q];
-          bc[q] = a_g * ug[q] - a_n * un[q];
+          est[q] = fn[q] / gam - f1[q] / (gam * (1.0 - gam));
+          bc[q] = a_g * u[q] - a_n * un[q];
         }
         conv = newton<QM>(M, L, lane, u, bc, c_bdf * h, atol_n, rtol_n, max_newton, jint, cnt);
       }
@@ -446,15 +597,15 @@ __device__ void integrate_cell(const IntegratorArgs& a, const CellLds<TA>& L, in
       failed = true;
       break;
     }
-    eval_rhs<QM>(M, L, lane, ug, fg);
     eval_rhs<QM>(M, L, lane, u, f1);
     double e = 0.0;
 #pragma unroll
     for (int q = 0; q < QM; ++q) {
-      const int i = lane + kWave * q;
-      if (i < m) {
-        const double est = 2.0 * k_err * h * (fn[q] / gam - fg[q] / (gam * (1.0 - gam)) + f1[q] / (1.0 - gam));
-        double w = fabs(est) / (atl[q] + rtl[q] * fabs(u[q]));
+      const int ai = lane + kWave * q;
+      if (ai < m) {
+        const int x = M.perm()[ai];
+        const double lte = 2.0 * k_err * h * (est[q] + f1[q] / (1.0 - gam));
+        double w = fabs(lte) / (a.tol_time[2 * x] + a.tol_time[2 * x + 1] * fabs(u[q]));
         if (!(w == w)) w = INFINITY;
         e = fmax(e, w);
       }
@@ -476,14 +627,16 @@ __device__ void integrate_cell(const IntegratorArgs& a, const CellLds<TA>& L, in
     ++cnt.accepted;
   }
 
+  TCHEM_PT_END(cell, 6)
   double* rowo = a.state_out + size_t(cell) * dim;
 #pragma unroll
   for (int q = 0; q < QM; ++q) {
-    const int i = lane + kWave * q;
-    if (i < m) {
+    const int ai = lane + kWave * q;
+    if (ai < m) {
+      const int x = M.perm()[ai];
       const double v = failed ? 0.0 : un[q];
-      row[3 + i] = v;
-      if (rowo != row) rowo[3 + i] = v;
+      row[3 + x] = v;
+      if (rowo != row) rowo[3 + x] = v;
     }
   }
   if (rowo != row)
@@ -533,7 +686,13 @@ __global__ void update_tadv_kernel(int n, tchem_time_advance_t* tadv, const doub
   }
 }
 
-// Source term per cell: one wavefront per cell.
+template <typename TA>
+__device__ void load_concentrations(const DevMech& M, const CellLds<TA>& L, int lane, const double* row) {
+  for (int a = lane; a < M.m; a += kWave) L.c[a] = row[3 + M.perm()[a]];
+  for (int i = M.m + lane; i < M.nSpec; i += kWave) L.c[i] = row[3 + i];
+}
+
+// Source term per cell (one wavefront per cell), reference species order.
 template <int QM>
 __global__ __launch_bounds__(64) void sourceterm_kernel(DevMech M, int nBatch, const double* state,
                                                         const double* photo, const double* ext, double* omega) {
@@ -543,18 +702,21 @@ __global__ __launch_bounds__(64) void sourceterm_kernel(DevMech M, int nBatch, c
   if (cell >= nBatch) return;
   const double* row = state + size_t(cell) * (M.nSpec + 3);
   load_rate_constants(M, L, lane, row[2], row[1], photo ? photo + size_t(cell) * M.nPhoto : nullptr);
-  for (int i = lane; i < M.nSpec; i += kWave) L.c[i] = row[3 + i];
+  load_concentrations(M, L, lane, row);
   wave_sync();
   double f[QM];
   source_term_from_lds<QM>(M, L, lane, f);
 #pragma unroll
   for (int q = 0; q < QM; ++q) {
-    const int i = lane + kWave * q;
-    if (i < M.m) omega[size_t(cell) * M.m + i] = f[q] + (ext ? ext[size_t(cell) * M.m + i] : 0.0);
+    const int a = lane + kWave * q;
+    if (a < M.m) {
+      const size_t o = size_t(cell) * M.m + M.perm()[a];
+      omega[o] = f[q] + (ext ? ext[o] : 0.0);
+    }
   }
 }
 
-// Jacobian per cell: assembled in LDS, written out coalesced.
+// Jacobian per cell: assembled in LDS in reference order, written out coalesced.
 __global__ __launch_bounds__(64) void jacobian_kernel(DevMech M, int nBatch, const double* state, const double* photo,
                                                       double* jac) {
   extern __shared__ __align__(16) char lds_raw[];
@@ -563,9 +725,9 @@ __global__ __launch_bounds__(64) void jacobian_kernel(DevMech M, int nBatch, con
   if (cell >= nBatch) return;
   const double* row = state + size_t(cell) * (M.nSpec + 3);
   load_rate_constants(M, L, lane, row[2], row[1], photo ? photo + size_t(cell) * M.nPhoto : nullptr);
-  for (int i = lane; i < M.nSpec; i += kWave) L.c[i] = row[3 + i];
+  load_concentrations(M, L, lane, row);
   wave_sync();
-  assemble_jacobian_from_lds(M, L, lane, 0.0, 1.0);
+  assemble_jacobian_from_lds(M, L, lane, 0.0, 1.0, true);
   const int total = M.m * M.m;
   double* out = jac + size_t(cell) * total;
   for (int idx = lane; idx < total; idx += kWave) out[idx] = L.A[idx];
@@ -641,3 +803,15 @@ hipError_t launchJacobian(const DevMech& M, int nBatch, const double* state, con
 }
 
 }  // namespace tchem_amd
+
+#ifdef TCHEM_PHASE_TIMERS
+extern "C" int tchem_debug_phase_cycles(unsigned long long* out, int reset) {
+  if (hipMemcpyFromSymbol(out, HIP_SYMBOL(tchem_amd::g_phase_cycles), sizeof(unsigned long long) * 8) != hipSuccess)
+    return -1;
+  if (reset) {
+    const unsigned long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (hipMemcpyToSymbol(HIP_SYMBOL(tchem_amd::g_phase_cycles), z, sizeof(z)) != hipSuccess) return -1;
+  }
+  return 0;
+}
+#endif

@@ -47,21 +47,9 @@ MechTables compileMechanism(const KineticModel& km) {
   t.nPhoto = int(t.photoReac.size());
   for (int q = 0; q < t.nPhoto; ++q) t.photoSlot[t.photoReac[q]] = q;
 
-  // reactants (ordered by species index, as the reference's std::map)
-  int maxR = 0;
-  for (int r = 0; r < nR; ++r) maxR = std::max(maxR, km.reacNreac[r]);
-  t.maxReactants = std::max(maxR, 1);
-  t.reacSpec.assign(size_t(nR) * t.maxReactants, -1);
-  t.reacPow.assign(size_t(nR) * t.maxReactants, 0);
-  for (int r = 0; r < nR; ++r)
-    for (int j = 0; j < km.reacNreac[r]; ++j) {
-      t.reacSpec[size_t(r) * t.maxReactants + j] = km.sidx(r, j);
-      t.reacPow[size_t(r) * t.maxReactants + j] = std::abs(int(km.nuki(r, j)));
-    }
-
   // net coefficient of species i in reaction r (active species only);
   // reactant and product entries of the same species are combined
-  std::vector<std::map<int, double>> net(nR);  // r -> (species -> nu)
+  std::vector<std::map<int, double>> net(nR);  // r -> (species -> nu), reference numbering
   for (int r = 0; r < nR; ++r) {
     for (int j = 0; j < km.reacNreac[r]; ++j) {
       const int s = km.sidx(r, j);
@@ -75,22 +63,57 @@ MechTables compileMechanism(const KineticModel& km) {
     }
   }
 
-  // source term CSR
+  // structure of I - J in the reference numbering -> elimination order
+  {
+    std::vector<char> pat(size_t(m) * m, 0);
+    for (int i = 0; i < m; ++i) pat[size_t(i) * m + i] = 1;
+    for (int r = 0; r < nR; ++r)
+      for (int j = 0; j < km.reacNreac[r]; ++j) {
+        const int s = km.sidx(r, j);
+        if (s >= m) continue;
+        for (const auto& e : net[r])
+          if (e.second != 0.0) pat[size_t(e.first) * m + s] = 1;
+      }
+    t.perm = markowitzOrder(m, pat);
+    t.iperm.assign(m, 0);
+    for (int a = 0; a < m; ++a) t.iperm[t.perm[a]] = a;
+  }
+  auto in = [&](int s) { return s < m ? t.iperm[s] : s; };  // reference -> internal index
+
+  // reactants (ordered by species index, as the reference's std::map)
+  int maxR = 0;
+  for (int r = 0; r < nR; ++r) maxR = std::max(maxR, km.reacNreac[r]);
+  t.maxReactants = std::max(maxR, 1);
+  t.reacSpec.assign(size_t(nR) * t.maxReactants, -1);
+  t.reacPow.assign(size_t(nR) * t.maxReactants, 0);
+  for (int r = 0; r < nR; ++r)
+    for (int j = 0; j < km.reacNreac[r]; ++j) {
+      t.reacSpec[size_t(r) * t.maxReactants + j] = in(km.sidx(r, j));
+      t.reacPow[size_t(r) * t.maxReactants + j] = std::abs(int(km.nuki(r, j)));
+    }
+
+  // source term CSR (internal rows, entries in reaction order) and segments
   std::vector<std::vector<std::pair<int, double>>> rows(m);
   for (int r = 0; r < nR; ++r)
     for (const auto& e : net[r])
-      if (e.second != 0.0) rows[e.first].push_back({r, e.second});
+      if (e.second != 0.0) rows[t.iperm[e.first]].push_back({r, e.second});
   t.omegaPtr.assign(m + 1, 0);
-  for (int i = 0; i < m; ++i) {
-    t.omegaPtr[i + 1] = t.omegaPtr[i] + int(rows[i].size());
-    for (const auto& e : rows[i]) {
+  t.rowSegPtr.assign(m + 1, 0);
+  for (int a = 0; a < m; ++a) {
+    t.omegaPtr[a + 1] = t.omegaPtr[a] + int(rows[a].size());
+    for (const auto& e : rows[a]) {
       t.omegaReac.push_back(e.first);
       t.omegaCoef.push_back(e.second);
     }
+    for (int b = t.omegaPtr[a]; b < t.omegaPtr[a + 1]; b += MechTables::kOmegaSegment) {
+      t.segBegin.push_back(b);
+      t.segEnd.push_back(std::min(b + MechTables::kOmegaSegment, t.omegaPtr[a + 1]));
+    }
+    t.rowSegPtr[a + 1] = int(t.segBegin.size());
   }
   t.emission.assign(m, 0.0);
   for (const auto& e : km.emission) {
-    if (e.species < m) t.emission[e.species] = e.rate;  // assignment: the last entry wins
+    if (e.species < m) t.emission[t.iperm[e.species]] = e.rate;  // assignment: the last entry wins
     else std::fprintf(stderr, "tchem: emission for constant species %d ignored\n", e.species);
   }
 
@@ -103,7 +126,7 @@ MechTables compileMechanism(const KineticModel& km) {
       if (s < m) {
         t.pairSlot.push_back(j);
         pairReac.push_back(r);
-        pairSpec.push_back(s);
+        pairSpec.push_back(t.iperm[s]);
       }
     }
     t.pairPtr[r + 1] = int(t.pairSlot.size());
@@ -113,7 +136,7 @@ MechTables compileMechanism(const KineticModel& km) {
   for (int p = 0; p < t.nPairs(); ++p) {
     const int r = pairReac[p], s = pairSpec[p];
     for (const auto& e : net[r])
-      if (e.second != 0.0) jmap[{e.first, s}].push_back({p, e.second});
+      if (e.second != 0.0) jmap[{t.iperm[e.first], s}].push_back({p, e.second});
   }
   t.jnzPtr.push_back(0);
   for (const auto& e : jmap) {
@@ -126,6 +149,35 @@ MechTables compileMechanism(const KineticModel& km) {
     t.jnzPtr.push_back(int(t.jcPair.size()));
   }
   return t;
+}
+
+std::vector<int> markowitzOrder(int m, std::vector<char> pat) {
+  std::vector<int> order;
+  std::vector<char> done(m, 0);
+  order.reserve(m);
+  for (int step = 0; step < m; ++step) {
+    long best = -1;
+    int bi = -1;
+    for (int i = 0; i < m; ++i) {
+      if (done[i]) continue;
+      long rc = 0, cc = 0;
+      for (int j = 0; j < m; ++j)
+        if (!done[j]) {
+          rc += pat[size_t(i) * m + j];
+          cc += pat[size_t(j) * m + i];
+        }
+      const long score = (rc - 1) * (cc - 1);
+      if (bi < 0 || score < best) { best = score; bi = i; }
+    }
+    order.push_back(bi);
+    done[bi] = 1;
+    for (int r = 0; r < m; ++r) {
+      if (done[r] || !pat[size_t(r) * m + bi]) continue;
+      for (int c = 0; c < m; ++c)
+        if (!done[c] && pat[size_t(bi) * m + c]) pat[size_t(r) * m + c] = 1;
+    }
+  }
+  return order;
 }
 
 }  // namespace tchem_amd

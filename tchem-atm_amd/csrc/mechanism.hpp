@@ -31,6 +31,13 @@ struct MechTables {
   int maxReactants = 0;
   double convPPM = kConvPPM;
 
+  // Active species are renumbered in a fill-reducing elimination order
+  // (greedy minimum Markowitz count on the structure of I - J).  Every table
+  // below uses the internal numbering; perm[a] is the reference's species
+  // index of internal species a, iperm its inverse.  Constant species keep
+  // their indices (>= m).
+  std::vector<int> perm, iperm;  // [m]
+
   // rate constants, per reaction
   std::vector<int> rateKind;     // RateKind
   std::vector<double> ratePar;   // [nReac][8]
@@ -42,10 +49,16 @@ struct MechTables {
   std::vector<int> reacSpec;  // [nReac][maxReactants], -1 padded
   std::vector<int> reacPow;   // [nReac][maxReactants]
 
-  // source term gather (species-major), emission per active species
+  // source term gather (species-major), emission per active species.  Rows
+  // are cut into segments of at most kOmegaSegment entries so long rows
+  // (radicals in ~100 reactions) spread over several lanes; segment sums are
+  // added per row in segment order.
+  static constexpr int kOmegaSegment = 16;
   std::vector<int> omegaPtr;     // [m+1]
   std::vector<int> omegaReac;    // [nnzOmega]
   std::vector<double> omegaCoef; // [nnzOmega]
+  std::vector<int> segBegin, segEnd;  // [nSeg] entry ranges
+  std::vector<int> rowSegPtr;         // [m+1] segments of row a
   std::vector<double> emission;  // [m]
 
   // Jacobian
@@ -57,6 +70,7 @@ struct MechTables {
   std::vector<double> jcCoef;    // [nContrib]
 
   int nPairs() const { return int(pairSlot.size()); }
+  int nSeg() const { return int(segBegin.size()); }
   int nnzJ() const { return int(jnzRow.size()); }
 };
 
@@ -64,5 +78,11 @@ struct MechTables {
 // evaluate (JPL / R_JPL_ARRHENIUS / PHOTO_RATE reactions belong to the E3SM
 // kinetics path).
 MechTables compileMechanism(const KineticModel& km);
+
+// Greedy minimum-Markowitz elimination order of the m x m structure `pattern`
+// (row-major booleans, diagonal included): at each step the remaining index
+// with the smallest (row count - 1) * (column count - 1) is eliminated (ties:
+// lowest index) and its fill is added.  Returns the order (position -> index).
+std::vector<int> markowitzOrder(int m, std::vector<char> pattern);
 
 }  // namespace tchem_amd
