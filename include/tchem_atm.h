@@ -53,11 +53,15 @@ typedef struct {
 #define TCHEM_PRECISION_MIXED 1 /* FP32 iteration matrix + LU, FP64 state/residual/error/step control */
 #define TCHEM_SCHEDULE_GRID 0       /* one workgroup per cell, grid = nBatch */
 #define TCHEM_SCHEDULE_PERSISTENT 1 /* resident workgroups pull cells from a device work queue */
+#define TCHEM_LU_AUTO 0   /* sparse when the mechanism has a sparse schedule, dense otherwise */
+#define TCHEM_LU_DENSE 1  /* dense LU with partial pivoting (the oracle's arithmetic) */
+#define TCHEM_LU_SPARSE 2 /* level-scheduled sparse LU, pivots fixed by the fill-reducing order */
 typedef struct {
   int precision;         /* TCHEM_PRECISION_* */
   int schedule;          /* TCHEM_SCHEDULE_* */
   int blocks_per_cu;     /* persistent schedule: resident workgroups per CU (0 = from occupancy) */
-  int reserved[5];       /* must be zero */
+  int lu;                /* TCHEM_LU_* */
+  int reserved[4];       /* must be zero */
 } tchem_atmchem_options_t;
 
 /* Last error message of the calling thread ("" when none). */
@@ -97,8 +101,8 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out);
 int tchem_kmcd_destroy(tchem_kmcd_t kmcd);
 
 /* Replaces AtmosphericChemistry::getWorkSpaceSize (src/core/TChem_AtmosphericChemistry.hpp:47-81):
- * LDS bytes one cell uses in the given precision. */
-int tchem_atmchem_workspace_bytes(tchem_kmcd_t kmcd, int precision, size_t* bytes);
+ * LDS bytes one cell uses in the form opts selects (NULL: defaults). */
+int tchem_atmchem_workspace_bytes(tchem_kmcd_t kmcd, const tchem_atmchem_options_t* opts, size_t* bytes);
 
 /* ---- batched kernels ------------------------------------------------------ */
 

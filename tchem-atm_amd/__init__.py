@@ -34,6 +34,9 @@ PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 SCHEDULE_GRID = 0
 SCHEDULE_PERSISTENT = 1
+LU_AUTO = 0     # sparse when the mechanism has a sparse schedule, dense otherwise
+LU_DENSE = 1    # dense LU with partial pivoting (the oracle's arithmetic)
+LU_SPARSE = 2   # level-scheduled sparse LU, pivots fixed by the fill-reducing order
 
 TIME_ADVANCE_DTYPE = np.dtype([
     ("_tbeg", "f8"), ("_tend", "f8"), ("_dt", "f8"), ("_dtmin", "f8"), ("_dtmax", "f8"),
@@ -59,7 +62,7 @@ class TChemError(RuntimeError):
 
 class _Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("schedule", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
-                ("reserved", ctypes.c_int * 5)]
+                ("lu", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
 
 
 def _load():
@@ -77,7 +80,7 @@ def _load():
         "tchem_scenario_load": [vp, cp, vp, i, ctypes.POINTER(i)],
         "tchem_kmcd_create": [vp, ctypes.POINTER(vp)],
         "tchem_kmcd_destroy": [vp],
-        "tchem_atmchem_workspace_bytes": [vp, i, ctypes.POINTER(ctypes.c_size_t)],
+        "tchem_atmchem_workspace_bytes": [vp, ctypes.POINTER(_Options), ctypes.POINTER(ctypes.c_size_t)],
         "tchem_atmchem_run_device_batch": [ctypes.POINTER(_Options), i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                            vp],
         "tchem_atmchem_update_time_advance": [i, vp, vp, vp, vp],
@@ -194,21 +197,23 @@ def setScenarioConditions(filename: str, kmd: KineticModelData) -> np.ndarray:
 
 class AtmosphericChemistry:
     @staticmethod
-    def getWorkSpaceSize(kmcd: KineticModelConstData, precision: int = PRECISION_FP64) -> int:
-        """Bytes of LDS one cell uses."""
+    def getWorkSpaceSize(kmcd: KineticModelConstData, precision: int = PRECISION_FP64, lu: int = LU_AUTO) -> int:
+        """Bytes of LDS one cell uses in the given form."""
         b = ctypes.c_size_t()
-        _check(_lib.tchem_atmchem_workspace_bytes(kmcd._h, precision, ctypes.byref(b)))
+        opts = _Options(precision, SCHEDULE_GRID, 0, lu)
+        _check(_lib.tchem_atmchem_workspace_bytes(kmcd._h, ctypes.byref(opts), ctypes.byref(b)))
         return b.value
 
     @staticmethod
     def runDeviceBatch(tol_newton, tol_time, fac, tadv, state, t_out, dt_out, state_out, kmcd, photo_rates=None,
-                       stats=None, precision=PRECISION_FP64, schedule=SCHEDULE_GRID, blocks_per_cu=0, stream=None):
+                       stats=None, precision=PRECISION_FP64, schedule=SCHEDULE_GRID, blocks_per_cu=0, lu=LU_AUTO,
+                       stream=None):
         """One integration interval over the batch (asynchronous on `stream`).
 
         tadv is a uint8 device tensor holding TIME_ADVANCE_DTYPE records (or any
         tensor with 56 bytes per cell); stats likewise CELL_STATS_DTYPE."""
         nBatch = state.shape[0]
-        opts = _Options(precision, schedule, blocks_per_cu)
+        opts = _Options(precision, schedule, blocks_per_cu, lu)
         _check(_lib.tchem_atmchem_run_device_batch(
             ctypes.byref(opts), nBatch, _ptr(tol_newton, what="tol_newton"), _ptr(tol_time, what="tol_time"),
             _ptr(fac, None, "fac"), _ptr(tadv, None, "tadv"), _ptr(state, what="state"),

@@ -14,11 +14,14 @@
 #include "device_mech.hpp"
 #include "kinetic_model.hpp"
 #include "mechanism.hpp"
+#include "schedule.hpp"
 
 namespace tchem_amd {
 hipError_t launchIntegrator(const IntegratorArgs& a, int precision, int grid, size_t lds, hipStream_t stream);
-hipError_t integratorMaxBlocksPerCU(int precision, int m, size_t lds, int* blocks);
+hipError_t integratorMaxBlocksPerCU(int precision, int sparse, int m, size_t lds, int* blocks);
 hipError_t launchResetCounter(int* counter, hipStream_t stream);
+hipError_t launchRates(const DevMech& M, int nBatch, const double* state, const double* photo, double* kf,
+                       hipStream_t stream);
 hipError_t launchUpdateTadv(int n, tchem_time_advance_t* tadv, const double* t_out, const double* dt_out,
                             hipStream_t stream);
 hipError_t launchSourceTerm(const DevMech& M, int nBatch, const double* state, const double* photo, const double* ext,
@@ -35,13 +38,18 @@ struct tchem_kmd_s {
 
 struct tchem_kmcd_s {
   MechTables tables;
+  SparseSchedule sched;
   DevMech dev{};
+  DevSparse sparse{};
   std::vector<void*> allocations;
   int device = 0;
   int numCUs = 0;
   size_t maxLds = 0;
   int* queue = nullptr;
+  double* kf = nullptr;  // [capacity][nReac] rate constants of the last batch
+  size_t kfCapacity = 0;  // cells
   ~tchem_kmcd_s() {
+    if (kf) allocations.push_back(kf);
     int cur = 0;
     if (hipGetDevice(&cur) == hipSuccess && cur != device) (void)hipSetDevice(device);
     for (void* p : allocations) (void)hipFree(p);
@@ -76,6 +84,31 @@ int upload(tchem_kmcd_s* k, const std::vector<T>& v, const T** out) {
   if (e != hipSuccess) return hipFail(e, "hipMemcpy");
   *out = static_cast<const T*>(p);
   return TCHEM_OK;
+}
+
+// Options after validation, with the LU form resolved.
+int resolveOptions(const tchem_atmchem_options_t* opts, const tchem_kmcd_s* k, tchem_atmchem_options_t* o,
+                   const char* who) {
+  *o = tchem_atmchem_options_t{};
+  if (opts) *o = *opts;
+  for (int r : o->reserved)
+    if (r != 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": reserved option fields must be zero");
+  if (o->precision != TCHEM_PRECISION_FP64 && o->precision != TCHEM_PRECISION_MIXED)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": unknown precision");
+  if (o->schedule != TCHEM_SCHEDULE_GRID && o->schedule != TCHEM_SCHEDULE_PERSISTENT)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": unknown schedule");
+  if (o->blocks_per_cu < 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": negative blocks_per_cu");
+  if (o->lu != TCHEM_LU_AUTO && o->lu != TCHEM_LU_DENSE && o->lu != TCHEM_LU_SPARSE)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": unknown lu");
+  if (o->lu == TCHEM_LU_AUTO) o->lu = k->sparse.enabled ? TCHEM_LU_SPARSE : TCHEM_LU_DENSE;
+  if (o->lu == TCHEM_LU_SPARSE && !k->sparse.enabled)
+    return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": mechanism has no sparse schedule");
+  return TCHEM_OK;
+}
+
+size_t ldsBytes(const tchem_kmcd_s* k, const tchem_atmchem_options_t& o) {
+  const size_t bytesA = o.precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float);
+  return o.lu == TCHEM_LU_SPARSE ? sparseLdsBytes(k->dev, k->sparse, bytesA) : integratorLdsBytes(k->dev, bytesA);
 }
 
 }  // namespace
@@ -218,6 +251,49 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
   d.oOmegaCoef = addD(t.omegaCoef);
   d.oEmission = addD(t.emission);
   d.oJcCoef = addD(t.jcCoef);
+  // sparse schedule (mechanisms it cannot express run the dense form only)
+  try {
+    k->sched = buildSparseSchedule(t);
+    const SparseSchedule& sc = k->sched;
+    DevSparse& sp = k->sparse;
+    sp.enabled = 1;
+    sp.nnzF = sc.nnzF;
+    sp.zeroSlot = sc.zeroSlot;
+    sp.nLevels = sc.nLevels;
+    sp.nFwdLevels = sc.nFwdLevels;
+    sp.nBackLevels = sc.nBackLevels;
+    sp.ropRounds = sc.ropRounds;
+    sp.segRounds = sc.segRounds;
+    sp.pairRounds = sc.pairRounds;
+    sp.pairSlots = sc.pairSlots;
+    sp.jacRounds = sc.jacRounds;
+    sp.oRopRec = addI(sc.ropRec);
+    sp.oSegRound = addI(sc.segRound);
+    sp.oSegTarget = addI(sc.segTarget);
+    sp.oSegReac = addI(sc.segReac);
+    sp.oPairRec = addI(sc.pairRec);
+    sp.oJacRound = addI(sc.jacRound);
+    sp.oJacTarget = addI(sc.jacTarget);
+    sp.oJacPair = addI(sc.jacPair);
+    sp.oLevMul = addI(sc.levMul);
+    sp.oLevUpd = addI(sc.levUpd);
+    sp.oMulRec = addI(sc.mulRec);
+    sp.oUpdRound = addI(sc.updRound);
+    sp.oUpdTarget = addI(sc.updTarget);
+    sp.oUpdPair = addI(sc.updPair);
+    sp.oFwdLev = addI(sc.fwdLev);
+    sp.oFwdRound = addI(sc.fwdRound);
+    sp.oFwdTarget = addI(sc.fwdTarget);
+    sp.oFwdPair = addI(sc.fwdPair);
+    sp.oBackLev = addI(sc.backLev);
+    sp.oBackRound = addI(sc.backRound);
+    sp.oBackTarget = addI(sc.backTarget);
+    sp.oBackPair = addI(sc.backPair);
+    sp.oSegCoef = addD(sc.segCoef);
+    sp.oJacCoef = addD(sc.jacCoef);
+  } catch (const std::exception&) {
+    k->sparse = DevSparse{};
+  }
   ib.push_back(0);
   db.push_back(0.0);
   int rc = TCHEM_OK;
@@ -236,11 +312,12 @@ int tchem_kmcd_destroy(tchem_kmcd_t kmcd) {
   return TCHEM_OK;
 }
 
-int tchem_atmchem_workspace_bytes(tchem_kmcd_t kmcd, int precision, size_t* bytes) {
+int tchem_atmchem_workspace_bytes(tchem_kmcd_t kmcd, const tchem_atmchem_options_t* opts, size_t* bytes) {
   if (!kmcd || !bytes) return setError(TCHEM_ERR_INVALID, "tchem_atmchem_workspace_bytes: null argument");
-  if (precision != TCHEM_PRECISION_FP64 && precision != TCHEM_PRECISION_MIXED)
-    return setError(TCHEM_ERR_INVALID, "unknown precision");
-  *bytes = integratorLdsBytes(kmcd->dev, precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float));
+  tchem_atmchem_options_t o;
+  const int rc = resolveOptions(opts, kmcd, &o, "tchem_atmchem_workspace_bytes");
+  if (rc) return rc;
+  *bytes = ldsBytes(kmcd, o);
   return TCHEM_OK;
 }
 
@@ -249,24 +326,17 @@ int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBat
                                    double* state, const double* photo_rates, double* t_out, double* dt_out,
                                    double* state_out, tchem_cell_stats_t* stats, tchem_kmcd_t kmcd, void* stream) {
   (void)fac;
-  tchem_atmchem_options_t o{};
-  if (opts) o = *opts;
   if (!kmcd) return setError(TCHEM_ERR_INVALID, "run_device_batch: null kmcd");
   if (nBatch < 0) return setError(TCHEM_ERR_INVALID, "run_device_batch: negative nBatch");
-  for (int r : o.reserved)
-    if (r != 0) return setError(TCHEM_ERR_INVALID, "run_device_batch: reserved option fields must be zero");
-  if (o.precision != TCHEM_PRECISION_FP64 && o.precision != TCHEM_PRECISION_MIXED)
-    return setError(TCHEM_ERR_INVALID, "run_device_batch: unknown precision");
-  if (o.schedule != TCHEM_SCHEDULE_GRID && o.schedule != TCHEM_SCHEDULE_PERSISTENT)
-    return setError(TCHEM_ERR_INVALID, "run_device_batch: unknown schedule");
-  if (o.blocks_per_cu < 0) return setError(TCHEM_ERR_INVALID, "run_device_batch: negative blocks_per_cu");
+  tchem_atmchem_options_t o;
+  const int rc = resolveOptions(opts, kmcd, &o, "run_device_batch");
+  if (rc) return rc;
   if (nBatch == 0) return TCHEM_OK;
   if (!tol_newton || !tol_time || !tadv || !state || !t_out || !dt_out || !state_out)
     return setError(TCHEM_ERR_INVALID, "run_device_batch: null device array");
   if (kmcd->dev.nPhoto == 0) photo_rates = nullptr;
 
-  const size_t lds =
-      integratorLdsBytes(kmcd->dev, o.precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float));
+  const size_t lds = ldsBytes(kmcd, o);
   if (lds > kmcd->maxLds)
     return setError(TCHEM_ERR_UNSUPPORTED, "run_device_batch: mechanism needs " + std::to_string(lds) +
                                                " bytes of LDS per cell, device limit " + std::to_string(kmcd->maxLds));
@@ -276,6 +346,8 @@ int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBat
 
   IntegratorArgs a{};
   a.M = kmcd->dev;
+  a.S = kmcd->sparse;
+  a.sparse = o.lu == TCHEM_LU_SPARSE ? 1 : 0;
   a.nBatch = nBatch;
   a.tol_newton = tol_newton;
   a.tol_time = tol_time;
@@ -289,12 +361,30 @@ int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBat
   a.queue = kmcd->queue;
   a.persistent = (o.schedule == TCHEM_SCHEDULE_PERSISTENT) ? 1 : 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if (size_t(nBatch) > kmcd->kfCapacity) {
+    // grows only: the previous buffer may still be read by work queued on s
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hipFail(e, "hipStreamSynchronize");
+    if (kmcd->kf) (void)hipFree(kmcd->kf);
+    kmcd->kf = nullptr;
+    kmcd->kfCapacity = 0;
+    void* p = nullptr;
+    e = hipMalloc(&p, sizeof(double) * size_t(nBatch) * size_t(kmcd->dev.nReac > 0 ? kmcd->dev.nReac : 1));
+    if (e != hipSuccess) return hipFail(e, "hipMalloc (rate constants)");
+    kmcd->kf = static_cast<double*>(p);
+    kmcd->kfCapacity = size_t(nBatch);
+  }
+  a.kf = kmcd->kf;
+  {
+    hipError_t e = launchRates(kmcd->dev, nBatch, state, photo_rates, kmcd->kf, s);
+    if (e != hipSuccess) return hipFail(e, "rates_kernel launch");
+  }
 
   int grid = nBatch;
   if (a.persistent) {
     int bpc = o.blocks_per_cu;
     if (bpc == 0) {
-      hipError_t e = integratorMaxBlocksPerCU(o.precision, kmcd->dev.m, lds, &bpc);
+      hipError_t e = integratorMaxBlocksPerCU(o.precision, a.sparse, kmcd->dev.m, lds, &bpc);
       if (e != hipSuccess) return hipFail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
       if (bpc < 1) bpc = 1;
     }

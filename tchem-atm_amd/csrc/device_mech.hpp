@@ -42,20 +42,36 @@ struct DevMech {
   __host__ __device__ const double* jcCoef() const { return db + oJcCoef; }      // [nContrib]
 };
 
+// Device view of the compiled sparse schedule (schedule.hpp); arrays live in
+// the mechanism's int / double buffers, addressed by offsets.
+struct DevSparse {
+  int enabled;  // 0: mechanism has no sparse schedule
+  int nnzF, zeroSlot, nLevels, nFwdLevels, nBackLevels;
+  int ropRounds, segRounds, pairRounds, pairSlots, jacRounds;
+  int oRopRec, oSegRound, oSegTarget, oSegReac, oPairRec, oJacRound, oJacTarget, oJacPair;
+  int oLevMul, oLevUpd, oMulRec, oUpdRound, oUpdTarget, oUpdPair;
+  int oFwdLev, oFwdRound, oFwdTarget, oFwdPair;
+  int oBackLev, oBackRound, oBackTarget, oBackPair;
+  int oSegCoef, oJacCoef;  // double buffer
+};
+
 struct IntegratorArgs {
   DevMech M;
+  DevSparse S;
   int nBatch;
   const double* tol_newton;          // [2]
   const double* tol_time;            // [m][2]
   const tchem_time_advance_t* tadv;  // [nBatch]
   double* state;                     // [nBatch][nSpec+3]
   const double* photo;               // [nBatch][nPhoto] or null
+  const double* kf;                  // [nBatch][nReac] rate constants (rates_kernel)
   double* t_out;
   double* dt_out;
   double* state_out;
   tchem_cell_stats_t* stats;         // or null
   int* queue;                        // persistent schedule: next-cell counter
   int persistent;
+  int sparse;                        // 1: sparse static-pivot linear algebra
 };
 
 // LDS bytes of one cell: the iteration matrix in TA precision (bytesA per
@@ -69,6 +85,22 @@ inline size_t integratorLdsBytes(const DevMech& M, size_t bytesA) {
   a = (a + 15) & ~size_t(15);
   size_t b = sizeof(double) * (size_t(M.nSpec) + size_t(M.nReac) + nScr + size_t(M.nSeg));
   return a + b + sizeof(double) * size_t(M.m) + sizeof(int) * 2 * size_t(M.m);
+}
+
+
+// LDS bytes of one cell for the sparse form: factor entries + RHS + zero
+// slot and reciprocal pivots in TA precision, then doubles for
+// concentrations, rate constants (+ a zero), rates of progress (+ zero
+// slot), derivative pairs (+ zero slot, padded to whole rounds) and
+// source-term segment sums.
+inline size_t sparseLdsBytes(const DevMech& M, const DevSparse& S, size_t bytesA) {
+  size_t a = bytesA * size_t(S.zeroSlot + 1);
+  a = (a + 15) & ~size_t(15);
+  size_t r = bytesA * size_t(M.m);
+  r = (r + 15) & ~size_t(15);
+  const size_t d = sizeof(double) * (size_t(M.nSpec) + size_t(M.nReac + 1) + size_t(M.nReac + 1) +
+                                     size_t(S.pairSlots) + size_t(M.nSeg));
+  return a + r + d;
 }
 
 }  // namespace tchem_amd

@@ -453,71 +453,341 @@ __device__ void lu_back(const CellLds<TA>& L, int m, int lane, TA (&b)[QM]) {
   }
 }
 
+// ---- sparse form ------------------------------------------------------------
+// Compiled schedules (schedule.hpp): the iteration matrix is held as the
+// nonzeros of its symbolic LU pattern, factorised without interchanges in
+// the fill-reducing order, level by level; the RHS is stored after the
+// matrix entries and eliminated alongside.
+
+template <typename TA>
+struct SpLds {
+  TA* A;        // [zeroSlot+1] factor entries, RHS, zero
+  TA* rdiag;    // [m] reciprocal pivots
+  double* c;    // [nSpec]
+  double* kf;   // [nReac+1], kf[nReac] = 0
+  double* rop;  // [nReac+1], rop[nReac] = 0
+  double* dpr;  // [pairSlots], dpr[nPairs] = 0
+  double* seg;  // [nSeg]
+};
+
+template <typename TA>
+__device__ SpLds<TA> carve_sparse(char* base, const DevMech& M, const DevSparse& S) {
+  SpLds<TA> L;
+  size_t off = sizeof(TA) * size_t(S.zeroSlot + 1);
+  off = (off + 15) & ~size_t(15);
+  L.A = reinterpret_cast<TA*>(base);
+  L.rdiag = reinterpret_cast<TA*>(base + off);
+  off += sizeof(TA) * size_t(M.m);
+  off = (off + 15) & ~size_t(15);
+  L.c = reinterpret_cast<double*>(base + off);
+  L.kf = L.c + M.nSpec;
+  L.rop = L.kf + M.nReac + 1;
+  L.dpr = L.rop + M.nReac + 1;
+  L.seg = L.dpr + S.pairSlots;
+  return L;
+}
+
+// x^n for small n >= 0 by repeated multiplication (exact for n <= 2; at
+// most n-1 roundings above)
+__device__ __forceinline__ double ipow(double x, int n) {
+  double r = n >= 1 ? x : 1.0;
+  for (int i = 1; i < n; ++i) r *= x;
+  return r;
+}
+// packed reactant field: species | power << 16, 0xffff for none
+__device__ __forceinline__ double reactant_factor(const double* c, int e) {
+  return e == 0xffff ? 1.0 : ipow(c[e & 0xffff], e >> 16);
+}
+__device__ __forceinline__ int2 uniform(int2 v) {
+  return make_int2(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y));
+}
+
+// rates of progress (and, with pairs, the derivatives d rop_r / d c_s) at
+// the concentrations in LDS; same products, same order as the dense form
+template <typename TA>
+__device__ void sp_rates(const DevMech& M, const DevSparse& S, const SpLds<TA>& L, int lane, bool pairs) {
+  // padding records name reaction nReac (kf = 0, no reactants) and write 0
+  // into the zero slots
+  const int4* rr = reinterpret_cast<const int4*>(M.ib + S.oRopRec);
+  for (int g = 0; g < S.ropRounds; ++g) {
+    const int4 e = rr[g * kWave + lane];
+    L.rop[e.x] = L.kf[e.x] * reactant_factor(L.c, e.y) * reactant_factor(L.c, e.z) * reactant_factor(L.c, e.w);
+  }
+  if (!pairs) return;
+  const int4* pr = reinterpret_cast<const int4*>(M.ib + S.oPairRec);
+  for (int g = 0; g < S.pairRounds; ++g) {
+    const int4 e = pr[g * kWave + lane];
+    const int r = e.x & 0xffffff, js = e.x >> 24;
+    const int es = js == 0 ? e.y : (js == 1 ? e.z : e.w);
+    const int ns = es >> 16;
+    double d = L.kf[r] * double(ns) * ipow(L.c[es & 0xffff], ns - 1);
+    if (js != 0) d *= reactant_factor(L.c, e.y);
+    if (js != 1) d *= reactant_factor(L.c, e.z);
+    if (js != 2) d *= reactant_factor(L.c, e.w);
+    L.dpr[g * kWave + lane] = d;
+  }
+}
+
+// source-term segment sums and, with jac, A = diag * I + scale * J on the
+// factor pattern (fill positions cleared)
+template <typename TA>
+__device__ void sp_sums(const DevMech& M, const DevSparse& S, const SpLds<TA>& L, int lane, bool jac, double diag,
+                        double scale) {
+  {
+    const int2* rd = reinterpret_cast<const int2*>(M.ib + S.oSegRound);
+    const int* tg = M.ib + S.oSegTarget;
+    const int* er = M.ib + S.oSegReac;
+    const double* ec = M.db + S.oSegCoef;
+    for (int g = 0; g < S.segRounds; ++g) {
+      const int2 r = uniform(rd[g]);
+      double acc = 0.0;
+#pragma unroll 4
+      for (int k = 0; k < r.y; ++k) {
+        const int e = (r.x + k) * kWave + lane;
+        acc += ec[e] * L.rop[er[e]];
+      }
+      const int t = tg[g * kWave + lane];
+      if (t >= 0) L.seg[t] = acc;
+    }
+  }
+  if (!jac) return;
+  const int2* rd = reinterpret_cast<const int2*>(M.ib + S.oJacRound);
+  const int* tg = M.ib + S.oJacTarget;
+  const int* ep = M.ib + S.oJacPair;
+  const double* ec = M.db + S.oJacCoef;
+  for (int g = 0; g < S.jacRounds; ++g) {
+    const int2 r = uniform(rd[g]);
+    double s = 0.0;
+#pragma unroll 4
+    for (int k = 0; k < r.y; ++k) {
+      const int e = (r.x + k) * kWave + lane;
+      s += ec[e] * L.dpr[ep[e]];
+    }
+    const int t = tg[g * kWave + lane];
+    if (t >= 0) L.A[t & 0x3fffffff] = TA(((t >> 30) ? diag : 0.0) + scale * s);
+  }
+}
+
+// per-species sums of the segment sums plus emission (register layout)
+template <int QM>
+__device__ __forceinline__ void row_sums(const DevMech& M, const double* seg, int lane, double (&f)[QM]) {
+#pragma unroll
+  for (int q = 0; q < QM; ++q) {
+    const int a = lane + kWave * q;
+    double s = 0.0;
+    if (a < M.m) {
+      s = M.emission()[a];
+      const int s1 = M.rowSegPtr()[a + 1];
+      for (int g = M.rowSegPtr()[a]; g < s1; ++g) s += seg[g];
+    }
+    f[q] = s;
+  }
+}
+
+// Rounds of chained updates: target -= sum A[lo] * A[hi] over its chain.
+// BACK: the target is row i of the RHS and the result is scaled by rdiag_i.
+template <bool BACK, typename TA>
+__device__ __forceinline__ void sp_update_rounds(const int2* rd, int g0, int g1, const int* target, const int* ent,
+                                                 TA* A, const TA* rdiag, int rhs0, int lane) {
+  for (int g = g0; g < g1; ++g) {
+    const int2 r = uniform(rd[g]);
+    const int t = target[g * kWave + lane];
+    const int pos = BACK ? rhs0 + t : t;
+    TA acc = t >= 0 ? A[pos] : TA(0);
+#pragma unroll 4
+    for (int k = 0; k < r.y; ++k) {
+      const int p = ent[(r.x + k) * kWave + lane];
+      acc -= A[p & 0xffff] * A[p >> 16];
+    }
+    if (t >= 0) A[pos] = BACK ? acc * rdiag[t] : acc;
+  }
+}
+
+// Level-scheduled LU without interchanges, RHS eliminated alongside.
+// Multipliers l = a_ik * (1 / a_kk).  Returns false for a zero or
+// non-finite pivot (the Newton iteration then fails and the step is retried
+// with a smaller h).
+template <typename TA>
+__device__ bool sp_factor(const DevMech& M, const DevSparse& S, const SpLds<TA>& L, int lane) {
+  const int2* lm = reinterpret_cast<const int2*>(M.ib + S.oLevMul);
+  const int2* lu = reinterpret_cast<const int2*>(M.ib + S.oLevUpd);
+  const int2* mr = reinterpret_cast<const int2*>(M.ib + S.oMulRec);
+  const int2* ur = reinterpret_cast<const int2*>(M.ib + S.oUpdRound);
+  const int* ut = M.ib + S.oUpdTarget;
+  const int* up = M.ib + S.oUpdPair;
+  bool bad = false;
+  for (int lev = 0; lev < S.nLevels; ++lev) {
+    const int2 mb = uniform(lm[lev]);
+    for (int g = mb.x; g < mb.y; ++g) {
+      const int2 r = mr[g * kWave + lane];
+      if (r.y >= 0) {
+        const TA piv = L.A[r.y];
+        const TA inv = TA(1) / piv;
+        if (r.x < 0) {
+          L.rdiag[-r.x - 1] = inv;
+          bad |= !(piv != TA(0) && isfinite(piv) && isfinite(inv));
+        } else {
+          L.A[r.x] = L.A[r.x] * inv;
+        }
+      }
+    }
+    wave_sync();
+    const int2 ub = uniform(lu[lev]);
+    sp_update_rounds<false>(ur, ub.x, ub.y, ut, up, L.A, L.rdiag, 0, lane);
+    wave_sync();
+  }
+  return ballot(bad) == 0;
+}
+
+template <typename TA>
+__device__ void sp_forward(const DevMech& M, const DevSparse& S, const SpLds<TA>& L, int lane) {
+  const int2* lv = reinterpret_cast<const int2*>(M.ib + S.oFwdLev);
+  const int2* rd = reinterpret_cast<const int2*>(M.ib + S.oFwdRound);
+  for (int lev = 0; lev < S.nFwdLevels; ++lev) {
+    const int2 b = uniform(lv[lev]);
+    sp_update_rounds<false>(rd, b.x, b.y, M.ib + S.oFwdTarget, M.ib + S.oFwdPair, L.A, L.rdiag, 0, lane);
+    wave_sync();
+  }
+}
+
+template <typename TA>
+__device__ void sp_back(const DevMech& M, const DevSparse& S, const SpLds<TA>& L, int lane) {
+  const int2* lv = reinterpret_cast<const int2*>(M.ib + S.oBackLev);
+  const int2* rd = reinterpret_cast<const int2*>(M.ib + S.oBackRound);
+  for (int lev = 0; lev < S.nBackLevels; ++lev) {
+    const int2 b = uniform(lv[lev]);
+    sp_update_rounds<true>(rd, b.x, b.y, M.ib + S.oBackTarget, M.ib + S.oBackPair, L.A, L.rdiag, S.nnzF, lane);
+    wave_sync();
+  }
+}
+
+// ---- backends -----------------------------------------------------------------
+// Both provide: Lds (per-cell LDS view), Fact (state of the stored factors),
+// carve, init, evaluate (f at u; for a Newton iteration also the residual
+// and, when refreshing, the iteration matrix), factor_solve (factorise +
+// solve; false on a bad pivot) and solve (stored factors).
+
+template <typename TA_, int QM>
+struct DenseBackend {
+  using TA = TA_;
+  using Lds = CellLds<TA>;
+  struct Fact { int nswp = 0; };
+  static __device__ Lds carve(char* base, const IntegratorArgs& a) { return tchem_amd::carve<TA>(base, a.M); }
+  static __device__ void init(const IntegratorArgs&, const Lds&, int) {}
+  // f at u; for a Newton iteration also x = -(u - bc - ch f) and, when
+  // refreshing, the iteration matrix I - ch J at the same u
+  static __device__ void evaluate(const IntegratorArgs& a, const Lds& L, int lane, const double (&u)[QM],
+                                  const double (&bc)[QM], double ch, bool newton, bool refresh, double (&f)[QM],
+                                  TA (&x)[QM]) {
+    eval_rhs<QM>(a.M, L, lane, u, f);
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const int i = lane + kWave * q;
+      x[q] = (newton && i < a.M.m) ? TA(-(u[q] - bc[q] - ch * f[q])) : TA(0);
+    }
+    if (refresh) assemble_jacobian_from_lds(a.M, L, lane, 1.0, -ch, false);
+  }
+  static __device__ bool factor_solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM], Fact& fs) {
+    if (!lu_factor_fused<QM>(L, a.M.m, lane, x, fs.nswp)) return false;
+    lu_back<QM>(L, a.M.m, lane, x);
+    return true;
+  }
+  static __device__ void solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM], const Fact& fs) {
+    lu_forward<QM>(L, a.M.m, lane, x, fs.nswp);
+    lu_back<QM>(L, a.M.m, lane, x);
+  }
+};
+
+template <typename TA_, int QM>
+struct SparseBackend {
+  using TA = TA_;
+  using Lds = SpLds<TA>;
+  struct Fact {};
+  static __device__ Lds carve(char* base, const IntegratorArgs& a) { return carve_sparse<TA>(base, a.M, a.S); }
+  // zero slots read by padded chain entries
+  static __device__ void init(const IntegratorArgs& a, const Lds& L, int lane) {
+    if (lane == 0) {
+      L.kf[a.M.nReac] = 0.0;
+      L.rop[a.M.nReac] = 0.0;
+      L.dpr[a.M.nPairs] = 0.0;
+      L.A[a.S.zeroSlot] = TA(0);
+    }
+  }
+  static __device__ void put_c(const IntegratorArgs& a, const Lds& L, int lane, const double (&u)[QM]) {
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const int i = lane + kWave * q;
+      if (i < a.M.m) L.c[i] = u[q];
+    }
+    wave_sync();
+  }
+  static __device__ void evaluate(const IntegratorArgs& a, const Lds& L, int lane, const double (&u)[QM],
+                                  const double (&bc)[QM], double ch, bool newton, bool refresh, double (&f)[QM],
+                                  TA (&x)[QM]) {
+    put_c(a, L, lane, u);
+    sp_rates(a.M, a.S, L, lane, refresh);
+    wave_sync();
+    sp_sums(a.M, a.S, L, lane, refresh, 1.0, -ch);
+    wave_sync();
+    row_sums<QM>(a.M, L.seg, lane, f);
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const int i = lane + kWave * q;
+      x[q] = (newton && i < a.M.m) ? TA(-(u[q] - bc[q] - ch * f[q])) : TA(0);
+      if (newton && i < a.M.m) L.A[a.S.nnzF + i] = x[q];
+    }
+    if (newton) wave_sync();
+  }
+  static __device__ void get_x(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM]) {
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const int i = lane + kWave * q;
+      x[q] = (i < a.M.m) ? L.A[a.S.nnzF + i] : TA(0);
+    }
+  }
+  static __device__ bool factor_solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM], Fact&) {
+    if (!sp_factor(a.M, a.S, L, lane)) return false;
+    sp_back(a.M, a.S, L, lane);
+    get_x(a, L, lane, x);
+    return true;
+  }
+  static __device__ void solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM], const Fact&) {
+    sp_forward(a.M, a.S, L, lane);
+    sp_back(a.M, a.S, L, lane);
+    get_x(a, L, lane, x);
+  }
+};
+
 struct CellCounters {
   int accepted = 0, rejected = 0, newton_failures = 0, newton_iterations = 0, factorizations = 0, failed = 0;
 };
 
-// Newton iteration for one TR-BDF2 stage: G(u) = u - bc - ch f(u), iteration
-// matrix I - ch J refreshed every jacobian_interval iterations; converged when
-// max_i |dx_i| / (atol + rtol |u_i|) <= 1 (oracle/oracle.c: newton).  The
-// residual and the Jacobian are evaluated at the same iterate, the residual
-// first so the factorisation eliminates it on the way.
-template <int QM, typename TA>
-__device__ bool newton(const DevMech& M, const CellLds<TA>& L, int lane, double (&u)[QM], const double (&bc)[QM],
-                       double ch, double atol, double rtol, int max_it, int jint, CellCounters& cnt) {
-  const int m = M.m;
-  if (jint < 1) jint = 1;
-  int nswp = 0;
-  for (int it = 0; it < max_it; ++it) {
-    double f[QM];
-    TCHEM_PT_BEGIN(rhs)
-    eval_rhs<QM>(M, L, lane, u, f);
-    TCHEM_PT_END(rhs, 0)
-    TA x[QM];
-#pragma unroll
-    for (int q = 0; q < QM; ++q) {
-      const int a = lane + kWave * q;
-      x[q] = (a < m) ? TA(-(u[q] - bc[q] - ch * f[q])) : TA(0);
-    }
-    if (it % jint == 0) {
-      TCHEM_PT_BEGIN(jac)
-      assemble_jacobian_from_lds(M, L, lane, 1.0, -ch, false);
-      TCHEM_PT_END(jac, 1)
-      TCHEM_PT_BEGIN(lu)
-      const bool factored = lu_factor_fused<QM>(L, m, lane, x, nswp);
-      TCHEM_PT_END(lu, 2)
-      if (!factored) return false;
-      ++cnt.factorizations;
-    } else {
-      lu_forward<QM>(L, m, lane, x, nswp);
-    }
-    TCHEM_PT_BEGIN(back)
-    lu_back<QM>(L, m, lane, x);
-    TCHEM_PT_END(back, 3)
-    double nrm = 0.0;
-#pragma unroll
-    for (int q = 0; q < QM; ++q) {
-      const int a = lane + kWave * q;
-      if (a < m) {
-        const double dx = double(x[q]);
-        u[q] += dx;
-        double w = fabs(dx) / (atol + rtol * fabs(u[q]));
-        if (!(w == w)) w = INFINITY;
-        nrm = fmax(nrm, w);
-      }
-    }
-    nrm = wave_max(nrm);
-    ++cnt.newton_iterations;
-    if (nrm <= 1.0) return true;
-  }
-  return false;
+// wavefront-uniform copy of a value every lane holds (lets the compiler branch on it with scalar code)
+__device__ __forceinline__ double uniform_d(double v) {
+  const long long x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane(int(x & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane(int(x >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
 // One cell of AtmosphericChemistry::runDeviceBatch
 // (src/core/TChem_AtmosphericChemistry.cpp:78-163) with the TR-BDF2 step
-// control of oracle/oracle.c: integrate_cell.
-template <typename TA, int QM>
-__device__ void integrate_cell(const IntegratorArgs& a, const CellLds<TA>& L, int cell, int lane) {
+// control of oracle/oracle.c: integrate_cell, written as a state machine so
+// the function evaluation and the linear solve each have a single call site:
+//   STEP   start or retry a step: h = min(max(dt, dtmin), tend - t); fail when
+//          t + h == t; fetch f(u_n) first if it is not known;
+//   FN     f(u_n);
+//   NEWTON one Newton iteration of the current stage: residual
+//          G(u) = u - bc - ch f(u) and, every jacobian_interval iterations,
+//          the iteration matrix I - ch J at the same u; converged when
+//          max_i |dx_i| / (atol + rtol |u_i|) <= 1; after max_newton
+//          iterations without convergence, or on a zero or non-finite
+//          pivot, dt = h/2 and the step is retried (at most 4 attempts);
+//   STAGE  f at the converged stage value: after the TR stage set up BDF2;
+//          after BDF2 estimate the local error and accept or reject.
+template <class B, int QM>
+__device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L, int cell, int lane) {
+  using TA = typename B::TA;
   const DevMech& M = a.M;
   const int m = M.m, nSpec = M.nSpec, dim = nSpec + 3;
   const tchem_time_advance_t ta = a.tadv[cell];
@@ -539,78 +809,158 @@ __device__ void integrate_cell(const IntegratorArgs& a, const CellLds<TA>& L, in
   const double k_err = (-3.0 * gam * gam + 4.0 * gam - 2.0) / (12.0 * (2.0 - gam));
 
   TCHEM_PT_BEGIN(cell)
-  const double p = row[1], T = row[2];
-  load_rate_constants(M, L, lane, T, p, a.photo ? a.photo + size_t(cell) * M.nPhoto : nullptr);
+  {
+    const double* kf = a.kf + size_t(cell) * M.nReac;
+    for (int r = lane; r < M.nReac; r += kWave) L.kf[r] = kf[r];
+  }
   for (int i = m + lane; i < nSpec; i += kWave) L.c[i] = row[3 + i];
 
-  // live across steps: un (state), fn (f at un); per step: u, bc, est, f1
-  double un[QM], u[QM], fn[QM], f1[QM], bc[QM], est[QM];
+  // un (state) and fn (f at un) live across steps; u, bc, est per step
+  double un[QM], u[QM], fn[QM], bc[QM], est[QM];
 #pragma unroll
   for (int q = 0; q < QM; ++q) {
     const int ai = lane + kWave * q;
     un[q] = (ai < m) ? row[3 + M.perm()[ai]] : 0.0;
-    fn[q] = 0.0;
+    u[q] = un[q];
+    fn[q] = bc[q] = est[q] = 0.0;
   }
   const double atol_n = a.tol_newton[0], rtol_n = a.tol_newton[1];
   wave_sync();
 
   const int max_it = ta._num_time_iterations_per_interval;
   const int max_newton = ta._max_num_newton_iterations;
-  const int jint = ta._jacobian_interval;
-  double t = ta._tbeg, dt = ta._dt;
+  const int jint = ta._jacobian_interval < 1 ? 1 : ta._jacobian_interval;
+  double t = ta._tbeg, dt = ta._dt, h = 0.0, ch = 0.0;
   bool fn_valid = false, failed = false;
+  int iter = 0, attempt = 0, stage = 0, nit = 0;
+  enum : int { kStep, kFn, kNewton, kStage, kDone };
+  int phase = kStep;
+  typename B::Fact fs;
 
-  for (int iter = 0; iter < max_it && t < tend; ++iter) {
-    bool conv = false;
-    double h = 0.0;
-    for (int attempt = 0; attempt < 4 && !conv; ++attempt) {
+  while (phase != kDone) {
+    if (phase == kStep) {
+      if (iter >= max_it || !(t < tend)) {
+        phase = kDone;
+        continue;
+      }
       h = dt > ta._dtmin ? dt : ta._dtmin;
       if (h > tend - t) h = tend - t;
-      if (!(t + h > t)) break;  // no representable progress: hard failure
+      if (!(t + h > t)) {  // no representable progress: hard failure
+        failed = true;
+        phase = kDone;
+        continue;
+      }
       if (!fn_valid) {
-        eval_rhs<QM>(M, L, lane, un, fn);
-        fn_valid = true;
+#pragma unroll
+        for (int q = 0; q < QM; ++q) u[q] = un[q];
+        phase = kFn;
+        continue;
       }
 #pragma unroll
       for (int q = 0; q < QM; ++q) {
         u[q] = un[q];
         bc[q] = un[q] + c_tr * h * fn[q];
       }
-      conv = newton<QM>(M, L, lane, u, bc, c_tr * h, atol_n, rtol_n, max_newton, jint, cnt);
-      if (conv) {
-        // f at the TR-stage value enters the error estimate; the stage value
-        // itself only enters the BDF2 constant term
-        eval_rhs<QM>(M, L, lane, u, f1);
+      ch = c_tr * h;
+      stage = 0;
+      nit = 0;
+      phase = kNewton;
+      continue;
+    }
+
+    if (phase == kNewton && nit >= max_newton) {  // Newton failed: halve the step and retry
+      ++cnt.newton_failures;
+      dt = 0.5 * h;
+      if (++attempt >= 4) {
+        failed = true;
+        phase = kDone;
+      } else {
+        phase = kStep;
+      }
+      continue;
+    }
+    const bool nw = phase == kNewton;
+    const bool refresh = nw && (nit % jint) == 0;
+    double f[QM];
+    TA x[QM];
+    TCHEM_PT_BEGIN(ev)
+    B::evaluate(a, L, lane, u, bc, ch, nw, refresh, f, x);
+    TCHEM_PT_END(ev, 0)
+
+    if (nw) {
+      TCHEM_PT_BEGIN(lin)
+      bool ok = true;
+      if (refresh) {
+        ok = B::factor_solve(a, L, lane, x, fs);
+        if (ok) ++cnt.factorizations;
+      } else {
+        B::solve(a, L, lane, x, fs);
+      }
+      TCHEM_PT_END(lin, refresh ? 2 : 3)
+      if (!ok) {  // zero or non-finite pivot
+        nit = max_newton;
+        continue;
+      }
+      {
+        double nrm = 0.0;
 #pragma unroll
         for (int q = 0; q < QM; ++q) {
-          est[q] = fn[q] / gam - f1[q] / (gam * (1.0 - gam));
-          bc[q] = a_g * u[q] - a_n * un[q];
+          const int i = lane + kWave * q;
+          if (i < m) {
+            const double dx = double(x[q]);
+            u[q] += dx;
+            double w = fabs(dx) / (atol_n + rtol_n * fabs(u[q]));
+            if (!(w == w)) w = INFINITY;
+            nrm = fmax(nrm, w);
+          }
         }
-        conv = newton<QM>(M, L, lane, u, bc, c_bdf * h, atol_n, rtol_n, max_newton, jint, cnt);
+        nrm = uniform_d(wave_max(nrm));
+        ++cnt.newton_iterations;
+        ++nit;
+        if (nrm <= 1.0) phase = kStage;
       }
-      if (!conv) {
-        ++cnt.newton_failures;
-        dt = 0.5 * h;
+      continue;
+    }
+
+    if (phase == kFn) {
+#pragma unroll
+      for (int q = 0; q < QM; ++q) fn[q] = f[q];
+      fn_valid = true;
+      phase = kStep;
+      continue;
+    }
+
+    // kStage
+    if (stage == 0) {
+      // f at the TR-stage value enters the error estimate; the stage value
+      // itself only enters the BDF2 constant term
+#pragma unroll
+      for (int q = 0; q < QM; ++q) {
+        est[q] = fn[q] / gam - f[q] / (gam * (1.0 - gam));
+        bc[q] = a_g * u[q] - a_n * un[q];
       }
+      ch = c_bdf * h;
+      stage = 1;
+      nit = 0;
+      phase = kNewton;
+      continue;
     }
-    if (!conv) {
-      failed = true;
-      break;
-    }
-    eval_rhs<QM>(M, L, lane, u, f1);
     double e = 0.0;
 #pragma unroll
     for (int q = 0; q < QM; ++q) {
       const int ai = lane + kWave * q;
       if (ai < m) {
-        const int x = M.perm()[ai];
-        const double lte = 2.0 * k_err * h * (est[q] + f1[q] / (1.0 - gam));
-        double w = fabs(lte) / (a.tol_time[2 * x] + a.tol_time[2 * x + 1] * fabs(u[q]));
+        const int xs = M.perm()[ai];
+        const double lte = 2.0 * k_err * h * (est[q] + f[q] / (1.0 - gam));
+        double w = fabs(lte) / (a.tol_time[2 * xs] + a.tol_time[2 * xs + 1] * fabs(u[q]));
         if (!(w == w)) w = INFINITY;
         e = fmax(e, w);
       }
     }
-    e = wave_max(e);
+    e = uniform_d(wave_max(e));
+    ++iter;
+    attempt = 0;
+    phase = kStep;
     if (e > 1.0 && h > ta._dtmin) {
       ++cnt.rejected;
       dt = 0.25 * h;
@@ -620,7 +970,7 @@ __device__ void integrate_cell(const IntegratorArgs& a, const CellLds<TA>& L, in
 #pragma unroll
     for (int q = 0; q < QM; ++q) {
       un[q] = u[q];
-      fn[q] = f1[q];
+      fn[q] = f[q];
     }
     dt = (e <= 0.125) ? 2.0 * h : h;
     if (dt > ta._dtmax) dt = ta._dtmax;
@@ -633,10 +983,10 @@ __device__ void integrate_cell(const IntegratorArgs& a, const CellLds<TA>& L, in
   for (int q = 0; q < QM; ++q) {
     const int ai = lane + kWave * q;
     if (ai < m) {
-      const int x = M.perm()[ai];
+      const int xs = M.perm()[ai];
       const double v = failed ? 0.0 : un[q];
-      row[3 + x] = v;
-      if (rowo != row) rowo[3 + x] = v;
+      row[3 + xs] = v;
+      if (rowo != row) rowo[3 + xs] = v;
     }
   }
   if (rowo != row)
@@ -655,13 +1005,14 @@ __device__ void integrate_cell(const IntegratorArgs& a, const CellLds<TA>& L, in
   }
 }
 
-template <typename TA, int QM>
+template <class B, int QM>
 __global__ __launch_bounds__(64) void trbdf2_kernel(IntegratorArgs a) {
   extern __shared__ __align__(16) char lds_raw[];
-  const CellLds<TA> L = carve<TA>(lds_raw, a.M);
+  const typename B::Lds L = B::carve(lds_raw, a);
   const int lane = threadIdx.x;
+  B::init(a, L, lane);
   if (!a.persistent) {
-    integrate_cell<TA, QM>(a, L, blockIdx.x, lane);
+    integrate_cell<B, QM>(a, L, blockIdx.x, lane);
     return;
   }
   for (;;) {
@@ -669,9 +1020,24 @@ __global__ __launch_bounds__(64) void trbdf2_kernel(IntegratorArgs a) {
     if (lane == 0) cell = atomicAdd(a.queue, 1);
     cell = __builtin_amdgcn_readfirstlane(__shfl(cell, 0, kWave));
     if (cell >= a.nBatch) break;
-    integrate_cell<TA, QM>(a, L, cell, lane);
+    integrate_cell<B, QM>(a, L, cell, lane);
     wave_sync();
   }
+}
+
+// Rate constants of every (cell, reaction), one thread each; read by the
+// integrator at the start of a cell.  Keeping the transcendental code out of
+// the integrator keeps its register footprint small.
+__global__ __launch_bounds__(256) void rates_kernel(DevMech M, int nBatch, const double* state, const double* photo,
+                                                   double* kf) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long long)nBatch * M.nReac) return;
+  const int cell = int(idx / M.nReac), r = int(idx - (long long)cell * M.nReac);
+  const double* row = state + size_t(cell) * (M.nSpec + 3);
+  const double p = row[1], T = row[2];
+  const double t_1 = 1.0 / T;
+  const double conv = M.convPPM * p * t_1;
+  kf[idx] = rate_constant(M, r, T, p, t_1, conv, photo ? photo + size_t(cell) * M.nPhoto : nullptr);
 }
 
 __global__ void reset_counter_kernel(int* counter) {
@@ -739,35 +1105,46 @@ __global__ __launch_bounds__(64) void jacobian_kernel(DevMech M, int nBatch, con
 
 static int qmOf(int m) { return m <= 64 ? 1 : (m <= 128 ? 2 : (m <= 256 ? 4 : 0)); }
 
+template <class B, int QM>
+static const void* integratorFn() {
+  return reinterpret_cast<const void*>(&trbdf2_kernel<B, QM>);
+}
+
+// kernel instance for (precision, sparse, species slots per lane)
+static const void* integratorKernel(int precision, int sparse, int qm) {
+#define TCHEM_PICK(BK, TA)                                              \
+  return qm == 1 ? integratorFn<BK<TA, 1>, 1>()                        \
+                 : (qm == 2 ? integratorFn<BK<TA, 2>, 2>() : integratorFn<BK<TA, 4>, 4>())
+  if (precision == TCHEM_PRECISION_FP64) {
+    if (sparse) TCHEM_PICK(SparseBackend, double);
+    TCHEM_PICK(DenseBackend, double);
+  }
+  if (sparse) TCHEM_PICK(SparseBackend, float);
+  TCHEM_PICK(DenseBackend, float);
+#undef TCHEM_PICK
+}
+
 hipError_t launchIntegrator(const IntegratorArgs& a, int precision, int grid, size_t lds, hipStream_t stream) {
   const int qm = qmOf(a.M.m);
   if (qm == 0) return hipErrorInvalidValue;
-  dim3 blk(64), grd(grid);
-#define TCHEM_LAUNCH(TA, QM) hipLaunchKernelGGL((trbdf2_kernel<TA, QM>), grd, blk, lds, stream, a)
-  if (precision == TCHEM_PRECISION_FP64) {
-    if (qm == 1) TCHEM_LAUNCH(double, 1);
-    else if (qm == 2) TCHEM_LAUNCH(double, 2);
-    else TCHEM_LAUNCH(double, 4);
-  } else {
-    if (qm == 1) TCHEM_LAUNCH(float, 1);
-    else if (qm == 2) TCHEM_LAUNCH(float, 2);
-    else TCHEM_LAUNCH(float, 4);
-  }
-#undef TCHEM_LAUNCH
-  return hipGetLastError();
+  const void* fn = integratorKernel(precision, a.sparse, qm);
+  IntegratorArgs arg = a;
+  void* params[] = {&arg};
+  return hipLaunchKernel(fn, dim3(grid), dim3(64), params, lds, stream);
 }
 
-hipError_t integratorMaxBlocksPerCU(int precision, int m, size_t lds, int* blocks) {
+hipError_t integratorMaxBlocksPerCU(int precision, int sparse, int m, size_t lds, int* blocks) {
   const int qm = qmOf(m);
   if (qm == 0) return hipErrorInvalidValue;
-  const void* fn = nullptr;
-  if (precision == TCHEM_PRECISION_FP64)
-    fn = qm == 1 ? (const void*)trbdf2_kernel<double, 1> : qm == 2 ? (const void*)trbdf2_kernel<double, 2>
-                                                                     : (const void*)trbdf2_kernel<double, 4>;
-  else
-    fn = qm == 1 ? (const void*)trbdf2_kernel<float, 1> : qm == 2 ? (const void*)trbdf2_kernel<float, 2>
-                                                                    : (const void*)trbdf2_kernel<float, 4>;
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, fn, 64, lds);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, integratorKernel(precision, sparse, qm), 64, lds);
+}
+
+hipError_t launchRates(const DevMech& M, int nBatch, const double* state, const double* photo, double* kf,
+                       hipStream_t stream) {
+  const long long n = (long long)nBatch * M.nReac;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(rates_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, M, nBatch, state, photo, kf);
+  return hipGetLastError();
 }
 
 hipError_t launchResetCounter(int* counter, hipStream_t stream) {

@@ -47,7 +47,7 @@ class DeviceRun:
                                           w.jacobian_interval, self.device)
 
     def run_intervals(self, precision=T.PRECISION_FP64, schedule=T.SCHEDULE_GRID, max_intervals=100000,
-                      blocks_per_cu=0, collect_stats=False, stream=None):
+                      blocks_per_cu=0, collect_stats=False, stream=None, lu=T.LU_AUTO):
         """Reference host loop; returns the number of launches."""
         torch = self.torch
         tend = self.w.tend
@@ -56,8 +56,9 @@ class DeviceRun:
         while it < max_intervals:
             T.AtmosphericChemistry.runDeviceBatch(self.tol_newton, self.tol_time, None, self.tadv, self.state,
                                                   self.t, self.dt, self.state, self.kmcd, self.photo,
-                                                  self.stats if collect_stats else None, precision, schedule,
-                                                  blocks_per_cu, stream)
+                                                  self.stats if collect_stats else None, precision=precision,
+                                                  schedule=schedule, blocks_per_cu=blocks_per_cu, lu=lu,
+                                                  stream=stream)
             if collect_stats:
                 s = T.stats_from_tensor(self.stats)
                 arr = np.stack([s[f] for f in s.dtype.names], 1).astype(np.int64)
@@ -75,15 +76,15 @@ class DeviceRun:
         return it
 
     def run_to_end(self, precision=T.PRECISION_MIXED, schedule=T.SCHEDULE_PERSISTENT, blocks_per_cu=0,
-                   collect_stats=False, stream=None):
+                   collect_stats=False, stream=None, lu=T.LU_AUTO):
         """Every cell to tend in one launch (iteration cap 10^7 per cell)."""
         w = self.w
         self.tadv = T.time_advance_tensor(self.n, w.tbeg, w.tend, w.dt_in, w.dtmin, w.dtmax, w.max_newton,
                                           10_000_000, w.jacobian_interval, self.device)
         T.AtmosphericChemistry.runDeviceBatch(self.tol_newton, self.tol_time, None, self.tadv, self.state, self.t,
                                               self.dt, self.state, self.kmcd, self.photo,
-                                              self.stats if collect_stats else None, precision, schedule,
-                                              blocks_per_cu, stream)
+                                              self.stats if collect_stats else None, precision=precision,
+                                              schedule=schedule, blocks_per_cu=blocks_per_cu, lu=lu, stream=stream)
         if collect_stats:
             s = T.stats_from_tensor(self.stats)
             self.total_stats = np.stack([s[f] for f in s.dtype.names], 1).astype(np.int64)

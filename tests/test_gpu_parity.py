@@ -99,12 +99,22 @@ def test_jacobian_matches_oracle(cb05, cb05_oracle, scen, gpu, pkg):
     assert (np.abs(got - ref) / scale).max() < 1e-14
 
 
-def test_cb05cl_ae5_golden_on_gpu(cb05, gpu, pkg):
-    """The reference's ctest case run through runDeviceBatch on the GPU, both precisions."""
+LU_FORMS = ("dense", "sparse")
+
+
+def lu_code(pkg, name):
+    return pkg.LU_DENSE if name == "dense" else pkg.LU_SPARSE
+
+
+@pytest.mark.parametrize("lu", LU_FORMS)
+def test_cb05cl_ae5_golden_on_gpu(cb05, gpu, pkg, lu):
+    """The reference's ctest case run through runDeviceBatch on the GPU, both
+    precisions, dense (partial pivoting) and sparse (static pivots) LU."""
     kmd, kmcd, base = cb05
     torch = gpu
     m = kmd.nActive
     gold = np.load(golden("cb05cl_ae5_full_gas.npy"))
+    print("case: CB05CL_AE5 golden, lu", lu, flush=True)
     for precision in (pkg.PRECISION_FP64, pkg.PRECISION_MIXED):
         st = torch.tensor(base, device="cuda")
         tn = torch.tensor([1e-18, 1e-8], dtype=torch.float64, device="cuda")
@@ -114,7 +124,8 @@ def test_cb05cl_ae5_golden_on_gpu(cb05, gpu, pkg):
         dt = torch.full((1,), 10.0, dtype=torch.float64, device="cuda")
         rows = [np.concatenate([[-1, 0.0, 10.0], base[0]])]
         for it in range(2):
-            pkg.AtmosphericChemistry.runDeviceBatch(tn, tt, None, tadv, st, t, dt, st, kmcd, precision=precision)
+            pkg.AtmosphericChemistry.runDeviceBatch(tn, tt, None, tadv, st, t, dt, st, kmcd, precision=precision,
+                                                    lu=lu_code(pkg, lu))
             pkg.AtmosphericChemistry.updateTimeAdvance(tadv, t, dt)
             torch.cuda.synchronize()
             rows.append(np.concatenate([[it, t.item(), dt.item()], st.cpu().numpy()[0]]))
@@ -128,17 +139,26 @@ def test_cb05cl_ae5_golden_on_gpu(cb05, gpu, pkg):
 
 
 @pytest.fixture(scope="module")
-def fp64_vs_oracle(cb05, cb05_oracle, oracle_mod, scen, drv, gpu, pkg):
+def oracle_config0_64(cb05, cb05_oracle, oracle_mod, scen):
     kmd, kmcd, base = cb05
     w = scen.config0(kmd, base[0], n=64, seed=0)
+    return w, oracle_intervals(cb05_oracle, oracle_mod, w)
+
+
+@pytest.fixture(scope="module", params=LU_FORMS)
+def fp64_vs_oracle(request, cb05, oracle_config0_64, drv, gpu, pkg):
+    kmd, kmcd, base = cb05
+    w, (o_state, o_t, o_dt, o_stats) = oracle_config0_64
+    lu = lu_code(pkg, request.param)
+    print("case: config0 seed 0, 64 cells, fp64, lu", request.param, flush=True)
     run = drv.DeviceRun(kmcd, w)
     run.reset()
-    run.run_intervals(precision=pkg.PRECISION_FP64, schedule=pkg.SCHEDULE_GRID, collect_stats=True)
+    run.run_intervals(precision=pkg.PRECISION_FP64, schedule=pkg.SCHEDULE_GRID, collect_stats=True, lu=lu)
     gpu.cuda.synchronize()
     g_state = run.state.cpu().numpy()
     g_t = run.t.cpu().numpy()
     g_stats = run.total_stats
-    o_state, o_t, o_dt, o_stats = oracle_intervals(cb05_oracle, oracle_mod, w)
+    run.lu = lu
     return w, run, g_state, g_t, g_stats, o_state, o_t, o_stats
 
 
@@ -177,7 +197,7 @@ def test_persistent_schedule_is_bitwise_identical(fp64_vs_oracle, gpu, pkg):
     """The work-queue launch runs the same per-cell code: identical bits."""
     w, run, g_state, g_t, g_stats, *_ = fp64_vs_oracle
     run.reset()
-    run.run_intervals(precision=pkg.PRECISION_FP64, schedule=pkg.SCHEDULE_PERSISTENT)
+    run.run_intervals(precision=pkg.PRECISION_FP64, schedule=pkg.SCHEDULE_PERSISTENT, lu=run.lu)
     gpu.cuda.synchronize()
     assert np.array_equal(run.state.cpu().numpy(), g_state)
     assert np.array_equal(run.t.cpu().numpy(), g_t)
@@ -189,7 +209,7 @@ def test_mixed_precision_error_against_fp64_oracle(fp64_vs_oracle, cb05, gpu, pk
     w, run, g_state, g_t, g_stats, o_state, o_t, o_stats = fp64_vs_oracle
     m = cb05[0].nActive
     run.reset()
-    run.run_to_end(precision=pkg.PRECISION_MIXED, schedule=pkg.SCHEDULE_PERSISTENT, collect_stats=True)
+    run.run_to_end(precision=pkg.PRECISION_MIXED, schedule=pkg.SCHEDULE_PERSISTENT, collect_stats=True, lu=run.lu)
     gpu.cuda.synchronize()
     assert np.all(run.t.cpu().numpy() == w.tend)
     assert run.total_stats[:, 5].sum() == 0
@@ -202,10 +222,12 @@ def test_mixed_precision_error_against_fp64_oracle(fp64_vs_oracle, cb05, gpu, pk
     assert err.max() < 1e-3
 
 
-def test_forced_newton_failure_on_gpu(cb05, gpu, pkg):
+@pytest.mark.parametrize("lu", LU_FORMS)
+def test_forced_newton_failure_on_gpu(cb05, gpu, pkg, lu):
     """One Newton iteration and a minimum step of 100 s: every cell fails the
     reference way (species zeroed, t_out = tend, dt_out = -1) and the kernel exits."""
     kmd, kmcd, base = cb05
+    print("case: forced Newton failure, lu", lu, flush=True)
     torch = gpu
     m = kmd.nActive
     n = 4
@@ -216,7 +238,8 @@ def test_forced_newton_failure_on_gpu(cb05, gpu, pkg):
     t = torch.zeros(n, dtype=torch.float64, device="cuda")
     dt = torch.zeros(n, dtype=torch.float64, device="cuda")
     stats = pkg.stats_tensor(n)
-    pkg.AtmosphericChemistry.runDeviceBatch(tn, tt, None, tadv, st, t, dt, st, kmcd, stats=stats)
+    pkg.AtmosphericChemistry.runDeviceBatch(tn, tt, None, tadv, st, t, dt, st, kmcd, stats=stats,
+                                            lu=lu_code(pkg, lu))
     torch.cuda.synchronize()
     s = pkg.stats_from_tensor(stats)
     assert np.all(s["failed"] == 1) and np.all(s["newton_failures"] == 4)
@@ -224,7 +247,7 @@ def test_forced_newton_failure_on_gpu(cb05, gpu, pkg):
     out = st.cpu().numpy()
     assert np.all(out[:, 3:3 + m] == 0.0)
     # a second launch skips the failed cells
-    pkg.AtmosphericChemistry.runDeviceBatch(tn, tt, None, tadv, st, t, dt, st, kmcd)
+    pkg.AtmosphericChemistry.runDeviceBatch(tn, tt, None, tadv, st, t, dt, st, kmcd, lu=lu_code(pkg, lu))
     torch.cuda.synchronize()
     assert np.array_equal(st.cpu().numpy(), out)
 
@@ -242,3 +265,15 @@ def test_invalid_arguments_are_rejected(cb05, gpu, pkg):
     assert lib.tchem_atmchem_run_device_batch(opts, 1, None, None, None, None, None, None, None, None, None, None,
                                               kmcd._h, None) == -1
     assert "precision" in lib.tchem_last_error().decode()
+    opts = pkg._Options(0, 0, 0, 7)
+    assert lib.tchem_atmchem_run_device_batch(opts, 1, None, None, None, None, None, None, None, None, None, None,
+                                              kmcd._h, None) == -1
+    assert "lu" in lib.tchem_last_error().decode()
+    opts = pkg._Options(0, 0, 0, 0, (0, 1, 0, 0))
+    assert lib.tchem_atmchem_run_device_batch(opts, 1, None, None, None, None, None, None, None, None, None, None,
+                                              kmcd._h, None) == -1
+    assert "reserved" in lib.tchem_last_error().decode()
+    # workspace of each form; the sparse form needs far less LDS than the dense one
+    dense = pkg.AtmosphericChemistry.getWorkSpaceSize(kmcd, pkg.PRECISION_MIXED, pkg.LU_DENSE)
+    sparse = pkg.AtmosphericChemistry.getWorkSpaceSize(kmcd, pkg.PRECISION_MIXED, pkg.LU_SPARSE)
+    assert sparse < dense / 2
