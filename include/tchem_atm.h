@@ -61,7 +61,9 @@ typedef struct {
   int schedule;          /* TCHEM_SCHEDULE_* */
   int blocks_per_cu;     /* persistent schedule: resident workgroups per CU (0 = from occupancy) */
   int lu;                /* TCHEM_LU_* */
-  int reserved[4];       /* must be zero */
+  int cells_per_block;   /* sparse form: cells (wavefronts) per workgroup sharing the LDS-resident
+                            mechanism program (0 = as many as fit); the dense form runs 1 */
+  int reserved[3];       /* must be zero */
 } tchem_atmchem_options_t;
 
 /* Last error message of the calling thread ("" when none). */

@@ -62,7 +62,7 @@ class TChemError(RuntimeError):
 
 class _Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("schedule", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
-                ("lu", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
+                ("lu", ctypes.c_int), ("cells_per_block", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
 
 
 def _load():
@@ -197,23 +197,24 @@ def setScenarioConditions(filename: str, kmd: KineticModelData) -> np.ndarray:
 
 class AtmosphericChemistry:
     @staticmethod
-    def getWorkSpaceSize(kmcd: KineticModelConstData, precision: int = PRECISION_FP64, lu: int = LU_AUTO) -> int:
-        """Bytes of LDS one cell uses in the given form."""
+    def getWorkSpaceSize(kmcd: KineticModelConstData, precision: int = PRECISION_FP64, lu: int = LU_AUTO,
+                         cells_per_block: int = 0) -> int:
+        """Bytes of LDS one workgroup uses in the given form."""
         b = ctypes.c_size_t()
-        opts = _Options(precision, SCHEDULE_GRID, 0, lu)
+        opts = _Options(precision, SCHEDULE_GRID, 0, lu, cells_per_block)
         _check(_lib.tchem_atmchem_workspace_bytes(kmcd._h, ctypes.byref(opts), ctypes.byref(b)))
         return b.value
 
     @staticmethod
     def runDeviceBatch(tol_newton, tol_time, fac, tadv, state, t_out, dt_out, state_out, kmcd, photo_rates=None,
                        stats=None, precision=PRECISION_FP64, schedule=SCHEDULE_GRID, blocks_per_cu=0, lu=LU_AUTO,
-                       stream=None):
+                       cells_per_block=0, stream=None):
         """One integration interval over the batch (asynchronous on `stream`).
 
         tadv is a uint8 device tensor holding TIME_ADVANCE_DTYPE records (or any
         tensor with 56 bytes per cell); stats likewise CELL_STATS_DTYPE."""
         nBatch = state.shape[0]
-        opts = _Options(precision, schedule, blocks_per_cu, lu)
+        opts = _Options(precision, schedule, blocks_per_cu, lu, cells_per_block)
         _check(_lib.tchem_atmchem_run_device_batch(
             ctypes.byref(opts), nBatch, _ptr(tol_newton, what="tol_newton"), _ptr(tol_time, what="tol_time"),
             _ptr(fac, None, "fac"), _ptr(tadv, None, "tadv"), _ptr(state, what="state"),

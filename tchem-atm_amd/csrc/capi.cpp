@@ -18,7 +18,9 @@
 
 namespace tchem_amd {
 hipError_t launchIntegrator(const IntegratorArgs& a, int precision, int grid, size_t lds, hipStream_t stream);
-hipError_t integratorMaxBlocksPerCU(int precision, int sparse, int m, size_t lds, int* blocks);
+hipError_t integratorMaxBlocksPerCU(int precision, int sparse, int m, int threads, size_t lds, int* blocks);
+int sparseMaxCellsPerGroup(int precision, int m);
+size_t integratorStaticLds(int precision, int sparse, int m);
 hipError_t launchResetCounter(int* counter, hipStream_t stream);
 hipError_t launchRates(const DevMech& M, int nBatch, const double* state, const double* photo, double* kf,
                        hipStream_t stream);
@@ -100,15 +102,43 @@ int resolveOptions(const tchem_atmchem_options_t* opts, const tchem_kmcd_s* k, t
   if (o->blocks_per_cu < 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": negative blocks_per_cu");
   if (o->lu != TCHEM_LU_AUTO && o->lu != TCHEM_LU_DENSE && o->lu != TCHEM_LU_SPARSE)
     return setError(TCHEM_ERR_INVALID, std::string(who) + ": unknown lu");
-  if (o->lu == TCHEM_LU_AUTO) o->lu = k->sparse.enabled ? TCHEM_LU_SPARSE : TCHEM_LU_DENSE;
-  if (o->lu == TCHEM_LU_SPARSE && !k->sparse.enabled)
-    return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": mechanism has no sparse schedule");
+  if (o->cells_per_block < 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": negative cells_per_block");
+  const size_t bytesA = o->precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float);
+  const size_t staticLds = integratorStaticLds(o->precision, 1, k->dev.m);
+  const size_t maxLds = k->maxLds > staticLds ? k->maxLds - staticLds : 0;
+  auto sparseFits = [&](int cells) {
+    return sparseProgramBytes(k->sparse) + size_t(cells) * sparseCellBytes(k->dev, k->sparse, bytesA) <= maxLds;
+  };
+  if (o->lu == TCHEM_LU_AUTO)
+    o->lu = (k->sparse.enabled && sparseFits(1)) ? TCHEM_LU_SPARSE : TCHEM_LU_DENSE;
+  if (o->lu == TCHEM_LU_SPARSE) {
+    if (!k->sparse.enabled)
+      return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": mechanism has no sparse schedule");
+    const int cap = sparseMaxCellsPerGroup(o->precision, k->dev.m);
+    if (o->cells_per_block == 0) {
+      int c = cap;
+      while (c > 1 && !sparseFits(c)) --c;
+      o->cells_per_block = c;
+    }
+    if (o->cells_per_block > cap)
+      return setError(TCHEM_ERR_INVALID, std::string(who) + ": cells_per_block above " + std::to_string(cap));
+    if (!sparseFits(o->cells_per_block))
+      return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": " + std::to_string(o->cells_per_block) +
+                                                 " cells per block do not fit in LDS");
+  } else {
+    if (o->cells_per_block > 1)
+      return setError(TCHEM_ERR_INVALID, std::string(who) + ": the dense form runs one cell per block");
+    o->cells_per_block = 1;
+  }
   return TCHEM_OK;
 }
 
+// LDS bytes of one workgroup
 size_t ldsBytes(const tchem_kmcd_s* k, const tchem_atmchem_options_t& o) {
   const size_t bytesA = o.precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float);
-  return o.lu == TCHEM_LU_SPARSE ? sparseLdsBytes(k->dev, k->sparse, bytesA) : integratorLdsBytes(k->dev, bytesA);
+  if (o.lu == TCHEM_LU_SPARSE)
+    return sparseProgramBytes(k->sparse) + size_t(o.cells_per_block) * sparseCellBytes(k->dev, k->sparse, bytesA);
+  return integratorLdsBytes(k->dev, bytesA);
 }
 
 }  // namespace
@@ -259,9 +289,7 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
     sp.enabled = 1;
     sp.nnzF = sc.nnzF;
     sp.zeroSlot = sc.zeroSlot;
-    sp.nLevels = sc.nLevels;
-    sp.nFwdLevels = sc.nFwdLevels;
-    sp.nBackLevels = sc.nBackLevels;
+    sp.trashSlot = sc.trashSlot;
     sp.ropRounds = sc.ropRounds;
     sp.segRounds = sc.segRounds;
     sp.pairRounds = sc.pairRounds;
@@ -275,22 +303,15 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
     sp.oJacRound = addI(sc.jacRound);
     sp.oJacTarget = addI(sc.jacTarget);
     sp.oJacPair = addI(sc.jacPair);
-    sp.oLevMul = addI(sc.levMul);
-    sp.oLevUpd = addI(sc.levUpd);
-    sp.oMulRec = addI(sc.mulRec);
-    sp.oUpdRound = addI(sc.updRound);
-    sp.oUpdTarget = addI(sc.updTarget);
-    sp.oUpdPair = addI(sc.updPair);
-    sp.oFwdLev = addI(sc.fwdLev);
-    sp.oFwdRound = addI(sc.fwdRound);
-    sp.oFwdTarget = addI(sc.fwdTarget);
-    sp.oFwdPair = addI(sc.fwdPair);
-    sp.oBackLev = addI(sc.backLev);
-    sp.oBackRound = addI(sc.backRound);
-    sp.oBackTarget = addI(sc.backTarget);
-    sp.oBackPair = addI(sc.backPair);
     sp.oSegCoef = addD(sc.segCoef);
     sp.oJacCoef = addD(sc.jacCoef);
+    sp.oProgHdr = addI(std::vector<int>(sc.progHdr.begin(), sc.progHdr.end()));
+    sp.oProgBody = addI(std::vector<int>(sc.progBody.begin(), sc.progBody.end()));
+    sp.luRounds = sc.luRounds;
+    sp.backRounds = sc.backRounds;
+    sp.fwdRounds = sc.fwdRounds;
+    sp.backBody = sc.backBody;
+    sp.fwdBody = sc.fwdBody;
   } catch (const std::exception&) {
     k->sparse = DevSparse{};
   }
@@ -339,7 +360,8 @@ int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBat
   const size_t lds = ldsBytes(kmcd, o);
   if (lds > kmcd->maxLds)
     return setError(TCHEM_ERR_UNSUPPORTED, "run_device_batch: mechanism needs " + std::to_string(lds) +
-                                               " bytes of LDS per cell, device limit " + std::to_string(kmcd->maxLds));
+                                               " bytes of LDS per workgroup, device limit " +
+                                               std::to_string(kmcd->maxLds));
   int cur = 0;
   if (hipGetDevice(&cur) != hipSuccess || cur != kmcd->device)
     return setError(TCHEM_ERR_INVALID, "run_device_batch: kmcd belongs to another device");
@@ -360,6 +382,10 @@ int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBat
   a.stats = stats;
   a.queue = kmcd->queue;
   a.persistent = (o.schedule == TCHEM_SCHEDULE_PERSISTENT) ? 1 : 0;
+  a.cellsPerGroup = o.cells_per_block;
+  a.cellBytes = a.sparse ? int(sparseCellBytes(kmcd->dev, kmcd->sparse,
+                                               o.precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float)))
+                         : 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (size_t(nBatch) > kmcd->kfCapacity) {
     // grows only: the previous buffer may still be read by work queued on s
@@ -380,16 +406,18 @@ int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBat
     if (e != hipSuccess) return hipFail(e, "rates_kernel launch");
   }
 
-  int grid = nBatch;
+  const int cpg = a.cellsPerGroup;
+  const int groups = (nBatch + cpg - 1) / cpg;
+  int grid = groups;
   if (a.persistent) {
     int bpc = o.blocks_per_cu;
     if (bpc == 0) {
-      hipError_t e = integratorMaxBlocksPerCU(o.precision, a.sparse, kmcd->dev.m, lds, &bpc);
+      hipError_t e = integratorMaxBlocksPerCU(o.precision, a.sparse, kmcd->dev.m, 64 * cpg, lds, &bpc);
       if (e != hipSuccess) return hipFail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
       if (bpc < 1) bpc = 1;
     }
     const long long g = (long long)bpc * kmcd->numCUs;
-    grid = int(g < nBatch ? g : nBatch);
+    grid = int(g < groups ? g : groups);
     hipError_t e = launchResetCounter(kmcd->queue, s);
     if (e != hipSuccess) return hipFail(e, "reset_counter_kernel");
   }

@@ -46,14 +46,28 @@ struct DevMech {
 // the mechanism's int / double buffers, addressed by offsets.
 struct DevSparse {
   int enabled;  // 0: mechanism has no sparse schedule
-  int nnzF, zeroSlot, nLevels, nFwdLevels, nBackLevels;
+  int nnzF, zeroSlot, trashSlot;
   int ropRounds, segRounds, pairRounds, pairSlots, jacRounds;
   int oRopRec, oSegRound, oSegTarget, oSegReac, oPairRec, oJacRound, oJacTarget, oJacPair;
-  int oLevMul, oLevUpd, oMulRec, oUpdRound, oUpdTarget, oUpdPair;
-  int oFwdLev, oFwdRound, oFwdTarget, oFwdPair;
-  int oBackLev, oBackRound, oBackTarget, oBackPair;
   int oSegCoef, oJacCoef;  // double buffer
+  // LU / substitution program: round headers and bodies; the LU and BACK
+  // sections are staged into LDS, FWD (factor reuse) is read from memory
+  int oProgHdr, oProgBody;
+  int luRounds, backRounds, fwdRounds;
+  int backBody, fwdBody;  // body word offsets of the BACK and FWD sections
 };
+
+// LDS layout of a sparse-form workgroup: the staged program (headers of the
+// LU + BACK rounds, then their bodies), then one cell region per wavefront.
+__host__ __device__ inline size_t sparseProgramHeaderWords(const DevSparse& S) {
+  return (size_t(S.luRounds + S.backRounds) + 3) & ~size_t(3);
+}
+// body words staged: the LU and BACK sections plus the slack the
+// interpreter may read past the last round (1 + 4 slots of 64 words)
+__host__ __device__ inline size_t sparseStagedBodyWords(const DevSparse& S) { return size_t(S.fwdBody) + 64 * 5; }
+__host__ __device__ inline size_t sparseProgramBytes(const DevSparse& S) {
+  return 4 * (sparseProgramHeaderWords(S) + ((sparseStagedBodyWords(S) + 3) & ~size_t(3)));
+}
 
 struct IntegratorArgs {
   DevMech M;
@@ -72,6 +86,8 @@ struct IntegratorArgs {
   int* queue;                        // persistent schedule: next-cell counter
   int persistent;
   int sparse;                        // 1: sparse static-pivot linear algebra
+  int cellsPerGroup;                 // wavefronts (cells) per workgroup
+  int cellBytes;                     // LDS bytes of one cell region
 };
 
 // LDS bytes of one cell: the iteration matrix in TA precision (bytesA per
@@ -88,19 +104,16 @@ inline size_t integratorLdsBytes(const DevMech& M, size_t bytesA) {
 }
 
 
-// LDS bytes of one cell for the sparse form: factor entries + RHS + zero
-// slot and reciprocal pivots in TA precision, then doubles for
+// LDS bytes of one cell region of the sparse form: factor entries + RHS +
+// zero and trash slots, reciprocal pivots (+ a zero for column m and a
+// trash slot) and the
+// Jacobian derivative pairs in TA precision, then doubles for
 // concentrations, rate constants (+ a zero), rates of progress (+ zero
-// slot), derivative pairs (+ zero slot, padded to whole rounds) and
-// source-term segment sums.
-inline size_t sparseLdsBytes(const DevMech& M, const DevSparse& S, size_t bytesA) {
-  size_t a = bytesA * size_t(S.zeroSlot + 1);
-  a = (a + 15) & ~size_t(15);
-  size_t r = bytesA * size_t(M.m);
-  r = (r + 15) & ~size_t(15);
-  const size_t d = sizeof(double) * (size_t(M.nSpec) + size_t(M.nReac + 1) + size_t(M.nReac + 1) +
-                                     size_t(S.pairSlots) + size_t(M.nSeg));
-  return a + r + d;
+// slot) and source-term segment sums.
+inline size_t sparseCellBytes(const DevMech& M, const DevSparse& S, size_t bytesA) {
+  auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
+  return al(bytesA * size_t(S.trashSlot + 1)) + al(bytesA * size_t(M.m + 2)) + al(bytesA * size_t(S.pairSlots)) +
+         sizeof(double) * (size_t(M.nSpec) + size_t(M.nReac + 1) + size_t(M.nReac + 1) + size_t(M.nSeg));
 }
 
 }  // namespace tchem_amd

@@ -23,6 +23,7 @@
 #include <cstdint>
 
 #include "device_mech.hpp"
+#include "schedule.hpp"
 
 namespace tchem_amd {
 
@@ -30,22 +31,37 @@ namespace tchem_amd {
 // wavefront-summed shader-clock cycles of the source term, Jacobian assembly,
 // factorisation, back substitution and whole cells.
 #ifdef TCHEM_PHASE_TIMERS
-__device__ unsigned long long g_phase_cycles[8];
+__device__ unsigned long long g_phase_cycles[16];
+__shared__ unsigned long long tchem_pt_lds[16][16];  // [wavefront][slot], flushed per cell
 #define TCHEM_PT_BEGIN(nm) const unsigned long long tchem_pt_##nm = __builtin_readcyclecounter();
-#define TCHEM_PT_END(nm, slot) \
-  if (threadIdx.x == 0) atomicAdd(&g_phase_cycles[slot], __builtin_readcyclecounter() - tchem_pt_##nm);
+#define TCHEM_PT_END(nm, slot)          \
+  if ((threadIdx.x & 63) == 0)          \
+    tchem_pt_lds[threadIdx.x >> 6][slot] += __builtin_readcyclecounter() - tchem_pt_##nm;
+#define TCHEM_PT_RESET() \
+  if ((threadIdx.x & 63) < 16) tchem_pt_lds[threadIdx.x >> 6][threadIdx.x & 63] = 0;
+#define TCHEM_PT_FLUSH()                                                                          \
+  if ((threadIdx.x & 63) < 16 && tchem_pt_lds[threadIdx.x >> 6][threadIdx.x & 63])                \
+    atomicAdd(&g_phase_cycles[threadIdx.x & 63], tchem_pt_lds[threadIdx.x >> 6][threadIdx.x & 63]); \
+  if ((threadIdx.x & 63) < 16) tchem_pt_lds[threadIdx.x >> 6][threadIdx.x & 63] = 0;
 #else
 #define TCHEM_PT_BEGIN(nm)
 #define TCHEM_PT_END(nm, slot)
+#define TCHEM_PT_RESET()
+#define TCHEM_PT_FLUSH()
 #endif
 
 namespace {
 
 constexpr int kWave = 64;
 
-// A workgroup is one wavefront: the barrier orders the wavefront's LDS
-// traffic and fences the compiler.
-__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+// Wavefront-scope barrier for data exchanged through LDS between the lanes
+// of one wavefront: the LDS executes a wavefront's operations in order, so
+// only compiler reordering has to be prevented (no s_waitcnt, no s_barrier).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __device__ __forceinline__ double readlane(double v, int l) {
   const long long x = __double_as_longlong(v);
@@ -461,34 +477,39 @@ __device__ void lu_back(const CellLds<TA>& L, int m, int lane, TA (&b)[QM]) {
 
 template <typename TA>
 struct SpLds {
-  TA* A;        // [zeroSlot+1] factor entries, RHS, zero
-  TA* rdiag;    // [m] reciprocal pivots
+  const unsigned* hdr;   // staged program: round headers (LU + BACK), shared by the workgroup
+  const unsigned* body;  // staged program: round bodies (LU + BACK)
+  TA* A;        // [trashSlot+1] factor entries, RHS, zero slot, trash slot
+  TA* rdiag;    // [m+2] reciprocal pivots, rdiag[m] = 0, rdiag[m+1] = trash
+  TA* dpr;      // [pairSlots] Jacobian derivative pairs, dpr[nPairs] = 0
   double* c;    // [nSpec]
   double* kf;   // [nReac+1], kf[nReac] = 0
   double* rop;  // [nReac+1], rop[nReac] = 0
-  double* dpr;  // [pairSlots], dpr[nPairs] = 0
   double* seg;  // [nSeg]
 };
 
+// program at the start of the workgroup's LDS, then one cell region per wavefront
 template <typename TA>
-__device__ SpLds<TA> carve_sparse(char* base, const DevMech& M, const DevSparse& S) {
+__device__ SpLds<TA> carve_sparse(char* base, const DevMech& M, const DevSparse& S, int cellBytes, int wave) {
+  auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
   SpLds<TA> L;
-  size_t off = sizeof(TA) * size_t(S.zeroSlot + 1);
-  off = (off + 15) & ~size_t(15);
-  L.A = reinterpret_cast<TA*>(base);
-  L.rdiag = reinterpret_cast<TA*>(base + off);
-  off += sizeof(TA) * size_t(M.m);
-  off = (off + 15) & ~size_t(15);
-  L.c = reinterpret_cast<double*>(base + off);
+  L.hdr = reinterpret_cast<const unsigned*>(base);
+  L.body = L.hdr + sparseProgramHeaderWords(S);
+  char* cell = base + sparseProgramBytes(S) + size_t(wave) * size_t(cellBytes);
+  size_t off = 0;
+  L.A = reinterpret_cast<TA*>(cell);
+  off += al(sizeof(TA) * size_t(S.trashSlot + 1));
+  L.rdiag = reinterpret_cast<TA*>(cell + off);
+  off += al(sizeof(TA) * size_t(M.m + 2));
+  L.dpr = reinterpret_cast<TA*>(cell + off);
+  off += al(sizeof(TA) * size_t(S.pairSlots));
+  L.c = reinterpret_cast<double*>(cell + off);
   L.kf = L.c + M.nSpec;
   L.rop = L.kf + M.nReac + 1;
-  L.dpr = L.rop + M.nReac + 1;
-  L.seg = L.dpr + S.pairSlots;
+  L.seg = L.rop + M.nReac + 1;
   return L;
 }
 
-// x^n for small n >= 0 by repeated multiplication (exact for n <= 2; at
-// most n-1 roundings above)
 __device__ __forceinline__ double ipow(double x, int n) {
   double r = n >= 1 ? x : 1.0;
   for (int i = 1; i < n; ++i) r *= x;
@@ -540,14 +561,16 @@ __device__ void sp_sums(const DevMech& M, const DevSparse& S, const SpLds<TA>& L
     const double* ec = M.db + S.oSegCoef;
     for (int g = 0; g < S.segRounds; ++g) {
       const int2 r = uniform(rd[g]);
-      double acc = 0.0;
-#pragma unroll 4
-      for (int k = 0; k < r.y; ++k) {
-        const int e = (r.x + k) * kWave + lane;
-        acc += ec[e] * L.rop[er[e]];
-      }
       const int t = tg[g * kWave + lane];
-      if (t >= 0) L.seg[t] = acc;
+      if (t >= 0) {
+        double acc = 0.0;
+#pragma unroll 4
+        for (int k = 0; k < r.y; ++k) {
+          const int e = (r.x + k) * kWave + lane;
+          acc += ec[e] * L.rop[er[e]];
+        }
+        L.seg[t] = acc;
+      }
     }
   }
   if (!jac) return;
@@ -555,16 +578,20 @@ __device__ void sp_sums(const DevMech& M, const DevSparse& S, const SpLds<TA>& L
   const int* tg = M.ib + S.oJacTarget;
   const int* ep = M.ib + S.oJacPair;
   const double* ec = M.db + S.oJacCoef;
+  // accumulated in TA precision: FP32 assembly for the mixed form
+  const TA tdiag = TA(diag), tscale = TA(scale);
   for (int g = 0; g < S.jacRounds; ++g) {
     const int2 r = uniform(rd[g]);
-    double s = 0.0;
-#pragma unroll 4
-    for (int k = 0; k < r.y; ++k) {
-      const int e = (r.x + k) * kWave + lane;
-      s += ec[e] * L.dpr[ep[e]];
-    }
     const int t = tg[g * kWave + lane];
-    if (t >= 0) L.A[t & 0x3fffffff] = TA(((t >> 30) ? diag : 0.0) + scale * s);
+    if (t >= 0) {
+      TA s = TA(0);
+#pragma unroll 4
+      for (int k = 0; k < r.y; ++k) {
+        const int e = (r.x + k) * kWave + lane;
+        s += TA(ec[e]) * L.dpr[ep[e]];
+      }
+      L.A[t & 0x3fffffff] = ((t >> 30) ? tdiag : TA(0)) + tscale * s;
+    }
   }
 }
 
@@ -584,81 +611,99 @@ __device__ __forceinline__ void row_sums(const DevMech& M, const double* seg, in
   }
 }
 
-// Rounds of chained updates: target -= sum A[lo] * A[hi] over its chain.
-// BACK: the target is row i of the RHS and the result is scaled by rdiag_i.
-template <bool BACK, typename TA>
-__device__ __forceinline__ void sp_update_rounds(const int2* rd, int g0, int g1, const int* target, const int* ent,
-                                                 TA* A, const TA* rdiag, int rhs0, int lane) {
-  for (int g = g0; g < g1; ++g) {
-    const int2 r = uniform(rd[g]);
-    const int t = target[g * kWave + lane];
-    const int pos = BACK ? rhs0 + t : t;
-    TA acc = t >= 0 ? A[pos] : TA(0);
-#pragma unroll 4
-    for (int k = 0; k < r.y; ++k) {
-      const int p = ent[(r.x + k) * kWave + lane];
-      acc -= A[p & 0xffff] * A[p >> 16];
-    }
-    if (t >= 0) A[pos] = BACK ? acc * rdiag[t] : acc;
+// One chained-update round with LEN entries per lane (w[0] = target,
+// w[1..LEN] = entries): every load is issued before the first use.
+template <int LEN, typename TA>
+__device__ __forceinline__ void upd_round(const unsigned (&w)[1 + SparseSchedule::kMaxChain], TA* A) {
+  TA acc = A[w[0]];
+  TA l[LEN], u[LEN];
+#pragma unroll
+  for (int j = 0; j < LEN; ++j) {
+    l[j] = A[w[1 + j] & 0xffff];
+    u[j] = A[w[1 + j] >> 16];
   }
+#pragma unroll
+  for (int j = 0; j < LEN; ++j) acc -= l[j] * u[j];
+  A[w[0]] = acc;
+}
+template <int LEN, typename TA>
+__device__ __forceinline__ void back_round(const unsigned (&w)[1 + SparseSchedule::kMaxChain], TA* A,
+                                           const TA* rdiag, int rhs0) {
+  TA acc = A[w[0]];
+  TA U[LEN], b[LEN], r[LEN];
+#pragma unroll
+  for (int j = 0; j < LEN; ++j) {
+    const int k = int(w[1 + j] >> 16);
+    U[j] = A[w[1 + j] & 0xffff];
+    b[j] = A[rhs0 + k];
+    r[j] = rdiag[k];
+  }
+#pragma unroll
+  for (int j = 0; j < LEN; ++j) acc -= U[j] * (b[j] * r[j]);
+  A[w[0]] = acc;
 }
 
-// Level-scheduled LU without interchanges, RHS eliminated alongside.
-// Multipliers l = a_ik * (1 / a_kk).  Returns false for a zero or
-// non-finite pivot (the Newton iteration then fails and the step is retried
-// with a smaller h).
+// Runs rounds of the LU / substitution program (schedule.hpp): multiplier
+// and pivot records (kMul), chained updates A[t] -= sum A[lo] * A[hi] (kUpd)
+// and column-oriented back-substitution updates
+// A[t] -= sum A[lo] * (A[rhs0 + k] * rdiag[k]) (kBack).  Lanes of a round
+// touch disjoint targets; rounds run in order.  Branch free per lane
+// (padding reads zero slots, writes trash slots); the header two rounds
+// ahead and the next round's words are loaded while a round runs, so a
+// round waits only for its matrix data.  Returns false when a pivot is
+// zero or not finite.
 template <typename TA>
-__device__ bool sp_factor(const DevMech& M, const DevSparse& S, const SpLds<TA>& L, int lane) {
-  const int2* lm = reinterpret_cast<const int2*>(M.ib + S.oLevMul);
-  const int2* lu = reinterpret_cast<const int2*>(M.ib + S.oLevUpd);
-  const int2* mr = reinterpret_cast<const int2*>(M.ib + S.oMulRec);
-  const int2* ur = reinterpret_cast<const int2*>(M.ib + S.oUpdRound);
-  const int* ut = M.ib + S.oUpdTarget;
-  const int* up = M.ib + S.oUpdPair;
+__device__ bool run_program(const unsigned* hdr, const unsigned* body, int nRounds, TA* A, TA* rdiag, int m,
+                            int rhs0, int trash, int lane) {
+  constexpr int kW = 1 + SparseSchedule::kMaxChain;
   bool bad = false;
-  for (int lev = 0; lev < S.nLevels; ++lev) {
-    const int2 mb = uniform(lm[lev]);
-    for (int g = mb.x; g < mb.y; ++g) {
-      const int2 r = mr[g * kWave + lane];
-      if (r.y >= 0) {
-        const TA piv = L.A[r.y];
-        const TA inv = TA(1) / piv;
-        if (r.x < 0) {
-          L.rdiag[-r.x - 1] = inv;
-          bad |= !(piv != TA(0) && isfinite(piv) && isfinite(inv));
-        } else {
-          L.A[r.x] = L.A[r.x] * inv;
-        }
+  if (nRounds <= 0) return true;
+  unsigned h0 = __builtin_amdgcn_readfirstlane(hdr[0]);
+  unsigned h1 = nRounds > 1 ? hdr[1] : 0u;
+  unsigned w[kW];
+#pragma unroll
+  for (int j = 0; j < kW; ++j) w[j] = body[kWave * j + lane];
+  for (int r = 0; r < nRounds; ++r) {
+    const unsigned h2 = r + 2 < nRounds ? hdr[r + 2] : 0u;
+    const int kind = int(h0 >> 16), len = int(h0 & 0xffff);
+    const unsigned* bnext = body + (kind == SparseSchedule::kMul ? kWave : kWave * (1 + len));
+    unsigned wn[kW];
+#pragma unroll
+    for (int j = 0; j < kW; ++j) wn[j] = bnext[kWave * j + lane];
+    if (kind == SparseSchedule::kMul) {
+      const int dst = int(w[0] & 0xffff), src = int(w[0] >> 16);
+      const bool piv = (dst & 0x8000) != 0;
+      const int ai = piv ? trash : dst;
+      const int ri = piv ? (dst & 0x7fff) : m + 1;
+      const TA p = A[src];
+      const TA a = A[ai];
+      const TA inv = TA(1) / p;
+      A[ai] = a * inv;
+      rdiag[ri] = inv;
+      bad |= piv && !(p != TA(0) && isfinite(p) && isfinite(inv));
+    } else if (kind == SparseSchedule::kUpd) {
+      switch (len) {
+        case 1: upd_round<1>(w, A); break;
+        case 2: upd_round<2>(w, A); break;
+        case 3: upd_round<3>(w, A); break;
+        default: upd_round<4>(w, A); break;
+      }
+    } else {
+      switch (len) {
+        case 1: back_round<1>(w, A, rdiag, rhs0); break;
+        case 2: back_round<2>(w, A, rdiag, rhs0); break;
+        case 3: back_round<3>(w, A, rdiag, rhs0); break;
+        default: back_round<4>(w, A, rdiag, rhs0); break;
       }
     }
     wave_sync();
-    const int2 ub = uniform(lu[lev]);
-    sp_update_rounds<false>(ur, ub.x, ub.y, ut, up, L.A, L.rdiag, 0, lane);
-    wave_sync();
+    body = bnext;
+    h0 = __builtin_amdgcn_readfirstlane(h1);
+    h1 = h2;
+#pragma unroll
+    for (int j = 0; j < kW; ++j) w[j] = wn[j];
   }
   return ballot(bad) == 0;
-}
-
-template <typename TA>
-__device__ void sp_forward(const DevMech& M, const DevSparse& S, const SpLds<TA>& L, int lane) {
-  const int2* lv = reinterpret_cast<const int2*>(M.ib + S.oFwdLev);
-  const int2* rd = reinterpret_cast<const int2*>(M.ib + S.oFwdRound);
-  for (int lev = 0; lev < S.nFwdLevels; ++lev) {
-    const int2 b = uniform(lv[lev]);
-    sp_update_rounds<false>(rd, b.x, b.y, M.ib + S.oFwdTarget, M.ib + S.oFwdPair, L.A, L.rdiag, 0, lane);
-    wave_sync();
-  }
-}
-
-template <typename TA>
-__device__ void sp_back(const DevMech& M, const DevSparse& S, const SpLds<TA>& L, int lane) {
-  const int2* lv = reinterpret_cast<const int2*>(M.ib + S.oBackLev);
-  const int2* rd = reinterpret_cast<const int2*>(M.ib + S.oBackRound);
-  for (int lev = 0; lev < S.nBackLevels; ++lev) {
-    const int2 b = uniform(lv[lev]);
-    sp_update_rounds<true>(rd, b.x, b.y, M.ib + S.oBackTarget, M.ib + S.oBackPair, L.A, L.rdiag, S.nnzF, lane);
-    wave_sync();
-  }
 }
 
 // ---- backends -----------------------------------------------------------------
@@ -672,7 +717,8 @@ struct DenseBackend {
   using TA = TA_;
   using Lds = CellLds<TA>;
   struct Fact { int nswp = 0; };
-  static __device__ Lds carve(char* base, const IntegratorArgs& a) { return tchem_amd::carve<TA>(base, a.M); }
+  static __device__ void stage(const IntegratorArgs&, char*) {}
+  static __device__ Lds carve(char* base, const IntegratorArgs& a, int) { return tchem_amd::carve<TA>(base, a.M); }
   static __device__ void init(const IntegratorArgs&, const Lds&, int) {}
   // f at u; for a Newton iteration also x = -(u - bc - ch f) and, when
   // refreshing, the iteration matrix I - ch J at the same u
@@ -703,15 +749,32 @@ struct SparseBackend {
   using TA = TA_;
   using Lds = SpLds<TA>;
   struct Fact {};
-  static __device__ Lds carve(char* base, const IntegratorArgs& a) { return carve_sparse<TA>(base, a.M, a.S); }
-  // zero slots read by padded chain entries
+  // the mechanism program (LU and BACK sections) is copied into the
+  // workgroup's LDS once; every wavefront of the workgroup reads it
+  static __device__ void stage(const IntegratorArgs& a, char* base) {
+    const DevSparse& S = a.S;
+    unsigned* dst = reinterpret_cast<unsigned*>(base);
+    const unsigned* hsrc = reinterpret_cast<const unsigned*>(a.M.ib + S.oProgHdr);
+    const unsigned* bsrc = reinterpret_cast<const unsigned*>(a.M.ib + S.oProgBody);
+    const int nh = S.luRounds + S.backRounds, hw = int(sparseProgramHeaderWords(S));
+    for (int i = threadIdx.x; i < nh; i += blockDim.x) dst[i] = hsrc[i];
+    const int nb = int(sparseStagedBodyWords(S));
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) dst[hw + i] = bsrc[i];
+    __syncthreads();
+  }
+  static __device__ Lds carve(char* base, const IntegratorArgs& a, int wave) {
+    return carve_sparse<TA>(base, a.M, a.S, a.cellBytes, wave);
+  }
+  // zero slots read by padded records
   static __device__ void init(const IntegratorArgs& a, const Lds& L, int lane) {
     if (lane == 0) {
       L.kf[a.M.nReac] = 0.0;
       L.rop[a.M.nReac] = 0.0;
-      L.dpr[a.M.nPairs] = 0.0;
+      L.dpr[a.M.nPairs] = TA(0);
       L.A[a.S.zeroSlot] = TA(0);
+      L.rdiag[a.M.m] = TA(0);
     }
+    wave_sync();
   }
   static __device__ void put_c(const IntegratorArgs& a, const Lds& L, int lane, const double (&u)[QM]) {
 #pragma unroll
@@ -724,11 +787,16 @@ struct SparseBackend {
   static __device__ void evaluate(const IntegratorArgs& a, const Lds& L, int lane, const double (&u)[QM],
                                   const double (&bc)[QM], double ch, bool newton, bool refresh, double (&f)[QM],
                                   TA (&x)[QM]) {
+    TCHEM_PT_BEGIN(rates)
     put_c(a, L, lane, u);
     sp_rates(a.M, a.S, L, lane, refresh);
     wave_sync();
+    TCHEM_PT_END(rates, 11)
+    TCHEM_PT_BEGIN(sums)
     sp_sums(a.M, a.S, L, lane, refresh, 1.0, -ch);
     wave_sync();
+    TCHEM_PT_END(sums, 12)
+    TCHEM_PT_BEGIN(rows)
     row_sums<QM>(a.M, L.seg, lane, f);
 #pragma unroll
     for (int q = 0; q < QM; ++q) {
@@ -737,23 +805,36 @@ struct SparseBackend {
       if (newton && i < a.M.m) L.A[a.S.nnzF + i] = x[q];
     }
     if (newton) wave_sync();
+    TCHEM_PT_END(rows, 13)
   }
+  // after the BACK section the RHS holds u_ii x_i
   static __device__ void get_x(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM]) {
 #pragma unroll
     for (int q = 0; q < QM; ++q) {
       const int i = lane + kWave * q;
-      x[q] = (i < a.M.m) ? L.A[a.S.nnzF + i] : TA(0);
+      x[q] = (i < a.M.m) ? L.A[a.S.nnzF + i] * L.rdiag[i] : TA(0);
     }
   }
+  static __device__ void back(const IntegratorArgs& a, const Lds& L, int lane) {
+    TCHEM_PT_BEGIN(back)
+    run_program(L.hdr + a.S.luRounds, L.body + a.S.backBody, a.S.backRounds, L.A, L.rdiag, a.M.m, a.S.nnzF,
+                a.S.trashSlot, lane);
+    TCHEM_PT_END(back, 10)
+  }
   static __device__ bool factor_solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM], Fact&) {
-    if (!sp_factor(a.M, a.S, L, lane)) return false;
-    sp_back(a.M, a.S, L, lane);
+    TCHEM_PT_BEGIN(lu)
+    const bool ok = run_program(L.hdr, L.body, a.S.luRounds, L.A, L.rdiag, a.M.m, a.S.nnzF, a.S.trashSlot, lane);
+    TCHEM_PT_END(lu, 9)
+    if (!ok) return false;
+    back(a, L, lane);
     get_x(a, L, lane, x);
     return true;
   }
   static __device__ void solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM], const Fact&) {
-    sp_forward(a.M, a.S, L, lane);
-    sp_back(a.M, a.S, L, lane);
+    const unsigned* fh = reinterpret_cast<const unsigned*>(a.M.ib + a.S.oProgHdr) + a.S.luRounds + a.S.backRounds;
+    const unsigned* fb = reinterpret_cast<const unsigned*>(a.M.ib + a.S.oProgBody) + a.S.fwdBody;
+    run_program(fh, fb, a.S.fwdRounds, L.A, L.rdiag, a.M.m, a.S.nnzF, a.S.trashSlot, lane);
+    back(a, L, lane);
     get_x(a, L, lane, x);
   }
 };
@@ -978,6 +1059,8 @@ __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L
   }
 
   TCHEM_PT_END(cell, 6)
+  wave_sync();
+  TCHEM_PT_FLUSH()
   double* rowo = a.state_out + size_t(cell) * dim;
 #pragma unroll
   for (int q = 0; q < QM; ++q) {
@@ -1005,14 +1088,19 @@ __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L
   }
 }
 
-template <class B, int QM>
-__global__ __launch_bounds__(64) void trbdf2_kernel(IntegratorArgs a) {
+// Workgroup = cellsPerGroup wavefronts, one cell per wavefront at a time;
+// after staging, wavefronts never synchronise with each other.
+template <class B, int QM, int kThreads>
+__global__ __launch_bounds__(kThreads) void trbdf2_kernel(IntegratorArgs a) {
   extern __shared__ __align__(16) char lds_raw[];
-  const typename B::Lds L = B::carve(lds_raw, a);
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  B::stage(a, lds_raw);
+  const typename B::Lds L = B::carve(lds_raw, a, wave);
   B::init(a, L, lane);
+  TCHEM_PT_RESET()
   if (!a.persistent) {
-    integrate_cell<B, QM>(a, L, blockIdx.x, lane);
+    const int cell = blockIdx.x * a.cellsPerGroup + wave;
+    if (cell < a.nBatch) integrate_cell<B, QM>(a, L, cell, lane);
     return;
   }
   for (;;) {
@@ -1105,23 +1193,32 @@ __global__ __launch_bounds__(64) void jacobian_kernel(DevMech M, int nBatch, con
 
 static int qmOf(int m) { return m <= 64 ? 1 : (m <= 128 ? 2 : (m <= 256 ? 4 : 0)); }
 
-template <class B, int QM>
+template <class B, int QM, int kThreads>
 static const void* integratorFn() {
-  return reinterpret_cast<const void*>(&trbdf2_kernel<B, QM>);
+  return reinterpret_cast<const void*>(&trbdf2_kernel<B, QM, kThreads>);
 }
+
+// Largest workgroup of the sparse form: 16 wavefronts (cells) for the mixed
+// form with m <= 128 (register budget 128 VGPRs), 8 otherwise (256 VGPRs).
+int sparseMaxCellsPerGroup(int precision, int m) { return (precision == TCHEM_PRECISION_MIXED && m <= 128) ? 16 : 8; }
 
 // kernel instance for (precision, sparse, species slots per lane)
 static const void* integratorKernel(int precision, int sparse, int qm) {
-#define TCHEM_PICK(BK, TA)                                              \
-  return qm == 1 ? integratorFn<BK<TA, 1>, 1>()                        \
-                 : (qm == 2 ? integratorFn<BK<TA, 2>, 2>() : integratorFn<BK<TA, 4>, 4>())
+#define TCHEM_PICK_D(TA) \
+  return qm == 1 ? integratorFn<DenseBackend<TA, 1>, 1, 64>() \
+                 : (qm == 2 ? integratorFn<DenseBackend<TA, 2>, 2, 64>() : integratorFn<DenseBackend<TA, 4>, 4, 64>())
+#define TCHEM_PICK_S(TA, T12)                                                                 \
+  return qm == 1 ? integratorFn<SparseBackend<TA, 1>, 1, T12>()                               \
+                 : (qm == 2 ? integratorFn<SparseBackend<TA, 2>, 2, T12>()                    \
+                            : integratorFn<SparseBackend<TA, 4>, 4, 512>())
   if (precision == TCHEM_PRECISION_FP64) {
-    if (sparse) TCHEM_PICK(SparseBackend, double);
-    TCHEM_PICK(DenseBackend, double);
+    if (sparse) TCHEM_PICK_S(double, 512);
+    TCHEM_PICK_D(double);
   }
-  if (sparse) TCHEM_PICK(SparseBackend, float);
-  TCHEM_PICK(DenseBackend, float);
-#undef TCHEM_PICK
+  if (sparse) TCHEM_PICK_S(float, 1024);
+  TCHEM_PICK_D(float);
+#undef TCHEM_PICK_D
+#undef TCHEM_PICK_S
 }
 
 hipError_t launchIntegrator(const IntegratorArgs& a, int precision, int grid, size_t lds, hipStream_t stream) {
@@ -1130,13 +1227,22 @@ hipError_t launchIntegrator(const IntegratorArgs& a, int precision, int grid, si
   const void* fn = integratorKernel(precision, a.sparse, qm);
   IntegratorArgs arg = a;
   void* params[] = {&arg};
-  return hipLaunchKernel(fn, dim3(grid), dim3(64), params, lds, stream);
+  return hipLaunchKernel(fn, dim3(grid), dim3(64 * a.cellsPerGroup), params, lds, stream);
 }
 
-hipError_t integratorMaxBlocksPerCU(int precision, int sparse, int m, size_t lds, int* blocks) {
+// static LDS of the integrator instance (profiling builds declare some)
+size_t integratorStaticLds(int precision, int sparse, int m) {
+  const int qm = qmOf(m);
+  if (qm == 0) return 0;
+  hipFuncAttributes attr;
+  if (hipFuncGetAttributes(&attr, integratorKernel(precision, sparse, qm)) != hipSuccess) return 0;
+  return attr.sharedSizeBytes;
+}
+
+hipError_t integratorMaxBlocksPerCU(int precision, int sparse, int m, int threads, size_t lds, int* blocks) {
   const int qm = qmOf(m);
   if (qm == 0) return hipErrorInvalidValue;
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, integratorKernel(precision, sparse, qm), 64, lds);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, integratorKernel(precision, sparse, qm), threads, lds);
 }
 
 hipError_t launchRates(const DevMech& M, int nBatch, const double* state, const double* photo, double* kf,
@@ -1183,10 +1289,10 @@ hipError_t launchJacobian(const DevMech& M, int nBatch, const double* state, con
 
 #ifdef TCHEM_PHASE_TIMERS
 extern "C" int tchem_debug_phase_cycles(unsigned long long* out, int reset) {
-  if (hipMemcpyFromSymbol(out, HIP_SYMBOL(tchem_amd::g_phase_cycles), sizeof(unsigned long long) * 8) != hipSuccess)
+  if (hipMemcpyFromSymbol(out, HIP_SYMBOL(tchem_amd::g_phase_cycles), sizeof(unsigned long long) * 16) != hipSuccess)
     return -1;
   if (reset) {
-    const unsigned long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const unsigned long long z[16] = {};
     if (hipMemcpyToSymbol(HIP_SYMBOL(tchem_amd::g_phase_cycles), z, sizeof(z)) != hipSuccess) return -1;
   }
   return 0;

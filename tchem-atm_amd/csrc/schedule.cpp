@@ -177,28 +177,63 @@ SparseSchedule buildSparseSchedule(const MechTables& t) {
     lev[k] = L + 1;
   }
   s.nLevels = m ? *std::max_element(lev.begin(), lev.end()) : 0;
+  const int T = Z + 1;
+  s.trashSlot = T;
+  if (T >= 0x8000) throw std::runtime_error("sparse schedule: factor pattern too large for 15-bit positions");
+
+  // Packs chains (target, entries) into rounds of 64 lanes with at most
+  // kMaxChain entries per lane; a long chain continues in a later round of
+  // the same group (its partial sum is in the target entry).
+  auto emitChains = [&](std::vector<Chain> chains, int kind, unsigned padEntry) {
+    int emitted = 0;
+    for (auto& c : chains) std::reverse(c.a.begin(), c.a.end());  // pop_back yields the original order
+    while (true) {
+      std::vector<Chain*> live;
+      for (auto& c : chains)
+        if (!c.a.empty()) live.push_back(&c);
+      if (live.empty()) break;
+      std::stable_sort(live.begin(), live.end(),
+                       [](const Chain* x, const Chain* y) { return x->a.size() > y->a.size(); });
+      if (live.size() > size_t(kLanes)) live.resize(kLanes);
+      const int len = std::min<int>(SparseSchedule::kMaxChain, int(live.front()->a.size()));
+      std::vector<std::vector<int>> part(kLanes);
+      for (size_t l = 0; l < live.size(); ++l)
+        for (int j = 0; j < len && !live[l]->a.empty(); ++j) {
+          part[l].push_back(live[l]->a.back());
+          live[l]->a.pop_back();
+        }
+      s.progHdr.push_back(unsigned(len) | (unsigned(kind) << 16));
+      for (int l = 0; l < kLanes; ++l)
+        s.progBody.push_back(size_t(l) < live.size() ? unsigned(live[l]->target) : unsigned(T));
+      for (int j = 0; j < len; ++j)
+        for (int l = 0; l < kLanes; ++l)
+          s.progBody.push_back(size_t(j) < part[l].size() ? unsigned(part[l][size_t(j)]) : padEntry);
+      ++emitted;
+    }
+    return emitted;
+  };
+
+  const unsigned zz = unsigned(Z) | (unsigned(Z) << 16);
+  s.luBody = 0;
   for (int L = 1; L <= s.nLevels; ++L) {
-    std::vector<std::pair<int, int>> recs;
+    // multiplier and pivot records
+    std::vector<unsigned> recs;
     for (int k = 0; k < m; ++k) {
       if (lev[k] != L) continue;
-      recs.push_back({-(k + 1), P(k, k)});
+      recs.push_back((0x8000u | unsigned(k)) | (unsigned(P(k, k)) << 16));
       for (int i = k + 1; i < m; ++i)
         if (Fz(i, k)) {
-          recs.push_back({P(i, k), P(k, k)});
+          recs.push_back(unsigned(P(i, k)) | (unsigned(P(k, k)) << 16));
           ++s.nMultipliers;
         }
     }
-    const int mb = int(s.mulRec.size() / (2 * kLanes));
-    for (size_t base = 0; base < recs.size(); base += kLanes)
-      for (int l = 0; l < kLanes; ++l) {
-        const bool live = base + l < recs.size();
-        s.mulRec.push_back(live ? recs[base + l].first : 0);
-        s.mulRec.push_back(live ? recs[base + l].second : -1);
-      }
-    s.levMul.push_back(mb);
-    s.levMul.push_back(int(s.mulRec.size() / (2 * kLanes)));
-
-    // updates grouped by target, contributions in pivot order
+    for (size_t base = 0; base < recs.size(); base += kLanes) {
+      s.progHdr.push_back(unsigned(SparseSchedule::kMul) << 16);
+      for (int l = 0; l < kLanes; ++l)
+        s.progBody.push_back(base + l < recs.size() ? recs[base + l] : (unsigned(T) | (unsigned(Z) << 16)));
+      ++s.luRounds;
+    }
+    // updates grouped by target, contributions in pivot order; the RHS rides along as column m
     std::map<int, Chain> byTarget;
     for (int k = 0; k < m; ++k) {
       if (lev[k] != L) continue;
@@ -211,47 +246,19 @@ SparseSchedule buildSparseSchedule(const MechTables& t) {
           c.a.push_back(pack16(P(i, k), P(k, j)));
           ++s.nUpdates;
         }
-        Chain& c = byTarget[rhs0 + i];  // the right-hand side rides along as column m
+        Chain& c = byTarget[rhs0 + i];
         c.target = rhs0 + i;
         c.a.push_back(pack16(P(i, k), rhs0 + k));
       }
     }
     std::vector<Chain> ch;
     for (auto& e : byTarget) ch.push_back(e.second);
-    const int ub = rounds(s.updRound);
-    layRounds(ch, s.updRound, s.updTarget, s.updPair, nullptr, pack16(Z, Z));
-    s.levUpd.push_back(ub);
-    s.levUpd.push_back(rounds(s.updRound));
+    s.luRounds += emitChains(ch, SparseSchedule::kUpd, zz);
   }
 
-  // ---- forward substitution levels (unit lower factor)
-  std::vector<int> fl(m, 0);
-  for (int i = 0; i < m; ++i) {
-    int L = 0;
-    for (int k = 0; k < i; ++k)
-      if (Fz(i, k)) L = std::max(L, fl[k]);
-    fl[i] = L + 1;
-  }
-  s.nFwdLevels = 0;
-  for (int i = 0; i < m; ++i)
-    if (fl[i] > 1) s.nFwdLevels = std::max(s.nFwdLevels, fl[i] - 1);
-  for (int L = 2; L <= s.nFwdLevels + 1; ++L) {  // rows of level 1 have nothing to subtract
-    std::vector<Chain> ch;
-    for (int i = 0; i < m; ++i) {
-      if (fl[i] != L) continue;
-      Chain c;
-      c.target = rhs0 + i;
-      for (int k = 0; k < i; ++k)
-        if (Fz(i, k)) c.a.push_back(pack16(P(i, k), rhs0 + k));
-      ch.push_back(c);
-    }
-    const int b = rounds(s.fwdRound);
-    layRounds(ch, s.fwdRound, s.fwdTarget, s.fwdPair, nullptr, pack16(Z, Z));
-    s.fwdLev.push_back(b);
-    s.fwdLev.push_back(rounds(s.fwdRound));
-  }
-
-  // ---- back substitution levels
+  // ---- back substitution, column oriented: when column k is final
+  // (x_k = b_k * rdiag_k), every row i < k with U_ik != 0 gets
+  // b_i -= U_ik * x_k.  Columns of one level touch disjoint rows of later levels.
   std::vector<int> bl(m, 0);
   for (int i = m - 1; i >= 0; --i) {
     int L = 0;
@@ -260,21 +267,45 @@ SparseSchedule buildSparseSchedule(const MechTables& t) {
     bl[i] = L + 1;
   }
   s.nBackLevels = m ? *std::max_element(bl.begin(), bl.end()) : 0;
+  s.backBody = int(s.progBody.size());
+  const unsigned zcol = unsigned(Z) | (unsigned(m) << 16);  // U = 0, column m: b = 0, rdiag = 0
   for (int L = 1; L <= s.nBackLevels; ++L) {
-    std::vector<Chain> ch;
-    for (int i = 0; i < m; ++i) {
-      if (bl[i] != L) continue;
-      Chain c;
-      c.target = i;
-      for (int j = i + 1; j < m; ++j)
-        if (Fz(i, j)) c.a.push_back(pack16(P(i, j), rhs0 + j));
-      ch.push_back(c);
+    std::map<int, Chain> byTarget;
+    for (int k = 0; k < m; ++k) {
+      if (bl[k] != L) continue;
+      for (int i = 0; i < k; ++i)
+        if (Fz(i, k)) {
+          Chain& c = byTarget[rhs0 + i];
+          c.target = rhs0 + i;
+          c.a.push_back(pack16(P(i, k), k));
+        }
     }
-    const int b = rounds(s.backRound);
-    layRounds(ch, s.backRound, s.backTarget, s.backPair, nullptr, pack16(Z, Z));
-    s.backLev.push_back(b);
-    s.backLev.push_back(rounds(s.backRound));
+    std::vector<Chain> ch;
+    for (auto& e : byTarget) ch.push_back(e.second);
+    s.backRounds += emitChains(ch, SparseSchedule::kBack, zcol);
   }
+
+  // ---- forward substitution with stored factors (unit lower), column oriented by LU level
+  s.fwdBody = int(s.progBody.size());
+  for (int L = 1; L <= s.nLevels; ++L) {
+    std::map<int, Chain> byTarget;
+    for (int k = 0; k < m; ++k) {
+      if (lev[k] != L) continue;
+      for (int i = k + 1; i < m; ++i)
+        if (Fz(i, k)) {
+          Chain& c = byTarget[rhs0 + i];
+          c.target = rhs0 + i;
+          c.a.push_back(pack16(P(i, k), rhs0 + k));
+        }
+    }
+    std::vector<Chain> ch;
+    for (auto& e : byTarget) ch.push_back(e.second);
+    s.fwdRounds += emitChains(ch, SparseSchedule::kUpd, zz);
+  }
+  s.nFwdLevels = s.nLevels;
+  // slack after the last round: the interpreter reads the next round's words
+  // (up to 1 + kMaxChain slots) before it knows there is none
+  s.progBody.insert(s.progBody.end(), size_t(kLanes) * (1 + SparseSchedule::kMaxChain), 0u);
   return s;
 }
 

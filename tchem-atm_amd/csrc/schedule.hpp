@@ -65,22 +65,33 @@ struct SparseSchedule {
   std::vector<int> jacRound, jacTarget, jacPair;
   std::vector<double> jacCoef;
 
-  // factorisation, per level: multiplier rounds, then update rounds.
-  // multiplier records, int2 per (round, lane) = (dst, src):
-  //   dst >= 0: A[dst] *= 1 / A[src]; dst < 0: pivot record, rdiag[-dst-1] = 1 / A[src]; src = -1: none
-  std::vector<int> levMul, levUpd;  // int2 (first round, end round) per level
-  std::vector<int> mulRec;
-  // updates: target position (matrix entry or RHS entry), entries packed (l | u << 16): target -= A[l] * A[u]
-  std::vector<int> updRound, updTarget, updPair;
-
-  // forward substitution with stored factors (RHS target rhs0 + i, entries (L_ik | b_k << 16))
-  std::vector<int> fwdLev, fwdRound, fwdTarget, fwdPair;
-  // back substitution: target row i, entries (U_ij | x_j << 16); x_i = (b_i - sum) * rdiag_i
-  std::vector<int> backLev, backRound, backTarget, backPair;
+  // ---- LU / substitution program (copied into LDS, shared by the cells of
+  // a workgroup).  A sequence of rounds; header word per round = len | kind << 16:
+  //   kMul: 64 words (dst | src << 16): dst < 0x8000: A[dst] *= 1 / A[src];
+  //         dst = 0x8000 | k: pivot record, rdiag[k] = 1 / A[src] (zero or
+  //         non-finite pivot: factorisation fails);
+  //   kUpd: 64 target words, then len x 64 entry words (lo | hi << 16):
+  //         A[target] -= sum A[lo] * A[hi];
+  //   kBack: as kUpd with entries (U position | column k << 16):
+  //         A[target] -= sum A[lo] * (A[rhs0 + k] * rdiag[k]).
+  // Entry words of one lane are at [(round body + 64 * (1 + j)) + lane].
+  // Padding is branch free: idle lanes name the trash slot (zeroSlot + 1)
+  // as target (multiplier padding: dst = trash, src = zero slot) and idle
+  // chain entries read the zero slot (column m for kBack, whose RHS entry and
+  // reciprocal pivot are zero).  Sections, in order: factorisation with the RHS
+  // eliminated alongside (LU), column-oriented back substitution leaving
+  // b_i = u_ii x_i (BACK: x_i = b_i * rdiag_i afterwards), and forward
+  // substitution with stored factors (FWD, kUpd rounds).
+  enum : int { kMul = 0, kUpd = 1, kBack = 2 };
+  static constexpr int kMaxChain = 4;  // entries per lane per round
+  int trashSlot = 0;                   // zeroSlot + 1
+  std::vector<unsigned> progHdr, progBody;  // progBody ends with (1 + kMaxChain) x 64 words of slack
+  int luRounds = 0, backRounds = 0, fwdRounds = 0;  // rounds per section
+  int luBody = 0, backBody = 0, fwdBody = 0;        // body word offset of each section
 };
 
 // Builds the schedules; requires at most three reactants per reaction and
-// nnzF + m + 1 <= 65535.  Throws std::runtime_error otherwise.
+// nnzF + m + 2 < 32768.  Throws std::runtime_error otherwise.
 SparseSchedule buildSparseSchedule(const MechTables& t);
 
 }  // namespace tchem_amd

@@ -269,11 +269,24 @@ def test_invalid_arguments_are_rejected(cb05, gpu, pkg):
     assert lib.tchem_atmchem_run_device_batch(opts, 1, None, None, None, None, None, None, None, None, None, None,
                                               kmcd._h, None) == -1
     assert "lu" in lib.tchem_last_error().decode()
-    opts = pkg._Options(0, 0, 0, 0, (0, 1, 0, 0))
+    opts = pkg._Options()
+    opts.reserved[1] = 1
     assert lib.tchem_atmchem_run_device_batch(opts, 1, None, None, None, None, None, None, None, None, None, None,
                                               kmcd._h, None) == -1
     assert "reserved" in lib.tchem_last_error().decode()
-    # workspace of each form; the sparse form needs far less LDS than the dense one
+    opts = pkg._Options(lu=pkg.LU_DENSE, cells_per_block=2)
+    assert lib.tchem_atmchem_run_device_batch(opts, 1, None, None, None, None, None, None, None, None, None, None,
+                                              kmcd._h, None) == -1
+    assert "one cell per block" in lib.tchem_last_error().decode()
+    opts = pkg._Options(lu=pkg.LU_SPARSE, cells_per_block=64)
+    assert lib.tchem_atmchem_run_device_batch(opts, 1, None, None, None, None, None, None, None, None, None, None,
+                                              kmcd._h, None) == -1
+    assert "cells_per_block" in lib.tchem_last_error().decode()
+    # workspace of each form: one cell of the dense form; the sparse form's
+    # shared program plus its cells
     dense = pkg.AtmosphericChemistry.getWorkSpaceSize(kmcd, pkg.PRECISION_MIXED, pkg.LU_DENSE)
-    sparse = pkg.AtmosphericChemistry.getWorkSpaceSize(kmcd, pkg.PRECISION_MIXED, pkg.LU_SPARSE)
-    assert sparse < dense / 2
+    one = pkg.AtmosphericChemistry.getWorkSpaceSize(kmcd, pkg.PRECISION_MIXED, pkg.LU_SPARSE, 1)
+    two = pkg.AtmosphericChemistry.getWorkSpaceSize(kmcd, pkg.PRECISION_MIXED, pkg.LU_SPARSE, 2)
+    full = pkg.AtmosphericChemistry.getWorkSpaceSize(kmcd, pkg.PRECISION_MIXED, pkg.LU_SPARSE)
+    cell = two - one
+    assert cell < dense / 2 and full <= 160 * 1024 and full + cell > 160 * 1024
