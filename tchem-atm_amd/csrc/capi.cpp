@@ -290,23 +290,22 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
     sp.nnzF = sc.nnzF;
     sp.zeroSlot = sc.zeroSlot;
     sp.trashSlot = sc.trashSlot;
-    sp.ropRounds = sc.ropRounds;
-    sp.segRounds = sc.segRounds;
-    sp.pairRounds = sc.pairRounds;
     sp.pairSlots = sc.pairSlots;
+    auto words = [](const std::vector<unsigned>& v) { return std::vector<int>(v.begin(), v.end()); };
+    sp.oEvalHdr = addI(words(sc.evalHdr));
+    sp.oEvalBody = addI(words(sc.evalBody));
+    sp.evalBodyWords = int(sc.evalBody.size());
+    sp.oCoef = addD(sc.coefTable);
+    sp.nCoef = int(sc.coefTable.size());
+    sp.ropRounds = sc.ropRounds;
+    sp.pairRounds = sc.pairRounds;
+    sp.segRounds = sc.segRounds;
     sp.jacRounds = sc.jacRounds;
-    sp.oRopRec = addI(sc.ropRec);
-    sp.oSegRound = addI(sc.segRound);
-    sp.oSegTarget = addI(sc.segTarget);
-    sp.oSegReac = addI(sc.segReac);
-    sp.oPairRec = addI(sc.pairRec);
-    sp.oJacRound = addI(sc.jacRound);
-    sp.oJacTarget = addI(sc.jacTarget);
-    sp.oJacPair = addI(sc.jacPair);
-    sp.oSegCoef = addD(sc.segCoef);
-    sp.oJacCoef = addD(sc.jacCoef);
-    sp.oProgHdr = addI(std::vector<int>(sc.progHdr.begin(), sc.progHdr.end()));
-    sp.oProgBody = addI(std::vector<int>(sc.progBody.begin(), sc.progBody.end()));
+    sp.pairBody = sc.pairBody;
+    sp.segBody = sc.segBody;
+    sp.jacBody = sc.jacBody;
+    sp.oProgHdr = addI(words(sc.progHdr));
+    sp.oProgBody = addI(words(sc.progBody));
     sp.luRounds = sc.luRounds;
     sp.backRounds = sc.backRounds;
     sp.fwdRounds = sc.fwdRounds;

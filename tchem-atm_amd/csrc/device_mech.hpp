@@ -42,31 +42,40 @@ struct DevMech {
   __host__ __device__ const double* jcCoef() const { return db + oJcCoef; }      // [nContrib]
 };
 
-// Device view of the compiled sparse schedule (schedule.hpp); arrays live in
-// the mechanism's int / double buffers, addressed by offsets.
+// Device view of the compiled sparse programs (schedule.hpp); arrays live
+// in the mechanism's int / double buffers, addressed by offsets.
 struct DevSparse {
-  int enabled;  // 0: mechanism has no sparse schedule
-  int nnzF, zeroSlot, trashSlot;
-  int ropRounds, segRounds, pairRounds, pairSlots, jacRounds;
-  int oRopRec, oSegRound, oSegTarget, oSegReac, oPairRec, oJacRound, oJacTarget, oJacPair;
-  int oSegCoef, oJacCoef;  // double buffer
-  // LU / substitution program: round headers and bodies; the LU and BACK
-  // sections are staged into LDS, FWD (factor reuse) is read from memory
+  int enabled;  // 0: mechanism has no sparse programs
+  int nnzF, zeroSlot, trashSlot, pairSlots;
+  // evaluation program (staged into LDS) and its coefficient table
+  int oEvalHdr, oEvalBody, evalBodyWords, oCoef, nCoef;
+  int ropRounds, pairRounds, segRounds, jacRounds;
+  int pairBody, segBody, jacBody;
+  // LU program: the LU and BACK sections are staged into LDS, FWD (factor
+  // reuse) is read from memory
   int oProgHdr, oProgBody;
   int luRounds, backRounds, fwdRounds;
-  int backBody, fwdBody;  // body word offsets of the BACK and FWD sections
+  int backBody, fwdBody;
 };
 
-// LDS layout of a sparse-form workgroup: the staged program (headers of the
-// LU + BACK rounds, then their bodies), then one cell region per wavefront.
-__host__ __device__ inline size_t sparseProgramHeaderWords(const DevSparse& S) {
-  return (size_t(S.luRounds + S.backRounds) + 3) & ~size_t(3);
+// LDS layout of a sparse-form workgroup: LU program headers and staged
+// bodies, evaluation program headers and bodies, coefficient table, then one
+// cell region per wavefront.  Word counts are rounded up to 4 (16 bytes).
+__host__ __device__ inline size_t al4(size_t words) { return (words + 3) & ~size_t(3); }
+__host__ __device__ inline size_t sparseLuHdrWords(const DevSparse& S) { return al4(size_t(S.luRounds + S.backRounds)); }
+// LU and BACK bodies plus the slack the interpreter may read past the last round
+__host__ __device__ inline size_t sparseLuBodyWords(const DevSparse& S) { return al4(size_t(S.fwdBody) + 64 * 5); }
+__host__ __device__ inline size_t sparseEvalHdrWords(const DevSparse& S) {
+  return al4(size_t(S.ropRounds + S.pairRounds + S.segRounds + S.jacRounds));
 }
-// body words staged: the LU and BACK sections plus the slack the
-// interpreter may read past the last round (1 + 4 slots of 64 words)
-__host__ __device__ inline size_t sparseStagedBodyWords(const DevSparse& S) { return size_t(S.fwdBody) + 64 * 5; }
+__host__ __device__ inline size_t sparseEvalOffsetWords(const DevSparse& S) {
+  return sparseLuHdrWords(S) + sparseLuBodyWords(S);
+}
+__host__ __device__ inline size_t sparseCoefOffsetWords(const DevSparse& S) {
+  return sparseEvalOffsetWords(S) + sparseEvalHdrWords(S) + al4(size_t(S.evalBodyWords));
+}
 __host__ __device__ inline size_t sparseProgramBytes(const DevSparse& S) {
-  return 4 * (sparseProgramHeaderWords(S) + ((sparseStagedBodyWords(S) + 3) & ~size_t(3)));
+  return 4 * sparseCoefOffsetWords(S) + ((8 * size_t(S.nCoef) + 15) & ~size_t(15));
 }
 
 struct IntegratorArgs {
@@ -106,14 +115,13 @@ inline size_t integratorLdsBytes(const DevMech& M, size_t bytesA) {
 
 // LDS bytes of one cell region of the sparse form: factor entries + RHS +
 // zero and trash slots, reciprocal pivots (+ a zero for column m and a
-// trash slot) and the
-// Jacobian derivative pairs in TA precision, then doubles for
-// concentrations, rate constants (+ a zero), rates of progress (+ zero
-// slot) and source-term segment sums.
+// trash slot) and the Jacobian derivative pairs in TA precision, then
+// doubles for concentrations, rate constants (+ a zero), rates of progress
+// (+ zero slot) and source-term segment sums (+ trash slot).
 inline size_t sparseCellBytes(const DevMech& M, const DevSparse& S, size_t bytesA) {
   auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
   return al(bytesA * size_t(S.trashSlot + 1)) + al(bytesA * size_t(M.m + 2)) + al(bytesA * size_t(S.pairSlots)) +
-         sizeof(double) * (size_t(M.nSpec) + size_t(M.nReac + 1) + size_t(M.nReac + 1) + size_t(M.nSeg));
+         al(sizeof(double) * (size_t(M.nSpec) + size_t(M.nReac + 1) + size_t(M.nReac + 1) + size_t(M.nSeg + 1)));
 }
 
 }  // namespace tchem_amd

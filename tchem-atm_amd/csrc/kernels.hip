@@ -477,24 +477,31 @@ __device__ void lu_back(const CellLds<TA>& L, int m, int lane, TA (&b)[QM]) {
 
 template <typename TA>
 struct SpLds {
-  const unsigned* hdr;   // staged program: round headers (LU + BACK), shared by the workgroup
-  const unsigned* body;  // staged program: round bodies (LU + BACK)
+  const unsigned* hdr;    // LU program headers (LU + BACK sections), shared by the workgroup
+  const unsigned* body;   // LU program bodies
+  const unsigned* ehdr;   // evaluation program headers
+  const unsigned* ebody;  // evaluation program bodies
+  const double* coef;     // stoichiometric coefficient table
   TA* A;        // [trashSlot+1] factor entries, RHS, zero slot, trash slot
   TA* rdiag;    // [m+2] reciprocal pivots, rdiag[m] = 0, rdiag[m+1] = trash
   TA* dpr;      // [pairSlots] Jacobian derivative pairs, dpr[nPairs] = 0
   double* c;    // [nSpec]
   double* kf;   // [nReac+1], kf[nReac] = 0
   double* rop;  // [nReac+1], rop[nReac] = 0
-  double* seg;  // [nSeg]
+  double* seg;  // [nSeg+1], seg[nSeg] = trash
 };
 
-// program at the start of the workgroup's LDS, then one cell region per wavefront
+// programs at the start of the workgroup's LDS, then one cell region per wavefront
 template <typename TA>
 __device__ SpLds<TA> carve_sparse(char* base, const DevMech& M, const DevSparse& S, int cellBytes, int wave) {
   auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
   SpLds<TA> L;
-  L.hdr = reinterpret_cast<const unsigned*>(base);
-  L.body = L.hdr + sparseProgramHeaderWords(S);
+  const unsigned* w = reinterpret_cast<const unsigned*>(base);
+  L.hdr = w;
+  L.body = w + sparseLuHdrWords(S);
+  L.ehdr = w + sparseEvalOffsetWords(S);
+  L.ebody = L.ehdr + sparseEvalHdrWords(S);
+  L.coef = reinterpret_cast<const double*>(w + sparseCoefOffsetWords(S));
   char* cell = base + sparseProgramBytes(S) + size_t(wave) * size_t(cellBytes);
   size_t off = 0;
   L.A = reinterpret_cast<TA*>(cell);
@@ -510,88 +517,129 @@ __device__ SpLds<TA> carve_sparse(char* base, const DevMech& M, const DevSparse&
   return L;
 }
 
+// x^n for small n >= 0 by repeated multiplication (exact for n <= 2; at
+// most n-1 roundings above)
 __device__ __forceinline__ double ipow(double x, int n) {
   double r = n >= 1 ? x : 1.0;
   for (int i = 1; i < n; ++i) r *= x;
   return r;
 }
-// packed reactant field: species | power << 16, 0xffff for none
-__device__ __forceinline__ double reactant_factor(const double* c, int e) {
-  return e == 0xffff ? 1.0 : ipow(c[e & 0xffff], e >> 16);
-}
-__device__ __forceinline__ int2 uniform(int2 v) {
-  return make_int2(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y));
+// packed reactant field: species | power << 12, kNone for no reactant
+__device__ __forceinline__ double reactant_factor(const double* c, unsigned f) {
+  return f == SparseSchedule::kNone ? 1.0 : ipow(c[f & 0xfff], int(f >> 12));
 }
 
-// rates of progress (and, with pairs, the derivatives d rop_r / d c_s) at
-// the concentrations in LDS; same products, same order as the dense form
+// Rates of progress (kRop rounds) and, with pairs, the derivatives
+// d rop_r / d c_s (kPair rounds) at the concentrations in LDS: the same
+// products in the same order as the dense form.  Rounds are independent;
+// the next round's two words are loaded while a round computes.
 template <typename TA>
-__device__ void sp_rates(const DevMech& M, const DevSparse& S, const SpLds<TA>& L, int lane, bool pairs) {
-  // padding records name reaction nReac (kf = 0, no reactants) and write 0
-  // into the zero slots
-  const int4* rr = reinterpret_cast<const int4*>(M.ib + S.oRopRec);
-  for (int g = 0; g < S.ropRounds; ++g) {
-    const int4 e = rr[g * kWave + lane];
-    L.rop[e.x] = L.kf[e.x] * reactant_factor(L.c, e.y) * reactant_factor(L.c, e.z) * reactant_factor(L.c, e.w);
+__device__ void sp_rates(const DevSparse& S, const SpLds<TA>& L, int lane, bool pairs) {
+  {
+    const unsigned* b = L.ebody;
+    unsigned w0 = b[lane], w1 = b[kWave + lane];
+    for (int g = 0; g < S.ropRounds; ++g) {
+      b += 2 * kWave;
+      const unsigned n0 = b[lane], n1 = b[kWave + lane];
+      const int r = int(w0 & 0xffff);
+      L.rop[r] = L.kf[r] * reactant_factor(L.c, w0 >> 16) * reactant_factor(L.c, w1 & 0xffff) *
+                 reactant_factor(L.c, w1 >> 16);
+      w0 = n0;
+      w1 = n1;
+    }
   }
   if (!pairs) return;
-  const int4* pr = reinterpret_cast<const int4*>(M.ib + S.oPairRec);
+  const unsigned* b = L.ebody + S.pairBody;
+  unsigned w0 = b[lane], w1 = b[kWave + lane];
   for (int g = 0; g < S.pairRounds; ++g) {
-    const int4 e = pr[g * kWave + lane];
-    const int r = e.x & 0xffffff, js = e.x >> 24;
-    const int es = js == 0 ? e.y : (js == 1 ? e.z : e.w);
-    const int ns = es >> 16;
-    double d = L.kf[r] * double(ns) * ipow(L.c[es & 0xffff], ns - 1);
-    if (js != 0) d *= reactant_factor(L.c, e.y);
-    if (js != 1) d *= reactant_factor(L.c, e.z);
-    if (js != 2) d *= reactant_factor(L.c, e.w);
-    L.dpr[g * kWave + lane] = d;
+    b += 2 * kWave;
+    const unsigned n0 = b[lane], n1 = b[kWave + lane];
+    const int r = int(w0 & 0x3fff), js = int((w0 >> 14) & 3);
+    const unsigned f0 = w0 >> 16, f1 = w1 & 0xffff, f2 = w1 >> 16;
+    const unsigned fs = js == 0 ? f0 : (js == 1 ? f1 : f2);
+    const int ns = int(fs >> 12);
+    double d = L.kf[r] * double(ns) * ipow(L.c[fs & 0xfff], ns - 1);
+    if (js != 0) d *= reactant_factor(L.c, f0);
+    if (js != 1) d *= reactant_factor(L.c, f1);
+    if (js != 2) d *= reactant_factor(L.c, f2);
+    L.dpr[g * kWave + lane] = TA(d);
+    w0 = n0;
+    w1 = n1;
   }
 }
 
-// source-term segment sums and, with jac, A = diag * I + scale * J on the
-// factor pattern (fill positions cleared)
-template <typename TA>
-__device__ void sp_sums(const DevMech& M, const DevSparse& S, const SpLds<TA>& L, int lane, bool jac, double diag,
-                        double scale) {
-  {
-    const int2* rd = reinterpret_cast<const int2*>(M.ib + S.oSegRound);
-    const int* tg = M.ib + S.oSegTarget;
-    const int* er = M.ib + S.oSegReac;
-    const double* ec = M.db + S.oSegCoef;
-    for (int g = 0; g < S.segRounds; ++g) {
-      const int2 r = uniform(rd[g]);
-      const int t = tg[g * kWave + lane];
-      if (t >= 0) {
-        double acc = 0.0;
-#pragma unroll 4
-        for (int k = 0; k < r.y; ++k) {
-          const int e = (r.x + k) * kWave + lane;
-          acc += ec[e] * L.rop[er[e]];
-        }
-        L.seg[t] = acc;
-      }
-    }
+// one segment-sum round, LEN entries per lane (w[0] = target, w[1..] = (reaction | coef << 16))
+template <int LEN, typename TA>
+__device__ __forceinline__ void seg_round(const unsigned (&w)[1 + SparseSchedule::kMaxChain], const SpLds<TA>& L) {
+  const unsigned t = w[0] & ~SparseSchedule::kCont;
+  double acc = (w[0] & SparseSchedule::kCont) ? L.seg[t] : 0.0;
+  double cf[LEN], rv[LEN];
+#pragma unroll
+  for (int j = 0; j < LEN; ++j) {
+    cf[j] = L.coef[w[1 + j] >> 16];
+    rv[j] = L.rop[w[1 + j] & 0xffff];
   }
-  if (!jac) return;
-  const int2* rd = reinterpret_cast<const int2*>(M.ib + S.oJacRound);
-  const int* tg = M.ib + S.oJacTarget;
-  const int* ep = M.ib + S.oJacPair;
-  const double* ec = M.db + S.oJacCoef;
-  // accumulated in TA precision: FP32 assembly for the mixed form
+#pragma unroll
+  for (int j = 0; j < LEN; ++j) acc += cf[j] * rv[j];
+  L.seg[t] = acc;
+}
+// one Jacobian round: A[t] = (cont ? A[t] : diag) + scale * sum coef * dpr, in TA precision
+template <int LEN, typename TA>
+__device__ __forceinline__ void jac_round(const unsigned (&w)[1 + SparseSchedule::kMaxChain], const SpLds<TA>& L,
+                                          TA tdiag, TA tscale) {
+  const unsigned t = w[0] & 0x3fffffffu;
+  const TA init = (w[0] & SparseSchedule::kCont) ? L.A[t] : (((w[0] >> 30) & 1u) ? tdiag : TA(0));
+  TA cf[LEN], dv[LEN];
+#pragma unroll
+  for (int j = 0; j < LEN; ++j) {
+    cf[j] = TA(L.coef[w[1 + j] >> 16]);
+    dv[j] = L.dpr[w[1 + j] & 0xffff];
+  }
+  TA sum = TA(0);
+#pragma unroll
+  for (int j = 0; j < LEN; ++j) sum += cf[j] * dv[j];
+  L.A[t] = init + tscale * sum;
+}
+
+// Source-term segment sums (kSeg rounds) and, with jac, A = diag * I +
+// scale * J on the factor pattern (kJac rounds; fill positions cleared).
+template <typename TA>
+__device__ void sp_sums(const DevSparse& S, const SpLds<TA>& L, int lane, bool jac, double diag, double scale) {
+  constexpr int kW = 1 + SparseSchedule::kMaxChain;
+  const int r0 = S.ropRounds + S.pairRounds;
+  const int nr = S.segRounds + (jac ? S.jacRounds : 0);
+  const unsigned* hdr = L.ehdr + r0;
+  const unsigned* b = L.ebody + S.segBody;
   const TA tdiag = TA(diag), tscale = TA(scale);
-  for (int g = 0; g < S.jacRounds; ++g) {
-    const int2 r = uniform(rd[g]);
-    const int t = tg[g * kWave + lane];
-    if (t >= 0) {
-      TA s = TA(0);
-#pragma unroll 4
-      for (int k = 0; k < r.y; ++k) {
-        const int e = (r.x + k) * kWave + lane;
-        s += TA(ec[e]) * L.dpr[ep[e]];
+  unsigned w[kW];
+#pragma unroll
+  for (int j = 0; j < kW; ++j) w[j] = b[kWave * j + lane];
+  unsigned h = nr > 0 ? __builtin_amdgcn_readfirstlane(hdr[0]) : 0u;
+  for (int r = 0; r < nr; ++r) {
+    const unsigned hn = r + 1 < nr ? hdr[r + 1] : 0u;
+    const int kind = int(h >> 16), len = int(h & 0xffff);
+    b += kWave * (1 + len);
+    unsigned wn[kW];
+#pragma unroll
+    for (int j = 0; j < kW; ++j) wn[j] = b[kWave * j + lane];
+    if (kind == SparseSchedule::kSeg) {
+      switch (len) {
+        case 1: seg_round<1>(w, L); break;
+        case 2: seg_round<2>(w, L); break;
+        case 3: seg_round<3>(w, L); break;
+        default: seg_round<4>(w, L); break;
       }
-      L.A[t & 0x3fffffff] = ((t >> 30) ? tdiag : TA(0)) + tscale * s;
+    } else {
+      switch (len) {
+        case 1: jac_round<1>(w, L, tdiag, tscale); break;
+        case 2: jac_round<2>(w, L, tdiag, tscale); break;
+        case 3: jac_round<3>(w, L, tdiag, tscale); break;
+        default: jac_round<4>(w, L, tdiag, tscale); break;
+      }
     }
+    h = __builtin_amdgcn_readfirstlane(hn);
+#pragma unroll
+    for (int j = 0; j < kW; ++j) w[j] = wn[j];
   }
 }
 
@@ -754,12 +802,16 @@ struct SparseBackend {
   static __device__ void stage(const IntegratorArgs& a, char* base) {
     const DevSparse& S = a.S;
     unsigned* dst = reinterpret_cast<unsigned*>(base);
-    const unsigned* hsrc = reinterpret_cast<const unsigned*>(a.M.ib + S.oProgHdr);
-    const unsigned* bsrc = reinterpret_cast<const unsigned*>(a.M.ib + S.oProgBody);
-    const int nh = S.luRounds + S.backRounds, hw = int(sparseProgramHeaderWords(S));
-    for (int i = threadIdx.x; i < nh; i += blockDim.x) dst[i] = hsrc[i];
-    const int nb = int(sparseStagedBodyWords(S));
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) dst[hw + i] = bsrc[i];
+    auto copy = [&](size_t at, const int* src, int n) {
+      for (int i = threadIdx.x; i < n; i += blockDim.x) dst[at + size_t(i)] = unsigned(src[i]);
+    };
+    copy(0, a.M.ib + S.oProgHdr, S.luRounds + S.backRounds);
+    copy(sparseLuHdrWords(S), a.M.ib + S.oProgBody, S.fwdBody + 64 * SparseSchedule::kSlack);
+    const size_t e0 = sparseEvalOffsetWords(S);
+    copy(e0, a.M.ib + S.oEvalHdr, S.ropRounds + S.pairRounds + S.segRounds + S.jacRounds);
+    copy(e0 + sparseEvalHdrWords(S), a.M.ib + S.oEvalBody, S.evalBodyWords);
+    double* cd = reinterpret_cast<double*>(dst + sparseCoefOffsetWords(S));
+    for (int i = threadIdx.x; i < S.nCoef; i += blockDim.x) cd[i] = a.M.db[S.oCoef + i];
     __syncthreads();
   }
   static __device__ Lds carve(char* base, const IntegratorArgs& a, int wave) {
@@ -789,11 +841,11 @@ struct SparseBackend {
                                   TA (&x)[QM]) {
     TCHEM_PT_BEGIN(rates)
     put_c(a, L, lane, u);
-    sp_rates(a.M, a.S, L, lane, refresh);
+    sp_rates(a.S, L, lane, refresh);
     wave_sync();
     TCHEM_PT_END(rates, 11)
     TCHEM_PT_BEGIN(sums)
-    sp_sums(a.M, a.S, L, lane, refresh, 1.0, -ch);
+    sp_sums(a.S, L, lane, refresh, 1.0, -ch);
     wave_sync();
     TCHEM_PT_END(sums, 12)
     TCHEM_PT_BEGIN(rows)
@@ -1198,9 +1250,9 @@ static const void* integratorFn() {
   return reinterpret_cast<const void*>(&trbdf2_kernel<B, QM, kThreads>);
 }
 
-// Largest workgroup of the sparse form: 16 wavefronts (cells) for the mixed
-// form with m <= 128 (register budget 128 VGPRs), 8 otherwise (256 VGPRs).
-int sparseMaxCellsPerGroup(int precision, int m) { return (precision == TCHEM_PRECISION_MIXED && m <= 128) ? 16 : 8; }
+// Largest workgroup of the sparse form: 12 wavefronts (cells) for the mixed
+// form with m <= 128 (register budget 168 VGPRs), 8 otherwise (256 VGPRs).
+int sparseMaxCellsPerGroup(int precision, int m) { return (precision == TCHEM_PRECISION_MIXED && m <= 128) ? 12 : 8; }
 
 // kernel instance for (precision, sparse, species slots per lane)
 static const void* integratorKernel(int precision, int sparse, int qm) {
@@ -1215,7 +1267,7 @@ static const void* integratorKernel(int precision, int sparse, int qm) {
     if (sparse) TCHEM_PICK_S(double, 512);
     TCHEM_PICK_D(double);
   }
-  if (sparse) TCHEM_PICK_S(float, 1024);
+  if (sparse) TCHEM_PICK_S(float, 768);
   TCHEM_PICK_D(float);
 #undef TCHEM_PICK_D
 #undef TCHEM_PICK_S

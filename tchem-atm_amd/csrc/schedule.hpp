@@ -1,21 +1,23 @@
-// Compiled per-wavefront schedules for the sparse form of the per-cell
-// linear algebra: rates of progress, source term, Jacobian assembly, LU
-// factorisation of I - c h J in the fill-reducing species order with the
-// pivot sequence fixed at compile time (no row interchanges), and the
-// triangular solves.
+// Compiled per-wavefront programs for the sparse form of the per-cell work:
+// rates of progress, derivative pairs, source-term segment sums, Jacobian
+// assembly (the evaluation program), and the LU factorisation of I - c h J in
+// the fill-reducing species order with the pivot sequence fixed at compile
+// time, back and forward substitution (the LU program).
 //
-// Every phase is a list of "rounds"; in a round each of the 64 lanes of the
-// wavefront executes at most one record (a reaction, a segment, a matrix
-// entry ...).  Records carry every index the lane needs, laid out
-// [slot][lane] so the loads of a round are one coalesced access with no
-// pointer chasing.  Chains of work on one target (a sum over contributions,
-// the updates one LU entry receives within a level) are padded to the
-// longest chain of their round with entries that read a zero slot.
+// A program is a sequence of "rounds"; in a round each of the 64 lanes of
+// the wavefront executes one record.  A round is a header word
+// (len | kind << 16, read by the whole wavefront) and a body of 64-word
+// slots, slot s of lane l at body[64 * s + l], so every load of a round is
+// one conflict-free LDS access.  Both programs are copied into LDS once per
+// workgroup and read by every wavefront (cell) of the workgroup.
 //
-// Matrix storage (LDS, per cell): the nnzF entries of the symbolic LU
-// pattern F (structure of I - J plus fill) in row-major order, then the m
-// right-hand-side entries, then one zero entry.  Positions fit in 16 bits,
-// so a pair of positions packs into one int (lo | hi << 16).
+// Per-cell matrix storage: the nnzF entries of the symbolic LU pattern F
+// (structure of I - J plus fill) in row-major order, the m right-hand-side
+// entries, a zero slot and a trash slot.  Positions fit in 15 bits, so two
+// indices pack into one word (lo | hi << 16).
+//
+// Padding is branch free: idle lanes write trash slots, idle chain entries
+// read zero slots (a zero coefficient, a zero rate, a zero matrix entry).
 #pragma once
 
 #include <vector>
@@ -25,73 +27,55 @@
 namespace tchem_amd {
 
 struct SparseSchedule {
-  int m = 0, nnzF = 0;  // matrix entries; RHS at [nnzF, nnzF+m); zero at nnzF+m
-  int zeroSlot = 0;     // = nnzF + m
-  int nLevels = 0;      // factorisation levels
-  int nFwdLevels = 0;   // forward-substitution levels (factors reused)
-  int nBackLevels = 0;  // back-substitution levels
+  enum : int { kMul = 0, kUpd = 1, kBack = 2, kRop = 3, kPair = 4, kSeg = 5, kJac = 6 };
+  static constexpr int kMaxChain = 4;            // chain entries per lane per round
+  static constexpr int kSlack = 1 + kMaxChain;   // slots read past a section's last round
+  static constexpr unsigned kCont = 0x80000000u;  // target flag: continues a chain split over rounds
+  static constexpr unsigned kNone = 0xffffu;      // packed reactant field: no reactant
 
-  // structure (internal order), for tests and diagnostics
-  std::vector<int> rowPtr, colIdx;  // CSR of F
-  std::vector<int> diagIdx;         // [m] position of (k,k)
+  int m = 0, nnzF = 0;
+  int zeroSlot = 0, trashSlot = 0;  // nnzF + m, nnzF + m + 1
+  int nLevels = 0, nBackLevels = 0;
   int nMultipliers = 0, nUpdates = 0;
+  std::vector<int> rowPtr, colIdx;  // CSR of F (internal order)
+  std::vector<int> diagIdx;         // [m] position of (k,k)
 
-  // rate of progress: one int4 per (round, lane):
-  //   x = reaction (nReac on padding lanes: rate constant 0), y/z/w = species | power << 16 (0xffff: none)
-  int ropRounds = 0;
-  std::vector<int> ropRec;  // [ropRounds*64*4]
+  // ---- evaluation program
+  //   kRop:  2 words: (r | fa << 16), (fb | fc << 16); rop[r] = kf[r] * fa * fb * fc,
+  //          field f = species | power << 12 (kNone: factor 1); padding r = nReac (kf = 0)
+  //   kPair: 2 words: (r | slot << 14 | fa << 16), (fb | fc << 16); derivative of
+  //          rop[r] by the reactant in `slot`, written to dpr[64 * round + lane]
+  //   kSeg:  target (segment | kCont), len entry words (reaction | coef << 16):
+  //          seg[t] = (cont ? seg[t] : 0) + sum coefTable[coef] * rop[reaction]
+  //   kJac:  target (position | diag << 30 | kCont), entries (pair | coef << 16):
+  //          A[t] = (cont ? A[t] : diag) + scale * sum coefTable[coef] * dpr[pair]
+  // Sections in order: rop, pair, seg, jac.  Padding: seg target nSeg (trash),
+  // jac target trashSlot, entries (zero slot | coefficient 0).
+  std::vector<double> coefTable;            // [0] = 0
+  std::vector<unsigned> evalHdr, evalBody;  // evalBody ends with kSlack x 64 words of slack
+  int ropRounds = 0, pairRounds = 0, segRounds = 0, jacRounds = 0;
+  int pairBody = 0, segBody = 0, jacBody = 0;  // body word offsets of the sections
+  int pairSlots = 0;                           // derivative slots of a cell (zero slot nPairs included)
 
-  // chained phases share one layout: per round g an int2 (offset, length)
-  // in units of 64-entry slots, a target per lane ([g*64 + lane], -1 none)
-  // and chain entries at [(offset + k) * 64 + lane].
-
-  // source-term segments: target = segment id; entries (reaction or the
-  // rate-of-progress zero slot nReac, coefficient)
-  int segRounds = 0;
-  std::vector<int> segRound, segTarget, segReac;
-  std::vector<double> segCoef;
-
-  // Jacobian derivative pairs: int4 per (round, lane): x = reaction | slot << 24, y/z/w as ropRec;
-  // lane l of round g writes derivative slot g*64 + l (padding lanes write 0);
-  // pairSlots = derivative slots a cell needs (zero slot nPairs included)
-  int pairRounds = 0, pairSlots = 0;
-  std::vector<int> pairRec;
-
-  // Jacobian entries: target = matrix position | diag << 30 (-1 none); every
-  // position of F is a target (fill positions with an empty chain, so the
-  // assembly also clears them); entries (pair or the derivative zero slot
-  // nPairs, coefficient)
-  int jacRounds = 0;
-  std::vector<int> jacRound, jacTarget, jacPair;
-  std::vector<double> jacCoef;
-
-  // ---- LU / substitution program (copied into LDS, shared by the cells of
-  // a workgroup).  A sequence of rounds; header word per round = len | kind << 16:
-  //   kMul: 64 words (dst | src << 16): dst < 0x8000: A[dst] *= 1 / A[src];
-  //         dst = 0x8000 | k: pivot record, rdiag[k] = 1 / A[src] (zero or
-  //         non-finite pivot: factorisation fails);
-  //   kUpd: 64 target words, then len x 64 entry words (lo | hi << 16):
-  //         A[target] -= sum A[lo] * A[hi];
-  //   kBack: as kUpd with entries (U position | column k << 16):
-  //         A[target] -= sum A[lo] * (A[rhs0 + k] * rdiag[k]).
-  // Entry words of one lane are at [(round body + 64 * (1 + j)) + lane].
-  // Padding is branch free: idle lanes name the trash slot (zeroSlot + 1)
-  // as target (multiplier padding: dst = trash, src = zero slot) and idle
-  // chain entries read the zero slot (column m for kBack, whose RHS entry and
-  // reciprocal pivot are zero).  Sections, in order: factorisation with the RHS
-  // eliminated alongside (LU), column-oriented back substitution leaving
-  // b_i = u_ii x_i (BACK: x_i = b_i * rdiag_i afterwards), and forward
-  // substitution with stored factors (FWD, kUpd rounds).
-  enum : int { kMul = 0, kUpd = 1, kBack = 2 };
-  static constexpr int kMaxChain = 4;  // entries per lane per round
-  int trashSlot = 0;                   // zeroSlot + 1
-  std::vector<unsigned> progHdr, progBody;  // progBody ends with (1 + kMaxChain) x 64 words of slack
-  int luRounds = 0, backRounds = 0, fwdRounds = 0;  // rounds per section
-  int luBody = 0, backBody = 0, fwdBody = 0;        // body word offset of each section
+  // ---- LU program
+  //   kMul:  1 word (dst | src << 16): dst < 0x8000: A[dst] *= 1 / A[src];
+  //          dst = 0x8000 | k: pivot record, rdiag[k] = 1 / A[src] (a zero or
+  //          non-finite pivot fails the factorisation); padding (trash | zero)
+  //   kUpd:  target, len entries (lo | hi << 16): A[t] -= sum A[lo] * A[hi]
+  //   kBack: target, len entries (U position | k << 16):
+  //          A[t] -= sum A[lo] * (A[rhs0 + k] * rdiag[k]); padding (zero | m << 16)
+  // Sections in order: factorisation with the RHS eliminated alongside (LU),
+  // column-oriented back substitution leaving b_i = u_ii x_i (BACK; then
+  // x_i = b_i * rdiag_i), forward substitution with stored factors (FWD).
+  std::vector<unsigned> progHdr, progBody;  // progBody ends with kSlack x 64 words of slack
+  int luRounds = 0, backRounds = 0, fwdRounds = 0;
+  int backBody = 0, fwdBody = 0;
 };
 
-// Builds the schedules; requires at most three reactants per reaction and
-// nnzF + m + 2 < 32768.  Throws std::runtime_error otherwise.
+// Builds both programs.  Requires at most three reactants per reaction,
+// nSpec < 4095, reactant powers < 16, nReac < 16384, at most 65535 distinct
+// stoichiometric coefficients and nnzF + m + 2 < 32768; throws
+// std::runtime_error otherwise.
 SparseSchedule buildSparseSchedule(const MechTables& t);
 
 }  // namespace tchem_amd

@@ -1,0 +1,30 @@
+"""The compiled sparse programs (tchem-atm_amd/csrc/schedule.cpp), checked on
+the CPU by tests/native/schedule_check.cpp: bounds, race freedom of every
+round, padding isolation, and agreement with direct formulas and a
+long-double dense solve.  The checker is built from the library's host
+sources with g++."""
+import os
+import subprocess
+
+import pytest
+
+from conftest import ROOT, golden
+
+CSRC = os.path.join(ROOT, "tchem-atm_amd", "csrc")
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("native") / "schedule_check")
+    srcs = [os.path.join(ROOT, "tests", "native", "schedule_check.cpp")] + [
+        os.path.join(CSRC, f) for f in ("yaml_lite.cpp", "kinetic_model.cpp", "mechanism.cpp", "schedule.cpp")]
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I" + CSRC, *srcs, "-o", exe], check=True)
+    return exe
+
+
+@pytest.mark.parametrize("mech", ["cb05cl_ae5.yaml", "carbonbond05.yaml", "troe.yaml", "photolysis.yaml",
+                                  "emission.yaml"])
+def test_sparse_programs(checker, mech):
+    r = subprocess.run([checker, golden(mech)], capture_output=True, text=True)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout[-3000:]
