@@ -46,6 +46,8 @@ typedef struct {
  * prints a warning for failed cells, src/core/problems/TChem_Impl_AtmosphericChemistry.hpp:213-221). */
 typedef struct {
   int accepted, rejected, newton_failures, newton_iterations, factorizations, failed;
+  unsigned long long t_start, t_end; /* device real-time counter (100 MHz) when the cell started and ended in
+                                        this launch (0 for skipped cells) */
 } tchem_cell_stats_t;
 
 /* Kernel form and launch shape of tchem_atmchem_run_device_batch. */

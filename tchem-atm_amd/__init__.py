@@ -45,7 +45,11 @@ TIME_ADVANCE_DTYPE = np.dtype([
 ])
 assert TIME_ADVANCE_DTYPE.itemsize == 56
 CELL_STATS_DTYPE = np.dtype([("accepted", "i4"), ("rejected", "i4"), ("newton_failures", "i4"),
-                             ("newton_iterations", "i4"), ("factorizations", "i4"), ("failed", "i4")])
+                             ("newton_iterations", "i4"), ("factorizations", "i4"), ("failed", "i4"),
+                             ("t_start", "u8"), ("t_end", "u8")])
+assert CELL_STATS_DTYPE.itemsize == 40
+COUNTERS = ("accepted", "rejected", "newton_failures", "newton_iterations", "factorizations", "failed")
+REALTIME_HZ = 100e6  # device real-time counter of t_start / t_end
 
 # every symbol include/tchem_atm.h declares
 EXPORTED_SYMBOLS = (

@@ -106,8 +106,11 @@ int resolveOptions(const tchem_atmchem_options_t* opts, const tchem_kmcd_s* k, t
   const size_t bytesA = o->precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float);
   const size_t staticLds = integratorStaticLds(o->precision, 1, k->dev.m);
   const size_t maxLds = k->maxLds > staticLds ? k->maxLds - staticLds : 0;
+  // programs are staged into LDS when they leave room for at least 4 cells
+  const size_t cellB = sparseCellBytes(k->dev, k->sparse, bytesA);
+  const bool staged = sparseProgramBytes(k->sparse) + 4 * cellB <= maxLds;
   auto sparseFits = [&](int cells) {
-    return sparseProgramBytes(k->sparse) + size_t(cells) * sparseCellBytes(k->dev, k->sparse, bytesA) <= maxLds;
+    return (staged ? sparseProgramBytes(k->sparse) : 0) + size_t(cells) * cellB <= maxLds;
   };
   if (o->lu == TCHEM_LU_AUTO)
     o->lu = (k->sparse.enabled && sparseFits(1)) ? TCHEM_LU_SPARSE : TCHEM_LU_DENSE;
@@ -134,10 +137,19 @@ int resolveOptions(const tchem_atmchem_options_t* opts, const tchem_kmcd_s* k, t
 }
 
 // LDS bytes of one workgroup
+// whether the sparse programs are staged into LDS for this precision
+bool sparseStaged(const tchem_kmcd_s* k, int precision) {
+  const size_t bytesA = precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float);
+  const size_t staticLds = integratorStaticLds(precision, 1, k->dev.m);
+  const size_t maxLds = k->maxLds > staticLds ? k->maxLds - staticLds : 0;
+  return sparseProgramBytes(k->sparse) + 4 * sparseCellBytes(k->dev, k->sparse, bytesA) <= maxLds;
+}
+
 size_t ldsBytes(const tchem_kmcd_s* k, const tchem_atmchem_options_t& o) {
   const size_t bytesA = o.precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float);
   if (o.lu == TCHEM_LU_SPARSE)
-    return sparseProgramBytes(k->sparse) + size_t(o.cells_per_block) * sparseCellBytes(k->dev, k->sparse, bytesA);
+    return (sparseStaged(k, o.precision) ? sparseProgramBytes(k->sparse) : 0) +
+           size_t(o.cells_per_block) * sparseCellBytes(k->dev, k->sparse, bytesA);
   return integratorLdsBytes(k->dev, bytesA);
 }
 
@@ -368,7 +380,7 @@ int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBat
   IntegratorArgs a{};
   a.M = kmcd->dev;
   a.S = kmcd->sparse;
-  a.sparse = o.lu == TCHEM_LU_SPARSE ? 1 : 0;
+  a.sparse = o.lu == TCHEM_LU_SPARSE ? (sparseStaged(kmcd, o.precision) ? 1 : 2) : 0;
   a.nBatch = nBatch;
   a.tol_newton = tol_newton;
   a.tol_time = tol_time;

@@ -94,7 +94,7 @@ struct IntegratorArgs {
   tchem_cell_stats_t* stats;         // or null
   int* queue;                        // persistent schedule: next-cell counter
   int persistent;
-  int sparse;                        // 1: sparse static-pivot linear algebra
+  int sparse;                        // 0 dense LU; 1 sparse, programs staged in LDS; 2 sparse, programs in memory
   int cellsPerGroup;                 // wavefronts (cells) per workgroup
   int cellBytes;                     // LDS bytes of one cell region
 };

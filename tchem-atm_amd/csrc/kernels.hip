@@ -491,18 +491,30 @@ struct SpLds {
   double* seg;  // [nSeg+1], seg[nSeg] = trash
 };
 
-// programs at the start of the workgroup's LDS, then one cell region per wavefront
-template <typename TA>
+// STAGED: programs at the start of the workgroup's LDS, then one cell region
+// per wavefront; otherwise the programs are read from device memory (they
+// stay resident in L2) and the cell regions start at 0.
+template <typename TA, bool STAGED>
 __device__ SpLds<TA> carve_sparse(char* base, const DevMech& M, const DevSparse& S, int cellBytes, int wave) {
   auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
   SpLds<TA> L;
-  const unsigned* w = reinterpret_cast<const unsigned*>(base);
-  L.hdr = w;
-  L.body = w + sparseLuHdrWords(S);
-  L.ehdr = w + sparseEvalOffsetWords(S);
-  L.ebody = L.ehdr + sparseEvalHdrWords(S);
-  L.coef = reinterpret_cast<const double*>(w + sparseCoefOffsetWords(S));
-  char* cell = base + sparseProgramBytes(S) + size_t(wave) * size_t(cellBytes);
+  char* cell;
+  if (STAGED) {
+    const unsigned* w = reinterpret_cast<const unsigned*>(base);
+    L.hdr = w;
+    L.body = w + sparseLuHdrWords(S);
+    L.ehdr = w + sparseEvalOffsetWords(S);
+    L.ebody = L.ehdr + sparseEvalHdrWords(S);
+    L.coef = reinterpret_cast<const double*>(w + sparseCoefOffsetWords(S));
+    cell = base + sparseProgramBytes(S) + size_t(wave) * size_t(cellBytes);
+  } else {
+    L.hdr = reinterpret_cast<const unsigned*>(M.ib + S.oProgHdr);
+    L.body = reinterpret_cast<const unsigned*>(M.ib + S.oProgBody);
+    L.ehdr = reinterpret_cast<const unsigned*>(M.ib + S.oEvalHdr);
+    L.ebody = reinterpret_cast<const unsigned*>(M.ib + S.oEvalBody);
+    L.coef = M.db + S.oCoef;
+    cell = base + size_t(wave) * size_t(cellBytes);
+  }
   size_t off = 0;
   L.A = reinterpret_cast<TA*>(cell);
   off += al(sizeof(TA) * size_t(S.trashSlot + 1));
@@ -792,7 +804,7 @@ struct DenseBackend {
   }
 };
 
-template <typename TA_, int QM>
+template <typename TA_, int QM, bool STAGED>
 struct SparseBackend {
   using TA = TA_;
   using Lds = SpLds<TA>;
@@ -800,6 +812,7 @@ struct SparseBackend {
   // the mechanism program (LU and BACK sections) is copied into the
   // workgroup's LDS once; every wavefront of the workgroup reads it
   static __device__ void stage(const IntegratorArgs& a, char* base) {
+    if (!STAGED) return;
     const DevSparse& S = a.S;
     unsigned* dst = reinterpret_cast<unsigned*>(base);
     auto copy = [&](size_t at, const int* src, int n) {
@@ -815,7 +828,7 @@ struct SparseBackend {
     __syncthreads();
   }
   static __device__ Lds carve(char* base, const IntegratorArgs& a, int wave) {
-    return carve_sparse<TA>(base, a.M, a.S, a.cellBytes, wave);
+    return carve_sparse<TA, STAGED>(base, a.M, a.S, a.cellBytes, wave);
   }
   // zero slots read by padded records
   static __device__ void init(const IntegratorArgs& a, const Lds& L, int lane) {
@@ -930,10 +943,11 @@ __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L
 
   const double t_in = a.t_out[cell];
   if (t_in >= tend || ta._dt < 0.0) {  // skipped (.cpp:83) or already failed
-    if (a.stats && lane == 0) a.stats[cell] = tchem_cell_stats_t{0, 0, 0, 0, 0, ta._dt < 0.0 ? 1 : 0};
+    if (a.stats && lane == 0) a.stats[cell] = tchem_cell_stats_t{0, 0, 0, 0, 0, ta._dt < 0.0 ? 1 : 0, 0, 0};
     return;
   }
 
+  const unsigned long long t_start = a.stats ? (unsigned long long)__builtin_amdgcn_s_memrealtime() : 0ull;
   const double gam = 2.0 - 1.41421356237309504880;
   const double c_tr = 0.5 * gam;
   const double a_g = 1.0 / (gam * (2.0 - gam));
@@ -1135,7 +1149,8 @@ __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L
     if (a.stats) {
       cnt.failed = failed ? 1 : 0;
       a.stats[cell] = tchem_cell_stats_t{cnt.accepted, cnt.rejected, cnt.newton_failures, cnt.newton_iterations,
-                                         cnt.factorizations, cnt.failed};
+                                         cnt.factorizations, cnt.failed, t_start,
+                                         (unsigned long long)__builtin_amdgcn_s_memrealtime()};
     }
   }
 }
@@ -1254,20 +1269,23 @@ static const void* integratorFn() {
 // form with m <= 128 (register budget 168 VGPRs), 8 otherwise (256 VGPRs).
 int sparseMaxCellsPerGroup(int precision, int m) { return (precision == TCHEM_PRECISION_MIXED && m <= 128) ? 12 : 8; }
 
-// kernel instance for (precision, sparse, species slots per lane)
+// kernel instance for (precision, sparse form: 0 dense, 1 staged programs,
+// 2 programs in device memory; species slots per lane)
 static const void* integratorKernel(int precision, int sparse, int qm) {
 #define TCHEM_PICK_D(TA) \
   return qm == 1 ? integratorFn<DenseBackend<TA, 1>, 1, 64>() \
                  : (qm == 2 ? integratorFn<DenseBackend<TA, 2>, 2, 64>() : integratorFn<DenseBackend<TA, 4>, 4, 64>())
-#define TCHEM_PICK_S(TA, T12)                                                                 \
-  return qm == 1 ? integratorFn<SparseBackend<TA, 1>, 1, T12>()                               \
-                 : (qm == 2 ? integratorFn<SparseBackend<TA, 2>, 2, T12>()                    \
-                            : integratorFn<SparseBackend<TA, 4>, 4, 512>())
+#define TCHEM_PICK_S(TA, T12, ST)                                                             \
+  return qm == 1 ? integratorFn<SparseBackend<TA, 1, ST>, 1, T12>()                           \
+                 : (qm == 2 ? integratorFn<SparseBackend<TA, 2, ST>, 2, T12>()                \
+                            : integratorFn<SparseBackend<TA, 4, ST>, 4, 512>())
   if (precision == TCHEM_PRECISION_FP64) {
-    if (sparse) TCHEM_PICK_S(double, 512);
+    if (sparse == 1) TCHEM_PICK_S(double, 512, true);
+    if (sparse == 2) TCHEM_PICK_S(double, 512, false);
     TCHEM_PICK_D(double);
   }
-  if (sparse) TCHEM_PICK_S(float, 768);
+  if (sparse == 1) TCHEM_PICK_S(float, 768, true);
+  if (sparse == 2) TCHEM_PICK_S(float, 768, false);
   TCHEM_PICK_D(float);
 #undef TCHEM_PICK_D
 #undef TCHEM_PICK_S

@@ -61,7 +61,7 @@ class DeviceRun:
                                                   stream=stream)
             if collect_stats:
                 s = T.stats_from_tensor(self.stats)
-                arr = np.stack([s[f] for f in s.dtype.names], 1).astype(np.int64)
+                arr = np.stack([s[f] for f in T.COUNTERS], 1).astype(np.int64)
                 arr[:, 5] = s["failed"]
                 if total is None:
                     total = arr.copy()
@@ -87,5 +87,6 @@ class DeviceRun:
                                               schedule=schedule, blocks_per_cu=blocks_per_cu, lu=lu, stream=stream)
         if collect_stats:
             s = T.stats_from_tensor(self.stats)
-            self.total_stats = np.stack([s[f] for f in s.dtype.names], 1).astype(np.int64)
+            self.total_stats = np.stack([s[f] for f in T.COUNTERS], 1).astype(np.int64)
+            self.t_start, self.t_end = s["t_start"].copy(), s["t_end"].copy()
         return 1

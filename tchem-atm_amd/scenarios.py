@@ -90,3 +90,116 @@ def tol_arrays(w: Workload, m: int):
     tol_newton = np.array([w.atol_newton, w.rtol_newton])
     tol_time = np.tile([w.atol_time, w.rtol_time], (m, 1))
     return tol_newton, tol_time
+
+
+def config4_batches(seed: int = 3, n_batches: int = 48, lo: int = 256, hi: int = 65536,
+                    size_scale: float = 1.0) -> np.ndarray:
+    """Cell counts of the uneven batches: log-uniform in [lo, hi], times
+    size_scale (at least one cell)."""
+    g = np.random.Generator(np.random.Philox(key=seed + 1000))
+    sizes = 10.0 ** g.uniform(np.log10(lo), np.log10(hi), n_batches)
+    return np.maximum(1, np.round(sizes * size_scale)).astype(np.int64)
+
+
+def config4(kmd, base_state_row, seed: int = 3, n_batches: int = 48, size_scale: float = 1.0):
+    """Uneven batches of the synthetic mechanism, states drawn as in config 1
+    (T ~ U[220,320] K; the synthetic mechanism has no photolysis), all
+    batches concatenated into one workload for one persistent launch.
+    Returns (workload, batch sizes)."""
+    sizes = config4_batches(seed, n_batches, size_scale=size_scale)
+    m = kmd.nSpec - kmd.nConstSpec
+    st, ph = cb05_cells(int(sizes.sum()), seed, m, kmd.nSpec, kmd.nPhoto, base_state_row, t_range=(220.0, 320.0),
+                        night_fraction=0.5)
+    return Workload("synthetic250_config4", st, ph, 0.0, 3600.0, 1e-20, 1e-20, 60.0, 1e-12, 1e-6), sizes
+
+
+def synthetic_mechanism_yaml(path: str, seed: int = 3, n_species: int = 250, n_reactions: int = 750,
+                             n_radicals: int = 12, window: int = 40) -> str:
+    """Writes a seeded synthetic mechanism in the NCAR YAML format the loader
+    reads: n_species active species S000.. plus the constant third body M and
+    n_reactions reactions with 1-2 reactants (integer stoichiometry, a
+    repeated reactant becomes nu = 2) and 1-4 distinct products with yields
+    ~ U[0.1, 1].  Rate parameters: first-order A ~ 10^U[-6,-1] s^-1,
+    bimolecular A ~ 10^U[-16,-10] cm^3 molecule^-1 s^-1, C = -Ea/R with
+    Ea/R ~ U[-500, 3000] K; 10 % of the reactions are TROE with k0_A and
+    kinf_A drawn from the bimolecular range, Fc = 0.6, N = 1.
+
+    Structure (an oxidation cascade, as in atmospheric mechanisms): species
+    0..n_radicals-1 are radicals, the rest organics in generation order.
+    60 % of the reactions are organic + radical, 20 % organic photolysis-like
+    first order, 20 % radical + radical; organic products come from the
+    next `window` organics, radical products from the radicals.  The
+    Jacobian is then a cascade plus dense radical rows and columns, which a
+    fill-reducing order factors with little fill; uniformly random coupling
+    would instead make every 250 x 250 factorisation nearly dense."""
+    g = np.random.Generator(np.random.Philox(key=seed))
+    names = [f"S{i:03d}" for i in range(n_species)]
+    nr = n_radicals
+    out = ["NCAR-version: v1.0",
+           "environmental_conditions:",
+           "  pressure:", "    evolving: 'false'", "    initial_value: [101325]", "    units: Pa",
+           "  temperature:", "    evolving: 'false'", "    initial_value: [298.0]", "    units: K",
+           "model_info:",
+           "  chemistry_time_step:", "    units: min", "    value: 1.0",
+           "  evolving_conditions:", "    exist: 'false'",
+           "  output_time_step:", "    units: min", "    value: 1.0",
+           "  simulation_length:", "    units: hr", "    value: 1.0",
+           "initial_state:"]
+    for nm in names:
+        out += [f"  {nm}:", "    initial_value: [0.001]", "    units: mol m-3"]
+    out += ["  M:", "    initial_value: [1000000.0]", "    units: mol m-3"]
+    out += ["constant_species:", "- description: third body", "  name: M", "species:"]
+    for i, nm in enumerate(names):
+        out += [f"- description: synthetic {'radical' if i < nr else 'organic'}", f"  name: {nm}"]
+    out += ["reactions:"]
+
+    def later_organics(i, k):
+        lo = max(i + 1, nr)
+        hi = min(n_species, lo + window)
+        if hi - lo < k:
+            lo = max(nr, hi - max(k, window))
+        return [int(x) for x in g.choice(np.arange(lo, hi), k, replace=False)]
+
+    for r in range(n_reactions):
+        u = g.uniform()
+        nprod = 1 + int(g.integers(0, 4))
+        if u < 0.6:    # organic + radical
+            org = int(g.integers(nr, n_species - 1))
+            reac = [org, int(g.integers(0, nr))]
+            n_org = max(1, nprod - int(g.integers(0, 2)))
+            prods = later_organics(org, min(n_org, nprod))
+            prods += [int(x) for x in g.choice(nr, nprod - len(prods), replace=False)]
+        elif u < 0.8:  # organic first order
+            org = int(g.integers(nr, n_species - 1))
+            reac = [org]
+            prods = later_organics(org, max(1, nprod - 1))
+            prods += [int(x) for x in g.choice(nr, nprod - len(prods), replace=False)]
+        else:          # radical + radical
+            reac = [int(x) for x in g.integers(0, nr, 2)]
+            prods = [int(x) for x in g.choice(np.arange(0, nr + window), nprod, replace=False)]
+        nu = {}
+        for x in reac:
+            nu[x] = nu.get(x, 0) + 1
+        prods = list(dict.fromkeys(prods))
+        yields = g.uniform(0.1, 1.0, len(prods))
+        out.append(f"- MUSICA_name: R{r + 1}")
+        out.append("  reactants:")
+        for x, v in nu.items():
+            out.append(f"    {names[x]}: {v}")
+        out.append("  products:")
+        for x, y in zip(prods, yields):
+            out.append(f"    {names[x]}: {float(y)!r}")
+        ea_r = float(g.uniform(-500.0, 3000.0))
+        bimolecular = sum(nu.values()) == 2
+        if bimolecular and g.uniform() < 0.125:  # 10 % of all reactions (80 % are bimolecular)
+            k0 = 10.0 ** float(g.uniform(-16.0, -10.0))
+            kinf = 10.0 ** float(g.uniform(-16.0, -10.0))
+            out += ["  type: TROE", "  coefficients:", f"    k0_A: {k0!r}", "    k0_B: 0.0", f"    k0_C: {-ea_r!r}",
+                    f"    kinf_A: {kinf!r}", "    kinf_B: 0.0", f"    kinf_C: {-ea_r!r}", "    Fc: 0.6", "    N: 1.0"]
+        else:
+            A = 10.0 ** float(g.uniform(-16.0, -10.0) if bimolecular else g.uniform(-6.0, -1.0))
+            out += ["  type: ARRHENIUS", "  coefficients:", f"    A: {A!r}", "    B: 0.0", f"    C: {-ea_r!r}"]
+    text = "\n".join(out) + "\n"
+    with open(path, "w") as f:
+        f.write(text)
+    return path
