@@ -153,6 +153,54 @@ int tchem_sourceterm_run_device_batch(int nBatch, const double* state, const dou
 int tchem_jacobian_run_device_batch(int nBatch, const double* state, const double* photo_rates, double* jac,
                                     tchem_kmcd_t kmcd, void* stream);
 
+/* ---- discrete adjoint ----------------------------------------------------- */
+/* No counterpart in the reference, which has no derivative path: gradients
+ * of a scalar loss on the final concentrations of a recorded TR-BDF2 run
+ * with respect to the initial active species and to per-reaction log-scale
+ * parameters theta_r (kf_r = k_r exp(theta_r); for photolysis reactions a
+ * uniform log-scale on the per-cell rate).  The forward run is the FP64
+ * sparse persistent integrator of tchem_atmchem_run_device_batch with its
+ * accepted step sizes and a checkpoint of the state every 16 accepted steps
+ * recorded into a caller-owned device workspace; the backward sweep
+ * replays each 16-step segment from its checkpoint and runs the transposed
+ * stage solves in reverse.  Step sizes are held fixed (the derivative of the
+ * discrete map along the accepted steps).
+ *
+ * Workspace bytes for nBatch cells and at most maxSteps accepted steps per
+ * cell (1 <= maxSteps <= 2^24). */
+int tchem_adjoint_workspace_bytes(tchem_kmcd_t kmcd, int nBatch, int maxSteps, size_t* bytes);
+
+/* Forward run with recording, from tadv[i]._tbeg to ._tend in one launch
+ * (at most _num_time_iterations_per_interval iterations).  Arguments as
+ * tchem_atmchem_run_device_batch, plus
+ *   log_rate_scale [device] theta, NULL for 0: [nReac] shared by every cell
+ *                  (log_rate_scale_stride 0) or [nBatch][nReac] (stride nReac)
+ *   workspace      [device] tchem_adjoint_workspace_bytes(kmcd, nBatch, maxSteps) bytes
+ * A cell that fails, or takes more than maxSteps accepted steps, is marked
+ * in the record; its backward results are NaN. */
+int tchem_adjoint_forward(int nBatch, const double* tol_newton, const double* tol_time,
+                          const tchem_time_advance_t* tadv, double* state, const double* photo_rates,
+                          const double* log_rate_scale, int log_rate_scale_stride, double* t_out, double* dt_out,
+                          tchem_cell_stats_t* stats, void* workspace, int maxSteps, tchem_kmcd_t kmcd,
+                          void* stream);
+
+/* Backward sweep over the record of the last tchem_adjoint_forward on this
+ * workspace (same nBatch and maxSteps; a mismatch gives NaN results).
+ *   lambda     [device] [nBatch][nSpec] dL/d(final species), reference order
+ *              (constant species ignored)
+ *   lambda_out [device] [nBatch][nSpec] dL/d(initial species); 0 for constant
+ *              species; may alias lambda
+ *   grad       [device] [nBatch][nReac] dL/dtheta_r of each cell
+ * The workspace's replay scratch is overwritten; the record is kept, so the
+ * sweep may be repeated with another lambda. */
+int tchem_adjoint_backward(int nBatch, const double* lambda, double* lambda_out, double* grad, void* workspace,
+                           int maxSteps, tchem_kmcd_t kmcd, void* stream);
+
+/* Copies the record's accepted-step counts nsteps[nBatch] (-1: failed or
+ * overflowed) and, when h is not NULL, step sizes h[nBatch][maxSteps]. */
+int tchem_adjoint_steps(int nBatch, const void* workspace, int maxSteps, int* nsteps, double* h, tchem_kmcd_t kmcd,
+                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -81,6 +81,10 @@ def lib():
         _lib.oracle_jacobian_batch.argtypes = [vp, i, vp, vp, vp]
         _lib.oracle_run_batch.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp, vp, i, i]
         _lib.oracle_trbdf2_constants.argtypes = [vp]
+        _lib.oracle_record_steps.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i]
+        _lib.oracle_record_steps.restype = i
+        _lib.oracle_tlm_fixed_steps.argtypes = [vp, vp, i, i, vp, vp, vp, vp, i, i, vp, vp, vp]
+        _lib.oracle_tlm_fixed_steps.restype = i
     return _lib
 
 
@@ -151,6 +155,38 @@ class Oracle:
         ph = None if photo is None else np.ascontiguousarray(photo, np.float64)
         lib().oracle_run_batch(self.ptr, state.shape[0], _p(tn), _p(tt), _p(tadv), _p(state), _p(ph),
                                _p(t_out), _p(dt_out), _p(stats), int(jacobian_mode), int(nthreads))
+
+    def record_steps(self, tol_newton, tol_time, ta, row, photo=None, lscale=None, max_steps=4096):
+        """One cell (analytic Jacobian) with rate constants scaled by exp(lscale);
+        returns (final row, accepted step sizes) or (row, None) on failure."""
+        assert ta.dtype == TADV_DTYPE and ta.shape == (1,)
+        row = np.array(row, np.float64)
+        h = np.zeros(max_steps)
+        tn = np.ascontiguousarray(tol_newton, np.float64)
+        tt = np.ascontiguousarray(tol_time, np.float64)
+        ph = None if photo is None else np.ascontiguousarray(photo, np.float64)
+        ls = None if lscale is None else np.ascontiguousarray(lscale, np.float64)
+        n = lib().oracle_record_steps(self.ptr, _p(tn), _p(tt), _p(ta), _p(row), _p(ph), _p(ls), _p(h), max_steps)
+        return row, (h[:n].copy() if n >= 0 else None)
+
+
+    def tlm_fixed_steps(self, tol_newton, max_newton, jint, row, h, sens=(), photo=None, lscale=None):
+        """Fixed step sequence h with the discrete tangent linear of the
+        parameters in sens (>= 0: reaction log-scale, < 0: initial species -s-1).
+        Returns (u_final [m], S [m][len(sens)]); raises on a Newton failure."""
+        row = np.ascontiguousarray(row, np.float64)
+        h = np.ascontiguousarray(h, np.float64)
+        sens = np.ascontiguousarray(sens, np.int32)
+        u = np.zeros(self.m)
+        S = np.zeros((self.m, max(len(sens), 1)))
+        tn = np.ascontiguousarray(tol_newton, np.float64)
+        ph = None if photo is None else np.ascontiguousarray(photo, np.float64)
+        ls = None if lscale is None else np.ascontiguousarray(lscale, np.float64)
+        rc = lib().oracle_tlm_fixed_steps(self.ptr, _p(tn), int(max_newton), int(jint), _p(row), _p(ph), _p(ls), _p(h),
+                                          len(h), len(sens), _p(sens), _p(u), _p(S))
+        if rc != 0:
+            raise RuntimeError("oracle_tlm_fixed_steps: Newton failure")
+        return u, S[:, :len(sens)]
 
 
 def lu(A):

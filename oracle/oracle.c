@@ -191,6 +191,7 @@ typedef struct {
   int m, mode;
   double T, p;
   const double* photo;
+  const double* lscale; /* [nReac] log-scale of the rate constants, or NULL */
   double* kfor;  /* [nReac] */
   double* c;     /* [nSpec]: active part is the evaluation point */
   double* omega; /* [m] */
@@ -200,10 +201,17 @@ typedef struct {
   double* fh;    /* [m] FD scratch */
 } cell_ctx;
 
+/* rate constants of the cell, kf_r * exp(lscale_r) when lscale is given */
+static void cell_kforward(cell_ctx* x) {
+  oracle_kforward(x->k, x->T, x->p, x->photo, x->kfor);
+  if (x->lscale)
+    for (int r = 0; r < x->k->nReac; ++r) x->kfor[r] *= exp(x->lscale[r]);
+}
+
 /* source term at u (active part); the constant part of c is fixed per cell */
 static void rhs(cell_ctx* x, const double* u, double* f) {
   memcpy(x->c, u, sizeof(double) * x->m);
-  if (x->mode == 1) oracle_kforward(x->k, x->T, x->p, x->photo, x->kfor); /* reference: every call */
+  if (x->mode == 1) cell_kforward(x); /* reference: every call */
   oracle_omega(x->k, x->kfor, x->c, f);
 }
 
@@ -285,7 +293,8 @@ static int newton(cell_ctx* x, double* u, const double* b, double ch, const doub
  */
 static void integrate_cell(const oracle_kmd* k, const double* tol_newton, const double* tol_time,
                            const oracle_time_advance* ta, double* row, const double* photo, double* t_out,
-                           double* dt_out, oracle_stats* st, int mode, double* work, int* iwork) {
+                           double* dt_out, oracle_stats* st, int mode, double* work, int* iwork,
+                           const double* lscale, double* h_rec, int max_rec, int* n_rec) {
   const int nSpec = k->nSpec, m = nSpec - k->nConstSpec;
   const double tend = ta->tend;
   if (*t_out >= tend) return; /* .cpp:83 */
@@ -303,6 +312,7 @@ static void integrate_cell(const oracle_kmd* k, const double* tol_newton, const 
   x.p = row[1];
   x.T = row[2];
   x.photo = photo;
+  x.lscale = lscale;
   double* w = work;
   x.kfor = w; w += k->nReac;
   x.c = w; w += nSpec;
@@ -323,7 +333,8 @@ static void integrate_cell(const oracle_kmd* k, const double* tol_newton, const 
 
   for (int i = 0; i < nSpec; ++i) x.c[i] = row[3 + i];
   for (int i = 0; i < m; ++i) un[i] = row[3 + i];
-  if (mode == 0) oracle_kforward(k, x.T, x.p, photo, x.kfor); /* constant per cell */
+  if (mode == 0) cell_kforward(&x); /* constant per cell */
+  int nrec = 0;
 
   const int max_it = ta->num_time_iterations_per_interval;
   const int max_newton = ta->max_num_newton_iterations;
@@ -372,7 +383,12 @@ static void integrate_cell(const oracle_kmd* k, const double* tol_newton, const 
     dt = (e <= 0.125) ? 2.0 * h : h;
     if (dt > ta->dtmax) dt = ta->dtmax;
     if (st) st->accepted++;
+    if (h_rec) {
+      if (nrec < max_rec) h_rec[nrec] = h;
+      ++nrec;
+    }
   }
+  if (n_rec) *n_rec = (failed || nrec > max_rec) ? -1 : nrec;
 
   if (failed) {
     for (int i = 0; i < m; ++i) row[3 + i] = 0.0;
@@ -402,7 +418,7 @@ void oracle_run_batch(const oracle_kmd* k, int nBatch, const double* tol_newton,
     for (int c = 0; c < nBatch; ++c) {
       integrate_cell(k, tol_newton, tol_time, &tadv[c], state + (size_t)c * (nSpec + 3),
                      photo ? photo + (size_t)c * k->nPhoto : NULL, &t_out[c], &dt_out[c],
-                     stats ? &stats[c] : NULL, jacobian_mode, work, iwork);
+                     stats ? &stats[c] : NULL, jacobian_mode, work, iwork, NULL, NULL, 0, NULL);
     }
     free(work);
     free(iwork);
@@ -439,4 +455,179 @@ void oracle_jacobian_batch(const oracle_kmd* k, int nBatch, const double* state,
     }
     free(kfor);
   }
+}
+
+/* ---------------------------------------------------------------------------
+ * Derivatives of the discrete map (no counterpart in the reference).
+ */
+
+static size_t cell_work_len(const oracle_kmd* k) {
+  const int m = k->nSpec - k->nConstSpec;
+  return (size_t)k->nReac + k->nSpec + 2 * (size_t)m * m + 12 * (size_t)m;
+}
+
+/* One integrate_cell run (analytic Jacobian, rate constants scaled by
+ * exp(lscale)) recording the accepted step sizes: returns the number of
+ * accepted steps, or -1 when the cell failed or took more than max_steps.
+ * row is integrated in place.
+ * Pinned by: the integrator's own pins (the run is oracle_run_batch's). */
+int oracle_record_steps(const oracle_kmd* k, const double* tol_newton, const double* tol_time,
+                        const oracle_time_advance* ta, double* row, const double* photo, const double* lscale,
+                        double* h_out, int max_steps) {
+  const int m = k->nSpec - k->nConstSpec;
+  double* work = (double*)malloc(sizeof(double) * cell_work_len(k));
+  int* iwork = (int*)malloc(sizeof(int) * (size_t)(m > 0 ? m : 1));
+  double t_out = ta->tbeg, dt_out = ta->dt;
+  int n = -1;
+  integrate_cell(k, tol_newton, tol_time, ta, row, photo, &t_out, &dt_out, NULL, 0, work, iwork, lscale, h_out,
+                 max_steps, &n);
+  free(work);
+  free(iwork);
+  return n;
+}
+
+/* v += scale * nu_r * rop_r(c) over active species (reactants then products) */
+static void add_nu_rop(const oracle_kmd* k, const double* kfor, const double* c, int r, double scale, double* v,
+                       int stride) {
+  const int m = k->nSpec - k->nConstSpec;
+  const int ms = k->maxSpecInReac;
+  double rop = kfor[r];
+  for (int j = 0; j < k->reacNreac[r]; ++j) rop *= pow_int(c[k->reacSidx[r * ms + j]], abs((int)k->reacNuki[r * ms + j]));
+  for (int j = 0; j < k->reacNreac[r]; ++j) {
+    const int ks = k->reacSidx[r * ms + j];
+    if (ks < m) v[(size_t)ks * stride] += scale * k->reacNuki[r * ms + j] * rop;
+  }
+  const int joff = ms / 2;
+  for (int j = 0; j < k->reacNprod[r]; ++j) {
+    const int ks = k->reacSidx[r * ms + j + joff];
+    if (ks < m) v[(size_t)ks * stride] += scale * k->reacNuki[r * ms + j + joff] * rop;
+  }
+}
+
+/* Integrates one cell over a given sequence of step sizes h[0..nsteps) with
+ * the oracle's TR-BDF2 stages and Newton solver (no error control), and
+ * propagates the discrete tangent linear S = du/dp of nsens parameters:
+ * sens[c] >= 0 is the log-scale theta_r of reaction r (kf_r * exp(theta_r),
+ * d f / d theta_r = nu_r rop_r), sens[c] < 0 the initial value of active
+ * species -sens[c]-1.  Per step, with J0, Jg, J1 the Jacobians at u_n, u_g,
+ * u_{n+1} and F(u) the parameter columns of d f / d p,
+ *   (I - c_tr h Jg) Sg = (I + c_tr h J0) S + c_tr h (F(u_g) + F(u_n))
+ *   (I - c_bdf h J1) S' = a_g Sg - a_n S + c_bdf h F(u_{n+1}),
+ * the derivative of the stage equations solved exactly.
+ *   row: [nSpec+3] initial state (not modified); u_out [m]; S_out [m][nsens]
+ *   (nullable).  Returns 0, or -1 when a Newton solve failed.
+ * Pinned by: central differences of this same function in the parameters
+ * (tests/test_oracle_adjoint.py), and on y' = lambda y by the closed-form
+ * derivative of the TR-BDF2 amplification factor. */
+int oracle_tlm_fixed_steps(const oracle_kmd* k, const double* tol_newton, int max_newton, int jint,
+                           const double* row, const double* photo, const double* lscale, const double* h, int nsteps,
+                           int nsens, const int* sens, double* u_out, double* S_out) {
+  const int nSpec = k->nSpec, m = nSpec - k->nConstSpec;
+  double g6[6];
+  oracle_trbdf2_constants(g6);
+  const double c_tr = g6[1], a_g = g6[2], a_n = g6[3], c_bdf = g6[4];
+  cell_ctx x;
+  x.k = k;
+  x.m = m;
+  x.mode = 0;
+  x.p = row[1];
+  x.T = row[2];
+  x.photo = photo;
+  x.lscale = lscale;
+  double* work = (double*)malloc(sizeof(double) * cell_work_len(k));
+  int* iwork = (int*)malloc(sizeof(int) * (size_t)(m > 0 ? m : 1));
+  const size_t mm = (size_t)m * m, ms = (size_t)m * (nsens > 0 ? nsens : 1);
+  double* big = (double*)calloc(3 * mm + 5 * ms + 8 * (size_t)m, sizeof(double));
+  double* w = work;
+  x.kfor = w; w += k->nReac;
+  x.c = w; w += nSpec;
+  x.omega = w; w += m;
+  x.J = w; w += mm;
+  x.A = w; w += mm;
+  x.fh = w; w += m;
+  x.piv = iwork;
+  double* J0 = big;
+  double* Jg = J0 + mm;
+  double* M = Jg + mm;
+  double* S = M + mm;
+  double* Sg = S + ms;
+  double* R = Sg + ms;
+  double* col = R + ms;
+  double* tmp = col + ms;
+  double* un = tmp + ms;
+  double* u = un + m;
+  double* ug = u + m;
+  double* fn = ug + m;
+  double* f = fn + m;
+  double* dx = f + m;
+  double* b = dx + m;
+  int* piv = (int*)malloc(sizeof(int) * (size_t)(m > 0 ? m : 1));
+  for (int i = 0; i < nSpec; ++i) x.c[i] = row[3 + i];
+  for (int i = 0; i < m; ++i) un[i] = row[3 + i];
+  cell_kforward(&x);
+  for (int c = 0; c < nsens; ++c)
+    for (int i = 0; i < m; ++i) S[(size_t)i * nsens + c] = (sens[c] < 0 && i == -sens[c] - 1) ? 1.0 : 0.0;
+  int rc = 0;
+  rhs(&x, un, fn);
+  /* F(u) columns added with a scale into R ([m][nsens]) */
+  #define ADD_F(uvec, scale)                                                   \
+    do {                                                                       \
+      memcpy(x.c, (uvec), sizeof(double) * m);                                 \
+      for (int cc = 0; cc < nsens; ++cc)                                       \
+        if (sens[cc] >= 0) add_nu_rop(k, x.kfor, x.c, sens[cc], (scale), R + cc, nsens); \
+    } while (0)
+  for (int s = 0; s < nsteps && rc == 0; ++s) {
+    const double hh = h[s];
+    for (int i = 0; i < m; ++i) { u[i] = un[i]; b[i] = un[i] + c_tr * hh * fn[i]; }
+    if (!newton(&x, u, b, c_tr * hh, tol_newton, max_newton, jint, f, dx, NULL)) { rc = -1; break; }
+    for (int i = 0; i < m; ++i) { ug[i] = u[i]; b[i] = a_g * ug[i] - a_n * un[i]; }
+    if (!newton(&x, u, b, c_bdf * hh, tol_newton, max_newton, jint, f, dx, NULL)) { rc = -1; break; }
+    if (S_out && nsens > 0) {
+      /* R = (I + c_tr h J0) S + c_tr h (F(u_g) + F(u_n)) */
+      memcpy(x.c, un, sizeof(double) * m);
+      oracle_jacobian(k, x.kfor, x.c, J0);
+      for (int i = 0; i < m; ++i)
+        for (int c = 0; c < nsens; ++c) {
+          double acc = S[(size_t)i * nsens + c];
+          for (int j = 0; j < m; ++j) acc += c_tr * hh * J0[(size_t)i * m + j] * S[(size_t)j * nsens + c];
+          R[(size_t)i * nsens + c] = acc;
+        }
+      ADD_F(ug, c_tr * hh);
+      ADD_F(un, c_tr * hh);
+      memcpy(x.c, ug, sizeof(double) * m);
+      oracle_jacobian(k, x.kfor, x.c, Jg);
+      for (size_t e = 0; e < mm; ++e) M[e] = -c_tr * hh * Jg[e];
+      for (int i = 0; i < m; ++i) M[(size_t)i * m + i] += 1.0;
+      if (oracle_lu(m, M, m, piv) != 0) { rc = -1; break; }
+      for (int c = 0; c < nsens; ++c) {
+        for (int i = 0; i < m; ++i) col[i] = R[(size_t)i * nsens + c];
+        oracle_lu_solve(m, M, m, piv, col);
+        for (int i = 0; i < m; ++i) Sg[(size_t)i * nsens + c] = col[i];
+      }
+      /* R = a_g Sg - a_n S + c_bdf h F(u_{n+1}) */
+      for (size_t e = 0; e < ms; ++e) R[e] = a_g * Sg[e] - a_n * S[e];
+      ADD_F(u, c_bdf * hh);
+      memcpy(x.c, u, sizeof(double) * m);
+      oracle_jacobian(k, x.kfor, x.c, Jg);
+      for (size_t e = 0; e < mm; ++e) M[e] = -c_bdf * hh * Jg[e];
+      for (int i = 0; i < m; ++i) M[(size_t)i * m + i] += 1.0;
+      if (oracle_lu(m, M, m, piv) != 0) { rc = -1; break; }
+      for (int c = 0; c < nsens; ++c) {
+        for (int i = 0; i < m; ++i) col[i] = R[(size_t)i * nsens + c];
+        oracle_lu_solve(m, M, m, piv, col);
+        for (int i = 0; i < m; ++i) S[(size_t)i * nsens + c] = col[i];
+      }
+    }
+    for (int i = 0; i < m; ++i) un[i] = u[i];
+    rhs(&x, un, fn);
+  }
+  #undef ADD_F
+  (void)tmp;
+  if (u_out) memcpy(u_out, un, sizeof(double) * m);
+  if (S_out && nsens > 0) memcpy(S_out, S, sizeof(double) * ms);
+  free(piv);
+  free(big);
+  free(work);
+  free(iwork);
+  return rc;
 }

@@ -98,6 +98,20 @@ void oracle_run_batch(const oracle_kmd* k, int nBatch, const double* tol_newton,
                       const oracle_time_advance* tadv, double* state, const double* photo, double* t_out,
                       double* dt_out, oracle_stats* stats, int jacobian_mode, int nthreads);
 
+/* One cell with rate constants kf_r * exp(lscale_r) (lscale nullable),
+ * recording accepted step sizes h_out[max_steps]; row integrated in place.
+ * Returns the number of accepted steps, -1 on failure or overflow. */
+int oracle_record_steps(const oracle_kmd* k, const double* tol_newton, const double* tol_time,
+                        const oracle_time_advance* ta, double* row, const double* photo, const double* lscale,
+                        double* h_out, int max_steps);
+
+/* Fixed step sequence h[nsteps] with the discrete tangent linear of nsens
+ * parameters (sens[c] >= 0: log-scale of reaction sens[c]; < 0: initial
+ * value of active species -sens[c]-1).  u_out [m], S_out [m][nsens]. */
+int oracle_tlm_fixed_steps(const oracle_kmd* k, const double* tol_newton, int max_newton, int jint,
+                           const double* row, const double* photo, const double* lscale, const double* h, int nsteps,
+                           int nsens, const int* sens, double* u_out, double* S_out);
+
 /* TR-BDF2 constants used by the oracle (for tests) */
 void oracle_trbdf2_constants(double* out6);
 

@@ -56,7 +56,8 @@ EXPORTED_SYMBOLS = (
     "tchem_last_error", "tchem_kmd_load_yaml", "tchem_kmd_destroy", "tchem_kmd_info", "tchem_kmd_species_name",
     "tchem_kmd_photolysis_reactions", "tchem_scenario_load", "tchem_kmcd_create", "tchem_kmcd_destroy",
     "tchem_atmchem_workspace_bytes", "tchem_atmchem_run_device_batch", "tchem_atmchem_update_time_advance",
-    "tchem_sourceterm_run_device_batch", "tchem_jacobian_run_device_batch",
+    "tchem_sourceterm_run_device_batch", "tchem_jacobian_run_device_batch", "tchem_adjoint_workspace_bytes",
+    "tchem_adjoint_forward", "tchem_adjoint_backward", "tchem_adjoint_steps",
 )
 
 
@@ -90,6 +91,10 @@ def _load():
         "tchem_atmchem_update_time_advance": [i, vp, vp, vp, vp],
         "tchem_sourceterm_run_device_batch": [i, vp, vp, vp, vp, vp, vp],
         "tchem_jacobian_run_device_batch": [i, vp, vp, vp, vp, vp],
+        "tchem_adjoint_workspace_bytes": [vp, i, i, ctypes.POINTER(ctypes.c_size_t)],
+        "tchem_adjoint_forward": [i, vp, vp, vp, vp, vp, vp, i, vp, vp, vp, vp, i, vp, vp],
+        "tchem_adjoint_backward": [i, vp, vp, vp, vp, i, vp, vp],
+        "tchem_adjoint_steps": [i, vp, i, vp, vp, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -248,6 +253,60 @@ class Jacobian:
         _check(_lib.tchem_jacobian_run_device_batch(
             state.shape[0], _ptr(state, what="state"), _ptr(photo_rates, what="photo_rates"),
             _ptr(jacobian, what="jacobian"), kmcd._h, _stream_ptr(stream)))
+
+
+class Adjoint:
+    """Discrete adjoint of the recorded TR-BDF2 trajectory (no counterpart in
+    the reference): gradients of a loss on the final concentrations with
+    respect to the initial active species and per-reaction log-scale
+    parameters theta (kf_r = k_r exp(theta_r)).  The autograd wrapper is
+    adjoint.KineticsAdjoint."""
+
+    @staticmethod
+    def workspaceBytes(kmcd, nBatch, maxSteps):
+        b = ctypes.c_size_t()
+        _check(_lib.tchem_adjoint_workspace_bytes(kmcd._h, int(nBatch), int(maxSteps), ctypes.byref(b)))
+        return b.value
+
+    @staticmethod
+    def workspace(kmcd, nBatch, maxSteps, device="cuda"):
+        """Device workspace tensor (uint8) for nBatch cells and maxSteps accepted steps."""
+        import torch
+        return torch.empty(max(Adjoint.workspaceBytes(kmcd, nBatch, maxSteps), 1), dtype=torch.uint8,
+                           device=device)
+
+    @staticmethod
+    def forward(tol_newton, tol_time, tadv, state, t_out, dt_out, workspace, maxSteps, kmcd, photo_rates=None,
+                log_rate_scale=None, stats=None, stream=None):
+        """Recording forward run from tadv._tbeg to ._tend (state integrated in
+        place).  log_rate_scale: [nReac] shared, or [nBatch][nReac] per cell."""
+        stride = 0 if log_rate_scale is None or log_rate_scale.dim() == 1 else log_rate_scale.shape[-1]
+        _check(_lib.tchem_adjoint_forward(
+            state.shape[0], _ptr(tol_newton, what="tol_newton"), _ptr(tol_time, what="tol_time"),
+            _ptr(tadv, None, "tadv"), _ptr(state, what="state"), _ptr(photo_rates, what="photo_rates"),
+            _ptr(log_rate_scale, what="log_rate_scale"), stride, _ptr(t_out, what="t_out"),
+            _ptr(dt_out, what="dt_out"),
+            _ptr(stats, None, "stats"), _ptr(workspace, None, "workspace"), int(maxSteps), kmcd._h,
+            _stream_ptr(stream)))
+
+    @staticmethod
+    def backward(lam, lam_out, grad, workspace, maxSteps, kmcd, stream=None):
+        """lam [nBatch][nSpec] = dL/d(final species) -> lam_out = dL/d(initial
+        species), grad [nBatch][nReac] = dL/dtheta per cell."""
+        _check(_lib.tchem_adjoint_backward(
+            lam.shape[0], _ptr(lam, what="lambda"), _ptr(lam_out, what="lambda_out"), _ptr(grad, what="grad"),
+            _ptr(workspace, None, "workspace"), int(maxSteps), kmcd._h, _stream_ptr(stream)))
+
+    @staticmethod
+    def steps(workspace, nBatch, maxSteps, kmcd, with_h=True, stream=None):
+        """(nsteps [nBatch] int32, h [nBatch][maxSteps] or None) of the record."""
+        import torch
+        ns = torch.empty(max(nBatch, 1), dtype=torch.int32, device=workspace.device)
+        h = torch.empty((max(nBatch, 1), maxSteps), dtype=torch.float64, device=workspace.device) if with_h else None
+        _check(_lib.tchem_adjoint_steps(nBatch, _ptr(workspace, None, "workspace"), int(maxSteps),
+                                        _ptr(ns, None, "nsteps"), _ptr(h, "float64", "h") if with_h else None,
+                                        kmcd._h, _stream_ptr(stream)))
+        return ns[:nBatch], (h[:nBatch] if with_h else None)
 
 
 def time_advance_tensor(n, tbeg, tend, dt, dtmin, dtmax, max_newton, iters_per_interval, jacobian_interval=1,

@@ -3,6 +3,7 @@
 // returns a status code and records a message for tchem_last_error().
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <cstring>
 #include <fstream>
 #include <memory>
@@ -18,12 +19,17 @@
 
 namespace tchem_amd {
 hipError_t launchIntegrator(const IntegratorArgs& a, int precision, int grid, size_t lds, hipStream_t stream);
-hipError_t integratorMaxBlocksPerCU(int precision, int sparse, int m, int threads, size_t lds, int* blocks);
+hipError_t integratorMaxBlocksPerCU(int precision, int sparse, int m, int threads, size_t lds, int* blocks, bool rec);
 int sparseMaxCellsPerGroup(int precision, int m);
 size_t integratorStaticLds(int precision, int sparse, int m);
 hipError_t launchResetCounter(int* counter, hipStream_t stream);
-hipError_t launchRates(const DevMech& M, int nBatch, const double* state, const double* photo, double* kf,
-                       hipStream_t stream);
+hipError_t launchRates(const DevMech& M, int nBatch, const double* state, const double* photo, const double* lscale,
+                       int lstride, double* kf, hipStream_t stream);
+size_t adjointCellBytes(const DevMech& M, const DevSparse& S);
+hipError_t adjointMaxBlocksPerCU(int m, int threads, size_t lds, int* blocks);
+hipError_t launchAdjoint(const AdjointArgs& a, int grid, size_t lds, hipStream_t stream);
+hipError_t launchAdjointSetup(int nBatch, int maxSteps, const tchem_time_advance_t* tadv, const double* tol_newton,
+                              int* header, double* tol, int* newton, hipStream_t stream);
 hipError_t launchUpdateTadv(int n, tchem_time_advance_t* tadv, const double* t_out, const double* dt_out,
                             hipStream_t stream);
 hipError_t launchSourceTerm(const DevMech& M, int nBatch, const double* state, const double* photo, const double* ext,
@@ -40,9 +46,9 @@ struct tchem_kmd_s {
 
 struct tchem_kmcd_s {
   MechTables tables;
-  SparseSchedule sched;
+  SparseSchedule sched, schedT;  // forward and transposed (adjoint) programs
   DevMech dev{};
-  DevSparse sparse{};
+  DevSparse sparse{}, sparseT{};
   std::vector<void*> allocations;
   int device = 0;
   int numCUs = 0;
@@ -151,6 +157,84 @@ size_t ldsBytes(const tchem_kmcd_s* k, const tchem_atmchem_options_t& o) {
     return (sparseStaged(k, o.precision) ? sparseProgramBytes(k->sparse) : 0) +
            size_t(o.cells_per_block) * sparseCellBytes(k->dev, k->sparse, bytesA);
   return integratorLdsBytes(k->dev, bytesA);
+}
+
+// Rate constants (optionally log-scaled) into a.kf, then the integrator
+// launch for the resolved options; a carries the per-call arrays.
+int runIntegrator(tchem_kmcd_s* kmcd, const tchem_atmchem_options_t& o, const double* lscale, int lstride,
+                  IntegratorArgs& a, hipStream_t s, const char* who) {
+  const size_t lds = ldsBytes(kmcd, o);
+  if (lds > kmcd->maxLds)
+    return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": mechanism needs " + std::to_string(lds) +
+                                               " bytes of LDS per workgroup, device limit " +
+                                               std::to_string(kmcd->maxLds));
+  int cur = 0;
+  if (hipGetDevice(&cur) != hipSuccess || cur != kmcd->device)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": kmcd belongs to another device");
+  a.M = kmcd->dev;
+  a.S = kmcd->sparse;
+  a.sparse = o.lu == TCHEM_LU_SPARSE ? (sparseStaged(kmcd, o.precision) ? 1 : 2) : 0;
+  a.queue = kmcd->queue;
+  a.persistent = (o.schedule == TCHEM_SCHEDULE_PERSISTENT) ? 1 : 0;
+  a.cellsPerGroup = o.cells_per_block;
+  a.cellBytes = a.sparse ? int(sparseCellBytes(kmcd->dev, kmcd->sparse,
+                                               o.precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float)))
+                         : 0;
+  {
+    hipError_t e = launchRates(kmcd->dev, a.nBatch, a.state, a.photo, lscale, lstride, const_cast<double*>(a.kf), s);
+    if (e != hipSuccess) return hipFail(e, "rates_kernel launch");
+  }
+  const bool rec = a.rec.h != nullptr;
+  const int cpg = a.cellsPerGroup;
+  const int groups = (a.nBatch + cpg - 1) / cpg;
+  int grid = groups;
+  if (a.persistent) {
+    int bpc = o.blocks_per_cu;
+    if (bpc == 0) {
+      hipError_t e = integratorMaxBlocksPerCU(o.precision, a.sparse, kmcd->dev.m, 64 * cpg, lds, &bpc, rec);
+      if (e != hipSuccess) return hipFail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+      if (bpc < 1) bpc = 1;
+    }
+    const long long g = (long long)bpc * kmcd->numCUs;
+    grid = int(g < groups ? g : groups);
+    hipError_t e = launchResetCounter(kmcd->queue, s);
+    if (e != hipSuccess) return hipFail(e, "reset_counter_kernel");
+  }
+  hipError_t e = launchIntegrator(a, o.precision, grid, lds, s);
+  if (e != hipSuccess) return hipFail(e, "trbdf2_kernel launch");
+  return TCHEM_OK;
+}
+
+// Byte offsets of the adjoint workspace sections (256-byte aligned).
+struct AdjointLayout {
+  size_t header, tol, newton, nsteps, h, ckpt, consts, kf, traj, total;
+};
+AdjointLayout adjointLayout(const DevMech& M, int nBatch, int maxSteps) {
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  const size_t n = size_t(nBatch), nCkpt = size_t(maxSteps / kAdjointSegment + 1);
+  AdjointLayout L{};
+  size_t o = 0;
+  L.header = o; o += al(4 * sizeof(int));
+  L.tol = o; o += al(2 * sizeof(double));
+  L.newton = o; o += al(2 * sizeof(int) * n);
+  L.nsteps = o; o += al(sizeof(int) * n);
+  L.h = o; o += al(sizeof(double) * n * size_t(maxSteps));
+  L.ckpt = o; o += al(sizeof(double) * n * nCkpt * size_t(M.m));
+  L.consts = o; o += al(sizeof(double) * n * size_t(M.nSpec - M.m));
+  L.kf = o; o += al(sizeof(double) * n * size_t(M.nReac));
+  L.traj = o; o += al(sizeof(double) * n * size_t(2 * kAdjointSegment + 1) * size_t(M.m));
+  L.total = o;
+  return L;
+}
+
+int adjointCheck(const tchem_kmcd_s* k, int nBatch, int maxSteps, const char* who) {
+  if (!k) return setError(TCHEM_ERR_INVALID, std::string(who) + ": null kmcd");
+  if (nBatch < 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": negative nBatch");
+  if (maxSteps < 1 || maxSteps > (1 << 24))
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": maxSteps must be in [1, 2^24]");
+  if (!k->sparse.enabled || !k->sparseT.enabled)
+    return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": mechanism has no sparse (transposed) schedule");
+  return TCHEM_OK;
 }
 
 }  // namespace
@@ -293,17 +377,14 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
   d.oOmegaCoef = addD(t.omegaCoef);
   d.oEmission = addD(t.emission);
   d.oJcCoef = addD(t.jcCoef);
-  // sparse schedule (mechanisms it cannot express run the dense form only)
-  try {
-    k->sched = buildSparseSchedule(t);
-    const SparseSchedule& sc = k->sched;
-    DevSparse& sp = k->sparse;
+  // sparse schedules (mechanisms they cannot express run the dense form only)
+  auto words = [](const std::vector<unsigned>& v) { return std::vector<int>(v.begin(), v.end()); };
+  auto pack = [&](const SparseSchedule& sc, DevSparse& sp) {
     sp.enabled = 1;
     sp.nnzF = sc.nnzF;
     sp.zeroSlot = sc.zeroSlot;
     sp.trashSlot = sc.trashSlot;
     sp.pairSlots = sc.pairSlots;
-    auto words = [](const std::vector<unsigned>& v) { return std::vector<int>(v.begin(), v.end()); };
     sp.oEvalHdr = addI(words(sc.evalHdr));
     sp.oEvalBody = addI(words(sc.evalBody));
     sp.evalBodyWords = int(sc.evalBody.size());
@@ -316,6 +397,10 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
     sp.pairBody = sc.pairBody;
     sp.segBody = sc.segBody;
     sp.jacBody = sc.jacBody;
+    sp.nutRounds = sc.nutRounds;
+    sp.jtvRounds = sc.jtvRounds;
+    sp.nutBody = sc.nutBody;
+    sp.jtvBody = sc.jtvBody;
     sp.oProgHdr = addI(words(sc.progHdr));
     sp.oProgBody = addI(words(sc.progBody));
     sp.luRounds = sc.luRounds;
@@ -323,8 +408,23 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
     sp.fwdRounds = sc.fwdRounds;
     sp.backBody = sc.backBody;
     sp.fwdBody = sc.fwdBody;
+  };
+  try {
+    k->sched = buildSparseSchedule(t);
+    pack(k->sched, k->sparse);
   } catch (const std::exception&) {
     k->sparse = DevSparse{};
+  }
+  if (k->sparse.enabled) {
+    // the adjoint shares the cell layout of the forward programs: the
+    // transposed factor pattern has the same size
+    try {
+      k->schedT = buildSparseSchedule(t, true);
+      if (k->schedT.nnzF == k->sched.nnzF && k->schedT.pairSlots == k->sched.pairSlots)
+        pack(k->schedT, k->sparseT);
+    } catch (const std::exception&) {
+      k->sparseT = DevSparse{};
+    }
   }
   ib.push_back(0);
   db.push_back(0.0);
@@ -367,36 +467,6 @@ int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBat
   if (!tol_newton || !tol_time || !tadv || !state || !t_out || !dt_out || !state_out)
     return setError(TCHEM_ERR_INVALID, "run_device_batch: null device array");
   if (kmcd->dev.nPhoto == 0) photo_rates = nullptr;
-
-  const size_t lds = ldsBytes(kmcd, o);
-  if (lds > kmcd->maxLds)
-    return setError(TCHEM_ERR_UNSUPPORTED, "run_device_batch: mechanism needs " + std::to_string(lds) +
-                                               " bytes of LDS per workgroup, device limit " +
-                                               std::to_string(kmcd->maxLds));
-  int cur = 0;
-  if (hipGetDevice(&cur) != hipSuccess || cur != kmcd->device)
-    return setError(TCHEM_ERR_INVALID, "run_device_batch: kmcd belongs to another device");
-
-  IntegratorArgs a{};
-  a.M = kmcd->dev;
-  a.S = kmcd->sparse;
-  a.sparse = o.lu == TCHEM_LU_SPARSE ? (sparseStaged(kmcd, o.precision) ? 1 : 2) : 0;
-  a.nBatch = nBatch;
-  a.tol_newton = tol_newton;
-  a.tol_time = tol_time;
-  a.tadv = tadv;
-  a.state = state;
-  a.photo = photo_rates;
-  a.t_out = t_out;
-  a.dt_out = dt_out;
-  a.state_out = state_out;
-  a.stats = stats;
-  a.queue = kmcd->queue;
-  a.persistent = (o.schedule == TCHEM_SCHEDULE_PERSISTENT) ? 1 : 0;
-  a.cellsPerGroup = o.cells_per_block;
-  a.cellBytes = a.sparse ? int(sparseCellBytes(kmcd->dev, kmcd->sparse,
-                                               o.precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float)))
-                         : 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (size_t(nBatch) > kmcd->kfCapacity) {
     // grows only: the previous buffer may still be read by work queued on s
@@ -411,30 +481,19 @@ int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBat
     kmcd->kf = static_cast<double*>(p);
     kmcd->kfCapacity = size_t(nBatch);
   }
+  IntegratorArgs a{};
+  a.nBatch = nBatch;
+  a.tol_newton = tol_newton;
+  a.tol_time = tol_time;
+  a.tadv = tadv;
+  a.state = state;
+  a.photo = photo_rates;
+  a.t_out = t_out;
+  a.dt_out = dt_out;
+  a.state_out = state_out;
+  a.stats = stats;
   a.kf = kmcd->kf;
-  {
-    hipError_t e = launchRates(kmcd->dev, nBatch, state, photo_rates, kmcd->kf, s);
-    if (e != hipSuccess) return hipFail(e, "rates_kernel launch");
-  }
-
-  const int cpg = a.cellsPerGroup;
-  const int groups = (nBatch + cpg - 1) / cpg;
-  int grid = groups;
-  if (a.persistent) {
-    int bpc = o.blocks_per_cu;
-    if (bpc == 0) {
-      hipError_t e = integratorMaxBlocksPerCU(o.precision, a.sparse, kmcd->dev.m, 64 * cpg, lds, &bpc);
-      if (e != hipSuccess) return hipFail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-      if (bpc < 1) bpc = 1;
-    }
-    const long long g = (long long)bpc * kmcd->numCUs;
-    grid = int(g < groups ? g : groups);
-    hipError_t e = launchResetCounter(kmcd->queue, s);
-    if (e != hipSuccess) return hipFail(e, "reset_counter_kernel");
-  }
-  hipError_t e = launchIntegrator(a, o.precision, grid, lds, s);
-  if (e != hipSuccess) return hipFail(e, "trbdf2_kernel launch");
-  return TCHEM_OK;
+  return runIntegrator(kmcd, o, nullptr, 0, a, s, "run_device_batch");
 }
 
 int tchem_atmchem_update_time_advance(int nBatch, tchem_time_advance_t* tadv, const double* t_out,
@@ -469,6 +528,143 @@ int tchem_jacobian_run_device_batch(int nBatch, const double* state, const doubl
   if (lds > kmcd->maxLds) return setError(TCHEM_ERR_UNSUPPORTED, "jacobian: mechanism too large for LDS");
   hipError_t e = launchJacobian(kmcd->dev, nBatch, state, photo_rates, jac, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hipFail(e, "jacobian_kernel launch");
+  return TCHEM_OK;
+}
+
+int tchem_adjoint_workspace_bytes(tchem_kmcd_t kmcd, int nBatch, int maxSteps, size_t* bytes) {
+  int rc = adjointCheck(kmcd, nBatch, maxSteps, "tchem_adjoint_workspace_bytes");
+  if (rc) return rc;
+  if (!bytes) return setError(TCHEM_ERR_INVALID, "tchem_adjoint_workspace_bytes: null bytes");
+  *bytes = adjointLayout(kmcd->dev, nBatch, maxSteps).total;
+  return TCHEM_OK;
+}
+
+int tchem_adjoint_forward(int nBatch, const double* tol_newton, const double* tol_time,
+                          const tchem_time_advance_t* tadv, double* state, const double* photo_rates,
+                          const double* log_rate_scale, int log_rate_scale_stride, double* t_out, double* dt_out,
+                          tchem_cell_stats_t* stats, void* workspace, int maxSteps, tchem_kmcd_t kmcd,
+                          void* stream) {
+  const char* who = "tchem_adjoint_forward";
+  int rc = adjointCheck(kmcd, nBatch, maxSteps, who);
+  if (rc) return rc;
+  if (log_rate_scale_stride != 0 && log_rate_scale_stride != kmcd->dev.nReac)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": log_rate_scale_stride must be 0 or nReac");
+  if (nBatch == 0) return TCHEM_OK;
+  if (!tol_newton || !tol_time || !tadv || !state || !t_out || !dt_out || !workspace)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": null device array");
+  if (reinterpret_cast<uintptr_t>(workspace) % 256)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": workspace must be 256-byte aligned");
+  if (kmcd->dev.nPhoto == 0) photo_rates = nullptr;
+  tchem_atmchem_options_t req{};
+  req.precision = TCHEM_PRECISION_FP64;
+  req.schedule = TCHEM_SCHEDULE_PERSISTENT;
+  req.lu = TCHEM_LU_SPARSE;
+  tchem_atmchem_options_t o;
+  rc = resolveOptions(&req, kmcd, &o, who);
+  if (rc) return rc;
+  const AdjointLayout W = adjointLayout(kmcd->dev, nBatch, maxSteps);
+  char* ws = static_cast<char*>(workspace);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = launchAdjointSetup(nBatch, maxSteps, tadv, tol_newton, reinterpret_cast<int*>(ws + W.header),
+                                    reinterpret_cast<double*>(ws + W.tol), reinterpret_cast<int*>(ws + W.newton), s);
+  if (e != hipSuccess) return hipFail(e, "adjoint_setup_kernel");
+  IntegratorArgs a{};
+  a.nBatch = nBatch;
+  a.tol_newton = tol_newton;
+  a.tol_time = tol_time;
+  a.tadv = tadv;
+  a.state = state;
+  a.photo = photo_rates;
+  a.t_out = t_out;
+  a.dt_out = dt_out;
+  a.state_out = state;
+  a.stats = stats;
+  a.kf = reinterpret_cast<double*>(ws + W.kf);
+  a.rec.nsteps = reinterpret_cast<int*>(ws + W.nsteps);
+  a.rec.h = reinterpret_cast<double*>(ws + W.h);
+  a.rec.ckpt = reinterpret_cast<double*>(ws + W.ckpt);
+  a.rec.consts = reinterpret_cast<double*>(ws + W.consts);
+  a.rec.maxSteps = maxSteps;
+  return runIntegrator(kmcd, o, log_rate_scale, log_rate_scale_stride, a, s, who);
+}
+
+int tchem_adjoint_backward(int nBatch, const double* lambda, double* lambda_out, double* grad, void* workspace,
+                           int maxSteps, tchem_kmcd_t kmcd, void* stream) {
+  const char* who = "tchem_adjoint_backward";
+  int rc = adjointCheck(kmcd, nBatch, maxSteps, who);
+  if (rc) return rc;
+  if (nBatch == 0) return TCHEM_OK;
+  if (!lambda || !lambda_out || !grad || !workspace)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": null device array");
+  if (reinterpret_cast<uintptr_t>(workspace) % 256)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": workspace must be 256-byte aligned");
+  int cur = 0;
+  if (hipGetDevice(&cur) != hipSuccess || cur != kmcd->device)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": kmcd belongs to another device");
+  const DevMech& M = kmcd->dev;
+  const size_t cellB = adjointCellBytes(M, kmcd->sparse);
+  int cpg = 8;
+  while (cpg > 1 && size_t(cpg) * cellB > kmcd->maxLds) --cpg;
+  if (cellB > kmcd->maxLds)
+    return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": one cell needs " + std::to_string(cellB) +
+                                               " bytes of LDS, device limit " + std::to_string(kmcd->maxLds));
+  const size_t lds = size_t(cpg) * cellB;
+  const AdjointLayout W = adjointLayout(M, nBatch, maxSteps);
+  char* ws = static_cast<char*>(workspace);
+  AdjointArgs a{};
+  a.fw.M = M;
+  a.fw.S = kmcd->sparse;
+  a.fw.nBatch = nBatch;
+  a.fw.kf = reinterpret_cast<const double*>(ws + W.kf);
+  a.fw.queue = kmcd->queue;
+  a.fw.persistent = 1;
+  a.fw.sparse = 2;
+  a.fw.cellsPerGroup = cpg;
+  a.fw.cellBytes = int(cellB);
+  a.fw.rec.nsteps = reinterpret_cast<int*>(ws + W.nsteps);
+  a.fw.rec.h = reinterpret_cast<double*>(ws + W.h);
+  a.fw.rec.ckpt = reinterpret_cast<double*>(ws + W.ckpt);
+  a.fw.rec.consts = reinterpret_cast<double*>(ws + W.consts);
+  a.fw.rec.maxSteps = maxSteps;
+  a.ST = kmcd->sparseT;
+  a.header = reinterpret_cast<const int*>(ws + W.header);
+  a.tol = reinterpret_cast<const double*>(ws + W.tol);
+  a.newton = reinterpret_cast<const int*>(ws + W.newton);
+  a.traj = reinterpret_cast<double*>(ws + W.traj);
+  a.lambda_in = lambda;
+  a.lambda_out = lambda_out;
+  a.grad = grad;
+  a.cellsPerGroup = cpg;
+  a.cellBytes = int(cellB);
+  int bpc = 0;
+  hipError_t e = adjointMaxBlocksPerCU(M.m, 64 * cpg, lds, &bpc);
+  if (e != hipSuccess) return hipFail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+  if (bpc < 1) bpc = 1;
+  const int groups = (nBatch + cpg - 1) / cpg;
+  const long long g = (long long)bpc * kmcd->numCUs;
+  const int grid = int(g < groups ? g : groups);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  e = launchResetCounter(kmcd->queue, s);
+  if (e != hipSuccess) return hipFail(e, "reset_counter_kernel");
+  e = launchAdjoint(a, grid, lds, s);
+  if (e != hipSuccess) return hipFail(e, "adjoint_kernel launch");
+  return TCHEM_OK;
+}
+
+int tchem_adjoint_steps(int nBatch, const void* workspace, int maxSteps, int* nsteps, double* h, tchem_kmcd_t kmcd,
+                        void* stream) {
+  const char* who = "tchem_adjoint_steps";
+  int rc = adjointCheck(kmcd, nBatch, maxSteps, who);
+  if (rc) return rc;
+  if (nBatch == 0) return TCHEM_OK;
+  if (!workspace || !nsteps) return setError(TCHEM_ERR_INVALID, std::string(who) + ": null device array");
+  const AdjointLayout W = adjointLayout(kmcd->dev, nBatch, maxSteps);
+  const char* ws = static_cast<const char*>(workspace);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = hipMemcpyAsync(nsteps, ws + W.nsteps, sizeof(int) * size_t(nBatch), hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess && h)
+    e = hipMemcpyAsync(h, ws + W.h, sizeof(double) * size_t(nBatch) * size_t(maxSteps), hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) return hipFail(e, "hipMemcpyAsync");
   return TCHEM_OK;
 }
 

@@ -51,6 +51,7 @@ struct DevSparse {
   int oEvalHdr, oEvalBody, evalBodyWords, oCoef, nCoef;
   int ropRounds, pairRounds, segRounds, jacRounds;
   int pairBody, segBody, jacBody;
+  int nutRounds, jtvRounds, nutBody, jtvBody;  // adjoint sections (transposed programs only)
   // LU program: the LU and BACK sections are staged into LDS, FWD (factor
   // reuse) is read from memory
   int oProgHdr, oProgBody;
@@ -78,6 +79,18 @@ __host__ __device__ inline size_t sparseProgramBytes(const DevSparse& S) {
   return 4 * sparseCoefOffsetWords(S) + ((8 * size_t(S.nCoef) + 15) & ~size_t(15));
 }
 
+// Forward record of the discrete adjoint: accepted step sizes and the state
+// every kAdjointSegment accepted steps (checkpoint j = state after 16 j
+// steps, internal species order).
+constexpr int kAdjointSegment = 16;
+struct AdjointRecord {
+  int* nsteps;      // [nBatch] accepted steps, -1 when the cell failed or the record overflowed
+  double* h;        // [nBatch][maxSteps]
+  double* ckpt;     // [nBatch][maxSteps / 16 + 1][m]
+  double* consts;   // [nBatch][nConst] constant-species concentrations
+  int maxSteps;
+};
+
 struct IntegratorArgs {
   DevMech M;
   DevSparse S;
@@ -97,6 +110,7 @@ struct IntegratorArgs {
   int sparse;                        // 0 dense LU; 1 sparse, programs staged in LDS; 2 sparse, programs in memory
   int cellsPerGroup;                 // wavefronts (cells) per workgroup
   int cellBytes;                     // LDS bytes of one cell region
+  AdjointRecord rec;                 // h == nullptr: no record
 };
 
 // LDS bytes of one cell: the iteration matrix in TA precision (bytesA per
@@ -118,10 +132,31 @@ inline size_t integratorLdsBytes(const DevMech& M, size_t bytesA) {
 // trash slot) and the Jacobian derivative pairs in TA precision, then
 // doubles for concentrations, rate constants (+ a zero), rates of progress
 // (+ zero slot) and source-term segment sums (+ trash slot).
-inline size_t sparseCellBytes(const DevMech& M, const DevSparse& S, size_t bytesA) {
+__host__ __device__ inline size_t sparseCellBytes(const DevMech& M, const DevSparse& S, size_t bytesA) {
   auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
   return al(bytesA * size_t(S.trashSlot + 1)) + al(bytesA * size_t(M.m + 2)) + al(bytesA * size_t(S.pairSlots)) +
          al(sizeof(double) * (size_t(M.nSpec) + size_t(M.nReac + 1) + size_t(M.nReac + 1) + size_t(M.nSeg + 1)));
+}
+
+// Arguments of the discrete adjoint kernel (backward sweep over a record).
+struct AdjointArgs {
+  IntegratorArgs fw;        // M, S (forward programs), nBatch, kf, rec, queue
+  DevSparse ST;             // transposed programs
+  const int* header;        // [4] nBatch, maxSteps of the record
+  const double* tol;        // [2] Newton atol, rtol of the recording run
+  const int* newton;        // [nBatch][2] max Newton iterations, Jacobian interval
+  double* traj;             // [nBatch][2 kAdjointSegment + 1][m] replayed states
+  const double* lambda_in;  // [nBatch][nSpec] dL/d(final species), reference order
+  double* lambda_out;       // [nBatch][nSpec] dL/d(initial species); constant species 0
+  double* grad;             // [nBatch][nReac] dL/dtheta per cell
+  int cellsPerGroup;
+  int cellBytes;            // LDS bytes of one cell region
+};
+
+// LDS of the adjoint beyond the sparse cell region: mu, w, v, grad
+__host__ __device__ inline size_t adjointExtraBytes(const DevMech& M) {
+  auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
+  return al(8 * size_t(M.m + 1)) + al(8 * size_t(M.nReac + 2)) + al(8 * size_t(M.m + 1)) + al(8 * size_t(M.nReac));
 }
 
 }  // namespace tchem_amd

@@ -489,6 +489,10 @@ struct SpLds {
   double* kf;   // [nReac+1], kf[nReac] = 0
   double* rop;  // [nReac+1], rop[nReac] = 0
   double* seg;  // [nSeg+1], seg[nSeg] = trash
+  // discrete adjoint only (nullptr otherwise)
+  double* mu;   // [m+1] adjoint vector, mu[m] = 0
+  double* w;    // [nReac+2] nu_r . mu, w[nReac] = 0, w[nReac+1] = trash
+  double* v;    // [m+1] J^T mu, v[m] = trash
 };
 
 // STAGED: programs at the start of the workgroup's LDS, then one cell region
@@ -526,6 +530,7 @@ __device__ SpLds<TA> carve_sparse(char* base, const DevMech& M, const DevSparse&
   L.kf = L.c + M.nSpec;
   L.rop = L.kf + M.nReac + 1;
   L.seg = L.rop + M.nReac + 1;
+  L.mu = L.w = L.v = nullptr;
   return L;
 }
 
@@ -546,7 +551,7 @@ __device__ __forceinline__ double reactant_factor(const double* c, unsigned f) {
 // products in the same order as the dense form.  Rounds are independent;
 // the next round's two words are loaded while a round computes.
 template <typename TA>
-__device__ void sp_rates(const DevSparse& S, const SpLds<TA>& L, int lane, bool pairs) {
+__device__ __forceinline__ void sp_rates(const DevSparse& S, const SpLds<TA>& L, int lane, bool pairs) {
   {
     const unsigned* b = L.ebody;
     unsigned w0 = b[lane], w1 = b[kWave + lane];
@@ -613,15 +618,47 @@ __device__ __forceinline__ void jac_round(const unsigned (&w)[1 + SparseSchedule
   L.A[t] = init + tscale * sum;
 }
 
-// Source-term segment sums (kSeg rounds) and, with jac, A = diag * I +
-// scale * J on the factor pattern (kJac rounds; fill positions cleared).
-template <typename TA>
-__device__ void sp_sums(const DevSparse& S, const SpLds<TA>& L, int lane, bool jac, double diag, double scale) {
+// nu_r . mu (kNuT): w[t] = (cont ? w[t] : 0) + sum coef * mu[species]
+template <int LEN, typename TA>
+__device__ __forceinline__ void nut_round(const unsigned (&w)[1 + SparseSchedule::kMaxChain], const SpLds<TA>& L) {
+  const unsigned t = w[0] & ~SparseSchedule::kCont;
+  double acc = (w[0] & SparseSchedule::kCont) ? L.w[t] : 0.0;
+  double cf[LEN], mv[LEN];
+#pragma unroll
+  for (int j = 0; j < LEN; ++j) {
+    cf[j] = L.coef[w[1 + j] >> 16];
+    mv[j] = L.mu[w[1 + j] & 0xffff];
+  }
+#pragma unroll
+  for (int j = 0; j < LEN; ++j) acc += cf[j] * mv[j];
+  L.w[t] = acc;
+}
+// J^T mu (kJtv): v[t] = (cont ? v[t] : 0) + sum dpr[pair] * w[reaction]
+template <int LEN, typename TA>
+__device__ __forceinline__ void jtv_round(const unsigned (&w)[1 + SparseSchedule::kMaxChain], const SpLds<TA>& L) {
+  const unsigned t = w[0] & ~SparseSchedule::kCont;
+  double acc = (w[0] & SparseSchedule::kCont) ? L.v[t] : 0.0;
+  double dv[LEN], wv[LEN];
+#pragma unroll
+  for (int j = 0; j < LEN; ++j) {
+    dv[j] = double(L.dpr[w[1 + j] & 0xffff]);
+    wv[j] = L.w[w[1 + j] >> 16];
+  }
+#pragma unroll
+  for (int j = 0; j < LEN; ++j) acc += dv[j] * wv[j];
+  L.v[t] = acc;
+}
+
+// Runs nr chained rounds of the evaluation program starting at header
+// index r0 and body word offset body0: kSeg / kJac rounds, and with ADJ
+// also kNuT / kJtv.  Rounds run in order (a chain split over rounds
+// continues in a later one); the next round's words are loaded while a
+// round computes.
+template <bool ADJ, typename TA>
+__device__ __forceinline__ void eval_rounds(const SpLds<TA>& L, int lane, int r0, int nr, int body0, double diag, double scale) {
   constexpr int kW = 1 + SparseSchedule::kMaxChain;
-  const int r0 = S.ropRounds + S.pairRounds;
-  const int nr = S.segRounds + (jac ? S.jacRounds : 0);
   const unsigned* hdr = L.ehdr + r0;
-  const unsigned* b = L.ebody + S.segBody;
+  const unsigned* b = L.ebody + body0;
   const TA tdiag = TA(diag), tscale = TA(scale);
   unsigned w[kW];
 #pragma unroll
@@ -641,18 +678,40 @@ __device__ void sp_sums(const DevSparse& S, const SpLds<TA>& L, int lane, bool j
         case 3: seg_round<3>(w, L); break;
         default: seg_round<4>(w, L); break;
       }
-    } else {
+    } else if (!ADJ || kind == SparseSchedule::kJac) {
       switch (len) {
         case 1: jac_round<1>(w, L, tdiag, tscale); break;
         case 2: jac_round<2>(w, L, tdiag, tscale); break;
         case 3: jac_round<3>(w, L, tdiag, tscale); break;
         default: jac_round<4>(w, L, tdiag, tscale); break;
       }
+    } else if (kind == SparseSchedule::kNuT) {
+      switch (len) {
+        case 1: nut_round<1>(w, L); break;
+        case 2: nut_round<2>(w, L); break;
+        case 3: nut_round<3>(w, L); break;
+        default: nut_round<4>(w, L); break;
+      }
+    } else {
+      switch (len) {
+        case 1: jtv_round<1>(w, L); break;
+        case 2: jtv_round<2>(w, L); break;
+        case 3: jtv_round<3>(w, L); break;
+        default: jtv_round<4>(w, L); break;
+      }
     }
     h = __builtin_amdgcn_readfirstlane(hn);
 #pragma unroll
     for (int j = 0; j < kW; ++j) w[j] = wn[j];
   }
+}
+
+// Source-term segment sums (kSeg rounds) and, with jac, A = diag * I +
+// scale * J on the factor pattern (kJac rounds; fill positions cleared).
+template <typename TA>
+__device__ void sp_sums(const DevSparse& S, const SpLds<TA>& L, int lane, bool jac, double diag, double scale) {
+  eval_rounds<false>(L, lane, S.ropRounds + S.pairRounds, S.segRounds + (jac ? S.jacRounds : 0), S.segBody, diag,
+                     scale);
 }
 
 // per-species sums of the segment sums plus emission (register layout)
@@ -841,7 +900,7 @@ struct SparseBackend {
     }
     wave_sync();
   }
-  static __device__ void put_c(const IntegratorArgs& a, const Lds& L, int lane, const double (&u)[QM]) {
+  static __device__ __forceinline__ void put_c(const IntegratorArgs& a, const Lds& L, int lane, const double (&u)[QM]) {
 #pragma unroll
     for (int q = 0; q < QM; ++q) {
       const int i = lane + kWave * q;
@@ -849,9 +908,9 @@ struct SparseBackend {
     }
     wave_sync();
   }
-  static __device__ void evaluate(const IntegratorArgs& a, const Lds& L, int lane, const double (&u)[QM],
-                                  const double (&bc)[QM], double ch, bool newton, bool refresh, double (&f)[QM],
-                                  TA (&x)[QM]) {
+  static __device__ __forceinline__ void evaluate(const IntegratorArgs& a, const Lds& L, int lane,
+                                                  const double (&u)[QM], const double (&bc)[QM], double ch,
+                                                  bool newton, bool refresh, double (&f)[QM], TA (&x)[QM]) {
     TCHEM_PT_BEGIN(rates)
     put_c(a, L, lane, u);
     sp_rates(a.S, L, lane, refresh);
@@ -873,7 +932,7 @@ struct SparseBackend {
     TCHEM_PT_END(rows, 13)
   }
   // after the BACK section the RHS holds u_ii x_i
-  static __device__ void get_x(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM]) {
+  static __device__ __forceinline__ void get_x(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM]) {
 #pragma unroll
     for (int q = 0; q < QM; ++q) {
       const int i = lane + kWave * q;
@@ -886,7 +945,8 @@ struct SparseBackend {
                 a.S.trashSlot, lane);
     TCHEM_PT_END(back, 10)
   }
-  static __device__ bool factor_solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM], Fact&) {
+  static __device__ __forceinline__ bool factor_solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM],
+                                                      Fact&) {
     TCHEM_PT_BEGIN(lu)
     const bool ok = run_program(L.hdr, L.body, a.S.luRounds, L.A, L.rdiag, a.M.m, a.S.nnzF, a.S.trashSlot, lane);
     TCHEM_PT_END(lu, 9)
@@ -895,7 +955,8 @@ struct SparseBackend {
     get_x(a, L, lane, x);
     return true;
   }
-  static __device__ void solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM], const Fact&) {
+  static __device__ __forceinline__ void solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM],
+                                               const Fact&) {
     const unsigned* fh = reinterpret_cast<const unsigned*>(a.M.ib + a.S.oProgHdr) + a.S.luRounds + a.S.backRounds;
     const unsigned* fb = reinterpret_cast<const unsigned*>(a.M.ib + a.S.oProgBody) + a.S.fwdBody;
     run_program(fh, fb, a.S.fwdRounds, L.A, L.rdiag, a.M.m, a.S.nnzF, a.S.trashSlot, lane);
@@ -931,7 +992,8 @@ __device__ __forceinline__ double uniform_d(double v) {
 //          pivot, dt = h/2 and the step is retried (at most 4 attempts);
 //   STAGE  f at the converged stage value: after the TR stage set up BDF2;
 //          after BDF2 estimate the local error and accept or reject.
-template <class B, int QM>
+// REC: also record the accepted steps for the discrete adjoint (a.rec).
+template <class B, int QM, bool REC>
 __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L, int cell, int lane) {
   using TA = typename B::TA;
   const DevMech& M = a.M;
@@ -944,6 +1006,12 @@ __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L
   const double t_in = a.t_out[cell];
   if (t_in >= tend || ta._dt < 0.0) {  // skipped (.cpp:83) or already failed
     if (a.stats && lane == 0) a.stats[cell] = tchem_cell_stats_t{0, 0, 0, 0, 0, ta._dt < 0.0 ? 1 : 0, 0, 0};
+    if (REC) {
+      const int nCkpt0 = a.rec.maxSteps / kAdjointSegment + 1;
+      for (int ai = lane; ai < m; ai += kWave) a.rec.ckpt[size_t(cell) * nCkpt0 * m + ai] = row[3 + M.perm()[ai]];
+      for (int i = m + lane; i < nSpec; i += kWave) a.rec.consts[size_t(cell) * (nSpec - m) + (i - m)] = row[3 + i];
+      if (lane == 0) a.rec.nsteps[cell] = ta._dt < 0.0 ? -1 : 0;
+    }
     return;
   }
 
@@ -972,6 +1040,16 @@ __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L
     fn[q] = bc[q] = est[q] = 0.0;
   }
   const double atol_n = a.tol_newton[0], rtol_n = a.tol_newton[1];
+  const int nCkpt = REC ? a.rec.maxSteps / kAdjointSegment + 1 : 0;
+  bool overflow = false;
+  if (REC) {
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const int ai = lane + kWave * q;
+      if (ai < m) a.rec.ckpt[size_t(cell) * nCkpt * m + ai] = un[q];
+    }
+    for (int i = m + lane; i < nSpec; i += kWave) a.rec.consts[size_t(cell) * (nSpec - m) + (i - m)] = row[3 + i];
+  }
   wave_sync();
 
   const int max_it = ta._num_time_iterations_per_interval;
@@ -1121,6 +1199,21 @@ __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L
     }
     dt = (e <= 0.125) ? 2.0 * h : h;
     if (dt > ta._dtmax) dt = ta._dtmax;
+    if (REC) {
+      const int k = cnt.accepted;  // index of this step
+      if (k < a.rec.maxSteps) {
+        if (lane == 0) a.rec.h[size_t(cell) * a.rec.maxSteps + k] = h;
+        if ((k + 1) % kAdjointSegment == 0) {
+#pragma unroll
+          for (int q = 0; q < QM; ++q) {
+            const int ai = lane + kWave * q;
+            if (ai < m) a.rec.ckpt[(size_t(cell) * nCkpt + (k + 1) / kAdjointSegment) * m + ai] = un[q];
+          }
+        }
+      } else {
+        overflow = true;
+      }
+    }
     ++cnt.accepted;
   }
 
@@ -1146,6 +1239,7 @@ __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L
     rowo[2] = row[2];
     a.t_out[cell] = failed ? tend : t;
     a.dt_out[cell] = failed ? -1.0 : dt;
+    if (REC) a.rec.nsteps[cell] = (failed || overflow) ? -1 : cnt.accepted;
     if (a.stats) {
       cnt.failed = failed ? 1 : 0;
       a.stats[cell] = tchem_cell_stats_t{cnt.accepted, cnt.rejected, cnt.newton_failures, cnt.newton_iterations,
@@ -1157,7 +1251,7 @@ __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L
 
 // Workgroup = cellsPerGroup wavefronts, one cell per wavefront at a time;
 // after staging, wavefronts never synchronise with each other.
-template <class B, int QM, int kThreads>
+template <class B, int QM, int kThreads, bool REC>
 __global__ __launch_bounds__(kThreads) void trbdf2_kernel(IntegratorArgs a) {
   extern __shared__ __align__(16) char lds_raw[];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -1167,7 +1261,7 @@ __global__ __launch_bounds__(kThreads) void trbdf2_kernel(IntegratorArgs a) {
   TCHEM_PT_RESET()
   if (!a.persistent) {
     const int cell = blockIdx.x * a.cellsPerGroup + wave;
-    if (cell < a.nBatch) integrate_cell<B, QM>(a, L, cell, lane);
+    if (cell < a.nBatch) integrate_cell<B, QM, REC>(a, L, cell, lane);
     return;
   }
   for (;;) {
@@ -1175,7 +1269,7 @@ __global__ __launch_bounds__(kThreads) void trbdf2_kernel(IntegratorArgs a) {
     if (lane == 0) cell = atomicAdd(a.queue, 1);
     cell = __builtin_amdgcn_readfirstlane(__shfl(cell, 0, kWave));
     if (cell >= a.nBatch) break;
-    integrate_cell<B, QM>(a, L, cell, lane);
+    integrate_cell<B, QM, REC>(a, L, cell, lane);
     wave_sync();
   }
 }
@@ -1183,8 +1277,11 @@ __global__ __launch_bounds__(kThreads) void trbdf2_kernel(IntegratorArgs a) {
 // Rate constants of every (cell, reaction), one thread each; read by the
 // integrator at the start of a cell.  Keeping the transcendental code out of
 // the integrator keeps its register footprint small.
+// lscale (nullable): kf_r *= exp(lscale[cell * lstride + r]), the log-scale
+// parameters the discrete adjoint differentiates with respect to (lstride 0:
+// one set for every cell).
 __global__ __launch_bounds__(256) void rates_kernel(DevMech M, int nBatch, const double* state, const double* photo,
-                                                   double* kf) {
+                                                   const double* lscale, int lstride, double* kf) {
   const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
   if (idx >= (long long)nBatch * M.nReac) return;
   const int cell = int(idx / M.nReac), r = int(idx - (long long)cell * M.nReac);
@@ -1192,7 +1289,289 @@ __global__ __launch_bounds__(256) void rates_kernel(DevMech M, int nBatch, const
   const double p = row[1], T = row[2];
   const double t_1 = 1.0 / T;
   const double conv = M.convPPM * p * t_1;
-  kf[idx] = rate_constant(M, r, T, p, t_1, conv, photo ? photo + size_t(cell) * M.nPhoto : nullptr);
+  const double k = rate_constant(M, r, T, p, t_1, conv, photo ? photo + size_t(cell) * M.nPhoto : nullptr);
+  kf[idx] = lscale ? k * exp(lscale[size_t(cell) * lstride + r]) : k;
+}
+
+// ---- discrete adjoint --------------------------------------------------------
+// Reverse-mode derivative of the recorded TR-BDF2 trajectory (accepted steps
+// only, step sizes held fixed) with respect to the initial active species
+// and the log-scale parameters theta_r of the rate constants
+// (kf_r = k_r exp(theta_r), so d f / d theta_r = nu_r rop_r).  For one step
+// u_n -> u_g -> u_{n+1},
+//   TR:   u_g - u_n - c_tr h (f(u_g) + f(u_n)) = 0
+//   BDF2: u_{n+1} - a_g u_g + a_n u_n - c_bdf h f(u_{n+1}) = 0,
+// lambda = dL/du_{n+1} propagates as
+//   (I - c_bdf h J(u_{n+1}))^T mu2 = lambda
+//   (I - c_tr h J(u_g))^T mu1     = a_g mu2
+//   lambda_n = -a_n mu2 + mu1 + c_tr h J(u_n)^T mu1
+//   dL/dtheta_r += c_bdf h rop_r(u_{n+1}) (nu_r . mu2)
+//                + c_tr h (rop_r(u_g) + rop_r(u_n)) (nu_r . mu1).
+// The transposed systems are assembled and factorised directly by the
+// transposed programs (I - c h J^T on the transposed factor pattern).
+// Checkpoints every kAdjointSegment accepted steps; each segment is replayed
+// forward with the forward programs (the same arithmetic as the recording
+// integrator, hence the same Newton iterates), its states kept in device
+// scratch, then swept in reverse.
+
+template <int QM>
+__device__ __forceinline__ void load_vec(const double* src, int m, int lane, double (&u)[QM]) {
+#pragma unroll
+  for (int q = 0; q < QM; ++q) {
+    const int i = lane + kWave * q;
+    u[q] = i < m ? src[i] : 0.0;
+  }
+}
+template <int QM>
+__device__ __forceinline__ void store_vec(double* dst, int m, int lane, const double (&u)[QM]) {
+#pragma unroll
+  for (int q = 0; q < QM; ++q) {
+    const int i = lane + kWave * q;
+    if (i < m) dst[i] = u[q];
+  }
+}
+
+// One Newton solve of a stage, as the integrator runs it (same refresh
+// rule, convergence test and arithmetic).
+template <int QM>
+__device__ __forceinline__ void replay_newton(const IntegratorArgs& f, const SpLds<double>& L, int lane, double (&u)[QM],
+                              const double (&bc)[QM], double ch, int max_newton, int jint, double atol, double rtol) {
+  using B = SparseBackend<double, QM, false>;
+  typename B::Fact fs;
+  for (int nit = 0; nit < max_newton; ++nit) {
+    const bool refresh = (nit % jint) == 0;
+    double fv[QM], x[QM];
+    B::evaluate(f, L, lane, u, bc, ch, true, refresh, fv, x);
+    if (refresh) {
+      if (!B::factor_solve(f, L, lane, x, fs)) return;
+    } else {
+      B::solve(f, L, lane, x, fs);
+    }
+    double nrm = 0.0;
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const int i = lane + kWave * q;
+      if (i < f.M.m) {
+        u[q] += x[q];
+        double w = fabs(x[q]) / (atol + rtol * fabs(u[q]));
+        if (!(w == w)) w = INFINITY;
+        nrm = fmax(nrm, w);
+      }
+    }
+    if (uniform_d(wave_max(nrm)) <= 1.0) return;
+  }
+}
+
+// Solves (I - ch J^T) mu = rhs at the Jacobian pairs in LDS; mu in registers
+// and in L.mu, w = nu . mu.  Returns false on a zero or non-finite pivot.
+template <int QM>
+__device__ __forceinline__ bool adjoint_solve(const AdjointArgs& a, const SpLds<double>& LT, int lane, double ch,
+                              const double (&rhs)[QM], double (&mu)[QM]) {
+  const DevSparse& S = a.ST;
+  const int m = a.fw.M.m;
+  eval_rounds<true>(LT, lane, S.ropRounds + S.pairRounds + S.segRounds, S.jacRounds, S.jacBody, 1.0, -ch);
+#pragma unroll
+  for (int q = 0; q < QM; ++q) {
+    const int i = lane + kWave * q;
+    if (i < m) LT.A[S.nnzF + i] = rhs[q];
+  }
+  wave_sync();
+  const bool ok = run_program(LT.hdr, LT.body, S.luRounds, LT.A, LT.rdiag, m, S.nnzF, S.trashSlot, lane);
+  run_program(LT.hdr + S.luRounds, LT.body + S.backBody, S.backRounds, LT.A, LT.rdiag, m, S.nnzF, S.trashSlot, lane);
+#pragma unroll
+  for (int q = 0; q < QM; ++q) {
+    const int i = lane + kWave * q;
+    mu[q] = i < m ? LT.A[S.nnzF + i] * LT.rdiag[i] : 0.0;
+    if (i < m) LT.mu[i] = mu[q];
+  }
+  wave_sync();
+  eval_rounds<true>(LT, lane, S.ropRounds + S.pairRounds + S.segRounds + S.jacRounds, S.nutRounds, S.nutBody, 0.0,
+                    0.0);
+  wave_sync();
+  return ok;
+}
+
+// grad_r += c rop_r w_r
+__device__ __forceinline__ void accumulate_grad(const SpLds<double>& L, double* g, int nReac, int lane, double c) {
+  for (int r = lane; r < nReac; r += kWave) g[r] += c * L.rop[r] * L.w[r];
+}
+
+template <int QM>
+__device__ void adjoint_cell(const AdjointArgs& a, const SpLds<double>& L, const SpLds<double>& LT, double* g,
+                             int cell, int lane) {
+  const IntegratorArgs& f = a.fw;
+  const DevMech& M = f.M;
+  const int m = M.m, nSpec = M.nSpec, nReac = M.nReac;
+  const int maxSteps = f.rec.maxSteps;
+  const int nCkpt = maxSteps / kAdjointSegment + 1;
+  constexpr int kTraj = 2 * kAdjointSegment + 1;
+  double* lam_out = a.lambda_out + size_t(cell) * nSpec;
+  double* gout = a.grad + size_t(cell) * nReac;
+  // a record of other sizes: only its header (offset 0) is read
+  const bool foreign = a.header[0] != f.nBatch || a.header[1] != maxSteps;
+  const int n = foreign ? -1 : f.rec.nsteps[cell];
+  if (n < 0) {  // failed, overflowed or foreign record
+    for (int i = lane; i < nSpec; i += kWave) lam_out[i] = NAN;
+    for (int r = lane; r < nReac; r += kWave) gout[r] = NAN;
+    return;
+  }
+  const double gam = 2.0 - 1.41421356237309504880;
+  const double c_tr = 0.5 * gam;
+  const double a_g = 1.0 / (gam * (2.0 - gam));
+  const double a_n = (1.0 - gam) * (1.0 - gam) / (gam * (2.0 - gam));
+  const double c_bdf = (1.0 - gam) / (2.0 - gam);
+  const double atol = a.tol[0], rtol = a.tol[1];
+  const int max_newton = a.newton[2 * cell], jint = a.newton[2 * cell + 1];
+
+  for (int r = lane; r < nReac; r += kWave) {
+    L.kf[r] = f.kf[size_t(cell) * nReac + r];
+    g[r] = 0.0;
+  }
+  for (int i = m + lane; i < nSpec; i += kWave) L.c[i] = f.rec.consts[size_t(cell) * (nSpec - m) + (i - m)];
+  double lam[QM];
+#pragma unroll
+  for (int q = 0; q < QM; ++q) {
+    const int ai = lane + kWave * q;
+    lam[q] = ai < m ? a.lambda_in[size_t(cell) * nSpec + M.perm()[ai]] : 0.0;
+  }
+  wave_sync();
+
+  using B = SparseBackend<double, QM, false>;
+  double* tr = a.traj + size_t(cell) * kTraj * m;
+  const double* hrec = f.rec.h + size_t(cell) * maxSteps;
+  bool ok = true;
+  for (int e = (n + kAdjointSegment - 1) / kAdjointSegment - 1; e >= 0; --e) {
+    const int k0 = e * kAdjointSegment;
+    const int len = n - k0 < kAdjointSegment ? n - k0 : kAdjointSegment;
+    // replay the segment from its checkpoint
+    {
+      double un[QM], u[QM], fn[QM], bc[QM], x[QM];
+      load_vec<QM>(f.rec.ckpt + (size_t(cell) * nCkpt + e) * m, m, lane, un);
+      store_vec<QM>(tr, m, lane, un);
+#pragma unroll
+      for (int q = 0; q < QM; ++q) bc[q] = 0.0;
+      B::evaluate(f, L, lane, un, bc, 0.0, false, false, fn, x);
+      for (int j = 0; j < len; ++j) {
+        const double h = hrec[k0 + j];
+#pragma unroll
+        for (int q = 0; q < QM; ++q) {
+          u[q] = un[q];
+          bc[q] = un[q] + c_tr * h * fn[q];
+        }
+        replay_newton<QM>(f, L, lane, u, bc, c_tr * h, max_newton, jint, atol, rtol);
+        store_vec<QM>(tr + size_t(kAdjointSegment + 1 + j) * m, m, lane, u);
+#pragma unroll
+        for (int q = 0; q < QM; ++q) bc[q] = a_g * u[q] - a_n * un[q];
+        replay_newton<QM>(f, L, lane, u, bc, c_bdf * h, max_newton, jint, atol, rtol);
+        store_vec<QM>(tr + size_t(j + 1) * m, m, lane, u);
+#pragma unroll
+        for (int q = 0; q < QM; ++q) un[q] = u[q];
+        if (j + 1 < len) B::evaluate(f, L, lane, un, bc, 0.0, false, false, fn, x);
+      }
+    }
+    // reverse sweep; rop and the Jacobian pairs at u_{n+1} are still in LDS
+    // from the previous (later) step except for the segment's last step
+    for (int j = len - 1; j >= 0; --j) {
+      const double h = hrec[k0 + j];
+      double u[QM], mu2[QM], mu1[QM], rhs[QM];
+      if (j == len - 1) {
+        load_vec<QM>(tr + size_t(j + 1) * m, m, lane, u);
+        B::put_c(f, L, lane, u);
+        sp_rates(a.ST, LT, lane, true);
+        wave_sync();
+      }
+      ok &= adjoint_solve<QM>(a, LT, lane, c_bdf * h, lam, mu2);
+      accumulate_grad(LT, g, nReac, lane, c_bdf * h);
+      load_vec<QM>(tr + size_t(kAdjointSegment + 1 + j) * m, m, lane, u);
+      B::put_c(f, L, lane, u);
+      sp_rates(a.ST, LT, lane, true);
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < QM; ++q) rhs[q] = a_g * mu2[q];
+      ok &= adjoint_solve<QM>(a, LT, lane, c_tr * h, rhs, mu1);
+      accumulate_grad(LT, g, nReac, lane, c_tr * h);
+      load_vec<QM>(tr + size_t(j) * m, m, lane, u);
+      B::put_c(f, L, lane, u);
+      sp_rates(a.ST, LT, lane, true);
+      wave_sync();
+      accumulate_grad(LT, g, nReac, lane, c_tr * h);
+      eval_rounds<true>(LT, lane, a.ST.ropRounds + a.ST.pairRounds + a.ST.segRounds + a.ST.jacRounds + a.ST.nutRounds,
+                        a.ST.jtvRounds, a.ST.jtvBody, 0.0, 0.0);
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < QM; ++q) {
+        const int i = lane + kWave * q;
+        lam[q] = i < m ? -a_n * mu2[q] + mu1[q] + c_tr * h * LT.v[i] : 0.0;
+      }
+      wave_sync();
+    }
+  }
+  ok = ballot(!ok) == 0;
+  for (int i = m + lane; i < nSpec; i += kWave) lam_out[i] = 0.0;
+#pragma unroll
+  for (int q = 0; q < QM; ++q) {
+    const int ai = lane + kWave * q;
+    if (ai < m) lam_out[M.perm()[ai]] = ok ? lam[q] : NAN;
+  }
+  for (int r = lane; r < nReac; r += kWave) gout[r] = ok ? g[r] : NAN;
+}
+
+// One wavefront per cell, cells taken from a device work queue.
+template <int QM>
+__global__ __launch_bounds__(512) __attribute__((flatten)) void adjoint_kernel(AdjointArgs a) {
+  extern __shared__ __align__(16) char lds_raw[];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const DevMech& M = a.fw.M;
+  SpLds<double> L = carve_sparse<double, false>(lds_raw, M, a.fw.S, a.cellBytes, wave);
+  auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
+  char* x = reinterpret_cast<char*>(L.A) + sparseCellBytes(M, a.fw.S, sizeof(double));
+  L.mu = reinterpret_cast<double*>(x);
+  x += al(8 * size_t(M.m + 1));
+  L.w = reinterpret_cast<double*>(x);
+  x += al(8 * size_t(M.nReac + 2));
+  L.v = reinterpret_cast<double*>(x);
+  x += al(8 * size_t(M.m + 1));
+  double* g = reinterpret_cast<double*>(x);
+  SpLds<double> LT = L;
+  LT.hdr = reinterpret_cast<const unsigned*>(M.ib + a.ST.oProgHdr);
+  LT.body = reinterpret_cast<const unsigned*>(M.ib + a.ST.oProgBody);
+  LT.ehdr = reinterpret_cast<const unsigned*>(M.ib + a.ST.oEvalHdr);
+  LT.ebody = reinterpret_cast<const unsigned*>(M.ib + a.ST.oEvalBody);
+  LT.coef = M.db + a.ST.oCoef;
+  if (lane == 0) {
+    L.kf[M.nReac] = 0.0;
+    L.rop[M.nReac] = 0.0;
+    L.dpr[M.nPairs] = 0.0;
+    L.A[a.fw.S.zeroSlot] = 0.0;
+    L.rdiag[M.m] = 0.0;
+    L.mu[M.m] = 0.0;
+    L.w[M.nReac] = 0.0;
+  }
+  wave_sync();
+  for (;;) {
+    int cell = 0;
+    if (lane == 0) cell = atomicAdd(a.fw.queue, 1);
+    cell = __builtin_amdgcn_readfirstlane(__shfl(cell, 0, kWave));
+    if (cell >= a.fw.nBatch) break;
+    adjoint_cell<QM>(a, L, LT, g, cell, lane);
+    wave_sync();
+  }
+}
+
+// Per-cell Newton settings and the record header of an adjoint forward run.
+__global__ void adjoint_setup_kernel(int nBatch, int maxSteps, const tchem_time_advance_t* tadv,
+                                     const double* tol_newton, int* header, double* tol, int* newton) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    header[0] = nBatch;
+    header[1] = maxSteps;
+    tol[0] = tol_newton[0];
+    tol[1] = tol_newton[1];
+  }
+  if (i < nBatch) {
+    newton[2 * i] = tadv[i]._max_num_newton_iterations;
+    newton[2 * i + 1] = tadv[i]._jacobian_interval < 1 ? 1 : tadv[i]._jacobian_interval;
+  }
 }
 
 __global__ void reset_counter_kernel(int* counter) {
@@ -1260,9 +1639,9 @@ __global__ __launch_bounds__(64) void jacobian_kernel(DevMech M, int nBatch, con
 
 static int qmOf(int m) { return m <= 64 ? 1 : (m <= 128 ? 2 : (m <= 256 ? 4 : 0)); }
 
-template <class B, int QM, int kThreads>
+template <class B, int QM, int kThreads, bool REC = false>
 static const void* integratorFn() {
-  return reinterpret_cast<const void*>(&trbdf2_kernel<B, QM, kThreads>);
+  return reinterpret_cast<const void*>(&trbdf2_kernel<B, QM, kThreads, REC>);
 }
 
 // Largest workgroup of the sparse form: 12 wavefronts (cells) for the mixed
@@ -1270,22 +1649,28 @@ static const void* integratorFn() {
 int sparseMaxCellsPerGroup(int precision, int m) { return (precision == TCHEM_PRECISION_MIXED && m <= 128) ? 12 : 8; }
 
 // kernel instance for (precision, sparse form: 0 dense, 1 staged programs,
-// 2 programs in device memory; species slots per lane)
-static const void* integratorKernel(int precision, int sparse, int qm) {
+// 2 programs in device memory; species slots per lane; recording for the
+// adjoint, FP64 sparse forms only); nullptr when there is none
+static const void* integratorKernel(int precision, int sparse, int qm, bool rec) {
 #define TCHEM_PICK_D(TA) \
   return qm == 1 ? integratorFn<DenseBackend<TA, 1>, 1, 64>() \
                  : (qm == 2 ? integratorFn<DenseBackend<TA, 2>, 2, 64>() : integratorFn<DenseBackend<TA, 4>, 4, 64>())
-#define TCHEM_PICK_S(TA, T12, ST)                                                             \
-  return qm == 1 ? integratorFn<SparseBackend<TA, 1, ST>, 1, T12>()                           \
-                 : (qm == 2 ? integratorFn<SparseBackend<TA, 2, ST>, 2, T12>()                \
-                            : integratorFn<SparseBackend<TA, 4, ST>, 4, 512>())
+#define TCHEM_PICK_S(TA, T12, ST, REC)                                                        \
+  return qm == 1 ? integratorFn<SparseBackend<TA, 1, ST>, 1, T12, REC>()                      \
+                 : (qm == 2 ? integratorFn<SparseBackend<TA, 2, ST>, 2, T12, REC>()           \
+                            : integratorFn<SparseBackend<TA, 4, ST>, 4, 512, REC>())
+  if (rec) {
+    if (precision != TCHEM_PRECISION_FP64 || sparse == 0) return nullptr;
+    if (sparse == 1) TCHEM_PICK_S(double, 512, true, true);
+    TCHEM_PICK_S(double, 512, false, true);
+  }
   if (precision == TCHEM_PRECISION_FP64) {
-    if (sparse == 1) TCHEM_PICK_S(double, 512, true);
-    if (sparse == 2) TCHEM_PICK_S(double, 512, false);
+    if (sparse == 1) TCHEM_PICK_S(double, 512, true, false);
+    if (sparse == 2) TCHEM_PICK_S(double, 512, false, false);
     TCHEM_PICK_D(double);
   }
-  if (sparse == 1) TCHEM_PICK_S(float, 768, true);
-  if (sparse == 2) TCHEM_PICK_S(float, 768, false);
+  if (sparse == 1) TCHEM_PICK_S(float, 768, true, false);
+  if (sparse == 2) TCHEM_PICK_S(float, 768, false, false);
   TCHEM_PICK_D(float);
 #undef TCHEM_PICK_D
 #undef TCHEM_PICK_S
@@ -1294,7 +1679,8 @@ static const void* integratorKernel(int precision, int sparse, int qm) {
 hipError_t launchIntegrator(const IntegratorArgs& a, int precision, int grid, size_t lds, hipStream_t stream) {
   const int qm = qmOf(a.M.m);
   if (qm == 0) return hipErrorInvalidValue;
-  const void* fn = integratorKernel(precision, a.sparse, qm);
+  const void* fn = integratorKernel(precision, a.sparse, qm, a.rec.h != nullptr);
+  if (!fn) return hipErrorInvalidValue;
   IntegratorArgs arg = a;
   void* params[] = {&arg};
   return hipLaunchKernel(fn, dim3(grid), dim3(64 * a.cellsPerGroup), params, lds, stream);
@@ -1305,21 +1691,56 @@ size_t integratorStaticLds(int precision, int sparse, int m) {
   const int qm = qmOf(m);
   if (qm == 0) return 0;
   hipFuncAttributes attr;
-  if (hipFuncGetAttributes(&attr, integratorKernel(precision, sparse, qm)) != hipSuccess) return 0;
+  if (hipFuncGetAttributes(&attr, integratorKernel(precision, sparse, qm, false)) != hipSuccess) return 0;
   return attr.sharedSizeBytes;
 }
 
-hipError_t integratorMaxBlocksPerCU(int precision, int sparse, int m, int threads, size_t lds, int* blocks) {
+hipError_t integratorMaxBlocksPerCU(int precision, int sparse, int m, int threads, size_t lds, int* blocks, bool rec) {
   const int qm = qmOf(m);
   if (qm == 0) return hipErrorInvalidValue;
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, integratorKernel(precision, sparse, qm), threads, lds);
+  const void* fn = integratorKernel(precision, sparse, qm, rec);
+  if (!fn) return hipErrorInvalidValue;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, fn, threads, lds);
 }
 
-hipError_t launchRates(const DevMech& M, int nBatch, const double* state, const double* photo, double* kf,
-                       hipStream_t stream) {
+hipError_t launchRates(const DevMech& M, int nBatch, const double* state, const double* photo, const double* lscale,
+                       int lstride, double* kf, hipStream_t stream) {
   const long long n = (long long)nBatch * M.nReac;
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(rates_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, M, nBatch, state, photo, kf);
+  hipLaunchKernelGGL(rates_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, M, nBatch, state, photo,
+                     lscale, lstride, kf);
+  return hipGetLastError();
+}
+
+static const void* adjointKernel(int qm) {
+  return qm == 1 ? reinterpret_cast<const void*>(&adjoint_kernel<1>)
+                 : (qm == 2 ? reinterpret_cast<const void*>(&adjoint_kernel<2>)
+                            : reinterpret_cast<const void*>(&adjoint_kernel<4>));
+}
+
+size_t adjointCellBytes(const DevMech& M, const DevSparse& S) {
+  return sparseCellBytes(M, S, sizeof(double)) + adjointExtraBytes(M);
+}
+
+hipError_t adjointMaxBlocksPerCU(int m, int threads, size_t lds, int* blocks) {
+  const int qm = qmOf(m);
+  if (qm == 0) return hipErrorInvalidValue;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, adjointKernel(qm), threads, lds);
+}
+
+hipError_t launchAdjoint(const AdjointArgs& a, int grid, size_t lds, hipStream_t stream) {
+  const int qm = qmOf(a.fw.M.m);
+  if (qm == 0) return hipErrorInvalidValue;
+  AdjointArgs arg = a;
+  void* params[] = {&arg};
+  return hipLaunchKernel(adjointKernel(qm), dim3(grid), dim3(64 * a.cellsPerGroup), params, lds, stream);
+}
+
+hipError_t launchAdjointSetup(int nBatch, int maxSteps, const tchem_time_advance_t* tadv, const double* tol_newton,
+                              int* header, double* tol, int* newton, hipStream_t stream) {
+  const int n = nBatch > 0 ? nBatch : 1;
+  hipLaunchKernelGGL(adjoint_setup_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, nBatch, maxSteps, tadv,
+                     tol_newton, header, tol, newton);
   return hipGetLastError();
 }
 

@@ -75,8 +75,9 @@ int emitChains(std::vector<Chain> chains, int kind, unsigned padTarget, unsigned
 
 }  // namespace
 
-SparseSchedule buildSparseSchedule(const MechTables& t) {
+SparseSchedule buildSparseSchedule(const MechTables& t, bool transposed) {
   SparseSchedule s;
+  s.transposed = transposed;
   const int m = t.m;
   if (t.maxReactants > 3) throw std::runtime_error("sparse schedule: more than three reactants in a reaction");
   if (t.nSpec >= 4095 || t.nReac >= 16384) throw std::runtime_error("sparse schedule: too many species or reactions");
@@ -87,7 +88,10 @@ SparseSchedule buildSparseSchedule(const MechTables& t) {
   // ---- symbolic LU of I - J, no interchanges
   std::vector<char> F(size_t(m) * m, 0);
   for (int i = 0; i < m; ++i) F[size_t(i) * m + i] = 1;
-  for (size_t n = 0; n < t.jnzRow.size(); ++n) F[size_t(t.jnzRow[n]) * m + t.jnzCol[n]] = 1;
+  for (size_t n = 0; n < t.jnzRow.size(); ++n) {
+    const int i = transposed ? t.jnzCol[n] : t.jnzRow[n], j = transposed ? t.jnzRow[n] : t.jnzCol[n];
+    F[size_t(i) * m + j] = 1;
+  }
   for (int k = 0; k < m; ++k)
     for (int i = k + 1; i < m; ++i) {
       if (!F[size_t(i) * m + k]) continue;
@@ -196,8 +200,11 @@ SparseSchedule buildSparseSchedule(const MechTables& t) {
   // Jacobian entries: every position of F is a target (fill positions get 0)
   s.jacBody = int(s.evalBody.size());
   {
-    std::vector<int> jn(size_t(m) * m, -1);
-    for (int n = 0; n < t.nnzJ(); ++n) jn[size_t(t.jnzRow[n]) * m + t.jnzCol[n]] = n;
+    std::vector<int> jn(size_t(m) * m, -1);  // structural nonzero of the assembled matrix at (i, j)
+    for (int n = 0; n < t.nnzJ(); ++n) {
+      const int i = transposed ? t.jnzCol[n] : t.jnzRow[n], j = transposed ? t.jnzRow[n] : t.jnzCol[n];
+      jn[size_t(i) * m + j] = n;
+    }
     std::vector<Chain> ch;
     for (int i = 0; i < m; ++i)
       for (int j = 0; j < m; ++j) {
@@ -213,6 +220,30 @@ SparseSchedule buildSparseSchedule(const MechTables& t) {
       }
     s.jacRounds = emitChains(ch, SparseSchedule::kJac, T, pack16(unsigned(t.nPairs()), 0), true, s.evalHdr,
                              s.evalBody);
+  }
+  if (transposed) {
+    // nu_r . mu: reaction-major stoichiometry (net coefficients of active species)
+    s.nutBody = int(s.evalBody.size());
+    std::vector<Chain> nu(t.nReac);
+    for (int r = 0; r < t.nReac; ++r) nu[r].target = unsigned(r);
+    for (int a = 0; a < m; ++a)
+      for (int e = t.omegaPtr[a]; e < t.omegaPtr[a + 1]; ++e)
+        nu[t.omegaReac[e]].a.push_back(pack16(unsigned(a), coef(t.omegaCoef[e])));
+    for (auto& c : nu)  // every w[r] is written (0 when reaction r touches no active species)
+      if (c.a.empty()) c.a.push_back(pack16(unsigned(m), 0));
+    s.nutRounds = emitChains(nu, SparseSchedule::kNuT, unsigned(t.nReac + 1), pack16(unsigned(m), 0), true,
+                             s.evalHdr, s.evalBody);
+    // J^T mu = sum over derivative pairs of species j of dpr[p] * w[reaction(p)]
+    s.jtvBody = int(s.evalBody.size());
+    std::vector<Chain> jt(m);
+    for (int j = 0; j < m; ++j) jt[j].target = unsigned(j);
+    for (int r = 0; r < t.nReac; ++r)
+      for (int p = t.pairPtr[r]; p < t.pairPtr[r + 1]; ++p)
+        jt[t.reacSpec[size_t(r) * R + t.pairSlot[p]]].a.push_back(pack16(unsigned(p), unsigned(r)));
+    for (auto& c : jt)  // every v[j] is written
+      if (c.a.empty()) c.a.push_back(pack16(unsigned(t.nPairs()), unsigned(t.nReac)));
+    s.jtvRounds = emitChains(jt, SparseSchedule::kJtv, unsigned(m), pack16(unsigned(t.nPairs()), unsigned(t.nReac)),
+                             true, s.evalHdr, s.evalBody);
   }
   s.evalBody.insert(s.evalBody.end(), slack, 0u);
 

@@ -27,7 +27,7 @@
 namespace tchem_amd {
 
 struct SparseSchedule {
-  enum : int { kMul = 0, kUpd = 1, kBack = 2, kRop = 3, kPair = 4, kSeg = 5, kJac = 6 };
+  enum : int { kMul = 0, kUpd = 1, kBack = 2, kRop = 3, kPair = 4, kSeg = 5, kJac = 6, kNuT = 7, kJtv = 8 };
   static constexpr int kMaxChain = 4;            // chain entries per lane per round
   static constexpr int kSlack = 1 + kMaxChain;   // slots read past a section's last round
   static constexpr unsigned kCont = 0x80000000u;  // target flag: continues a chain split over rounds
@@ -51,10 +51,19 @@ struct SparseSchedule {
   //          A[t] = (cont ? A[t] : diag) + scale * sum coefTable[coef] * dpr[pair]
   // Sections in order: rop, pair, seg, jac.  Padding: seg target nSeg (trash),
   // jac target trashSlot, entries (zero slot | coefficient 0).
+  // Transposed programs (adjoint) assemble and factor I - c h J^T instead
+  // and append two sections:
+  //   kNuT:  target (reaction | kCont), entries (species | coef << 16):
+  //          w[t] = (cont ? w[t] : 0) + sum coefTable[coef] * mu[species]   (nu_r . mu)
+  //   kJtv:  target (species | kCont), entries (pair | reaction << 16):
+  //          v[t] = (cont ? v[t] : 0) + sum dpr[pair] * w[reaction]         (J^T mu)
+  //          padding: w target nReac + 1, v target m; entries read the zero
+  //          slots mu[m], dpr[nPairs], w[nReac]
+  bool transposed = false;
   std::vector<double> coefTable;            // [0] = 0
   std::vector<unsigned> evalHdr, evalBody;  // evalBody ends with kSlack x 64 words of slack
-  int ropRounds = 0, pairRounds = 0, segRounds = 0, jacRounds = 0;
-  int pairBody = 0, segBody = 0, jacBody = 0;  // body word offsets of the sections
+  int ropRounds = 0, pairRounds = 0, segRounds = 0, jacRounds = 0, nutRounds = 0, jtvRounds = 0;
+  int pairBody = 0, segBody = 0, jacBody = 0, nutBody = 0, jtvBody = 0;  // body word offsets of the sections
   int pairSlots = 0;                           // derivative slots of a cell (zero slot nPairs included)
 
   // ---- LU program
@@ -76,6 +85,6 @@ struct SparseSchedule {
 // nSpec < 4095, reactant powers < 16, nReac < 16384, at most 65535 distinct
 // stoichiometric coefficients and nnzF + m + 2 < 32768; throws
 // std::runtime_error otherwise.
-SparseSchedule buildSparseSchedule(const MechTables& t);
+SparseSchedule buildSparseSchedule(const MechTables& t, bool transposed = false);
 
 }  // namespace tchem_amd

@@ -111,7 +111,10 @@ static void check_lu_program(const MechTables& t, const SparseSchedule& s) {
     Emu e(s, m);
     e.A.assign(s.trashSlot + 1, 0.0); e.rd.assign(m + 2, 0.0);
     std::vector<char> isJ(size_t(m) * m, 0);
-    for (size_t n = 0; n < t.jnzRow.size(); ++n) isJ[size_t(t.jnzRow[n]) * m + t.jnzCol[n]] = 1;
+    for (size_t n = 0; n < t.jnzRow.size(); ++n) {
+      const int i = s.transposed ? t.jnzCol[n] : t.jnzRow[n], j = s.transposed ? t.jnzRow[n] : t.jnzCol[n];
+      isJ[size_t(i) * m + j] = 1;
+    }
     const double scale = std::pow(10.0, trial - 3);
     for (int i = 0; i < m; ++i) for (int p = s.rowPtr[i]; p < s.rowPtr[i + 1]; ++p) {
       int j = s.colIdx[p]; double v = 0;
@@ -146,6 +149,74 @@ static void check_lu_program(const MechTables& t, const SparseSchedule& s) {
     // schedule gives O(1) errors
     CHECK(err <= 1e-6 * (xm + 1) && err2 <= 1e-6 * (xm + 1), "solve accuracy");
   }
+}
+
+// kNuT and kJtv sections of a transposed schedule: w = nu^T mu and v = J^T mu
+// against direct sums, starting at round rr / body word off.
+static void check_adjoint_sections(const MechTables& t, const SparseSchedule& s, const std::vector<double>& c,
+                                   const std::vector<double>& kf, const std::vector<double>& rop,
+                                   const std::vector<double>& dpr, int rr, size_t off) {
+  const int m = t.m, nR = t.nReac;
+  (void)c; (void)kf; (void)rop;
+  std::mt19937_64 rng(3);
+  std::uniform_real_distribution<double> U(-1.0, 1.0);
+  std::vector<double> mu(m + 1, 0.0), w(nR + 2, 0.0), v(m + 1, 0.0);
+  for (int i = 0; i < m; ++i) mu[i] = U(rng);
+  w[nR + 1] = std::nan("");
+  CHECK(off == size_t(s.nutBody), "nut body");
+  std::vector<int> wr(nR, 0), vr(m, 0);
+  const int nutEnd = rr + s.nutRounds;
+  for (; rr < (int)s.evalHdr.size(); ++rr) {
+    const int kind = s.evalHdr[rr] >> 16, len = s.evalHdr[rr] & 0xffff;
+    CHECK(kind == (rr < nutEnd ? SparseSchedule::kNuT : SparseSchedule::kJtv), "adjoint section kind");
+    if (rr == nutEnd) CHECK(off == size_t(s.jtvBody), "jtv body");
+    std::map<unsigned, int> tg;
+    for (int l = 0; l < 64; ++l) {
+      const unsigned tw = s.progBody.empty() ? 0 : s.evalBody[off + l];
+      const unsigned tt = tw & ~SparseSchedule::kCont;
+      double acc = 0.0;
+      if (kind == SparseSchedule::kNuT) {
+        CHECK((int)tt <= nR + 1, "w target");
+        if ((int)tt < nR) { CHECK(!tg.count(tt), "dup w target"); tg[tt] = l; if (!(tw & SparseSchedule::kCont)) wr[tt]++; }
+        acc = (tw & SparseSchedule::kCont) ? w[tt] : 0.0;
+        for (int j = 0; j < len; ++j) {
+          const unsigned e = s.evalBody[off + 64 * (1 + j) + l];
+          CHECK((int)(e & 0xffff) <= m && (e >> 16) < s.coefTable.size(), "nut entry");
+          acc += s.coefTable[e >> 16] * mu[e & 0xffff];
+        }
+        if ((int)tt <= nR + 1) w[tt] = acc;
+      } else {
+        CHECK((int)tt <= m, "v target");
+        if ((int)tt < m) { CHECK(!tg.count(tt), "dup v target"); tg[tt] = l; if (!(tw & SparseSchedule::kCont)) vr[tt]++; }
+        acc = (tw & SparseSchedule::kCont) ? v[tt] : 0.0;
+        for (int j = 0; j < len; ++j) {
+          const unsigned e = s.evalBody[off + 64 * (1 + j) + l];
+          CHECK((int)(e & 0xffff) < s.pairSlots && (int)(e >> 16) <= nR, "jtv entry");
+          acc += dpr[e & 0xffff] * w[e >> 16];
+        }
+        if ((int)tt <= m) v[tt] = acc;
+      }
+    }
+    off += 64 * (1 + len);
+  }
+  CHECK(off + 64 * SparseSchedule::kSlack == s.evalBody.size(), "adjoint body end");
+  for (int r = 0; r < nR; ++r) CHECK(wr[r] == 1, "w[%d] initialised %d times", r, wr[r]);
+  for (int j = 0; j < m; ++j) CHECK(vr[j] == 1, "v[%d] initialised %d times", j, vr[j]);
+  // direct: w_r = sum_a nu_ar mu_a; v = J^T mu with J = sum_p nu_r(p) dpr_p e_s(p)^T
+  std::vector<long double> wd(nR, 0.0L), vd(m, 0.0L), vs(m, 0.0L);
+  for (int a = 0; a < m; ++a)
+    for (int e = t.omegaPtr[a]; e < t.omegaPtr[a + 1]; ++e) wd[t.omegaReac[e]] += (long double)t.omegaCoef[e] * mu[a];
+  double we = 0, ve = 0;
+  for (int r = 0; r < nR; ++r) we = std::max(we, (double)fabsl(wd[r] - w[r]));
+  for (int n = 0; n < t.nnzJ(); ++n)
+    for (int e = t.jnzPtr[n]; e < t.jnzPtr[n + 1]; ++e) {
+      const long double jv = (long double)t.jcCoef[e] * dpr[t.jcPair[e]];
+      vd[t.jnzCol[n]] += jv * mu[t.jnzRow[n]];
+      vs[t.jnzCol[n]] += fabsl(jv * mu[t.jnzRow[n]]);
+    }
+  for (int j = 0; j < m; ++j) ve = std::max(ve, (double)(fabsl(vd[j] - v[j]) / (vs[j] + 1e-300)));
+  std::printf("adjoint sections: nu^T mu abs err %.2e, J^T mu rel err %.2e\n", we, ve);
+  CHECK(we < 1e-12 && ve < 1e-13, "adjoint sections");
 }
 
 static void check_eval_program(const MechTables& t, const SparseSchedule& s) {
@@ -189,7 +260,8 @@ static void check_eval_program(const MechTables& t, const SparseSchedule& s) {
     const double ch = 0.1 * (trial + 1);
     std::vector<double> A(s.trashSlot + 1, std::nan("")); A[s.zeroSlot] = 0;
     std::vector<int> jwrites(s.nnzF, 0);
-    for (; rr < (int)s.evalHdr.size(); ++rr) {
+    const int rEnd = s.ropRounds + s.pairRounds + s.segRounds + s.jacRounds;
+    for (; rr < rEnd; ++rr) {
       int kind = s.evalHdr[rr] >> 16, len = s.evalHdr[rr] & 0xffff;
       CHECK(len >= 1 && len <= SparseSchedule::kMaxChain, "len");
       std::map<unsigned, int> tg;
@@ -212,7 +284,8 @@ static void check_eval_program(const MechTables& t, const SparseSchedule& s) {
       }
       off += 64 * (1 + len);
     }
-    CHECK(off + 64 * SparseSchedule::kSlack == s.evalBody.size(), "eval body end");
+    if (s.transposed) check_adjoint_sections(t, s, c, kf, rop, dpr, rr, off);
+    else CHECK(off + 64 * SparseSchedule::kSlack == s.evalBody.size(), "eval body end");
     for (int p = 0; p < s.nnzF; ++p) CHECK(jwrites[p] == 1, "jac position %d initialised %d times", p, jwrites[p]);
     // omega vs direct
     double maxe = 0;
@@ -227,7 +300,14 @@ static void check_eval_program(const MechTables& t, const SparseSchedule& s) {
     std::vector<long double> D(size_t(m) * m, 0.0L);
     for (int i = 0; i < m; ++i) D[size_t(i) * m + i] = 1.0L;
     std::vector<long double> scl(size_t(m) * m, 0.0L);
-    for (int n = 0; n < t.nnzJ(); ++n) for (int e = t.jnzPtr[n]; e < t.jnzPtr[n + 1]; ++e) { long double v = (long double)t.jcCoef[e] * dpr[t.jcPair[e]]; D[size_t(t.jnzRow[n]) * m + t.jnzCol[n]] += -ch * v; scl[size_t(t.jnzRow[n]) * m + t.jnzCol[n]] += fabsl(ch * v); }
+    for (int n = 0; n < t.nnzJ(); ++n) {
+      const size_t at = s.transposed ? size_t(t.jnzCol[n]) * m + t.jnzRow[n] : size_t(t.jnzRow[n]) * m + t.jnzCol[n];
+      for (int e = t.jnzPtr[n]; e < t.jnzPtr[n + 1]; ++e) {
+        long double v = (long double)t.jcCoef[e] * dpr[t.jcPair[e]];
+        D[at] += -ch * v;
+        scl[at] += fabsl(ch * v);
+      }
+    }
     double je = 0;
     for (int i = 0; i < m; ++i) for (int p = s.rowPtr[i]; p < s.rowPtr[i + 1]; ++p) { int j = s.colIdx[p]; long double sc = scl[size_t(i) * m + j] + (i == j); je = std::max(je, (double)(fabsl(D[size_t(i) * m + j] - A[p]) / (sc + 1e-300))); }
     std::printf("trial %d: omega rel err %.2e, J rel err %.2e\n", trial, maxe, je);
@@ -239,9 +319,12 @@ int main(int argc, char** argv) {
   if (argc < 2) return 2;
   KineticModel km = loadKineticModel(argv[1], nullptr);
   MechTables t = compileMechanism(km);
-  SparseSchedule s = buildSparseSchedule(t);
-  check_eval_program(t, s);
-  check_lu_program(t, s);
+  for (bool tr : {false, true}) {
+    SparseSchedule s = buildSparseSchedule(t, tr);
+    std::printf("-- %s programs\n", tr ? "transposed" : "forward");
+    check_eval_program(t, s);
+    check_lu_program(t, s);
+  }
   std::printf("%s (%d failures)\n", fails ? "FAILED" : "OK", fails);
   return fails ? 1 : 0;
 }
