@@ -19,6 +19,10 @@ Forms:
          wavefront per cell, the reference's host interval loop (10 TR-BDF2
          iterations per launch); timed on --plain-cells cells.
 
+Other modes (one JSON line each, instead of the benchmark): --config3 times
+the discrete adjoint against the forward solve, --config4 the uneven-batch
+synthetic-mechanism workload, --precision-study the error study.
+
 Prints one JSON line on stdout; diagnostics go to stderr.  --copy-probe runs
 only a 1 GiB device-to-device copy (a known-size kernel for calibrating
 FETCH_SIZE / WRITE_SIZE counters).
@@ -232,6 +236,86 @@ def config4_run(size_scale, out_path=None):
             f.write(text + "\n")
 
 
+def config3_run(cells, out_path=None, max_steps=8192):
+    """Cost of the discrete adjoint on the differentiation workload: CB05,
+    config-0 draws with seed 2, advanced 1800 s.  Times the FP64 sparse
+    persistent forward solve, the recording forward run and the backward
+    sweep; loss L = sum_cells sum_k w_k log c_k(t_end), w ~ N(0, 1) over
+    species >= 1e-12 ppm at t_end."""
+    import torch
+    T = importlib.import_module("tchem-atm_amd")
+    S = importlib.import_module("tchem-atm_amd.scenarios")
+    kmd = T.KineticModelData(MECH)
+    kmcd = kmd.createConstData()
+    base = T.setScenarioConditions(MECH, kmd)
+    m, nspec, nR = kmd.nActive, kmd.nSpec, kmd.nReac
+    tend = 1800.0
+    w = S.config0(kmd, base[0], n=cells, seed=2)
+    dev = "cuda"
+    tn = torch.tensor([w.atol_newton, w.rtol_newton], dtype=torch.float64, device=dev)
+    tt = torch.tensor(np.tile([w.atol_time, w.rtol_time], (m, 1)), device=dev)
+    st0 = torch.tensor(w.state, device=dev)
+    ph = torch.tensor(w.photo, device=dev)
+    st = torch.empty_like(st0)
+    t = torch.empty(cells, dtype=torch.float64, device=dev)
+    dt = torch.empty_like(t)
+    ws = T.Adjoint.workspace(kmcd, cells, max_steps)
+
+    def reset():
+        st.copy_(st0)
+        t.fill_(0.0)
+        dt.zero_()
+        return T.time_advance_tensor(cells, 0.0, tend, w.dt_in, w.dtmin, w.dtmax, w.max_newton, 1 << 30, 1, dev)
+
+    def plain():
+        tadv = reset()
+        T.AtmosphericChemistry.runDeviceBatch(tn, tt, None, tadv, st, t, dt, st, kmcd, photo_rates=ph,
+                                              precision=T.PRECISION_FP64, schedule=T.SCHEDULE_PERSISTENT)
+
+    def record():
+        tadv = reset()
+        T.Adjoint.forward(tn, tt, tadv, st, t, dt, ws, max_steps, kmcd, photo_rates=ph)
+
+    def clock(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    plain()  # warm-up (module load)
+    t_plain = clock(plain)
+    log(f"config3: forward solve {t_plain:.3f} s")
+    t_rec = clock(record)
+    final = st.cpu().numpy()
+    ns, _ = T.Adjoint.steps(ws, cells, max_steps, kmcd, with_h=False)
+    ns = ns.cpu().numpy()
+    c = final[:, 3:3 + m]
+    wv = np.random.Generator(np.random.Philox(key=2)).standard_normal(m)
+    wts = np.where(c >= 1e-12, wv, 0.0)
+    lam = np.zeros((cells, nspec))
+    lam[:, :m] = np.where(wts != 0.0, wts / np.where(wts != 0.0, c, 1.0), 0.0)
+    lam_d = torch.tensor(lam, device=dev)
+    lam_out = torch.empty_like(lam_d)
+    grad = torch.empty((cells, nR), dtype=torch.float64, device=dev)
+    t_back = clock(lambda: T.Adjoint.backward(lam_d, lam_out, grad, ws, max_steps, kmcd))
+    g = grad.cpu().numpy()
+    ok = ns >= 0
+    gsum = g[ok].sum(0)
+    res = {"workload": "cb05_config3_adjoint", "cells": cells, "t_end_s": tend, "max_steps": max_steps,
+           "forward_solve_s": t_plain, "recording_forward_s": t_rec, "backward_s": t_back,
+           "gradient_cost_vs_forward": (t_rec + t_back) / t_plain, "backward_vs_forward": t_back / t_plain,
+           "accepted_steps_per_cell_mean": float(ns[ok].mean()), "accepted_steps_per_cell_max": int(ns[ok].max()),
+           "cells_without_gradient": int((~ok).sum()), "loss_species_per_cell_mean": float((wts != 0).sum(1).mean()),
+           "grad_norm_inf": float(np.abs(gsum).max()), "grad_finite": bool(np.isfinite(gsum).all()),
+           "workspace_GiB": ws.numel() / 2 ** 30}
+    text = json.dumps(res)
+    print(text, flush=True)
+    if out_path:
+        with open(out_path, "w") as f:
+            f.write(text + "\n")
+
+
 def copy_probe():
     import torch
     n = 1 << 27  # 1 GiB of float64
@@ -261,6 +345,9 @@ def main():
     ap.add_argument("--config4", metavar="OUT", nargs="?", const="",
                     help="run the uneven-batch synthetic-mechanism workload instead of the benchmark")
     ap.add_argument("--config4-scale", type=float, default=1 / 64, help="batch-size scale of --config4")
+    ap.add_argument("--config3", metavar="OUT", nargs="?", const="",
+                    help="time the discrete adjoint (forward, recording forward, backward) instead of the benchmark")
+    ap.add_argument("--config3-cells", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the path shards cells across GPUs with no exchange; "
@@ -271,6 +358,9 @@ def main():
         return
     if args.config4 is not None:
         config4_run(args.config4_scale, args.config4 or None)
+        return
+    if args.config3 is not None:
+        config3_run(args.config3_cells, args.config3 or None)
         return
     if args.precision_study is not None:
         precision_study(args.study_cells, args.cpu_threads or min(16, os.cpu_count() or 1),
