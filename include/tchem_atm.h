@@ -31,6 +31,7 @@ extern "C" {
 
 typedef struct tchem_kmd_s* tchem_kmd_t;   /* host kinetic model data */
 typedef struct tchem_kmcd_s* tchem_kmcd_t; /* device-resident constant data */
+typedef struct tchem_graph_s* tchem_graph_t; /* captured launch sequence (HIP graph) */
 
 /* Byte layout of TChem::TimeAdvance (src/core/TChem_Util.hpp:469-476). */
 typedef struct {
@@ -134,6 +135,40 @@ int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBat
  * tadv[i]._tbeg = t_out[i], tadv[i]._dt = dt_out[i]. */
 int tchem_atmchem_update_time_advance(int nBatch, tchem_time_advance_t* tadv, const double* t_out,
                                       const double* dt_out, void* stream);
+
+/* The driver's interval loop (src/examples/TChem_AtmosphericChemistry.cpp:312-352)
+ * captured into a HIP graph, bound to the given arrays by pointer: one
+ * launch replays nIntervals x (tchem_atmchem_run_device_batch, then
+ * tchem_atmchem_update_time_advance) on their current contents.  With
+ * stopFraction > 0 the driver's stop test runs on the device after every
+ * interval (mean t_out > stopFraction * tend, tend of cell 0; the driver uses
+ * 0.9999) and the remaining intervals of the launch do nothing once it holds,
+ * so a launch of nIntervals reproduces the eager loop exactly; stopFraction 0
+ * runs all nIntervals.  Options are resolved once at capture (blocks_per_cu 0
+ * -> the occupancy limit).  graph_create is a setup call (synchronises the
+ * device, allocates); tchem_graph_status reads the intervals run and the stop
+ * flag of the last launch (synchronous). */
+int tchem_atmchem_graph_create(const tchem_atmchem_options_t* opts, int nBatch, const double* tol_newton,
+                               const double* tol_time, tchem_time_advance_t* tadv, double* state,
+                               const double* photo_rates, double* t_out, double* dt_out, double* state_out,
+                               tchem_cell_stats_t* stats, int nIntervals, double stopFraction, tchem_kmcd_t kmcd,
+                               tchem_graph_t* out);
+int tchem_graph_launch(tchem_graph_t graph, void* stream);
+int tchem_graph_status(tchem_graph_t graph, int* intervals_run, int* stopped);
+int tchem_graph_destroy(tchem_graph_t graph);
+
+/* Autotuner (no counterpart in the reference, whose launch shape is fixed):
+ * times tchem_atmchem_run_device_batch on a strided sample of the batch
+ * (sampleCells, 0 = 48 per CU; each cell limited to sampleIterations TR-BDF2
+ * iterations from its tadv, 0 = 24) for every LU form opts->lu allows (AUTO:
+ * sparse and dense), every cells_per_block that fits, and the two largest
+ * blocks_per_cu of the persistent schedule, and writes the fastest into
+ * *opts (precision and schedule are kept).  The caller's arrays are not
+ * modified.  Synchronous; allocates scratch. */
+int tchem_atmchem_autotune(tchem_atmchem_options_t* opts, int nBatch, const double* tol_newton,
+                           const double* tol_time, const tchem_time_advance_t* tadv, const double* state,
+                           const double* photo_rates, int sampleCells, int sampleIterations, double* best_seconds,
+                           tchem_kmcd_t kmcd, void* stream);
 
 /* Source term omega[nBatch][m] at state rows [nBatch][nSpec+3]: the batched
  * computeFunction of src/examples/TChem_GasChemistry_RHSs.cpp:184-236

@@ -57,7 +57,8 @@ EXPORTED_SYMBOLS = (
     "tchem_kmd_photolysis_reactions", "tchem_scenario_load", "tchem_kmcd_create", "tchem_kmcd_destroy",
     "tchem_atmchem_workspace_bytes", "tchem_atmchem_run_device_batch", "tchem_atmchem_update_time_advance",
     "tchem_sourceterm_run_device_batch", "tchem_jacobian_run_device_batch", "tchem_adjoint_workspace_bytes",
-    "tchem_adjoint_forward", "tchem_adjoint_backward", "tchem_adjoint_steps",
+    "tchem_adjoint_forward", "tchem_adjoint_backward", "tchem_adjoint_steps", "tchem_atmchem_graph_create",
+    "tchem_graph_launch", "tchem_graph_status", "tchem_graph_destroy", "tchem_atmchem_autotune",
 )
 
 
@@ -95,6 +96,13 @@ def _load():
         "tchem_adjoint_forward": [i, vp, vp, vp, vp, vp, vp, i, vp, vp, vp, vp, i, vp, vp],
         "tchem_adjoint_backward": [i, vp, vp, vp, vp, i, vp, vp],
         "tchem_adjoint_steps": [i, vp, i, vp, vp, vp, vp],
+        "tchem_atmchem_graph_create": [ctypes.POINTER(_Options), i, vp, vp, vp, vp, vp, vp, vp, vp, vp, i,
+                                       ctypes.c_double, vp, ctypes.POINTER(vp)],
+        "tchem_graph_launch": [vp, vp],
+        "tchem_graph_status": [vp, ctypes.POINTER(i), ctypes.POINTER(i)],
+        "tchem_graph_destroy": [vp],
+        "tchem_atmchem_autotune": [ctypes.POINTER(_Options), i, vp, vp, vp, vp, vp, i, i,
+                                   ctypes.POINTER(ctypes.c_double), vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -231,11 +239,66 @@ class AtmosphericChemistry:
             _ptr(state_out, what="state_out"), _ptr(stats, None, "stats"), kmcd._h, _stream_ptr(stream)))
 
     @staticmethod
+    def captureGraph(tol_newton, tol_time, tadv, state, t_out, dt_out, state_out, kmcd, n_intervals,
+                     stop_fraction=0.9999, photo_rates=None, stats=None, precision=PRECISION_FP64,
+                     schedule=SCHEDULE_GRID, blocks_per_cu=0, lu=LU_AUTO, cells_per_block=0) -> "Graph":
+        """The interval loop (runDeviceBatch + updateTimeAdvance) x n_intervals
+        captured into a HIP graph bound to these arrays; with stop_fraction > 0
+        the driver's stop test (mean t > stop_fraction * tend) runs on the
+        device after each interval."""
+        opts = _Options(precision, schedule, blocks_per_cu, lu, cells_per_block)
+        h = ctypes.c_void_p()
+        _check(_lib.tchem_atmchem_graph_create(
+            ctypes.byref(opts), state.shape[0], _ptr(tol_newton, what="tol_newton"), _ptr(tol_time, what="tol_time"),
+            _ptr(tadv, None, "tadv"), _ptr(state, what="state"), _ptr(photo_rates, what="photo_rates"),
+            _ptr(t_out, what="t_out"), _ptr(dt_out, what="dt_out"), _ptr(state_out, what="state_out"),
+            _ptr(stats, None, "stats"), int(n_intervals), float(stop_fraction), kmcd._h, ctypes.byref(h)))
+        return Graph(h, (tol_newton, tol_time, tadv, state, photo_rates, t_out, dt_out, state_out, stats))
+
+    @staticmethod
+    def autotune(tol_newton, tol_time, tadv, state, kmcd, photo_rates=None, precision=PRECISION_MIXED,
+                 schedule=SCHEDULE_PERSISTENT, lu=LU_AUTO, sample_cells=0, sample_iterations=0, stream=None):
+        """Fastest (lu, cells_per_block, blocks_per_cu) on a sample of this
+        batch; returns a dict of runDeviceBatch keyword arguments plus
+        'seconds' (the sample's best time)."""
+        opts = _Options(precision, schedule, 0, lu, 0)
+        best = ctypes.c_double()
+        _check(_lib.tchem_atmchem_autotune(
+            ctypes.byref(opts), state.shape[0], _ptr(tol_newton, what="tol_newton"), _ptr(tol_time, what="tol_time"),
+            _ptr(tadv, None, "tadv"), _ptr(state, what="state"), _ptr(photo_rates, what="photo_rates"),
+            int(sample_cells), int(sample_iterations), ctypes.byref(best), kmcd._h, _stream_ptr(stream)))
+        return dict(precision=opts.precision, schedule=opts.schedule, lu=opts.lu,
+                    cells_per_block=opts.cells_per_block, blocks_per_cu=opts.blocks_per_cu, seconds=best.value)
+
+    @staticmethod
     def updateTimeAdvance(tadv, t_out, dt_out, stream=None):
         """tadv._tbeg = t_out, tadv._dt = dt_out (the driver's between-interval step)."""
         n = t_out.shape[0]
         _check(_lib.tchem_atmchem_update_time_advance(n, _ptr(tadv, None, "tadv"), _ptr(t_out), _ptr(dt_out),
                                                       _stream_ptr(stream)))
+
+
+class Graph:
+    """A captured interval loop (tchem_graph_t); keeps its arrays alive."""
+
+    def __init__(self, h, keep):
+        self._h = h
+        self._keep = keep
+
+    def launch(self, stream=None):
+        _check(_lib.tchem_graph_launch(self._h, _stream_ptr(stream)))
+
+    def status(self):
+        """(intervals run, stopped) of the last launch (synchronous)."""
+        n, st = ctypes.c_int(), ctypes.c_int()
+        _check(_lib.tchem_graph_status(self._h, ctypes.byref(n), ctypes.byref(st)))
+        return n.value, bool(st.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.tchem_graph_destroy(h)
+            self._h = None
 
 
 class SourceTerm:

@@ -3,6 +3,7 @@
 // returns a status code and records a message for tchem_last_error().
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
@@ -24,13 +25,15 @@ int sparseMaxCellsPerGroup(int precision, int m);
 size_t integratorStaticLds(int precision, int sparse, int m);
 hipError_t launchResetCounter(int* counter, hipStream_t stream);
 hipError_t launchRates(const DevMech& M, int nBatch, const double* state, const double* photo, const double* lscale,
-                       int lstride, double* kf, hipStream_t stream);
+                       int lstride, double* kf, const int* skip, hipStream_t stream);
+hipError_t launchLoopControl(int n, const double* t_out, const tchem_time_advance_t* tadv, double fraction, int* ctl,
+                             bool reset, hipStream_t stream);
 size_t adjointCellBytes(const DevMech& M, const DevSparse& S);
 hipError_t adjointMaxBlocksPerCU(int m, int threads, size_t lds, int* blocks);
 hipError_t launchAdjoint(const AdjointArgs& a, int grid, size_t lds, hipStream_t stream);
 hipError_t launchAdjointSetup(int nBatch, int maxSteps, const tchem_time_advance_t* tadv, const double* tol_newton,
                               int* header, double* tol, int* newton, hipStream_t stream);
-hipError_t launchUpdateTadv(int n, tchem_time_advance_t* tadv, const double* t_out, const double* dt_out,
+hipError_t launchUpdateTadv(int n, tchem_time_advance_t* tadv, const double* t_out, const double* dt_out, int* ctl,
                             hipStream_t stream);
 hipError_t launchSourceTerm(const DevMech& M, int nBatch, const double* state, const double* photo, const double* ext,
                             double* omega, hipStream_t stream);
@@ -62,6 +65,19 @@ struct tchem_kmcd_s {
     if (hipGetDevice(&cur) == hipSuccess && cur != device) (void)hipSetDevice(device);
     for (void* p : allocations) (void)hipFree(p);
     if (cur != device) (void)hipSetDevice(cur);
+  }
+};
+
+struct tchem_graph_s {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int* ctl = nullptr;  // [0] stopped, [1] intervals run (device)
+  int device = 0;
+  int intervals = 0;
+  ~tchem_graph_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (ctl) (void)hipFree(ctl);
   }
 };
 
@@ -159,6 +175,37 @@ size_t ldsBytes(const tchem_kmcd_s* k, const tchem_atmchem_options_t& o) {
   return integratorLdsBytes(k->dev, bytesA);
 }
 
+// Grows the per-cell rate-constant buffer to nBatch cells (allocation only
+// when a larger batch arrives; the previous buffer may still be read by work
+// queued on s, so s is synchronised first).
+int ensureKf(tchem_kmcd_s* kmcd, int nBatch, hipStream_t s) {
+  if (size_t(nBatch) <= kmcd->kfCapacity) return TCHEM_OK;
+  hipError_t e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hipFail(e, "hipStreamSynchronize");
+  if (kmcd->kf) (void)hipFree(kmcd->kf);
+  kmcd->kf = nullptr;
+  kmcd->kfCapacity = 0;
+  void* p = nullptr;
+  e = hipMalloc(&p, sizeof(double) * size_t(nBatch) * size_t(kmcd->dev.nReac > 0 ? kmcd->dev.nReac : 1));
+  if (e != hipSuccess) return hipFail(e, "hipMalloc (rate constants)");
+  kmcd->kf = static_cast<double*>(p);
+  kmcd->kfCapacity = size_t(nBatch);
+  return TCHEM_OK;
+}
+
+// Resident workgroups per CU of the persistent schedule (options value, or
+// the occupancy limit when 0).
+int resolveBlocksPerCU(const tchem_kmcd_s* kmcd, const tchem_atmchem_options_t& o, bool rec, int* bpc) {
+  *bpc = o.blocks_per_cu;
+  if (*bpc > 0) return TCHEM_OK;
+  const int sparse = o.lu == TCHEM_LU_SPARSE ? (sparseStaged(kmcd, o.precision) ? 1 : 2) : 0;
+  hipError_t e = integratorMaxBlocksPerCU(o.precision, sparse, kmcd->dev.m, 64 * o.cells_per_block,
+                                          ldsBytes(kmcd, o), bpc, rec);
+  if (e != hipSuccess) return hipFail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+  if (*bpc < 1) *bpc = 1;
+  return TCHEM_OK;
+}
+
 // Rate constants (optionally log-scaled) into a.kf, then the integrator
 // launch for the resolved options; a carries the per-call arrays.
 int runIntegrator(tchem_kmcd_s* kmcd, const tchem_atmchem_options_t& o, const double* lscale, int lstride,
@@ -181,7 +228,8 @@ int runIntegrator(tchem_kmcd_s* kmcd, const tchem_atmchem_options_t& o, const do
                                                o.precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float)))
                          : 0;
   {
-    hipError_t e = launchRates(kmcd->dev, a.nBatch, a.state, a.photo, lscale, lstride, const_cast<double*>(a.kf), s);
+    hipError_t e = launchRates(kmcd->dev, a.nBatch, a.state, a.photo, lscale, lstride, const_cast<double*>(a.kf), a.skip,
+                               s);
     if (e != hipSuccess) return hipFail(e, "rates_kernel launch");
   }
   const bool rec = a.rec.h != nullptr;
@@ -189,12 +237,9 @@ int runIntegrator(tchem_kmcd_s* kmcd, const tchem_atmchem_options_t& o, const do
   const int groups = (a.nBatch + cpg - 1) / cpg;
   int grid = groups;
   if (a.persistent) {
-    int bpc = o.blocks_per_cu;
-    if (bpc == 0) {
-      hipError_t e = integratorMaxBlocksPerCU(o.precision, a.sparse, kmcd->dev.m, 64 * cpg, lds, &bpc, rec);
-      if (e != hipSuccess) return hipFail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-      if (bpc < 1) bpc = 1;
-    }
+    int bpc = 0;
+    const int rc = resolveBlocksPerCU(kmcd, o, rec, &bpc);
+    if (rc) return rc;
     const long long g = (long long)bpc * kmcd->numCUs;
     grid = int(g < groups ? g : groups);
     hipError_t e = launchResetCounter(kmcd->queue, s);
@@ -468,18 +513,9 @@ int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBat
     return setError(TCHEM_ERR_INVALID, "run_device_batch: null device array");
   if (kmcd->dev.nPhoto == 0) photo_rates = nullptr;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (size_t(nBatch) > kmcd->kfCapacity) {
-    // grows only: the previous buffer may still be read by work queued on s
-    hipError_t e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return hipFail(e, "hipStreamSynchronize");
-    if (kmcd->kf) (void)hipFree(kmcd->kf);
-    kmcd->kf = nullptr;
-    kmcd->kfCapacity = 0;
-    void* p = nullptr;
-    e = hipMalloc(&p, sizeof(double) * size_t(nBatch) * size_t(kmcd->dev.nReac > 0 ? kmcd->dev.nReac : 1));
-    if (e != hipSuccess) return hipFail(e, "hipMalloc (rate constants)");
-    kmcd->kf = static_cast<double*>(p);
-    kmcd->kfCapacity = size_t(nBatch);
+  {
+    const int rk = ensureKf(kmcd, nBatch, s);
+    if (rk) return rk;
   }
   IntegratorArgs a{};
   a.nBatch = nBatch;
@@ -500,7 +536,7 @@ int tchem_atmchem_update_time_advance(int nBatch, tchem_time_advance_t* tadv, co
                                       const double* dt_out, void* stream) {
   if (nBatch < 0 || (nBatch > 0 && (!tadv || !t_out || !dt_out)))
     return setError(TCHEM_ERR_INVALID, "update_time_advance: bad argument");
-  hipError_t e = launchUpdateTadv(nBatch, tadv, t_out, dt_out, static_cast<hipStream_t>(stream));
+  hipError_t e = launchUpdateTadv(nBatch, tadv, t_out, dt_out, nullptr, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hipFail(e, "update_tadv_kernel");
   return TCHEM_OK;
 }
@@ -665,6 +701,263 @@ int tchem_adjoint_steps(int nBatch, const void* workspace, int maxSteps, int* ns
   if (e == hipSuccess && h)
     e = hipMemcpyAsync(h, ws + W.h, sizeof(double) * size_t(nBatch) * size_t(maxSteps), hipMemcpyDeviceToDevice, s);
   if (e != hipSuccess) return hipFail(e, "hipMemcpyAsync");
+  return TCHEM_OK;
+}
+
+int tchem_atmchem_graph_create(const tchem_atmchem_options_t* opts, int nBatch, const double* tol_newton,
+                               const double* tol_time, tchem_time_advance_t* tadv, double* state,
+                               const double* photo_rates, double* t_out, double* dt_out, double* state_out,
+                               tchem_cell_stats_t* stats, int nIntervals, double stopFraction, tchem_kmcd_t kmcd,
+                               tchem_graph_t* out) {
+  const char* who = "tchem_atmchem_graph_create";
+  if (!kmcd || !out) return setError(TCHEM_ERR_INVALID, std::string(who) + ": null argument");
+  *out = nullptr;
+  if (nBatch < 1 || nIntervals < 1) return setError(TCHEM_ERR_INVALID, std::string(who) + ": nBatch and nIntervals must be >= 1");
+  if (!tol_newton || !tol_time || !tadv || !state || !t_out || !dt_out || !state_out)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": null device array");
+  tchem_atmchem_options_t o;
+  int rc = resolveOptions(opts, kmcd, &o, who);
+  if (rc) return rc;
+  if (kmcd->dev.nPhoto == 0) photo_rates = nullptr;
+  // everything that queries or allocates happens before the capture
+  if ((rc = ensureKf(kmcd, nBatch, nullptr))) return rc;
+  if (o.schedule == TCHEM_SCHEDULE_PERSISTENT) {
+    int bpc = 0;
+    if ((rc = resolveBlocksPerCU(kmcd, o, false, &bpc))) return rc;
+    o.blocks_per_cu = bpc;
+  }
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hipFail(e, "hipDeviceSynchronize");
+  hipStream_t cs = nullptr;
+  e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e != hipSuccess) return hipFail(e, "hipStreamCreateWithFlags");
+  std::unique_ptr<tchem_graph_s> g(new tchem_graph_s);
+  g->device = kmcd->device;
+  g->intervals = nIntervals;
+  {
+    void* p = nullptr;
+    e = hipMalloc(&p, 256);
+    if (e != hipSuccess) {
+      (void)hipStreamDestroy(cs);
+      return hipFail(e, "hipMalloc");
+    }
+    g->ctl = static_cast<int*>(p);
+    e = hipMemset(g->ctl, 0, 256);  // outside the capture
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      (void)hipStreamDestroy(cs);
+      return hipFail(e, "hipMemset");
+    }
+  }
+  const bool stop = stopFraction > 0.0;
+  e = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    return hipFail(e, "hipStreamBeginCapture");
+  }
+  e = launchLoopControl(nBatch, t_out, tadv, stopFraction, g->ctl, true, cs);
+  if (e != hipSuccess) rc = hipFail(e, "loop_reset_kernel");
+  for (int it = 0; it < nIntervals && rc == TCHEM_OK; ++it) {
+    IntegratorArgs a{};
+    a.skip = g->ctl;
+    a.nBatch = nBatch;
+    a.tol_newton = tol_newton;
+    a.tol_time = tol_time;
+    a.tadv = tadv;
+    a.state = state;
+    a.photo = photo_rates;
+    a.t_out = t_out;
+    a.dt_out = dt_out;
+    a.state_out = state_out;
+    a.stats = stats;
+    a.kf = kmcd->kf;
+    rc = runIntegrator(kmcd, o, nullptr, 0, a, cs, who);
+    if (rc == TCHEM_OK) {
+      e = launchUpdateTadv(nBatch, tadv, t_out, dt_out, g->ctl, cs);
+      if (e != hipSuccess) rc = hipFail(e, "update_tadv_kernel");
+    }
+    if (rc == TCHEM_OK && stop) {
+      e = launchLoopControl(nBatch, t_out, tadv, stopFraction, g->ctl, false, cs);
+      if (e != hipSuccess) rc = hipFail(e, "stop_check_kernel");
+    }
+  }
+  hipError_t ee = hipStreamEndCapture(cs, &g->graph);
+  (void)hipStreamDestroy(cs);
+  if (rc) return rc;
+  if (ee != hipSuccess) return hipFail(ee, "hipStreamEndCapture");
+  e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) return hipFail(e, "hipGraphInstantiate");
+  *out = g.release();
+  return TCHEM_OK;
+}
+
+int tchem_graph_launch(tchem_graph_t graph, void* stream) {
+  if (!graph || !graph->exec) return setError(TCHEM_ERR_INVALID, "tchem_graph_launch: null graph");
+  int cur = 0;
+  if (hipGetDevice(&cur) != hipSuccess || cur != graph->device)
+    return setError(TCHEM_ERR_INVALID, "tchem_graph_launch: graph belongs to another device");
+  hipError_t e = hipGraphLaunch(graph->exec, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hipFail(e, "hipGraphLaunch");
+  return TCHEM_OK;
+}
+
+int tchem_graph_status(tchem_graph_t graph, int* intervals_run, int* stopped) {
+  if (!graph || !graph->ctl) return setError(TCHEM_ERR_INVALID, "tchem_graph_status: null graph");
+  int h[2] = {0, 0};
+  hipError_t e = hipDeviceSynchronize();  // the launch may be on any stream
+  if (e == hipSuccess) e = hipMemcpy(h, graph->ctl, sizeof(h), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hipFail(e, "hipMemcpy");
+  if (stopped) *stopped = h[0];
+  if (intervals_run) *intervals_run = h[1];
+  return TCHEM_OK;
+}
+
+int tchem_graph_destroy(tchem_graph_t graph) {
+  delete graph;
+  return TCHEM_OK;
+}
+
+int tchem_atmchem_autotune(tchem_atmchem_options_t* opts, int nBatch, const double* tol_newton,
+                           const double* tol_time, const tchem_time_advance_t* tadv, const double* state,
+                           const double* photo_rates, int sampleCells, int sampleIterations, double* best_seconds,
+                           tchem_kmcd_t kmcd, void* stream) {
+  const char* who = "tchem_atmchem_autotune";
+  if (!kmcd || !opts) return setError(TCHEM_ERR_INVALID, std::string(who) + ": null argument");
+  if (nBatch < 1 || sampleCells < 0 || sampleIterations < 0)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": bad size");
+  if (!tol_newton || !tol_time || !tadv || !state)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": null device array");
+  if (kmcd->dev.nPhoto == 0) photo_rates = nullptr;
+  tchem_atmchem_options_t base = *opts;
+  base.cells_per_block = 0;
+  base.blocks_per_cu = 0;
+  const int luIn = base.lu;
+  // candidate LU forms
+  std::vector<int> lus;
+  if (luIn == TCHEM_LU_AUTO || luIn == TCHEM_LU_SPARSE) {
+    tchem_atmchem_options_t t = base;
+    t.lu = TCHEM_LU_SPARSE;
+    tchem_atmchem_options_t r;
+    if (resolveOptions(&t, kmcd, &r, who) == TCHEM_OK) lus.push_back(TCHEM_LU_SPARSE);
+    else if (luIn == TCHEM_LU_SPARSE) return TCHEM_ERR_UNSUPPORTED;
+  }
+  if (luIn == TCHEM_LU_AUTO || luIn == TCHEM_LU_DENSE) lus.push_back(TCHEM_LU_DENSE);
+  {
+    tchem_atmchem_options_t r;
+    int rc = resolveOptions(&base, kmcd, &r, who);  // validates the remaining fields
+    if (rc) return rc;
+  }
+  const int nb = kmcd->dev.nSpec + 3, np = kmcd->dev.nPhoto;
+  const int n = sampleCells > 0 ? std::min(sampleCells, nBatch) : std::min(nBatch, kmcd->numCUs * 48);
+  const int iters = sampleIterations > 0 ? sampleIterations : 24;
+  const size_t stride = size_t(nBatch / n);  // sample rows 0, stride, 2 stride, ...
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int rc = ensureKf(kmcd, n, s);
+  if (rc) return rc;
+  // scratch: pristine sample rows, working rows, tadv, t/dt
+  std::vector<void*> bufs;
+  auto alloc = [&](size_t bytes) -> void* {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return nullptr;
+    bufs.push_back(p);
+    return p;
+  };
+  auto freeAll = [&]() {
+    (void)hipStreamSynchronize(s);
+    for (void* p : bufs) (void)hipFree(p);
+  };
+  double* st0 = static_cast<double*>(alloc(sizeof(double) * size_t(n) * nb));
+  double* st = static_cast<double*>(alloc(sizeof(double) * size_t(n) * nb));
+  double* ph = np ? static_cast<double*>(alloc(sizeof(double) * size_t(n) * np)) : nullptr;
+  tchem_time_advance_t* ta = static_cast<tchem_time_advance_t*>(alloc(sizeof(tchem_time_advance_t) * size_t(n)));
+  double* tt = static_cast<double*>(alloc(sizeof(double) * size_t(n) * 2));
+  if (!st0 || !st || (np && !ph) || !ta || !tt) {
+    freeAll();
+    return setError(TCHEM_ERR_HIP, std::string(who) + ": scratch allocation failed");
+  }
+  std::vector<tchem_time_advance_t> hta(n);
+  std::vector<double> ht(size_t(n) * 2, 0.0);
+  hipError_t e = hipMemcpy2DAsync(st0, sizeof(double) * nb, state, sizeof(double) * nb * stride, sizeof(double) * nb,
+                                  n, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess && ph)
+    e = hipMemcpy2DAsync(ph, sizeof(double) * np, photo_rates, sizeof(double) * np * stride, sizeof(double) * np, n,
+                         hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(hta.data(), sizeof(tchem_time_advance_t), tadv, sizeof(tchem_time_advance_t) * stride,
+                         sizeof(tchem_time_advance_t), n, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    freeAll();
+    return hipFail(e, "sample copy");
+  }
+  for (int i = 0; i < n; ++i) {
+    hta[i]._num_time_iterations_per_interval = iters;
+    ht[i] = hta[i]._tbeg;  // t_out starts at tbeg: no cell is skipped
+  }
+  e = hipMemcpyAsync(ta, hta.data(), sizeof(tchem_time_advance_t) * n, hipMemcpyHostToDevice, s);
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&ev0);
+  if (e == hipSuccess) e = hipEventCreate(&ev1);
+  if (e != hipSuccess) {
+    freeAll();
+    return hipFail(e, "autotune setup");
+  }
+  auto trial = [&](const tchem_atmchem_options_t& o, float* ms) -> int {
+    hipError_t ex = hipMemcpyAsync(st, st0, sizeof(double) * size_t(n) * nb, hipMemcpyDeviceToDevice, s);
+    if (ex == hipSuccess) ex = hipMemcpyAsync(tt, ht.data(), sizeof(double) * size_t(n) * 2, hipMemcpyHostToDevice, s);
+    if (ex != hipSuccess) return hipFail(ex, "trial reset");
+    IntegratorArgs a{};
+    a.nBatch = n;
+    a.tol_newton = tol_newton;
+    a.tol_time = tol_time;
+    a.tadv = ta;
+    a.state = st;
+    a.photo = ph;
+    a.t_out = tt;
+    a.dt_out = tt + n;
+    a.state_out = st;
+    a.kf = kmcd->kf;
+    if ((ex = hipEventRecord(ev0, s)) != hipSuccess) return hipFail(ex, "hipEventRecord");
+    const int r = runIntegrator(kmcd, o, nullptr, 0, a, s, who);
+    if (r) return r;
+    if ((ex = hipEventRecord(ev1, s)) != hipSuccess) return hipFail(ex, "hipEventRecord");
+    if ((ex = hipEventSynchronize(ev1)) != hipSuccess) return hipFail(ex, "hipEventSynchronize");
+    if ((ex = hipEventElapsedTime(ms, ev0, ev1)) != hipSuccess) return hipFail(ex, "hipEventElapsedTime");
+    return TCHEM_OK;
+  };
+  float best = -1.0f;
+  tchem_atmchem_options_t bestO{};
+  for (int lu : lus) {
+    tchem_atmchem_options_t t = base;
+    t.lu = lu;
+    tchem_atmchem_options_t r;
+    if ((rc = resolveOptions(&t, kmcd, &r, who))) break;
+    const int cmax = r.cells_per_block;  // as many as fit
+    for (int c = cmax; c >= 1 && rc == TCHEM_OK; --c) {
+      tchem_atmchem_options_t oc = r;
+      oc.cells_per_block = c;
+      int bmax = 1;
+      if (oc.schedule == TCHEM_SCHEDULE_PERSISTENT && (rc = resolveBlocksPerCU(kmcd, oc, false, &bmax))) break;
+      for (int b = bmax; b >= 1 && b >= bmax - 1; --b) {
+        oc.blocks_per_cu = oc.schedule == TCHEM_SCHEDULE_PERSISTENT ? b : 0;
+        float ms = 0.0f;
+        if ((rc = trial(oc, &ms)) || (rc = trial(oc, &ms))) break;  // first run warms the kernel instance
+        if (best < 0.0f || ms < best) {
+          best = ms;
+          bestO = oc;
+        }
+        if (oc.schedule != TCHEM_SCHEDULE_PERSISTENT) break;
+      }
+    }
+    if (rc) break;
+  }
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  freeAll();
+  if (rc) return rc;
+  if (best < 0.0f) return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": no launch configuration fits");
+  *opts = bestO;
+  if (best_seconds) *best_seconds = double(best) * 1e-3;
   return TCHEM_OK;
 }
 

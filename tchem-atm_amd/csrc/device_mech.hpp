@@ -111,6 +111,7 @@ struct IntegratorArgs {
   int cellsPerGroup;                 // wavefronts (cells) per workgroup
   int cellBytes;                     // LDS bytes of one cell region
   AdjointRecord rec;                 // h == nullptr: no record
+  const int* skip;                   // nullable: the launch does nothing when *skip != 0 (captured interval loops)
 };
 
 // LDS bytes of one cell: the iteration matrix in TA precision (bytesA per

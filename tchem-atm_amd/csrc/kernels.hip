@@ -1254,6 +1254,7 @@ __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L
 template <class B, int QM, int kThreads, bool REC>
 __global__ __launch_bounds__(kThreads) void trbdf2_kernel(IntegratorArgs a) {
   extern __shared__ __align__(16) char lds_raw[];
+  if (a.skip && __builtin_amdgcn_readfirstlane(*a.skip)) return;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   B::stage(a, lds_raw);
   const typename B::Lds L = B::carve(lds_raw, a, wave);
@@ -1281,7 +1282,8 @@ __global__ __launch_bounds__(kThreads) void trbdf2_kernel(IntegratorArgs a) {
 // parameters the discrete adjoint differentiates with respect to (lstride 0:
 // one set for every cell).
 __global__ __launch_bounds__(256) void rates_kernel(DevMech M, int nBatch, const double* state, const double* photo,
-                                                   const double* lscale, int lstride, double* kf) {
+                                                   const double* lscale, int lstride, double* kf, const int* skip) {
+  if (skip && *skip) return;
   const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
   if (idx >= (long long)nBatch * M.nReac) return;
   const int cell = int(idx / M.nReac), r = int(idx - (long long)cell * M.nReac);
@@ -1574,12 +1576,42 @@ __global__ void adjoint_setup_kernel(int nBatch, int maxSteps, const tchem_time_
   }
 }
 
+// Control words of a captured interval loop: ctl[0] = stopped, ctl[1] =
+// intervals run.
+__global__ void loop_reset_kernel(int* ctl) {
+  if (threadIdx.x == 0) {
+    ctl[0] = 0;
+    ctl[1] = 0;
+  }
+}
+
+// The driver's stop test after an interval (src/examples/TChem_AtmosphericChemistry.cpp:340-350):
+// stop when mean_i t_out[i] > fraction * tend (tend of cell 0); a fixed
+// reduction order, so the decision is reproducible.
+__global__ __launch_bounds__(1024) void stop_check_kernel(int n, const double* t_out, const tchem_time_advance_t* tadv,
+                                                         double fraction, int* ctl) {
+  __shared__ double part[1024];
+  if (ctl[0]) return;  // block-uniform
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) s += t_out[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && part[0] / double(n) > fraction * tadv[0]._tend) ctl[0] = 1;
+}
+
 __global__ void reset_counter_kernel(int* counter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
 }
 
-__global__ void update_tadv_kernel(int n, tchem_time_advance_t* tadv, const double* t_out, const double* dt_out) {
+__global__ void update_tadv_kernel(int n, tchem_time_advance_t* tadv, const double* t_out, const double* dt_out,
+                                   int* ctl) {
+  if (ctl && ctl[0]) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ctl && i == 0) ctl[1] += 1;  // intervals run
   if (i < n) {
     tadv[i]._tbeg = t_out[i];
     tadv[i]._dt = dt_out[i];
@@ -1704,11 +1736,11 @@ hipError_t integratorMaxBlocksPerCU(int precision, int sparse, int m, int thread
 }
 
 hipError_t launchRates(const DevMech& M, int nBatch, const double* state, const double* photo, const double* lscale,
-                       int lstride, double* kf, hipStream_t stream) {
+                       int lstride, double* kf, const int* skip, hipStream_t stream) {
   const long long n = (long long)nBatch * M.nReac;
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(rates_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, M, nBatch, state, photo,
-                     lscale, lstride, kf);
+                     lscale, lstride, kf, skip);
   return hipGetLastError();
 }
 
@@ -1749,10 +1781,17 @@ hipError_t launchResetCounter(int* counter, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launchUpdateTadv(int n, tchem_time_advance_t* tadv, const double* t_out, const double* dt_out,
+hipError_t launchUpdateTadv(int n, tchem_time_advance_t* tadv, const double* t_out, const double* dt_out, int* ctl,
                             hipStream_t stream) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(update_tadv_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, tadv, t_out, dt_out);
+  hipLaunchKernelGGL(update_tadv_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, tadv, t_out, dt_out, ctl);
+  return hipGetLastError();
+}
+
+hipError_t launchLoopControl(int n, const double* t_out, const tchem_time_advance_t* tadv, double fraction, int* ctl,
+                             bool reset, hipStream_t stream) {
+  if (reset) hipLaunchKernelGGL(loop_reset_kernel, dim3(1), dim3(64), 0, stream, ctl);
+  else hipLaunchKernelGGL(stop_check_kernel, dim3(1), dim3(1024), 0, stream, n, t_out, tadv, fraction, ctl);
   return hipGetLastError();
 }
 

@@ -43,11 +43,15 @@ class DeviceRun:
         self.t.fill_(w.tbeg)
         self.dt.fill_(w.dtmax)  # examples/...:227 (not read: tadv._dt is what the integrator uses)
         ipi = w.iters_per_interval if iters_per_interval is None else iters_per_interval
-        self.tadv = T.time_advance_tensor(self.n, w.tbeg, w.tend, w.dt_in, w.dtmin, w.dtmax, w.max_newton, ipi,
-                                          w.jacobian_interval, self.device)
+        tadv = T.time_advance_tensor(self.n, w.tbeg, w.tend, w.dt_in, w.dtmin, w.dtmax, w.max_newton, ipi,
+                                     w.jacobian_interval, self.device)
+        if getattr(self, "tadv", None) is not None and self.tadv.shape == tadv.shape:
+            self.tadv.copy_(tadv)  # in place: a captured graph holds this buffer
+        else:
+            self.tadv = tadv
 
     def run_intervals(self, precision=T.PRECISION_FP64, schedule=T.SCHEDULE_GRID, max_intervals=100000,
-                      blocks_per_cu=0, collect_stats=False, stream=None, lu=T.LU_AUTO):
+                      blocks_per_cu=0, collect_stats=False, stream=None, lu=T.LU_AUTO, cells_per_block=0):
         """Reference host loop; returns the number of launches."""
         torch = self.torch
         tend = self.w.tend
@@ -58,7 +62,7 @@ class DeviceRun:
                                                   self.t, self.dt, self.state, self.kmcd, self.photo,
                                                   self.stats if collect_stats else None, precision=precision,
                                                   schedule=schedule, blocks_per_cu=blocks_per_cu, lu=lu,
-                                                  stream=stream)
+                                                  cells_per_block=cells_per_block, stream=stream)
             if collect_stats:
                 s = T.stats_from_tensor(self.stats)
                 arr = np.stack([s[f] for f in T.COUNTERS], 1).astype(np.int64)
@@ -75,16 +79,40 @@ class DeviceRun:
         self.total_stats = total
         return it
 
+    def run_intervals_graph(self, precision=T.PRECISION_FP64, schedule=T.SCHEDULE_GRID, intervals_per_launch=32,
+                            max_intervals=100000, blocks_per_cu=0, lu=T.LU_AUTO, stream=None):
+        """The same loop as run_intervals, captured: each graph launch runs
+        up to intervals_per_launch intervals with the stop test on the
+        device; the host reads the stop flag once per launch.  Returns the
+        number of intervals run (equal to run_intervals')."""
+        key = (precision, schedule, intervals_per_launch, blocks_per_cu, lu)
+        if getattr(self, "_graph_key", None) != key:
+            self.graph = T.AtmosphericChemistry.captureGraph(
+                self.tol_newton, self.tol_time, self.tadv, self.state, self.t, self.dt, self.state, self.kmcd,
+                intervals_per_launch, 0.9999, photo_rates=self.photo, precision=precision, schedule=schedule,
+                blocks_per_cu=blocks_per_cu, lu=lu)
+            self._graph_key = key
+        total = 0
+        while total < max_intervals:
+            self.graph.launch(stream)
+            n, stopped = self.graph.status()
+            total += n
+            if stopped or n < intervals_per_launch:
+                break
+        return total
+
     def run_to_end(self, precision=T.PRECISION_MIXED, schedule=T.SCHEDULE_PERSISTENT, blocks_per_cu=0,
-                   collect_stats=False, stream=None, lu=T.LU_AUTO):
+                   collect_stats=False, stream=None, lu=T.LU_AUTO, cells_per_block=0):
         """Every cell to tend in one launch (iteration cap 10^7 per cell)."""
         w = self.w
         self.tadv = T.time_advance_tensor(self.n, w.tbeg, w.tend, w.dt_in, w.dtmin, w.dtmax, w.max_newton,
                                           10_000_000, w.jacobian_interval, self.device)
+        self._graph_key = None  # a captured loop holds the previous tadv buffer
         T.AtmosphericChemistry.runDeviceBatch(self.tol_newton, self.tol_time, None, self.tadv, self.state, self.t,
                                               self.dt, self.state, self.kmcd, self.photo,
                                               self.stats if collect_stats else None, precision=precision,
-                                              schedule=schedule, blocks_per_cu=blocks_per_cu, lu=lu, stream=stream)
+                                              schedule=schedule, blocks_per_cu=blocks_per_cu, lu=lu,
+                                              cells_per_block=cells_per_block, stream=stream)
         if collect_stats:
             s = T.stats_from_tensor(self.stats)
             self.total_stats = np.stack([s[f] for f in T.COUNTERS], 1).astype(np.int64)

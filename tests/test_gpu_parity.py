@@ -203,6 +203,74 @@ def test_persistent_schedule_is_bitwise_identical(fp64_vs_oracle, gpu, pkg):
     assert np.array_equal(run.t.cpu().numpy(), g_t)
 
 
+@pytest.mark.parametrize("schedule", ["grid", "persistent"])
+def test_captured_interval_loop_matches_eager_loop(fp64_vs_oracle, cb05, schedule, gpu, pkg):
+    """The driver loop captured into a HIP graph (7 intervals per launch, the
+    stop test on the device): after every replay the state equals the eager
+    loop's state after the same number of intervals, bit for bit, with eager
+    work and synchronisations between the replays; the loop stops after the
+    same interval count."""
+    torch = gpu
+    w, run, g_state, g_t, *_ = fp64_vs_oracle
+    kmd, kmcd, _ = cb05
+    sched = pkg.SCHEDULE_GRID if schedule == "grid" else pkg.SCHEDULE_PERSISTENT
+    k = 7
+    print("case: captured loop,", schedule, "lu", run.lu, flush=True)
+    run.reset()
+    snaps, n_eager = [], 0
+    while True:
+        pkg.AtmosphericChemistry.runDeviceBatch(run.tol_newton, run.tol_time, None, run.tadv, run.state, run.t,
+                                                run.dt, run.state, kmcd, run.photo, precision=pkg.PRECISION_FP64,
+                                                schedule=sched, lu=run.lu)
+        pkg.AtmosphericChemistry.updateTimeAdvance(run.tadv, run.t, run.dt)
+        n_eager += 1
+        stop = float(run.t.mean()) > w.tend * 0.9999
+        if n_eager % k == 0 or stop:
+            snaps.append(run.state.cpu().numpy().copy())
+        if stop:
+            break
+    np.testing.assert_array_equal(snaps[-1], g_state)
+    run.reset()
+    graph = pkg.AtmosphericChemistry.captureGraph(run.tol_newton, run.tol_time, run.tadv, run.state, run.t, run.dt,
+                                                  run.state, kmcd, k, 0.9999, photo_rates=run.photo,
+                                                  precision=pkg.PRECISION_FP64, schedule=sched, lu=run.lu)
+    other = torch.zeros(w.n, kmd.nActive, dtype=torch.float64, device="cuda")
+    total, launches = 0, 0
+    while True:
+        graph.launch()
+        n, stopped = graph.status()
+        total += n
+        launches += 1
+        np.testing.assert_array_equal(run.state.cpu().numpy(), snaps[launches - 1])
+        # eager work between replays
+        pkg.SourceTerm.runDeviceBatch(run.state, run.photo, None, other, kmcd)
+        torch.cuda.synchronize()
+        if stopped:
+            break
+        assert n == k
+    assert total == n_eager and launches == len(snaps)
+    np.testing.assert_array_equal(run.t.cpu().numpy(), g_t)
+
+
+def test_autotuned_launch_shape_gives_identical_results(fp64_vs_oracle, cb05, gpu, pkg):
+    """The autotuner returns a valid launch shape for the batch; the per-cell
+    arithmetic does not depend on it (bitwise equal to the grid launch)."""
+    w, run, g_state, g_t, *_ = fp64_vs_oracle
+    kmcd = cb05[1]
+    run.reset()
+    tuned = pkg.AtmosphericChemistry.autotune(run.tol_newton, run.tol_time, run.tadv, run.state, kmcd,
+                                              photo_rates=run.photo, precision=pkg.PRECISION_FP64,
+                                              schedule=pkg.SCHEDULE_PERSISTENT, lu=run.lu)
+    print("autotuned:", tuned, flush=True)
+    assert tuned["lu"] == run.lu and tuned["cells_per_block"] >= 1 and tuned["blocks_per_cu"] >= 1
+    assert tuned["seconds"] > 0
+    np.testing.assert_array_equal(run.state.cpu().numpy(), w.state)  # the caller's arrays are untouched
+    run.run_intervals(precision=pkg.PRECISION_FP64, schedule=pkg.SCHEDULE_PERSISTENT, lu=tuned["lu"],
+                      blocks_per_cu=tuned["blocks_per_cu"], cells_per_block=tuned["cells_per_block"])
+    gpu.cuda.synchronize()
+    np.testing.assert_array_equal(run.state.cpu().numpy(), g_state)
+
+
 def test_mixed_precision_error_against_fp64_oracle(fp64_vs_oracle, cb05, gpu, pkg):
     """FP32 iteration matrix and LU, FP64 state/residual/error/step control:
     within 1e-4 of the FP64 oracle per species."""
