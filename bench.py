@@ -333,7 +333,9 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--cells", type=int, default=131072)
-    ap.add_argument("--form", choices=["mixed", "fp64", "fp64-plain"], default="mixed")
+    ap.add_argument("--form", choices=["mixed", "fp64", "fp64-plain", "fp64-plain-graph"], default="mixed")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="persistent forms: default launch shape instead of the autotuner's")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 run of the same workload")
     ap.add_argument("--plain-cells", type=int, default=8192, help="cells timed in the fp64-plain form (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=16, help="cells in the CPU baseline sample (0: skip)")
@@ -381,7 +383,9 @@ def main():
         "mixed": dict(precision=T.PRECISION_MIXED, lu=T.LU_AUTO, persistent=True),
         "fp64": dict(precision=T.PRECISION_FP64, lu=T.LU_AUTO, persistent=True),
         "fp64-plain": dict(precision=T.PRECISION_FP64, lu=T.LU_DENSE, persistent=False),
+        "fp64-plain-graph": dict(precision=T.PRECISION_FP64, lu=T.LU_DENSE, persistent=False, graph=True),
     }
+    tuned_shapes = {}
 
     def timed(form, n, steps, warmup):
         """Runs `steps` timed steps of the workload on n cells; returns
@@ -389,12 +393,28 @@ def main():
         f = forms[form]
         w = S.config1(kmd, base[0], n=n, seed=1)
         run = D.DeviceRun(kmcd, w)
+        shape = dict(lu=f["lu"], cells_per_block=0, blocks_per_cu=0)
+        if f["persistent"] and not args.no_autotune:
+            # launch shape from the autotuner on this batch (outside the timed region)
+            run.reset(10_000_000)
+            t0 = time.perf_counter()
+            tuned = T.AtmosphericChemistry.autotune(run.tol_newton, run.tol_time, run.tadv, run.state, kmcd,
+                                                    photo_rates=run.photo, precision=f["precision"],
+                                                    schedule=T.SCHEDULE_PERSISTENT, lu=f["lu"])
+            shape = dict(lu=tuned["lu"], cells_per_block=tuned["cells_per_block"],
+                         blocks_per_cu=tuned["blocks_per_cu"])
+            tuned_shapes[form] = dict(shape, tuning_s=time.perf_counter() - t0)
+            log(f"autotuned {form}: {tuned_shapes[form]}")
 
         def step(collect):
             run.reset()
             if f["persistent"]:
-                return run.run_to_end(precision=f["precision"], schedule=T.SCHEDULE_PERSISTENT, lu=f["lu"],
-                                      collect_stats=collect)
+                return run.run_to_end(precision=f["precision"], schedule=T.SCHEDULE_PERSISTENT,
+                                      collect_stats=collect, **shape)
+            if f.get("graph"):
+                n_int = run.run_intervals_graph(precision=f["precision"], schedule=T.SCHEDULE_GRID, lu=f["lu"],
+                                                intervals_per_launch=64)
+                return (n_int + 63) // 64
             return run.run_intervals(precision=f["precision"], schedule=T.SCHEDULE_GRID, lu=f["lu"],
                                      collect_stats=collect)
 
@@ -455,9 +475,17 @@ def main():
     if args.form == "mixed" and args.plain_cells > 0:
         wallp, _, launchesp, runp, wp = timed("fp64-plain", args.plain_cells, 1, 0)
         vp = wp.cell_hours / wallp
+        wallg, _, launchesg, rung, _ = timed("fp64-plain-graph", args.plain_cells, 1, 1)
+        vg = wp.cell_hours / wallg
+        same = bool(np.array_equal(runp.state.cpu().numpy(), rung.state.cpu().numpy()))
         config.update({"fp64_plain_value": vp, "fp64_plain_cells": args.plain_cells,
-                       "fp64_plain_launches": launchesp, "mixed_over_fp64_plain": value / vp})
-        log(f"fp64-plain on {args.plain_cells} cells: {vp:.1f} cell-hours/s ({launchesp:.0f} launches)")
+                       "fp64_plain_launches": launchesp, "fp64_plain_graph_value": vg,
+                       "fp64_plain_graph_launches": launchesg, "fp64_plain_graph_identical": same,
+                       "mixed_over_fp64_plain": value / max(vp, vg)})
+        log(f"fp64-plain on {args.plain_cells} cells: {vp:.1f} cell-hours/s ({launchesp:.0f} interval launches); "
+            f"captured loop {vg:.1f} cell-hours/s ({launchesg:.0f} graph launches, identical {same})")
+    if tuned_shapes:
+        config["autotuned"] = tuned_shapes
 
     out = {
         "metric": METRIC,
