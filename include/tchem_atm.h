@@ -59,6 +59,9 @@ typedef struct {
 #define TCHEM_LU_AUTO 0   /* sparse when the mechanism has a sparse schedule, dense otherwise */
 #define TCHEM_LU_DENSE 1  /* dense LU with partial pivoting (the oracle's arithmetic) */
 #define TCHEM_LU_SPARSE 2 /* level-scheduled sparse LU, pivots fixed by the fill-reducing order */
+#define TCHEM_PROGRAMS_AUTO 0   /* LDS when they leave room for 4 cells, device memory otherwise */
+#define TCHEM_PROGRAMS_LDS 1    /* staged into LDS once per workgroup, shared by its cells */
+#define TCHEM_PROGRAMS_MEMORY 2 /* read from device memory (L2-resident); all LDS goes to cells */
 typedef struct {
   int precision;         /* TCHEM_PRECISION_* */
   int schedule;          /* TCHEM_SCHEDULE_* */
@@ -66,7 +69,8 @@ typedef struct {
   int lu;                /* TCHEM_LU_* */
   int cells_per_block;   /* sparse form: cells (wavefronts) per workgroup sharing the LDS-resident
                             mechanism program (0 = as many as fit); the dense form runs 1 */
-  int reserved[3];       /* must be zero */
+  int programs;          /* sparse form: where the compiled mechanism programs live, TCHEM_PROGRAMS_* */
+  int reserved[2];       /* must be zero */
 } tchem_atmchem_options_t;
 
 /* Last error message of the calling thread ("" when none). */
@@ -161,9 +165,10 @@ int tchem_graph_destroy(tchem_graph_t graph);
  * times tchem_atmchem_run_device_batch on a strided sample of the batch
  * (sampleCells, 0 = 48 per CU; each cell limited to sampleIterations TR-BDF2
  * iterations from its tadv, 0 = 24) for every LU form opts->lu allows (AUTO:
- * sparse and dense), every cells_per_block that fits, and the two largest
- * blocks_per_cu of the persistent schedule, and writes the fastest into
- * *opts (precision and schedule are kept).  The caller's arrays are not
+ * sparse and dense), every program placement opts->programs allows (AUTO:
+ * LDS and device memory), every cells_per_block that fits, and the two
+ * largest blocks_per_cu of the persistent schedule, and writes the fastest
+ * into *opts (precision and schedule are kept).  The caller's arrays are not
  * modified.  Synchronous; allocates scratch. */
 int tchem_atmchem_autotune(tchem_atmchem_options_t* opts, int nBatch, const double* tol_newton,
                            const double* tol_time, const tchem_time_advance_t* tadv, const double* state,

@@ -37,6 +37,9 @@ SCHEDULE_PERSISTENT = 1
 LU_AUTO = 0     # sparse when the mechanism has a sparse schedule, dense otherwise
 LU_DENSE = 1    # dense LU with partial pivoting (the oracle's arithmetic)
 LU_SPARSE = 2   # level-scheduled sparse LU, pivots fixed by the fill-reducing order
+PROGRAMS_AUTO = 0    # sparse form: programs in LDS when they leave room for 4 cells
+PROGRAMS_LDS = 1     # staged into LDS once per workgroup
+PROGRAMS_MEMORY = 2  # read from device memory (L2-resident), all LDS for cells
 
 TIME_ADVANCE_DTYPE = np.dtype([
     ("_tbeg", "f8"), ("_tend", "f8"), ("_dt", "f8"), ("_dtmin", "f8"), ("_dtmax", "f8"),
@@ -68,7 +71,8 @@ class TChemError(RuntimeError):
 
 class _Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("schedule", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
-                ("lu", ctypes.c_int), ("cells_per_block", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
+                ("lu", ctypes.c_int), ("cells_per_block", ctypes.c_int), ("programs", ctypes.c_int),
+                ("reserved", ctypes.c_int * 2)]
 
 
 def _load():
@@ -215,23 +219,23 @@ def setScenarioConditions(filename: str, kmd: KineticModelData) -> np.ndarray:
 class AtmosphericChemistry:
     @staticmethod
     def getWorkSpaceSize(kmcd: KineticModelConstData, precision: int = PRECISION_FP64, lu: int = LU_AUTO,
-                         cells_per_block: int = 0) -> int:
+                         cells_per_block: int = 0, programs: int = PROGRAMS_AUTO) -> int:
         """Bytes of LDS one workgroup uses in the given form."""
         b = ctypes.c_size_t()
-        opts = _Options(precision, SCHEDULE_GRID, 0, lu, cells_per_block)
+        opts = _Options(precision, SCHEDULE_GRID, 0, lu, cells_per_block, programs)
         _check(_lib.tchem_atmchem_workspace_bytes(kmcd._h, ctypes.byref(opts), ctypes.byref(b)))
         return b.value
 
     @staticmethod
     def runDeviceBatch(tol_newton, tol_time, fac, tadv, state, t_out, dt_out, state_out, kmcd, photo_rates=None,
                        stats=None, precision=PRECISION_FP64, schedule=SCHEDULE_GRID, blocks_per_cu=0, lu=LU_AUTO,
-                       cells_per_block=0, stream=None):
+                       cells_per_block=0, programs=PROGRAMS_AUTO, stream=None):
         """One integration interval over the batch (asynchronous on `stream`).
 
         tadv is a uint8 device tensor holding TIME_ADVANCE_DTYPE records (or any
         tensor with 56 bytes per cell); stats likewise CELL_STATS_DTYPE."""
         nBatch = state.shape[0]
-        opts = _Options(precision, schedule, blocks_per_cu, lu, cells_per_block)
+        opts = _Options(precision, schedule, blocks_per_cu, lu, cells_per_block, programs)
         _check(_lib.tchem_atmchem_run_device_batch(
             ctypes.byref(opts), nBatch, _ptr(tol_newton, what="tol_newton"), _ptr(tol_time, what="tol_time"),
             _ptr(fac, None, "fac"), _ptr(tadv, None, "tadv"), _ptr(state, what="state"),
@@ -241,12 +245,13 @@ class AtmosphericChemistry:
     @staticmethod
     def captureGraph(tol_newton, tol_time, tadv, state, t_out, dt_out, state_out, kmcd, n_intervals,
                      stop_fraction=0.9999, photo_rates=None, stats=None, precision=PRECISION_FP64,
-                     schedule=SCHEDULE_GRID, blocks_per_cu=0, lu=LU_AUTO, cells_per_block=0) -> "Graph":
+                     schedule=SCHEDULE_GRID, blocks_per_cu=0, lu=LU_AUTO, cells_per_block=0,
+                     programs=PROGRAMS_AUTO) -> "Graph":
         """The interval loop (runDeviceBatch + updateTimeAdvance) x n_intervals
         captured into a HIP graph bound to these arrays; with stop_fraction > 0
         the driver's stop test (mean t > stop_fraction * tend) runs on the
         device after each interval."""
-        opts = _Options(precision, schedule, blocks_per_cu, lu, cells_per_block)
+        opts = _Options(precision, schedule, blocks_per_cu, lu, cells_per_block, programs)
         h = ctypes.c_void_p()
         _check(_lib.tchem_atmchem_graph_create(
             ctypes.byref(opts), state.shape[0], _ptr(tol_newton, what="tol_newton"), _ptr(tol_time, what="tol_time"),
@@ -268,7 +273,8 @@ class AtmosphericChemistry:
             _ptr(tadv, None, "tadv"), _ptr(state, what="state"), _ptr(photo_rates, what="photo_rates"),
             int(sample_cells), int(sample_iterations), ctypes.byref(best), kmcd._h, _stream_ptr(stream)))
         return dict(precision=opts.precision, schedule=opts.schedule, lu=opts.lu,
-                    cells_per_block=opts.cells_per_block, blocks_per_cu=opts.blocks_per_cu, seconds=best.value)
+                    cells_per_block=opts.cells_per_block, blocks_per_cu=opts.blocks_per_cu, programs=opts.programs,
+                    seconds=best.value)
 
     @staticmethod
     def updateTimeAdvance(tadv, t_out, dt_out, stream=None):

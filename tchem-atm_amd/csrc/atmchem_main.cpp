@@ -1,0 +1,267 @@
+// Box-model driver with the command line and output files of the
+// reference's TChem_AtmosphericChemistry.x
+// (src/examples/TChem_AtmosphericChemistry.cpp:46-352): load the mechanism
+// (writing kmod.echo), read the scenario, relaunch the batched integrator
+// interval by interval until the mean cell time exceeds 0.9999 tend, write
+// the state of every cell after every interval (iter, t, dt, state row) and
+// the wall time of every interval (wall_times.json).  All compute goes
+// through the C ABI of libtchem_atm.so.
+//
+// Options are given as --name=value (booleans: true/false/1/0), as in the
+// reference's CommandLineParser (src/core/TChem_CommandLineParser.hpp).
+// team_thread_size / vector_thread_size / inputsPath are accepted and
+// ignored.  Additional options of this driver: --precision=fp64|mixed,
+// --lu=auto|dense|sparse, --schedule=grid|persistent.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/tchem_atm.h"
+
+namespace {
+
+struct Options {
+  std::string chemfile = "chem.yaml", outputfile = "AtmosphericChemistry.dat", times = "wall_times.json";
+  double tbeg = 0, tend = 1, dtmin = 1e-8, dtmax = 1e-1, rtol_time = 1e-4, atol_newton = 1e-10, rtol_newton = 1e-6;
+  int per_interval = 10, max_iters = 1000, max_newton = 100, jac_interval = 1, batch = 1;
+  bool verbose = false, write_profiles = true, cloned = false;
+  int precision = TCHEM_PRECISION_FP64, lu = TCHEM_LU_AUTO, schedule = TCHEM_SCHEDULE_GRID;
+};
+
+bool parseBool(const std::string& v) { return v == "true" || v == "1" || v == "True" || v == "TRUE"; }
+
+void usage(const char* argv0) {
+  std::printf(
+      "Usage: %s [options]\n  options (--name=value):\n"
+      "  --chemfile --outputfile --outputfile_times --inputsPath\n"
+      "  --tbeg --tend --dtmin --dtmax --tol-time --atol-newton --rtol-newton\n"
+      "  --time-iterations-per-interval --max-time-iterations --max-newton-iterations --jacobian-interval\n"
+      "  --batch_size --use_cloned_samples --write-time-profiles --verbose\n"
+      "  --team_thread_size --vector_thread_size (ignored)\n"
+      "  --precision=fp64|mixed --lu=auto|dense|sparse --schedule=grid|persistent\n",
+      argv0);
+}
+
+// returns 0 to continue, 1 when help was printed, -1 on a bad option
+int parse(int argc, char** argv, Options& o) {
+  for (int i = 1; i < argc; ++i) {
+    const std::string s = argv[i];
+    if (s == "--help") {
+      usage(argv[0]);
+      return 1;
+    }
+    if (s.rfind("--", 0) != 0) continue;
+    const size_t eq = s.find('=');
+    const std::string key = s.substr(2, eq == std::string::npos ? std::string::npos : eq - 2);
+    const std::string val = eq == std::string::npos ? "true" : s.substr(eq + 1);
+    try {
+      if (key == "chemfile") o.chemfile = val;
+      else if (key == "outputfile") o.outputfile = val;
+      else if (key == "outputfile_times") o.times = val;
+      else if (key == "tbeg") o.tbeg = std::stod(val);
+      else if (key == "tend") o.tend = std::stod(val);
+      else if (key == "dtmin") o.dtmin = std::stod(val);
+      else if (key == "dtmax") o.dtmax = std::stod(val);
+      else if (key == "tol-time") o.rtol_time = std::stod(val);
+      else if (key == "atol-newton") o.atol_newton = std::stod(val);
+      else if (key == "rtol-newton") o.rtol_newton = std::stod(val);
+      else if (key == "time-iterations-per-interval") o.per_interval = std::stoi(val);
+      else if (key == "max-time-iterations") o.max_iters = std::stoi(val);
+      else if (key == "max-newton-iterations") o.max_newton = std::stoi(val);
+      else if (key == "jacobian-interval") o.jac_interval = std::stoi(val);
+      else if (key == "batch_size") o.batch = std::stoi(val);
+      else if (key == "use_cloned_samples") o.cloned = parseBool(val);
+      else if (key == "write-time-profiles") o.write_profiles = parseBool(val);
+      else if (key == "verbose") o.verbose = parseBool(val);
+      else if (key == "precision") o.precision = val == "mixed" ? TCHEM_PRECISION_MIXED : TCHEM_PRECISION_FP64;
+      else if (key == "lu") o.lu = val == "dense" ? TCHEM_LU_DENSE : (val == "sparse" ? TCHEM_LU_SPARSE : TCHEM_LU_AUTO);
+      else if (key == "schedule")
+        o.schedule = val == "persistent" ? TCHEM_SCHEDULE_PERSISTENT : TCHEM_SCHEDULE_GRID;
+      else if (key == "team_thread_size" || key == "vector_thread_size" || key == "inputsPath" ||
+               key == "echo-command-line")
+        ;
+      else {
+        std::fprintf(stderr, "unknown option --%s\n", key.c_str());
+        return -1;
+      }
+    } catch (const std::exception&) {
+      std::fprintf(stderr, "bad value for --%s: %s\n", key.c_str(), val.c_str());
+      return -1;
+    }
+  }
+  return 0;
+}
+
+#define CHECK_T(call)                                                                  \
+  do {                                                                                 \
+    const int rc_ = (call);                                                            \
+    if (rc_) {                                                                         \
+      std::fprintf(stderr, "%s failed (%d): %s\n", #call, rc_, tchem_last_error());    \
+      return 1;                                                                        \
+    }                                                                                  \
+  } while (0)
+#define CHECK_H(call)                                                                  \
+  do {                                                                                 \
+    const hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess) {                                                            \
+      std::fprintf(stderr, "%s failed: %s\n", #call, hipGetErrorString(e_));           \
+      return 1;                                                                        \
+    }                                                                                  \
+  } while (0)
+
+template <typename T>
+struct DeviceArray {
+  T* p = nullptr;
+  ~DeviceArray() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+void writeState(FILE* f, int iter, const std::vector<double>& t, const std::vector<double>& dt,
+                const std::vector<double>& state, int n, int dim) {
+  for (int i = 0; i < n; ++i) {
+    std::fprintf(f, "%d \t %15.10e \t  %15.10e \t ", iter, t[i], dt[i]);
+    for (int k = 0; k < dim; ++k) std::fprintf(f, "%15.10e \t", state[size_t(i) * dim + k]);
+    std::fprintf(f, "\n");
+  }
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  Options o;
+  const int pr = parse(argc, argv, o);
+  if (pr) return pr > 0 ? 0 : 2;
+
+  tchem_kmd_t kmd = nullptr;
+  CHECK_T(tchem_kmd_load_yaml(o.chemfile.c_str(), "kmod.echo", &kmd));
+  int nSpec = 0, nConst = 0, nReac = 0, nPhoto = 0;
+  CHECK_T(tchem_kmd_info(kmd, &nSpec, &nConst, &nReac, &nPhoto));
+  std::printf("Number of Species %d \n", nSpec);
+  std::printf("Number of Reactions %d \n", nReac);
+  const int dim = nSpec + 3, m = nSpec - nConst;
+  std::vector<std::string> names(nSpec);
+  for (int k = 0; k < nSpec; ++k) {
+    char buf[64];
+    CHECK_T(tchem_kmd_species_name(kmd, k, buf, sizeof(buf)));
+    names[k] = buf;
+  }
+
+  int nFiles = 0;
+  CHECK_T(tchem_scenario_load(kmd, o.chemfile.c_str(), nullptr, 0, &nFiles));
+  std::vector<double> scen(size_t(nFiles) * dim);
+  CHECK_T(tchem_scenario_load(kmd, o.chemfile.c_str(), scen.data(), nFiles, &nFiles));
+  int n = nFiles;
+  std::vector<double> state;
+  if (nFiles == 1 && o.cloned && o.batch > 1) {
+    std::printf("Using cloned samples ... only for numerical experiments\n");
+    n = o.batch;
+    state.resize(size_t(n) * dim);
+    for (int i = 0; i < n; ++i) std::memcpy(&state[size_t(i) * dim], scen.data(), sizeof(double) * dim);
+  } else {
+    state = scen;
+  }
+
+  tchem_kmcd_t kmcd = nullptr;
+  CHECK_T(tchem_kmcd_create(kmd, &kmcd));
+
+  DeviceArray<double> d_state, d_t, d_dt, d_tn, d_tt;
+  DeviceArray<tchem_time_advance_t> d_tadv;
+  CHECK_H(hipMalloc(&d_state.p, sizeof(double) * state.size()));
+  CHECK_H(hipMalloc(&d_t.p, sizeof(double) * n));
+  CHECK_H(hipMalloc(&d_dt.p, sizeof(double) * n));
+  CHECK_H(hipMalloc(&d_tn.p, sizeof(double) * 2));
+  CHECK_H(hipMalloc(&d_tt.p, sizeof(double) * 2 * (m > 0 ? m : 1)));
+  CHECK_H(hipMalloc(&d_tadv.p, sizeof(tchem_time_advance_t) * n));
+  std::vector<double> t(n, o.tbeg), dt(n, o.dtmax);  // the reference fills dt with dtmax (examples/...:227)
+  const double tn[2] = {o.atol_newton, o.rtol_newton};
+  std::vector<double> tt(size_t(2) * (m > 0 ? m : 1));
+  for (int i = 0; i < m; ++i) {
+    tt[2 * i] = 1e-12;  // the reference's fixed absolute time tolerance
+    tt[2 * i + 1] = o.rtol_time;
+  }
+  tchem_time_advance_t ta{};
+  ta._tbeg = o.tbeg;
+  ta._tend = o.tend;
+  ta._dt = o.dtmin;
+  ta._dtmin = o.dtmin;
+  ta._dtmax = o.dtmax;
+  ta._max_num_newton_iterations = o.max_newton;
+  ta._num_time_iterations_per_interval = o.per_interval;
+  ta._jacobian_interval = o.jac_interval;
+  std::vector<tchem_time_advance_t> tadv(n, ta);
+  CHECK_H(hipMemcpy(d_state.p, state.data(), sizeof(double) * state.size(), hipMemcpyHostToDevice));
+  CHECK_H(hipMemcpy(d_t.p, t.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  CHECK_H(hipMemcpy(d_dt.p, dt.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  CHECK_H(hipMemcpy(d_tn.p, tn, sizeof(tn), hipMemcpyHostToDevice));
+  CHECK_H(hipMemcpy(d_tt.p, tt.data(), sizeof(double) * tt.size(), hipMemcpyHostToDevice));
+  CHECK_H(hipMemcpy(d_tadv.p, tadv.data(), sizeof(tchem_time_advance_t) * n, hipMemcpyHostToDevice));
+
+  FILE* fout = nullptr;
+  if (o.write_profiles) {
+    fout = std::fopen(o.outputfile.c_str(), "w");
+    if (!fout) {
+      std::fprintf(stderr, "cannot write %s\n", o.outputfile.c_str());
+      return 1;
+    }
+    std::fprintf(fout, "%s \t %s \t %s \t ", "iter", "t", "dt");
+    std::fprintf(fout, "%s \t %s \t %s \t", "Density[kg/m3]", "Pressure[Pascal]", "Temperature[K]");
+    for (int k = 0; k < nSpec; ++k) std::fprintf(fout, "%s \t", names[k].c_str());
+    std::fprintf(fout, "\n");
+    writeState(fout, -1, t, dt, state, n, dim);
+  }
+  FILE* ftimes = std::fopen(o.times.c_str(), "w");
+  if (!ftimes) {
+    std::fprintf(stderr, "cannot write %s\n", o.times.c_str());
+    return 1;
+  }
+  std::fprintf(ftimes, "{\n");
+  std::fprintf(ftimes, " \"Aerosol Chemistry\": \n {\n");
+
+  tchem_atmchem_options_t opts{};
+  opts.precision = o.precision;
+  opts.lu = o.lu;
+  opts.schedule = o.schedule;
+  int iter = 0;
+  double tsum = 0.0;
+  for (; iter < o.max_iters && tsum <= o.tend * 0.9999; ++iter) {
+    const auto w0 = std::chrono::steady_clock::now();
+    CHECK_T(tchem_atmchem_run_device_batch(&opts, n, d_tn.p, d_tt.p, nullptr, d_tadv.p, d_state.p, nullptr, d_t.p,
+                                           d_dt.p, d_state.p, nullptr, kmcd, nullptr));
+    CHECK_H(hipDeviceSynchronize());
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
+    std::fprintf(ftimes, "\"%s%d\": %20.14e, \n", "wall_time_iter_", iter, wall);
+    CHECK_H(hipMemcpy(t.data(), d_t.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    CHECK_H(hipMemcpy(dt.data(), d_dt.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (o.write_profiles) {
+      CHECK_H(hipMemcpy(state.data(), d_state.p, sizeof(double) * state.size(), hipMemcpyDeviceToHost));
+      writeState(fout, iter, t, dt, state, n, dim);
+    }
+    if (o.verbose) {
+      if (!o.write_profiles)
+        CHECK_H(hipMemcpy(state.data(), d_state.p, sizeof(double) * dim, hipMemcpyDeviceToHost));
+      std::printf("%e %e %e %e", t[0], state[0], state[1], state[2]);
+      for (int k = 3; k < dim; ++k) std::printf(" %e", state[k]);
+      std::printf("\n");
+    }
+    CHECK_T(tchem_atmchem_update_time_advance(n, d_tadv.p, d_t.p, d_dt.p, nullptr));
+    tsum = 0.0;
+    for (int i = 0; i < n; ++i) tsum += t[i];
+    tsum /= n;
+  }
+  std::fprintf(ftimes, "%s: %d, \n", "\"number_of_time_iters\"", iter);
+  std::fprintf(ftimes, "%s: %d \n", "\"number_of_samples\"", n);
+  std::fprintf(ftimes, "} \n ");
+  std::fprintf(ftimes, "}\n ");
+  std::fclose(ftimes);
+  if (fout) std::fclose(fout);
+  tchem_kmcd_destroy(kmcd);
+  tchem_kmd_destroy(kmd);
+  return 0;
+}

@@ -125,12 +125,23 @@ int resolveOptions(const tchem_atmchem_options_t* opts, const tchem_kmcd_s* k, t
   if (o->lu != TCHEM_LU_AUTO && o->lu != TCHEM_LU_DENSE && o->lu != TCHEM_LU_SPARSE)
     return setError(TCHEM_ERR_INVALID, std::string(who) + ": unknown lu");
   if (o->cells_per_block < 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": negative cells_per_block");
+  if (o->programs < TCHEM_PROGRAMS_AUTO || o->programs > TCHEM_PROGRAMS_MEMORY)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": unknown programs placement");
   const size_t bytesA = o->precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float);
   const size_t staticLds = integratorStaticLds(o->precision, 1, k->dev.m);
   const size_t maxLds = k->maxLds > staticLds ? k->maxLds - staticLds : 0;
-  // programs are staged into LDS when they leave room for at least 4 cells
   const size_t cellB = sparseCellBytes(k->dev, k->sparse, bytesA);
-  const bool staged = sparseProgramBytes(k->sparse) + 4 * cellB <= maxLds;
+  // AUTO: programs staged into LDS when that still leaves room for as many
+  // cells per workgroup as reading them from memory would (the kernel is
+  // latency bound: resident cells set its throughput); memory otherwise
+  if (o->programs == TCHEM_PROGRAMS_AUTO) {
+    const int cap = sparseMaxCellsPerGroup(o->precision, k->dev.m);
+    const size_t prog = sparseProgramBytes(k->sparse);
+    const size_t inMem = std::min<size_t>(cap, cellB ? maxLds / cellB : 0);
+    const size_t inLds = prog < maxLds ? std::min<size_t>(cap, cellB ? (maxLds - prog) / cellB : 0) : 0;
+    o->programs = (inLds >= 1 && inLds >= inMem) ? TCHEM_PROGRAMS_LDS : TCHEM_PROGRAMS_MEMORY;
+  }
+  const bool staged = o->programs == TCHEM_PROGRAMS_LDS;
   auto sparseFits = [&](int cells) {
     return (staged ? sparseProgramBytes(k->sparse) : 0) + size_t(cells) * cellB <= maxLds;
   };
@@ -158,19 +169,14 @@ int resolveOptions(const tchem_atmchem_options_t* opts, const tchem_kmcd_s* k, t
   return TCHEM_OK;
 }
 
-// LDS bytes of one workgroup
-// whether the sparse programs are staged into LDS for this precision
-bool sparseStaged(const tchem_kmcd_s* k, int precision) {
-  const size_t bytesA = precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float);
-  const size_t staticLds = integratorStaticLds(precision, 1, k->dev.m);
-  const size_t maxLds = k->maxLds > staticLds ? k->maxLds - staticLds : 0;
-  return sparseProgramBytes(k->sparse) + 4 * sparseCellBytes(k->dev, k->sparse, bytesA) <= maxLds;
-}
+// whether the sparse programs are staged into LDS (resolved options)
+bool sparseStaged(const tchem_atmchem_options_t& o) { return o.programs == TCHEM_PROGRAMS_LDS; }
 
+// LDS bytes of one workgroup (resolved options)
 size_t ldsBytes(const tchem_kmcd_s* k, const tchem_atmchem_options_t& o) {
   const size_t bytesA = o.precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float);
   if (o.lu == TCHEM_LU_SPARSE)
-    return (sparseStaged(k, o.precision) ? sparseProgramBytes(k->sparse) : 0) +
+    return (sparseStaged(o) ? sparseProgramBytes(k->sparse) : 0) +
            size_t(o.cells_per_block) * sparseCellBytes(k->dev, k->sparse, bytesA);
   return integratorLdsBytes(k->dev, bytesA);
 }
@@ -198,7 +204,7 @@ int ensureKf(tchem_kmcd_s* kmcd, int nBatch, hipStream_t s) {
 int resolveBlocksPerCU(const tchem_kmcd_s* kmcd, const tchem_atmchem_options_t& o, bool rec, int* bpc) {
   *bpc = o.blocks_per_cu;
   if (*bpc > 0) return TCHEM_OK;
-  const int sparse = o.lu == TCHEM_LU_SPARSE ? (sparseStaged(kmcd, o.precision) ? 1 : 2) : 0;
+  const int sparse = o.lu == TCHEM_LU_SPARSE ? (sparseStaged(o) ? 1 : 2) : 0;
   hipError_t e = integratorMaxBlocksPerCU(o.precision, sparse, kmcd->dev.m, 64 * o.cells_per_block,
                                           ldsBytes(kmcd, o), bpc, rec);
   if (e != hipSuccess) return hipFail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
@@ -220,7 +226,7 @@ int runIntegrator(tchem_kmcd_s* kmcd, const tchem_atmchem_options_t& o, const do
     return setError(TCHEM_ERR_INVALID, std::string(who) + ": kmcd belongs to another device");
   a.M = kmcd->dev;
   a.S = kmcd->sparse;
-  a.sparse = o.lu == TCHEM_LU_SPARSE ? (sparseStaged(kmcd, o.precision) ? 1 : 2) : 0;
+  a.sparse = o.lu == TCHEM_LU_SPARSE ? (sparseStaged(o) ? 1 : 2) : 0;
   a.queue = kmcd->queue;
   a.persistent = (o.schedule == TCHEM_SCHEDULE_PERSISTENT) ? 1 : 0;
   a.cellsPerGroup = o.cells_per_block;
@@ -927,11 +933,22 @@ int tchem_atmchem_autotune(tchem_atmchem_options_t* opts, int nBatch, const doub
   };
   float best = -1.0f;
   tchem_atmchem_options_t bestO{};
+  // (lu, programs placement) candidates; placement only matters for the sparse form
+  std::vector<std::pair<int, int>> forms;
   for (int lu : lus) {
+    if (lu == TCHEM_LU_SPARSE && base.programs == TCHEM_PROGRAMS_AUTO) {
+      forms.push_back({lu, TCHEM_PROGRAMS_LDS});
+      forms.push_back({lu, TCHEM_PROGRAMS_MEMORY});
+    } else {
+      forms.push_back({lu, base.programs});
+    }
+  }
+  for (const auto& form : forms) {
     tchem_atmchem_options_t t = base;
-    t.lu = lu;
+    t.lu = form.first;
+    t.programs = form.second;
     tchem_atmchem_options_t r;
-    if ((rc = resolveOptions(&t, kmcd, &r, who))) break;
+    if (resolveOptions(&t, kmcd, &r, who) != TCHEM_OK) continue;  // e.g. programs too large for LDS
     const int cmax = r.cells_per_block;  // as many as fit
     for (int c = cmax; c >= 1 && rc == TCHEM_OK; --c) {
       tchem_atmchem_options_t oc = r;

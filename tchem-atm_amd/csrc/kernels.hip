@@ -1676,14 +1676,23 @@ static const void* integratorFn() {
   return reinterpret_cast<const void*>(&trbdf2_kernel<B, QM, kThreads, REC>);
 }
 
-// Largest workgroup of the sparse form: 12 wavefronts (cells) for the mixed
-// form with m <= 128 (register budget 168 VGPRs), 8 otherwise (256 VGPRs).
-int sparseMaxCellsPerGroup(int precision, int m) { return (precision == TCHEM_PRECISION_MIXED && m <= 128) ? 12 : 8; }
+// Largest workgroup of the sparse form (m <= 128): 16 wavefronts (cells) for
+// the mixed form, 12 for FP64; 8 for larger mechanisms.  The occupancy of this
+// latency-bound kernel sets its throughput, so each workgroup size gets the
+// instance with the largest register budget that still lets its wavefronts
+// be resident: FP64 up to 8 cells 512 threads (256 VGPRs), 9-12 cells 768
+// (168); mixed up to 12 cells 768, 13-16 cells 1024 (128 VGPRs, 4 per SIMD,
+// a few registers spilled).
+int sparseMaxCellsPerGroup(int precision, int m) {
+  if (m > 128) return 8;
+  return precision == TCHEM_PRECISION_MIXED ? 16 : 12;
+}
 
 // kernel instance for (precision, sparse form: 0 dense, 1 staged programs,
 // 2 programs in device memory; species slots per lane; recording for the
-// adjoint, FP64 sparse forms only); nullptr when there is none
-static const void* integratorKernel(int precision, int sparse, int qm, bool rec) {
+// adjoint, FP64 sparse forms only; cells per workgroup); nullptr when there
+// is none
+static const void* integratorKernel(int precision, int sparse, int qm, bool rec, int cells) {
 #define TCHEM_PICK_D(TA) \
   return qm == 1 ? integratorFn<DenseBackend<TA, 1>, 1, 64>() \
                  : (qm == 2 ? integratorFn<DenseBackend<TA, 2>, 2, 64>() : integratorFn<DenseBackend<TA, 4>, 4, 64>())
@@ -1693,13 +1702,25 @@ static const void* integratorKernel(int precision, int sparse, int qm, bool rec)
                             : integratorFn<SparseBackend<TA, 4, ST>, 4, 512, REC>())
   if (rec) {
     if (precision != TCHEM_PRECISION_FP64 || sparse == 0) return nullptr;
+    if (cells > 8 && qm <= 2) {
+      if (sparse == 1) TCHEM_PICK_S(double, 768, true, true);
+      TCHEM_PICK_S(double, 768, false, true);
+    }
     if (sparse == 1) TCHEM_PICK_S(double, 512, true, true);
     TCHEM_PICK_S(double, 512, false, true);
   }
   if (precision == TCHEM_PRECISION_FP64) {
+    if (cells > 8 && qm <= 2) {
+      if (sparse == 1) TCHEM_PICK_S(double, 768, true, false);
+      if (sparse == 2) TCHEM_PICK_S(double, 768, false, false);
+    }
     if (sparse == 1) TCHEM_PICK_S(double, 512, true, false);
     if (sparse == 2) TCHEM_PICK_S(double, 512, false, false);
     TCHEM_PICK_D(double);
+  }
+  if (cells > 12 && qm <= 2) {
+    if (sparse == 1) TCHEM_PICK_S(float, 1024, true, false);
+    if (sparse == 2) TCHEM_PICK_S(float, 1024, false, false);
   }
   if (sparse == 1) TCHEM_PICK_S(float, 768, true, false);
   if (sparse == 2) TCHEM_PICK_S(float, 768, false, false);
@@ -1711,8 +1732,12 @@ static const void* integratorKernel(int precision, int sparse, int qm, bool rec)
 hipError_t launchIntegrator(const IntegratorArgs& a, int precision, int grid, size_t lds, hipStream_t stream) {
   const int qm = qmOf(a.M.m);
   if (qm == 0) return hipErrorInvalidValue;
-  const void* fn = integratorKernel(precision, a.sparse, qm, a.rec.h != nullptr);
+  const void* fn = integratorKernel(precision, a.sparse, qm, a.rec.h != nullptr, a.cellsPerGroup);
   if (!fn) return hipErrorInvalidValue;
+  hipFuncAttributes attr;
+  const hipError_t ea = hipFuncGetAttributes(&attr, fn);
+  if (ea != hipSuccess) return ea;
+  if (64 * a.cellsPerGroup > attr.maxThreadsPerBlock) return hipErrorInvalidConfiguration;  // instance bound
   IntegratorArgs arg = a;
   void* params[] = {&arg};
   return hipLaunchKernel(fn, dim3(grid), dim3(64 * a.cellsPerGroup), params, lds, stream);
@@ -1723,14 +1748,14 @@ size_t integratorStaticLds(int precision, int sparse, int m) {
   const int qm = qmOf(m);
   if (qm == 0) return 0;
   hipFuncAttributes attr;
-  if (hipFuncGetAttributes(&attr, integratorKernel(precision, sparse, qm, false)) != hipSuccess) return 0;
+  if (hipFuncGetAttributes(&attr, integratorKernel(precision, sparse, qm, false, 1)) != hipSuccess) return 0;
   return attr.sharedSizeBytes;
 }
 
 hipError_t integratorMaxBlocksPerCU(int precision, int sparse, int m, int threads, size_t lds, int* blocks, bool rec) {
   const int qm = qmOf(m);
   if (qm == 0) return hipErrorInvalidValue;
-  const void* fn = integratorKernel(precision, sparse, qm, rec);
+  const void* fn = integratorKernel(precision, sparse, qm, rec, threads / 64);
   if (!fn) return hipErrorInvalidValue;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, fn, threads, lds);
 }

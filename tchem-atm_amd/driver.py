@@ -51,7 +51,8 @@ class DeviceRun:
             self.tadv = tadv
 
     def run_intervals(self, precision=T.PRECISION_FP64, schedule=T.SCHEDULE_GRID, max_intervals=100000,
-                      blocks_per_cu=0, collect_stats=False, stream=None, lu=T.LU_AUTO, cells_per_block=0):
+                      blocks_per_cu=0, collect_stats=False, stream=None, lu=T.LU_AUTO, cells_per_block=0,
+                      programs=T.PROGRAMS_AUTO):
         """Reference host loop; returns the number of launches."""
         torch = self.torch
         tend = self.w.tend
@@ -62,7 +63,8 @@ class DeviceRun:
                                                   self.t, self.dt, self.state, self.kmcd, self.photo,
                                                   self.stats if collect_stats else None, precision=precision,
                                                   schedule=schedule, blocks_per_cu=blocks_per_cu, lu=lu,
-                                                  cells_per_block=cells_per_block, stream=stream)
+                                                  cells_per_block=cells_per_block, programs=programs,
+                                                  stream=stream)
             if collect_stats:
                 s = T.stats_from_tensor(self.stats)
                 arr = np.stack([s[f] for f in T.COUNTERS], 1).astype(np.int64)
@@ -102,7 +104,7 @@ class DeviceRun:
         return total
 
     def run_to_end(self, precision=T.PRECISION_MIXED, schedule=T.SCHEDULE_PERSISTENT, blocks_per_cu=0,
-                   collect_stats=False, stream=None, lu=T.LU_AUTO, cells_per_block=0):
+                   collect_stats=False, stream=None, lu=T.LU_AUTO, cells_per_block=0, programs=T.PROGRAMS_AUTO):
         """Every cell to tend in one launch (iteration cap 10^7 per cell)."""
         w = self.w
         self.tadv = T.time_advance_tensor(self.n, w.tbeg, w.tend, w.dt_in, w.dtmin, w.dtmax, w.max_newton,
@@ -112,7 +114,7 @@ class DeviceRun:
                                               self.dt, self.state, self.kmcd, self.photo,
                                               self.stats if collect_stats else None, precision=precision,
                                               schedule=schedule, blocks_per_cu=blocks_per_cu, lu=lu,
-                                              cells_per_block=cells_per_block, stream=stream)
+                                              cells_per_block=cells_per_block, programs=programs, stream=stream)
         if collect_stats:
             s = T.stats_from_tensor(self.stats)
             self.total_stats = np.stack([s[f] for f in T.COUNTERS], 1).astype(np.int64)

@@ -1,0 +1,62 @@
+"""The box-model executable (TChem_AtmosphericChemistry.x on the C ABI) run
+with the reference's CB05CL_AE5 ctest command line
+(src/unit-tests/CB05CL_AE5/CMakeLists.txt:23-35) and checked the way that
+ctest checks it: the written full_gas.dat against the reference's
+full_gas.hdf5 (kept as tests/golden/cb05cl_ae5_full_gas.npy) with the norm
+criterion of src/unit-tests/scripts/compare_tchem.py at 1e-8.  Also checks
+the other files the reference writes (kmod.echo, wall_times.json)."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, golden
+from test_oracle import compare_tchem_norms
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(ROOT, "tchem-atm_amd", "TChem_AtmosphericChemistry.x")
+
+
+def _run(tmp_path, *extra):
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} is not built (make -C tchem-atm_amd/csrc)")
+    args = [EXE, f"--chemfile={golden('cb05cl_ae5.yaml')}", "--outputfile=full_gas.dat",
+            "--outputfile_times=AtmosphericChemistry_wall_times.json", "--time-iterations-per-interval=10",
+            "--dtmin=1e-20", "--dtmax=10", "--tend=600", "--atol-newton=1e-18", "--rtol-newton=1e-8",
+            "--max-newton-iterations=20", "--max-time-iterations=2", *extra]
+    print("case:", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+@pytest.mark.parametrize("lu", ["dense", "sparse"])
+def test_cb05cl_ae5_ctest_through_the_executable(tmp_path, lu, gpu):
+    r = _run(tmp_path, f"--lu={lu}")
+    assert "Number of Species 81" in r.stdout and "Number of Reactions 187" in r.stdout
+    test = np.loadtxt(tmp_path / "full_gas.dat", skiprows=1)
+    ref = np.load(golden("cb05cl_ae5_full_gas.npy"))
+    assert test.shape == ref.shape
+    assert compare_tchem_norms(test, ref) < 1e-8
+    header = (tmp_path / "full_gas.dat").read_text().splitlines()[0].split("\t")
+    assert [h.strip() for h in header[:6]] == ["iter", "t", "dt", "Density[kg/m3]", "Pressure[Pascal]",
+                                                "Temperature[K]"]
+    assert (tmp_path / "kmod.echo").stat().st_size > 1000
+    body = json.loads((tmp_path / "AtmosphericChemistry_wall_times.json").read_text())
+    assert body["Aerosol Chemistry"]["number_of_time_iters"] == 2
+    assert body["Aerosol Chemistry"]["number_of_samples"] == 1
+
+
+def test_cloned_samples_and_mixed_precision(tmp_path, gpu):
+    _run(tmp_path, "--batch_size=64", "--use_cloned_samples=true", "--precision=mixed", "--schedule=persistent")
+    test = np.loadtxt(tmp_path / "full_gas.dat", skiprows=1)
+    ref = np.load(golden("cb05cl_ae5_full_gas.npy"))
+    assert test.shape == (3 * 64, ref.shape[1])
+    final = test[test[:, 0] == 1]
+    assert np.array_equal(final, np.repeat(final[:1], 64, axis=0))  # every clone identical
+    big = np.abs(ref[-1, 6:]) > 1e-23
+    rel = np.abs(final[0, 6:] - ref[-1, 6:])[big] / np.abs(ref[-1, 6:][big])
+    assert rel.max() < 1e-6

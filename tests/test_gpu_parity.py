@@ -357,4 +357,9 @@ def test_invalid_arguments_are_rejected(cb05, gpu, pkg):
     two = pkg.AtmosphericChemistry.getWorkSpaceSize(kmcd, pkg.PRECISION_MIXED, pkg.LU_SPARSE, 2)
     full = pkg.AtmosphericChemistry.getWorkSpaceSize(kmcd, pkg.PRECISION_MIXED, pkg.LU_SPARSE)
     cell = two - one
-    assert cell < dense / 2 and full <= 160 * 1024 and full + cell > 160 * 1024
+    cells = round((full - (one - cell)) / cell)  # one - cell: the programs' share (0 when read from memory)
+    assert cell < dense / 2 and full <= 160 * 1024 and (full + cell > 160 * 1024 or cells == 16)
+    opts = pkg._Options(lu=pkg.LU_SPARSE, programs=5)
+    assert lib.tchem_atmchem_run_device_batch(opts, 1, None, None, None, None, None, None, None, None, None, None,
+                                              kmcd._h, None) == -1
+    assert "programs" in lib.tchem_last_error().decode()
