@@ -193,6 +193,51 @@ int tchem_sourceterm_run_device_batch(int nBatch, const double* state, const dou
 int tchem_jacobian_run_device_batch(int nBatch, const double* state, const double* photo_rates, double* jac,
                                     tchem_kmcd_t kmcd, void* stream);
 
+/* ---- E3SM / UCI kinetics batch functions ----------------------------------- */
+/* The reference's E3SM kinetics (Arrhenius and Troe without the ppm
+ * conversion factor, CMAQ_H2O2, JPL and ratio-JPL falloff with [M], adjusted
+ * reactions and prod_O1D, photolysis rates given per cell).  State rows as
+ * above; species concentrations are the row's species block.  These work
+ * for every mechanism the loader reads, including those the box-model
+ * integrator rejects.
+ *
+ * Replaces ReactionRates::runDeviceBatch(policy, state, kfwd, kmcd)
+ * (src/core/TChem_ReactionRates.cpp:118-134; Impl: TChem_Impl_ReactionRates.hpp).
+ *   kfwd [device] [nBatch][nReac] */
+int tchem_reactionrates_run_device_batch(int nBatch, const double* state, double* kfwd, tchem_kmcd_t kmcd,
+                                         void* stream);
+
+/* Replaces RateofProgress::runDeviceBatch(policy, state, photo_rates, rate_of_progress, kmcd)
+ * (src/core/TChem_RateofProgress.cpp; Impl: TChem_Impl_RateofProgress.hpp:52-102).
+ *   photo_rates [device] [nBatch][nPhotoRates] or NULL: rate constants of
+ *               reactions 0..nPhotoRates-1 (the PHOTO_RATE reactions come first)
+ *   rop [device] [nBatch][nReac] */
+int tchem_rateofprogress_run_device_batch(int nBatch, const double* state, const double* photo_rates,
+                                          int nPhotoRates, double* rop, tchem_kmcd_t kmcd, void* stream);
+
+/* Replaces NetProductionRates::runDeviceBatch(policy, state, photo_rates, external_sources,
+ * net_production_rates, kmcd) (src/core/TChem_NetProductionRates.hpp:61-78;
+ * Impl: TChem_Impl_NetProductionRates.hpp:52-112).
+ *   external_sources [device] [nBatch][m] or NULL
+ *   omega [device] [nBatch][ldOmega], ldOmega >= m: active species' net
+ *         production rates, the remaining ldOmega - m entries of a row set to 0
+ *         (the reference's callers pass [nBatch][nSpec]) */
+int tchem_netproductionrates_run_device_batch(int nBatch, const double* state, const double* photo_rates,
+                                              int nPhotoRates, const double* external_sources, double* omega,
+                                              int ldOmega, tchem_kmcd_t kmcd, void* stream);
+
+/* Replaces AtmChemistry::setScenarioConditionsPhotolysisReactions
+ * (src/core/TChem_AtmosphericChemistry.cpp:511-560): the scenario file's
+ * photolysis_rates block as host rows [nBatch][capacity_cols] (count columns
+ * used; rates_host NULL queries count; count 0 without the block). */
+int tchem_scenario_photolysis_rates(tchem_kmd_t kmd, const char* path, int nBatch, double* rates_host,
+                                    int capacity_cols, int* count);
+
+/* Replaces AtmChemistry::setScenarioConditionsExternalForcing
+ * (src/core/TChem_AtmosphericChemistry.cpp:562-631): host rows [nBatch][m],
+ * zero for species without forcing; count = forced species (nullable). */
+int tchem_scenario_external_forcing(tchem_kmd_t kmd, const char* path, int nBatch, double* forcing_host, int* count);
+
 /* ---- discrete adjoint ----------------------------------------------------- */
 /* No counterpart in the reference, which has no derivative path: gradients
  * of a scalar loss on the final concentrations of a recorded TR-BDF2 run

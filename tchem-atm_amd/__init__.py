@@ -10,6 +10,10 @@ Python classes named after the reference's C++ types, calling the C ABI of
   AtmosphericChemistry.getWorkSpaceSize(kmcd)  src/core/TChem_AtmosphericChemistry.hpp:47-81
   SourceTerm.runDeviceBatch(...)               src/examples/TChem_GasChemistry_RHSs.cpp:184-236
   Jacobian.runDeviceBatch(...)                 src/examples/TChem_GasChemistry_RHSs.cpp:254-309
+  ReactionRates / RateofProgress / NetProductionRates.runDeviceBatch
+                                               src/core/TChem_{ReactionRates,RateofProgress,NetProductionRates}.hpp
+  setScenarioConditionsPhotolysisReactions / ExternalForcing
+                                               src/core/TChem_AtmosphericChemistry.cpp:511-631
 
 Device arrays are torch tensors on a HIP device (float64, contiguous); torch
 provides device memory and streams only.  Every compute call goes to the HIP
@@ -62,6 +66,8 @@ EXPORTED_SYMBOLS = (
     "tchem_sourceterm_run_device_batch", "tchem_jacobian_run_device_batch", "tchem_adjoint_workspace_bytes",
     "tchem_adjoint_forward", "tchem_adjoint_backward", "tchem_adjoint_steps", "tchem_atmchem_graph_create",
     "tchem_graph_launch", "tchem_graph_status", "tchem_graph_destroy", "tchem_atmchem_autotune",
+    "tchem_reactionrates_run_device_batch", "tchem_rateofprogress_run_device_batch",
+    "tchem_netproductionrates_run_device_batch", "tchem_scenario_photolysis_rates", "tchem_scenario_external_forcing",
 )
 
 
@@ -107,6 +113,11 @@ def _load():
         "tchem_graph_destroy": [vp],
         "tchem_atmchem_autotune": [ctypes.POINTER(_Options), i, vp, vp, vp, vp, vp, i, i,
                                    ctypes.POINTER(ctypes.c_double), vp, vp],
+        "tchem_reactionrates_run_device_batch": [i, vp, vp, vp, vp],
+        "tchem_rateofprogress_run_device_batch": [i, vp, vp, i, vp, vp, vp],
+        "tchem_netproductionrates_run_device_batch": [i, vp, vp, i, vp, vp, i, vp, vp],
+        "tchem_scenario_photolysis_rates": [vp, cp, i, vp, i, ctypes.POINTER(i)],
+        "tchem_scenario_external_forcing": [vp, cp, i, vp, ctypes.POINTER(i)],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -314,6 +325,61 @@ class SourceTerm:
             state.shape[0], _ptr(state, what="state"), _ptr(photo_rates, what="photo_rates"),
             _ptr(external_sources, what="external_sources"), _ptr(omega, what="omega"), kmcd._h,
             _stream_ptr(stream)))
+
+
+class ReactionRates:
+    """E3SM kinetics rate constants (ReactionRates::runDeviceBatch)."""
+
+    @staticmethod
+    def runDeviceBatch(state, kfwd, kmcd, stream=None):
+        _check(_lib.tchem_reactionrates_run_device_batch(state.shape[0], _ptr(state, what="state"),
+                                                         _ptr(kfwd, what="kfwd"), kmcd._h, _stream_ptr(stream)))
+
+
+class RateofProgress:
+    """E3SM kinetics rates of progress (RateofProgress::runDeviceBatch)."""
+
+    @staticmethod
+    def runDeviceBatch(state, photo_rates, rate_of_progress, kmcd, stream=None):
+        n_photo = 0 if photo_rates is None else photo_rates.shape[1]
+        _check(_lib.tchem_rateofprogress_run_device_batch(
+            state.shape[0], _ptr(state, what="state"), _ptr(photo_rates, what="photo_rates"), n_photo,
+            _ptr(rate_of_progress, what="rate_of_progress"), kmcd._h, _stream_ptr(stream)))
+
+
+class NetProductionRates:
+    """E3SM kinetics net production rates (NetProductionRates::runDeviceBatch);
+    net_production_rates is [nBatch][ld] with ld >= active species (the
+    reference's callers use nSpec; entries past the active species are 0)."""
+
+    @staticmethod
+    def runDeviceBatch(state, photo_rates, external_sources, net_production_rates, kmcd, stream=None):
+        n_photo = 0 if photo_rates is None else photo_rates.shape[1]
+        _check(_lib.tchem_netproductionrates_run_device_batch(
+            state.shape[0], _ptr(state, what="state"), _ptr(photo_rates, what="photo_rates"), n_photo,
+            _ptr(external_sources, what="external_sources"),
+            _ptr(net_production_rates, what="net_production_rates"), net_production_rates.shape[1], kmcd._h,
+            _stream_ptr(stream)))
+
+
+def setScenarioConditionsPhotolysisReactions(filename: str, kmd: KineticModelData, nBatch: int) -> np.ndarray:
+    """Host [nBatch][count] photolysis rates of a scenario file (count 0 without the block)."""
+    n = ctypes.c_int()
+    _check(_lib.tchem_scenario_photolysis_rates(kmd._h, filename.encode(), nBatch, None, 0, ctypes.byref(n)))
+    out = np.zeros((nBatch, n.value))
+    if n.value:
+        _check(_lib.tchem_scenario_photolysis_rates(kmd._h, filename.encode(), nBatch, out.ctypes.data, n.value,
+                                                    ctypes.byref(n)))
+    return out
+
+
+def setScenarioConditionsExternalForcing(filename: str, kmd: KineticModelData, nBatch: int) -> np.ndarray:
+    """Host [nBatch][active species] external forcing of a scenario file."""
+    out = np.zeros((nBatch, kmd.nActive))
+    n = ctypes.c_int()
+    _check(_lib.tchem_scenario_external_forcing(kmd._h, filename.encode(), nBatch, out.ctypes.data,
+                                                ctypes.byref(n)))
+    return out
 
 
 class Jacobian:

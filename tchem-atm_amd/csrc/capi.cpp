@@ -31,6 +31,8 @@ hipError_t launchLoopControl(int n, const double* t_out, const tchem_time_advanc
 size_t adjointCellBytes(const DevMech& M, const DevSparse& S);
 hipError_t adjointMaxBlocksPerCU(int m, int threads, size_t lds, int* blocks);
 hipError_t launchAdjoint(const AdjointArgs& a, int grid, size_t lds, hipStream_t stream);
+hipError_t launchE3smKinetics(const DevE3sm& E, int mode, int nBatch, const double* state, const double* photo,
+                              int nPhoto, const double* ext, double* out, int ldOut, hipStream_t stream);
 hipError_t launchAdjointSetup(int nBatch, int maxSteps, const tchem_time_advance_t* tadv, const double* tol_newton,
                               int* header, double* tol, int* newton, hipStream_t stream);
 hipError_t launchUpdateTadv(int n, tchem_time_advance_t* tadv, const double* t_out, const double* dt_out, int* ctl,
@@ -49,6 +51,9 @@ struct tchem_kmd_s {
 
 struct tchem_kmcd_s {
   MechTables tables;
+  bool boxModel = true;     // the box-model integrator (and source term / Jacobian / adjoint) is available
+  std::string boxError;     // why not
+  DevE3sm e3{};             // E3SM kinetics tables (every mechanism the loader reads)
   SparseSchedule sched, schedT;  // forward and transposed (adjoint) programs
   DevMech dev{};
   DevSparse sparse{}, sparseT{};
@@ -280,6 +285,7 @@ AdjointLayout adjointLayout(const DevMech& M, int nBatch, int maxSteps) {
 
 int adjointCheck(const tchem_kmcd_s* k, int nBatch, int maxSteps, const char* who) {
   if (!k) return setError(TCHEM_ERR_INVALID, std::string(who) + ": null kmcd");
+  if (!k->boxModel) return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": " + k->boxError);
   if (nBatch < 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": negative nBatch");
   if (maxSteps < 1 || maxSteps > (1 << 24))
     return setError(TCHEM_ERR_INVALID, std::string(who) + ": maxSteps must be in [1, 2^24]");
@@ -369,10 +375,17 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
   try {
     k->tables = compileMechanism(kmd->km);
   } catch (const std::exception& e) {
-    return setError(TCHEM_ERR_UNSUPPORTED, e.what());
+    // e.g. JPL / PHOTO_RATE reactions: only the E3SM kinetics entry points work
+    k->boxModel = false;
+    k->boxError = e.what();
+    k->tables = MechTables{};
+  }
+  if (k->boxModel && k->tables.m > 256) {
+    k->boxModel = false;
+    k->boxError = "more than 256 active species";
+    k->tables = MechTables{};
   }
   const MechTables& t = k->tables;
-  if (t.m > 256) return setError(TCHEM_ERR_UNSUPPORTED, "more than 256 active species");
   hipError_t e = hipGetDevice(&k->device);
   if (e != hipSuccess) return hipFail(e, "hipGetDevice");
   hipDeviceProp_t prop;
@@ -461,6 +474,7 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
     sp.fwdBody = sc.fwdBody;
   };
   try {
+    if (!k->boxModel) throw std::runtime_error("no box model");
     k->sched = buildSparseSchedule(t);
     pack(k->sched, k->sparse);
   } catch (const std::exception&) {
@@ -477,10 +491,76 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
       k->sparseT = DevSparse{};
     }
   }
+  {
+    // E3SM kinetics tables straight from the loader's data
+    const KineticModel& km = kmd->km;
+    DevE3sm& E = k->e3;
+    E.nSpec = km.nSpec;
+    E.nConst = km.nConstSpec;
+    E.nReac = km.nReac;
+    E.maxSpec = km.maxSpecInReac;
+    E.M_index = km.M_index;
+    E.convPPM = km.CONV_PPM;
+    std::vector<int> ai, ti, ci, ji, ri, adj, pI;
+    std::vector<double> ap, tp, cp, jp, rp, pD;
+    for (const auto& a : km.arrhenius) {
+      ai.push_back(a.reaction);
+      ap.insert(ap.end(), {a.A, a.B, a.C, a.D, a.E});
+    }
+    for (size_t i = 0; i < km.reacPfal.size(); ++i) {
+      ti.push_back(km.reacPfal[i]);
+      tp.insert(tp.end(), km.reacPpar[i].begin(), km.reacPpar[i].end());
+    }
+    for (const auto& c : km.cmaqH2O2) {
+      ci.push_back(c.reaction);
+      cp.insert(cp.end(), {c.A1, c.B1, c.C1, c.A2, c.B2, c.C2});
+    }
+    for (const auto& j : km.jpl) {
+      ji.push_back(j.reaction);
+      jp.insert(jp.end(), {j.k0_A, j.k0_B, j.k0_C, j.kinf_A, j.kinf_B, j.kinf_C, j.Fc});
+    }
+    for (const auto& j : km.rJplArrhenius) {
+      ri.push_back(j.reaction);
+      rp.insert(rp.end(), {j.k0_A, j.k0_B, j.k0_C, j.kinf_A, j.kinf_B, j.kinf_C, j.Fc, j.A, j.B, j.C});
+    }
+    for (const auto& a : km.adjustReaction) adj.insert(adj.end(), {a.reaction, a.species});
+    for (const auto& o : km.prodO1D) {
+      pI.insert(pI.end(), {o.reactions[0], o.reactions[1], o.reactions[2], o.photolysis_reaction, o.species1,
+                           o.species2, o.species3});
+      pD.insert(pD.end(), {o.A1, o.C1, o.A2, o.C2, o.A3, o.C3});
+    }
+    E.nArr = int(ai.size());
+    E.nTroe = int(ti.size());
+    E.nCmaq = int(ci.size());
+    E.nJpl = int(ji.size());
+    E.nRJpl = int(ri.size());
+    E.nAdjust = int(km.adjustReaction.size());
+    E.nProd = int(km.prodO1D.size());
+    E.oArrIdx = addI(ai);
+    E.oArrPar = addD(ap);
+    E.oTroeIdx = addI(ti);
+    E.oTroePar = addD(tp);
+    E.oCmaqIdx = addI(ci);
+    E.oCmaqPar = addD(cp);
+    E.oJplIdx = addI(ji);
+    E.oJplPar = addD(jp);
+    E.oRJplIdx = addI(ri);
+    E.oRJplPar = addD(rp);
+    E.oAdjust = addI(adj);
+    E.oProdI = addI(pI);
+    E.oProdD = addD(pD);
+    E.oConvExp = addI(km.convExponent);
+    E.oNreac = addI(km.reacNreac);
+    E.oNprod = addI(km.reacNprod);
+    E.oSidx = addI(km.reacSidx);
+    E.oNuki = addD(km.reacNuki);
+  }
   ib.push_back(0);
   db.push_back(0.0);
   int rc = TCHEM_OK;
   if ((rc = upload(k.get(), ib, &d.ib)) || (rc = upload(k.get(), db, &d.db))) return rc;
+  k->e3.ib = d.ib;
+  k->e3.db = d.db;
   void* q = nullptr;
   e = hipMalloc(&q, 256);
   if (e != hipSuccess) return hipFail(e, "hipMalloc");
@@ -496,6 +576,8 @@ int tchem_kmcd_destroy(tchem_kmcd_t kmcd) {
 }
 
 int tchem_atmchem_workspace_bytes(tchem_kmcd_t kmcd, const tchem_atmchem_options_t* opts, size_t* bytes) {
+  if (kmcd && !kmcd->boxModel)
+    return setError(TCHEM_ERR_UNSUPPORTED, std::string("tchem_atmchem_workspace_bytes: ") + kmcd->boxError);
   if (!kmcd || !bytes) return setError(TCHEM_ERR_INVALID, "tchem_atmchem_workspace_bytes: null argument");
   tchem_atmchem_options_t o;
   const int rc = resolveOptions(opts, kmcd, &o, "tchem_atmchem_workspace_bytes");
@@ -508,6 +590,8 @@ int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBat
                                    const double* tol_time, double* fac, const tchem_time_advance_t* tadv,
                                    double* state, const double* photo_rates, double* t_out, double* dt_out,
                                    double* state_out, tchem_cell_stats_t* stats, tchem_kmcd_t kmcd, void* stream) {
+  if (kmcd && !kmcd->boxModel)
+    return setError(TCHEM_ERR_UNSUPPORTED, std::string("run_device_batch: ") + kmcd->boxError);
   (void)fac;
   if (!kmcd) return setError(TCHEM_ERR_INVALID, "run_device_batch: null kmcd");
   if (nBatch < 0) return setError(TCHEM_ERR_INVALID, "run_device_batch: negative nBatch");
@@ -550,6 +634,8 @@ int tchem_atmchem_update_time_advance(int nBatch, tchem_time_advance_t* tadv, co
 int tchem_sourceterm_run_device_batch(int nBatch, const double* state, const double* photo_rates,
                                       const double* external_sources, double* omega, tchem_kmcd_t kmcd,
                                       void* stream) {
+  if (kmcd && !kmcd->boxModel)
+    return setError(TCHEM_ERR_UNSUPPORTED, std::string("sourceterm: ") + kmcd->boxError);
   if (!kmcd || nBatch < 0) return setError(TCHEM_ERR_INVALID, "sourceterm: bad argument");
   if (nBatch == 0) return TCHEM_OK;
   if (!state || !omega) return setError(TCHEM_ERR_INVALID, "sourceterm: null device array");
@@ -562,6 +648,8 @@ int tchem_sourceterm_run_device_batch(int nBatch, const double* state, const dou
 
 int tchem_jacobian_run_device_batch(int nBatch, const double* state, const double* photo_rates, double* jac,
                                     tchem_kmcd_t kmcd, void* stream) {
+  if (kmcd && !kmcd->boxModel)
+    return setError(TCHEM_ERR_UNSUPPORTED, std::string("jacobian: ") + kmcd->boxError);
   if (!kmcd || nBatch < 0) return setError(TCHEM_ERR_INVALID, "jacobian: bad argument");
   if (nBatch == 0) return TCHEM_OK;
   if (!state || !jac) return setError(TCHEM_ERR_INVALID, "jacobian: null device array");
@@ -715,6 +803,8 @@ int tchem_atmchem_graph_create(const tchem_atmchem_options_t* opts, int nBatch, 
                                const double* photo_rates, double* t_out, double* dt_out, double* state_out,
                                tchem_cell_stats_t* stats, int nIntervals, double stopFraction, tchem_kmcd_t kmcd,
                                tchem_graph_t* out) {
+  if (kmcd && !kmcd->boxModel)
+    return setError(TCHEM_ERR_UNSUPPORTED, std::string("tchem_atmchem_graph_create: ") + kmcd->boxError);
   const char* who = "tchem_atmchem_graph_create";
   if (!kmcd || !out) return setError(TCHEM_ERR_INVALID, std::string(who) + ": null argument");
   *out = nullptr;
@@ -827,6 +917,8 @@ int tchem_atmchem_autotune(tchem_atmchem_options_t* opts, int nBatch, const doub
                            const double* tol_time, const tchem_time_advance_t* tadv, const double* state,
                            const double* photo_rates, int sampleCells, int sampleIterations, double* best_seconds,
                            tchem_kmcd_t kmcd, void* stream) {
+  if (kmcd && !kmcd->boxModel)
+    return setError(TCHEM_ERR_UNSUPPORTED, std::string("tchem_atmchem_autotune: ") + kmcd->boxError);
   const char* who = "tchem_atmchem_autotune";
   if (!kmcd || !opts) return setError(TCHEM_ERR_INVALID, std::string(who) + ": null argument");
   if (nBatch < 1 || sampleCells < 0 || sampleIterations < 0)
@@ -976,6 +1068,82 @@ int tchem_atmchem_autotune(tchem_atmchem_options_t* opts, int nBatch, const doub
   *opts = bestO;
   if (best_seconds) *best_seconds = double(best) * 1e-3;
   return TCHEM_OK;
+}
+
+int tchem_scenario_photolysis_rates(tchem_kmd_t kmd, const char* path, int nBatch, double* rates_host,
+                                    int capacity_cols, int* count) {
+  if (!kmd || !path || !count || nBatch < 1)
+    return setError(TCHEM_ERR_INVALID, "tchem_scenario_photolysis_rates: bad argument");
+  try {
+    int n = 0;
+    const std::vector<double> r = loadScenarioPhotolysisRates(path, nBatch, &n);
+    *count = n;
+    if (rates_host) {
+      if (capacity_cols < n)
+        return setError(TCHEM_ERR_INVALID, "tchem_scenario_photolysis_rates: capacity below " + std::to_string(n));
+      for (int i = 0; i < nBatch; ++i)
+        for (int j = 0; j < n; ++j) rates_host[size_t(i) * capacity_cols + j] = r[size_t(i) * n + j];
+    }
+  } catch (const std::exception& e) {
+    return setError(TCHEM_ERR_PARSE, e.what());
+  }
+  return TCHEM_OK;
+}
+
+int tchem_scenario_external_forcing(tchem_kmd_t kmd, const char* path, int nBatch, double* forcing_host, int* count) {
+  if (!kmd || !path || !forcing_host || nBatch < 1)
+    return setError(TCHEM_ERR_INVALID, "tchem_scenario_external_forcing: bad argument");
+  try {
+    int n = 0;
+    const std::vector<double> f = loadScenarioExternalForcing(path, kmd->km, nBatch, &n);
+    std::copy(f.begin(), f.end(), forcing_host);
+    if (count) *count = n;
+  } catch (const std::exception& e) {
+    return setError(TCHEM_ERR_PARSE, e.what());
+  }
+  return TCHEM_OK;
+}
+
+namespace {
+int runE3sm(const char* who, int mode, int nBatch, const double* state, const double* photo, int nPhoto,
+            const double* ext, double* out, int ldOut, tchem_kmcd_t kmcd, void* stream) {
+  if (!kmcd || nBatch < 0 || nPhoto < 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": bad argument");
+  if (nBatch == 0) return TCHEM_OK;
+  if (!state || !out || (nPhoto > 0 && !photo))
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": null device array");
+  const DevE3sm& E = kmcd->e3;
+  if (nPhoto > E.nReac) return setError(TCHEM_ERR_INVALID, std::string(who) + ": more photolysis rates than reactions");
+  const size_t lds = sizeof(double) * (size_t(E.nSpec) + 2 * size_t(E.nReac) + size_t(E.nSpec - E.nConst));
+  if (lds > kmcd->maxLds) return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": mechanism too large for LDS");
+  int cur = 0;
+  if (hipGetDevice(&cur) != hipSuccess || cur != kmcd->device)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": kmcd belongs to another device");
+  hipError_t e = launchE3smKinetics(E, mode, nBatch, state, nPhoto ? photo : nullptr, nPhoto, ext, out, ldOut,
+                                    static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hipFail(e, "e3sm_kinetics_kernel launch");
+  return TCHEM_OK;
+}
+}  // namespace
+
+int tchem_reactionrates_run_device_batch(int nBatch, const double* state, double* kfwd, tchem_kmcd_t kmcd,
+                                         void* stream) {
+  return runE3sm("reactionrates", 0, nBatch, state, nullptr, 0, nullptr, kfwd, kmcd ? kmcd->e3.nReac : 0, kmcd,
+                 stream);
+}
+
+int tchem_rateofprogress_run_device_batch(int nBatch, const double* state, const double* photo_rates,
+                                          int nPhotoRates, double* rop, tchem_kmcd_t kmcd, void* stream) {
+  return runE3sm("rateofprogress", 1, nBatch, state, photo_rates, nPhotoRates, nullptr, rop,
+                 kmcd ? kmcd->e3.nReac : 0, kmcd, stream);
+}
+
+int tchem_netproductionrates_run_device_batch(int nBatch, const double* state, const double* photo_rates,
+                                              int nPhotoRates, const double* external_sources, double* omega,
+                                              int ldOmega, tchem_kmcd_t kmcd, void* stream) {
+  if (kmcd && ldOmega < kmcd->e3.nSpec - kmcd->e3.nConst)
+    return setError(TCHEM_ERR_INVALID, "netproductionrates: ldOmega below the number of active species");
+  return runE3sm("netproductionrates", 2, nBatch, state, photo_rates, nPhotoRates, external_sources, omega, ldOmega,
+                 kmcd, stream);
 }
 
 }  // extern "C"

@@ -139,6 +139,26 @@ __host__ __device__ inline size_t sparseCellBytes(const DevMech& M, const DevSpa
          al(sizeof(double) * (size_t(M.nSpec) + size_t(M.nReac + 1) + size_t(M.nReac + 1) + size_t(M.nSeg + 1)));
 }
 
+// Tables of the E3SM/UCI kinetics (ReactionRates / RateofProgress /
+// NetProductionRates of the reference): every rate-constant form the loader
+// reads, in the reference's parameter layout, indexed into the kmcd's int
+// and double buffers.
+struct DevE3sm {
+  const int* ib;
+  const double* db;
+  int nSpec, nConst, nReac, maxSpec, M_index;
+  double convPPM;
+  int nArr, nTroe, nCmaq, nJpl, nRJpl, nAdjust, nProd;
+  int oArrIdx, oArrPar;      // [nArr], [nArr][5] A B C D E
+  int oTroeIdx, oTroePar;    // [nTroe], [nTroe][8] k0_A(x1e6) k0_B k0_C kinf_A kinf_B kinf_C Fc N
+  int oCmaqIdx, oCmaqPar;    // [nCmaq], [nCmaq][6] A1 B1 C1 A2(x1e6) B2 C2
+  int oJplIdx, oJplPar;      // [nJpl], [nJpl][7] k0_A k0_B k0_C kinf_A kinf_B kinf_C Fc
+  int oRJplIdx, oRJplPar;    // [nRJpl], [nRJpl][10] + A B C
+  int oAdjust;               // [nAdjust][2] reaction, species
+  int oProdI, oProdD;        // [nProd][7] reactions[3], photolysis reaction, species 1-3; [nProd][6] A1 C1 A2 C2 A3 C3
+  int oConvExp, oNreac, oNprod, oSidx, oNuki;  // per reaction; [nReac][maxSpec]
+};
+
 // Arguments of the discrete adjoint kernel (backward sweep over a record).
 struct AdjointArgs {
   IntegratorArgs fw;        // M, S (forward programs), nBatch, kf, rec, queue

@@ -1295,6 +1295,124 @@ __global__ __launch_bounds__(256) void rates_kernel(DevMech M, int nBatch, const
   kf[idx] = lscale ? k * exp(lscale[size_t(cell) * lstride + r]) : k;
 }
 
+// ---- E3SM / UCI kinetics batch functions --------------------------------------
+// ReactionRates / RateofProgress / NetProductionRates of the reference's
+// E3SM kinetics (src/core/impl/TChem_Impl_{KForward,KForwardJPL,AdjustReactions,
+// RateofProgress,NetProductionRates}.hpp), one wavefront per cell:
+//   mode 0: kf   (ReactionRates)
+//   mode 1: rop  (RateofProgress; photolysis rates replace kf of the first nPhoto reactions)
+//   mode 2: omega of the active species (NetProductionRates: external sources + nu . rop)
+// Species concentrations are the state row's species block (active then constant).
+
+__device__ __forceinline__ double t300_pow(double T, double B) { return B == 0.0 ? 1.0 : pow(T / 300.0, B); }
+__device__ __forceinline__ double exp_or_one(double C, double t_1) { return C == 0.0 ? 1.0 : exp(C * t_1); }
+
+__global__ __launch_bounds__(64) void e3sm_kinetics_kernel(DevE3sm E, int mode, int nBatch, const double* state,
+                                                         const double* photo, int nPhoto, const double* ext,
+                                                         double* out, int ldOut) {
+  extern __shared__ __align__(16) char lds_raw[];
+  const int cell = blockIdx.x, lane = threadIdx.x;
+  if (cell >= nBatch) return;
+  double* x = reinterpret_cast<double*>(lds_raw);  // [nSpec]
+  double* kf = x + E.nSpec;                         // [nReac]
+  double* rop = kf + E.nReac;                       // [nReac]
+  double* om = rop + E.nReac;                       // [m]
+  const int m = E.nSpec - E.nConst;
+  const double* row = state + size_t(cell) * (E.nSpec + 3);
+  const double p = row[1], T = row[2];
+  const double t_1 = 1.0 / T;
+  const double conv = E.convPPM * p * t_1;
+  for (int i = lane; i < E.nSpec; i += kWave) x[i] = row[3 + i];
+  for (int r = lane; r < E.nReac; r += kWave)
+    kf[r] = (mode >= 1 && photo && r < nPhoto) ? photo[size_t(cell) * nPhoto + r] : 0.0;
+  wave_sync();
+  const int* ib = E.ib;
+  const double* db = E.db;
+  // KForward (TChem_Impl_KForward.hpp:52-141): Arrhenius and Troe without the
+  // conv factor, CMAQ_H2O2 with it
+  for (int i = lane; i < E.nArr; i += kWave) {
+    const double* P = db + E.oArrPar + 5 * i;
+    kf[ib[E.oArrIdx + i]] = P[0] * exp(P[2] * t_1) * (P[1] == 0.0 ? 1.0 : pow(T / P[3], P[1])) *
+                            (P[4] == 0.0 ? 1.0 : (1.0 + P[4] * p));
+  }
+  for (int i = lane; i < E.nTroe; i += kWave) {
+    const double* P = db + E.oTroePar + 8 * i;
+    const double k0 = P[0] * exp_or_one(P[2], t_1) * t300_pow(T, P[1]);
+    const double k0_kinf = k0 / (P[3] * exp_or_one(P[5], t_1) * t300_pow(T, P[4]));
+    kf[ib[E.oTroeIdx + i]] = k0 / (1.0 + k0_kinf) * pow(P[6], 1.0 / (1.0 + pow(log10(k0_kinf) / P[7], 2.0)));
+  }
+  for (int i = lane; i < E.nCmaq; i += kWave) {
+    const double* P = db + E.oCmaqPar + 6 * i;
+    const int r = ib[E.oCmaqIdx + i];
+    kf[r] = (P[0] * exp(P[2] * t_1) * t300_pow(T, P[1]) + P[3] * exp(P[5] * t_1) * t300_pow(T, P[4]) * conv) *
+            pow(conv, double(ib[E.oConvExp + r]));
+  }
+  // KForwardJPL (TChem_Impl_KForwardJPL.hpp:44-143), m = [M]
+  const double mM = E.M_index >= 0 ? x[E.M_index] : 0.0;
+  for (int i = lane; i < E.nJpl + E.nRJpl; i += kWave) {
+    const bool rj = i >= E.nJpl;
+    const int j = rj ? i - E.nJpl : i;
+    const double* P = db + (rj ? E.oRJplPar + 10 * j : E.oJplPar + 7 * j);
+    const double k0 = P[0] * exp_or_one(P[2], t_1) * t300_pow(T, P[1]);
+    const double kinf = P[3] * exp_or_one(P[5], t_1) * t300_pow(T, P[4]);
+    const double xpo = k0 * mM / kinf;
+    const double lx = log10(xpo);
+    const double factor = P[6] == 1.0 ? 1.0 : pow(P[6], 1.0 / (1.0 + lx * lx));
+    double k = k0 / (1.0 + xpo) * factor;
+    if (rj) k /= P[7] * exp_or_one(P[9], t_1) * t300_pow(T, P[8]);
+    kf[ib[(rj ? E.oRJplIdx : E.oJplIdx) + j]] = k;
+  }
+  wave_sync();
+  // AdjustReactions (TChem_Impl_AdjustReactions.hpp:52-96), in list order
+  if (lane == 0) {
+    for (int i = 0; i < E.nAdjust; ++i) kf[ib[E.oAdjust + 2 * i]] *= x[ib[E.oAdjust + 2 * i + 1]];
+    for (int i = 0; i < E.nProd; ++i) {
+      const int* I = ib + E.oProdI + 7 * i;
+      const double* P = db + E.oProdD + 6 * i;
+      const double fc = P[0] * exp(P[1] * t_1) * x[I[4]] + P[2] * exp(P[3] * t_1) * x[I[5]] +
+                        P[4] * exp(P[5] * t_1) * x[I[6]];
+      for (int k = 0; k < 3; ++k) kf[I[k]] *= kf[I[3]] / fc;
+    }
+  }
+  wave_sync();
+  double* o = out + size_t(cell) * ldOut;
+  if (mode == 0) {
+    for (int r = lane; r < E.nReac; r += kWave) o[r] = kf[r];
+    return;
+  }
+  // RateofProgress (TChem_Impl_RateofProgress.hpp:73-87)
+  for (int r = lane; r < E.nReac; r += kWave) {
+    double v = kf[r];
+    for (int j = 0; j < ib[E.oNreac + r]; ++j)
+      v *= pow(x[ib[E.oSidx + r * E.maxSpec + j]], double(abs(int(db[E.oNuki + r * E.maxSpec + j]))));
+    rop[r] = v;
+  }
+  wave_sync();
+  if (mode == 1) {
+    for (int r = lane; r < E.nReac; r += kWave) o[r] = rop[r];
+    return;
+  }
+  // NetProductionRates (TChem_Impl_NetProductionRates.hpp:60-104): external
+  // sources, then nu . rop over active species in reaction order
+  for (int a = lane; a < m; a += kWave) om[a] = ext ? ext[size_t(cell) * m + a] : 0.0;
+  wave_sync();
+  if (lane == 0) {
+    const int joff = E.maxSpec / 2;
+    for (int r = 0; r < E.nReac; ++r) {
+      for (int j = 0; j < ib[E.oNreac + r]; ++j) {
+        const int k = ib[E.oSidx + r * E.maxSpec + j];
+        if (k < m) om[k] += db[E.oNuki + r * E.maxSpec + j] * rop[r];
+      }
+      for (int j = 0; j < ib[E.oNprod + r]; ++j) {
+        const int k = ib[E.oSidx + r * E.maxSpec + j + joff];
+        if (k < m) om[k] += db[E.oNuki + r * E.maxSpec + j + joff] * rop[r];
+      }
+    }
+  }
+  wave_sync();
+  for (int a = lane; a < ldOut; a += kWave) o[a] = a < m ? om[a] : 0.0;
+}
+
 // ---- discrete adjoint --------------------------------------------------------
 // Reverse-mode derivative of the recorded TR-BDF2 trajectory (accepted steps
 // only, step sizes held fixed) with respect to the initial active species
@@ -1798,6 +1916,15 @@ hipError_t launchAdjointSetup(int nBatch, int maxSteps, const tchem_time_advance
   const int n = nBatch > 0 ? nBatch : 1;
   hipLaunchKernelGGL(adjoint_setup_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, nBatch, maxSteps, tadv,
                      tol_newton, header, tol, newton);
+  return hipGetLastError();
+}
+
+hipError_t launchE3smKinetics(const DevE3sm& E, int mode, int nBatch, const double* state, const double* photo,
+                              int nPhoto, const double* ext, double* out, int ldOut, hipStream_t stream) {
+  if (nBatch <= 0) return hipSuccess;
+  const size_t lds = sizeof(double) * (size_t(E.nSpec) + 2 * size_t(E.nReac) + size_t(E.nSpec - E.nConst));
+  hipLaunchKernelGGL(e3sm_kinetics_kernel, dim3(nBatch), dim3(64), lds, stream, E, mode, nBatch, state, photo, nPhoto,
+                     ext, out, ldOut);
   return hipGetLastError();
 }
 

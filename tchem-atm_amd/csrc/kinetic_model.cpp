@@ -379,4 +379,47 @@ std::vector<double> loadScenarioState(const std::string& path, const KineticMode
   return state;
 }
 
+std::vector<double> loadScenarioPhotolysisRates(const std::string& path, int nBatch, int* countOut) {
+  const Node root = yaml::parseFile(path);
+  const Node* pr = root.find("photolysis_rates");
+  int count = pr ? int(pr->size()) : 0;
+  std::vector<double> out(size_t(nBatch) * size_t(count), 0.0);
+  for (int j = 0; j < count; ++j) {
+    const Node& vals = (*pr)["r_" + std::to_string(j)]["value"];
+    if (int(vals.size()) != nBatch)
+      throw std::runtime_error("scenario: number of values of photolysis rate r_" + std::to_string(j) +
+                               " differs from the number of conditions");
+    for (int i = 0; i < nBatch; ++i) out[size_t(i) * count + j] = vals.items[i].asDouble();
+  }
+  *countOut = count;
+  return out;
+}
+
+std::vector<double> loadScenarioExternalForcing(const std::string& path, const KineticModel& km, int nBatch,
+                                                int* countOut) {
+  const Node root = yaml::parseFile(path);
+  const int m = km.nActive();
+  std::vector<double> out(size_t(nBatch) * size_t(m), 0.0);
+  int count = 0;
+  if (const Node* ef = root.find("external_forcing")) {
+    for (const auto& e : ef->entries) {
+      const std::string name = e.second["species_name"].str();
+      int idx = -1;
+      for (int i = 0; i < m; ++i)  // active species are listed first
+        if (std::strncmp(km.speciesNames[i].c_str(), name.c_str(), kLengthOfSpecName) == 0) {
+          idx = i;
+          break;
+        }
+      if (idx < 0) throw std::runtime_error("scenario: species " + name + " does not exist in the species list");
+      const Node& vals = e.second["value"];
+      if (int(vals.size()) != nBatch)
+        throw std::runtime_error("scenario: number of values of external forcing differs from the number of conditions");
+      for (int i = 0; i < nBatch; ++i) out[size_t(i) * m + idx] = vals.items[i].asDouble();
+      ++count;
+    }
+  }
+  *countOut = count;
+  return out;
+}
+
 }  // namespace tchem_amd

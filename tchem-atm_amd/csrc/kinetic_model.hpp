@@ -117,4 +117,15 @@ KineticModel loadKineticModel(const std::string& path, std::ostream* echo);
 // [nBatch][nSpec+3] = [density, pressure, temperature, species...].
 std::vector<double> loadScenarioState(const std::string& path, const KineticModel& km, int* nBatch);
 
+// `photolysis_rates` block (r_0, r_1, ... each with one value per cell) of a
+// scenario file (src/core/TChem_AtmosphericChemistry.cpp:511-560): returns
+// [nBatch][count] (count 0 when the block is absent).
+std::vector<double> loadScenarioPhotolysisRates(const std::string& path, int nBatch, int* count);
+
+// `external_forcing` block (species_name + one value per cell) of a scenario
+// file (src/core/TChem_AtmosphericChemistry.cpp:562-631): returns
+// [nBatch][nActive], zero for species without forcing.
+std::vector<double> loadScenarioExternalForcing(const std::string& path, const KineticModel& km, int nBatch,
+                                                int* count);
+
 }  // namespace tchem_amd

@@ -124,3 +124,25 @@ def test_c_abi_argument_errors_without_device(pkg):
     assert lib.tchem_sourceterm_run_device_batch(4, None, None, None, None, None, None) == -1
     assert lib.tchem_jacobian_run_device_batch(4, None, None, None, None, None) == -1
     assert lib.tchem_atmchem_update_time_advance(-1, None, None, None, None) == -1
+
+
+def test_scenario_photolysis_rates_and_external_forcing(pkg):
+    """setScenarioConditionsPhotolysisReactions / ExternalForcing
+    (src/core/TChem_AtmosphericChemistry.cpp:511-631) on the reference's
+    80-cell column input, against a direct reading of the file (the species
+    names read as strings: YAML 1.1 would turn NO into a boolean)."""
+    import yaml
+    f = golden("uci_col_input_conditions.yaml")
+    kmd = pkg.KineticModelData(golden("uci_v2_test3.yaml"))
+    photo = pkg.setScenarioConditionsPhotolysisReactions(f, kmd, 80)
+    ext = pkg.setScenarioConditionsExternalForcing(f, kmd, 80)
+    raw = yaml.load(open(f), Loader=yaml.BaseLoader)
+    assert photo.shape == (80, len(raw["photolysis_rates"]))
+    for j in range(photo.shape[1]):
+        np.testing.assert_array_equal(photo[:, j], [float(v) for v in raw["photolysis_rates"][f"r_{j}"]["value"]])
+    expect = np.zeros((80, kmd.nActive))
+    for e in raw["external_forcing"].values():
+        expect[:, kmd.speciesNames.index(e["species_name"])] = [float(v) for v in e["value"]]
+    np.testing.assert_array_equal(ext, expect)
+    assert pkg.setScenarioConditionsPhotolysisReactions(golden("cb05cl_ae5.yaml"),
+                                                        pkg.KineticModelData(golden("cb05cl_ae5.yaml")), 1).shape == (1, 0)
