@@ -140,6 +140,24 @@ int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBat
 int tchem_atmchem_update_time_advance(int nBatch, tchem_time_advance_t* tadv, const double* t_out,
                                       const double* dt_out, void* stream);
 
+/* Replaces AtmosphericChemistryE3SM::runDeviceBatch(policy, tol_newton, tol_time, fac, tadv,
+ * state, photo_rates, external_sources, t_out, dt_out, state_out, kmcd)
+ * (src/core/TChem_AtmosphericChemistryE3SM.hpp, .cpp:36-232; problem:
+ * src/core/problems/TChem_Impl_AtmosphericChemistryE3SM{,_Problem}.hpp): the
+ * same TR-BDF2 box model with the E3SM kinetics' net production rates
+ * (rate constants of the E3SM kinetics with per-cell photolysis rates, plus
+ * per-cell external sources).  Arguments as tchem_atmchem_run_device_batch,
+ * plus photo_rates [device] [nBatch][nPhotoRates] or NULL and
+ * external_sources [device] [nBatch][m] or NULL.  Requires every species the
+ * rate constants depend on ([M], adjusted reactions, prod_O1D) to be a
+ * constant species, as in the UCI mechanisms. */
+int tchem_atmchem_e3sm_run_device_batch(const tchem_atmchem_options_t* opts, int nBatch, const double* tol_newton,
+                                        const double* tol_time, double* fac, const tchem_time_advance_t* tadv,
+                                        double* state, const double* photo_rates, int nPhotoRates,
+                                        const double* external_sources, double* t_out, double* dt_out,
+                                        double* state_out, tchem_cell_stats_t* stats, tchem_kmcd_t kmcd,
+                                        void* stream);
+
 /* The driver's interval loop (src/examples/TChem_AtmosphericChemistry.cpp:312-352)
  * captured into a HIP graph, bound to the given arrays by pointer: one
  * launch replays nIntervals x (tchem_atmchem_run_device_batch, then

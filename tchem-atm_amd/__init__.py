@@ -68,6 +68,7 @@ EXPORTED_SYMBOLS = (
     "tchem_graph_launch", "tchem_graph_status", "tchem_graph_destroy", "tchem_atmchem_autotune",
     "tchem_reactionrates_run_device_batch", "tchem_rateofprogress_run_device_batch",
     "tchem_netproductionrates_run_device_batch", "tchem_scenario_photolysis_rates", "tchem_scenario_external_forcing",
+    "tchem_atmchem_e3sm_run_device_batch",
 )
 
 
@@ -118,6 +119,8 @@ def _load():
         "tchem_netproductionrates_run_device_batch": [i, vp, vp, i, vp, vp, i, vp, vp],
         "tchem_scenario_photolysis_rates": [vp, cp, i, vp, i, ctypes.POINTER(i)],
         "tchem_scenario_external_forcing": [vp, cp, i, vp, ctypes.POINTER(i)],
+        "tchem_atmchem_e3sm_run_device_batch": [ctypes.POINTER(_Options), i, vp, vp, vp, vp, vp, vp, i, vp, vp, vp,
+                                                vp, vp, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -316,6 +319,24 @@ class Graph:
         if h is not None and h.value and _lib is not None:
             _lib.tchem_graph_destroy(h)
             self._h = None
+
+
+class AtmosphericChemistryE3SM:
+    """The box model with the E3SM kinetics (AtmosphericChemistryE3SM::runDeviceBatch):
+    per-cell photolysis rates [nBatch][nPhoto] and external sources [nBatch][m]."""
+
+    @staticmethod
+    def runDeviceBatch(tol_newton, tol_time, fac, tadv, state, photo_rates, external_sources, t_out, dt_out,
+                       state_out, kmcd, stats=None, precision=PRECISION_FP64, schedule=SCHEDULE_GRID,
+                       blocks_per_cu=0, lu=LU_AUTO, cells_per_block=0, programs=PROGRAMS_AUTO, stream=None):
+        opts = _Options(precision, schedule, blocks_per_cu, lu, cells_per_block, programs)
+        n_photo = 0 if photo_rates is None else photo_rates.shape[1]
+        _check(_lib.tchem_atmchem_e3sm_run_device_batch(
+            ctypes.byref(opts), state.shape[0], _ptr(tol_newton, what="tol_newton"), _ptr(tol_time, what="tol_time"),
+            _ptr(fac, None, "fac"), _ptr(tadv, None, "tadv"), _ptr(state, what="state"),
+            _ptr(photo_rates, what="photo_rates"), n_photo, _ptr(external_sources, what="external_sources"),
+            _ptr(t_out, what="t_out"), _ptr(dt_out, what="dt_out"), _ptr(state_out, what="state_out"),
+            _ptr(stats, None, "stats"), kmcd._h, _stream_ptr(stream)))
 
 
 class SourceTerm:

@@ -1,6 +1,10 @@
 // Box-model driver with the command line and output files of the
 // reference's TChem_AtmosphericChemistry.x
-// (src/examples/TChem_AtmosphericChemistry.cpp:46-352): load the mechanism
+// (src/examples/TChem_AtmosphericChemistry.cpp:46-352) and, built with
+// TCHEM_E3SM_DRIVER, of TChem_AtmosphericChemistryE3SM.x
+// (src/examples/TChem_AtmosphericChemistryE3SM.cpp: E3SM kinetics, scenario
+// from --inputfile with per-cell photolysis rates and external forcing,
+// --atol-time): load the mechanism
 // (writing kmod.echo), read the scenario, relaunch the batched integrator
 // interval by interval until the mean cell time exceeds 0.9999 tend, write
 // the state of every cell after every interval (iter, t, dt, state row) and
@@ -26,9 +30,17 @@
 
 namespace {
 
+#ifdef TCHEM_E3SM_DRIVER
+constexpr bool kE3sm = true;
+#else
+constexpr bool kE3sm = false;
+#endif
+
 struct Options {
-  std::string chemfile = "chem.yaml", outputfile = "AtmosphericChemistry.dat", times = "wall_times.json";
+  std::string chemfile = "chem.yaml", inputfile = "None", times = "wall_times.json";
+  std::string outputfile = kE3sm ? "AtmosphericChemistryE3SM.dat" : "AtmosphericChemistry.dat";
   double tbeg = 0, tend = 1, dtmin = 1e-8, dtmax = 1e-1, rtol_time = 1e-4, atol_newton = 1e-10, rtol_newton = 1e-6;
+  double atol_time = 1e-12;
   int per_interval = 10, max_iters = 1000, max_newton = 100, jac_interval = 1, batch = 1;
   bool verbose = false, write_profiles = true, cloned = false;
   int precision = TCHEM_PRECISION_FP64, lu = TCHEM_LU_AUTO, schedule = TCHEM_SCHEDULE_GRID;
@@ -39,7 +51,7 @@ bool parseBool(const std::string& v) { return v == "true" || v == "1" || v == "T
 void usage(const char* argv0) {
   std::printf(
       "Usage: %s [options]\n  options (--name=value):\n"
-      "  --chemfile --outputfile --outputfile_times --inputsPath\n"
+      "  --chemfile --outputfile --outputfile_times --inputsPath (E3SM: --inputfile --atol-time)\n"
       "  --tbeg --tend --dtmin --dtmax --tol-time --atol-newton --rtol-newton\n"
       "  --time-iterations-per-interval --max-time-iterations --max-newton-iterations --jacobian-interval\n"
       "  --batch_size --use_cloned_samples --write-time-profiles --verbose\n"
@@ -62,6 +74,8 @@ int parse(int argc, char** argv, Options& o) {
     const std::string val = eq == std::string::npos ? "true" : s.substr(eq + 1);
     try {
       if (key == "chemfile") o.chemfile = val;
+      else if (kE3sm && key == "inputfile") o.inputfile = val;
+      else if (kE3sm && key == "atol-time") o.atol_time = std::stod(val);
       else if (key == "outputfile") o.outputfile = val;
       else if (key == "outputfile_times") o.times = val;
       else if (key == "tbeg") o.tbeg = std::stod(val);
@@ -153,25 +167,48 @@ int main(int argc, char** argv) {
     names[k] = buf;
   }
 
+  const std::string input = (kE3sm && o.inputfile != "None") ? o.inputfile : o.chemfile;
   int nFiles = 0;
-  CHECK_T(tchem_scenario_load(kmd, o.chemfile.c_str(), nullptr, 0, &nFiles));
+  CHECK_T(tchem_scenario_load(kmd, input.c_str(), nullptr, 0, &nFiles));
   std::vector<double> scen(size_t(nFiles) * dim);
-  CHECK_T(tchem_scenario_load(kmd, o.chemfile.c_str(), scen.data(), nFiles, &nFiles));
+  CHECK_T(tchem_scenario_load(kmd, input.c_str(), scen.data(), nFiles, &nFiles));
+  // E3SM: per-cell photolysis rates and external forcing of the scenario
+  int nPhotoCols = 0;
+  std::vector<double> photoScen, extScen;
+  if (kE3sm) {
+    CHECK_T(tchem_scenario_photolysis_rates(kmd, input.c_str(), nFiles, nullptr, 0, &nPhotoCols));
+    photoScen.resize(size_t(nFiles) * nPhotoCols);
+    if (nPhotoCols)
+      CHECK_T(tchem_scenario_photolysis_rates(kmd, input.c_str(), nFiles, photoScen.data(), nPhotoCols, &nPhotoCols));
+    extScen.resize(size_t(nFiles) * m);
+    int nExt = 0;
+    CHECK_T(tchem_scenario_external_forcing(kmd, input.c_str(), nFiles, extScen.data(), &nExt));
+  }
   int n = nFiles;
-  std::vector<double> state;
+  std::vector<double> state, photo, ext;
+  auto clone = [](const std::vector<double>& row0, int width, int count) {
+    std::vector<double> v(size_t(count) * width);
+    for (int i = 0; i < count; ++i) std::copy(row0.begin(), row0.begin() + width, v.begin() + size_t(i) * width);
+    return v;
+  };
   if (nFiles == 1 && o.cloned && o.batch > 1) {
     std::printf("Using cloned samples ... only for numerical experiments\n");
     n = o.batch;
-    state.resize(size_t(n) * dim);
-    for (int i = 0; i < n; ++i) std::memcpy(&state[size_t(i) * dim], scen.data(), sizeof(double) * dim);
+    state = clone(scen, dim, n);
+    if (kE3sm) {
+      photo = clone(photoScen, nPhotoCols, n);
+      ext = clone(extScen, m, n);
+    }
   } else {
     state = scen;
+    photo = photoScen;
+    ext = extScen;
   }
 
   tchem_kmcd_t kmcd = nullptr;
   CHECK_T(tchem_kmcd_create(kmd, &kmcd));
 
-  DeviceArray<double> d_state, d_t, d_dt, d_tn, d_tt;
+  DeviceArray<double> d_state, d_t, d_dt, d_tn, d_tt, d_photo, d_ext;
   DeviceArray<tchem_time_advance_t> d_tadv;
   CHECK_H(hipMalloc(&d_state.p, sizeof(double) * state.size()));
   CHECK_H(hipMalloc(&d_t.p, sizeof(double) * n));
@@ -183,8 +220,16 @@ int main(int argc, char** argv) {
   const double tn[2] = {o.atol_newton, o.rtol_newton};
   std::vector<double> tt(size_t(2) * (m > 0 ? m : 1));
   for (int i = 0; i < m; ++i) {
-    tt[2 * i] = 1e-12;  // the reference's fixed absolute time tolerance
+    tt[2 * i] = o.atol_time;  // 1e-12 (fixed in TChem_AtmosphericChemistry.x, --atol-time in the E3SM driver)
     tt[2 * i + 1] = o.rtol_time;
+  }
+  if (kE3sm) {
+    if (nPhotoCols) {
+      CHECK_H(hipMalloc(&d_photo.p, sizeof(double) * photo.size()));
+      CHECK_H(hipMemcpy(d_photo.p, photo.data(), sizeof(double) * photo.size(), hipMemcpyHostToDevice));
+    }
+    CHECK_H(hipMalloc(&d_ext.p, sizeof(double) * (ext.empty() ? 1 : ext.size())));
+    if (!ext.empty()) CHECK_H(hipMemcpy(d_ext.p, ext.data(), sizeof(double) * ext.size(), hipMemcpyHostToDevice));
   }
   tchem_time_advance_t ta{};
   ta._tbeg = o.tbeg;
@@ -222,7 +267,7 @@ int main(int argc, char** argv) {
     return 1;
   }
   std::fprintf(ftimes, "{\n");
-  std::fprintf(ftimes, " \"Aerosol Chemistry\": \n {\n");
+  std::fprintf(ftimes, kE3sm ? " \"Atmospheric Chemistry E3SM\": \n {\n" : " \"Aerosol Chemistry\": \n {\n");
 
   tchem_atmchem_options_t opts{};
   opts.precision = o.precision;
@@ -232,8 +277,13 @@ int main(int argc, char** argv) {
   double tsum = 0.0;
   for (; iter < o.max_iters && tsum <= o.tend * 0.9999; ++iter) {
     const auto w0 = std::chrono::steady_clock::now();
-    CHECK_T(tchem_atmchem_run_device_batch(&opts, n, d_tn.p, d_tt.p, nullptr, d_tadv.p, d_state.p, nullptr, d_t.p,
-                                           d_dt.p, d_state.p, nullptr, kmcd, nullptr));
+    if (kE3sm)
+      CHECK_T(tchem_atmchem_e3sm_run_device_batch(&opts, n, d_tn.p, d_tt.p, nullptr, d_tadv.p, d_state.p, d_photo.p,
+                                                  nPhotoCols, m > 0 ? d_ext.p : nullptr, d_t.p, d_dt.p, d_state.p,
+                                                  nullptr, kmcd, nullptr));
+    else
+      CHECK_T(tchem_atmchem_run_device_batch(&opts, n, d_tn.p, d_tt.p, nullptr, d_tadv.p, d_state.p, nullptr, d_t.p,
+                                             d_dt.p, d_state.p, nullptr, kmcd, nullptr));
     CHECK_H(hipDeviceSynchronize());
     const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
     std::fprintf(ftimes, "\"%s%d\": %20.14e, \n", "wall_time_iter_", iter, wall);

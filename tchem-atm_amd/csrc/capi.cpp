@@ -51,8 +51,10 @@ struct tchem_kmd_s {
 
 struct tchem_kmcd_s {
   MechTables tables;
-  bool boxModel = true;     // the box-model integrator (and source term / Jacobian / adjoint) is available
+  bool boxModel = true;     // the mechanism compiles for the box-model integrator
   std::string boxError;     // why not
+  bool cb05Kinetics = true;  // ... with the CB05 rate constants (runDeviceBatch, source term, Jacobian, adjoint)
+  std::string e3smBoxError;  // empty when the E3SM box model (AtmosphericChemistryE3SM) can run
   DevE3sm e3{};             // E3SM kinetics tables (every mechanism the loader reads)
   SparseSchedule sched, schedT;  // forward and transposed (adjoint) programs
   DevMech dev{};
@@ -220,7 +222,7 @@ int resolveBlocksPerCU(const tchem_kmcd_s* kmcd, const tchem_atmchem_options_t& 
 // Rate constants (optionally log-scaled) into a.kf, then the integrator
 // launch for the resolved options; a carries the per-call arrays.
 int runIntegrator(tchem_kmcd_s* kmcd, const tchem_atmchem_options_t& o, const double* lscale, int lstride,
-                  IntegratorArgs& a, hipStream_t s, const char* who) {
+                  IntegratorArgs& a, hipStream_t s, const char* who, bool ratesReady = false) {
   const size_t lds = ldsBytes(kmcd, o);
   if (lds > kmcd->maxLds)
     return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": mechanism needs " + std::to_string(lds) +
@@ -238,7 +240,7 @@ int runIntegrator(tchem_kmcd_s* kmcd, const tchem_atmchem_options_t& o, const do
   a.cellBytes = a.sparse ? int(sparseCellBytes(kmcd->dev, kmcd->sparse,
                                                o.precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float)))
                          : 0;
-  {
+  if (!ratesReady) {
     hipError_t e = launchRates(kmcd->dev, a.nBatch, a.state, a.photo, lscale, lstride, const_cast<double*>(a.kf), a.skip,
                                s);
     if (e != hipSuccess) return hipFail(e, "rates_kernel launch");
@@ -285,7 +287,7 @@ AdjointLayout adjointLayout(const DevMech& M, int nBatch, int maxSteps) {
 
 int adjointCheck(const tchem_kmcd_s* k, int nBatch, int maxSteps, const char* who) {
   if (!k) return setError(TCHEM_ERR_INVALID, std::string(who) + ": null kmcd");
-  if (!k->boxModel) return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": " + k->boxError);
+  if (!k->cb05Kinetics) return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": " + k->boxError);
   if (nBatch < 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": negative nBatch");
   if (maxSteps < 1 || maxSteps > (1 << 24))
     return setError(TCHEM_ERR_INVALID, std::string(who) + ": maxSteps must be in [1, 2^24]");
@@ -561,6 +563,24 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
   if ((rc = upload(k.get(), ib, &d.ib)) || (rc = upload(k.get(), db, &d.db))) return rc;
   k->e3.ib = d.ib;
   k->e3.db = d.db;
+  k->cb05Kinetics = k->boxModel && !t.externalRates;
+  if (!k->cb05Kinetics && k->boxModel)
+    k->boxError = "the mechanism has JPL / ratio-JPL / PHOTO_RATE reactions (E3SM kinetics): use the E3SM entry points";
+  {
+    // the E3SM box model takes its rate constants once per cell: every species
+    // they depend on ([M], adjusted reactions, prod_O1D) must be constant
+    const KineticModel& km = kmd->km;
+    const int m = km.nActive();
+    std::string why;
+    if (!k->boxModel) why = k->boxError;
+    if (why.empty() && km.M_index >= 0 && km.M_index < m && !km.jpl.empty()) why = "[M] is an active species";
+    for (const auto& a : km.adjustReaction)
+      if (why.empty() && a.species < m) why = "a reaction is adjusted by an active species";
+    for (const auto& o : km.prodO1D)
+      if (why.empty() && (o.species1 < m || o.species2 < m || o.species3 < m))
+        why = "prod_O1D depends on an active species";
+    k->e3smBoxError = why;
+  }
   void* q = nullptr;
   e = hipMalloc(&q, 256);
   if (e != hipSuccess) return hipFail(e, "hipMalloc");
@@ -576,7 +596,7 @@ int tchem_kmcd_destroy(tchem_kmcd_t kmcd) {
 }
 
 int tchem_atmchem_workspace_bytes(tchem_kmcd_t kmcd, const tchem_atmchem_options_t* opts, size_t* bytes) {
-  if (kmcd && !kmcd->boxModel)
+  if (kmcd && !kmcd->cb05Kinetics)
     return setError(TCHEM_ERR_UNSUPPORTED, std::string("tchem_atmchem_workspace_bytes: ") + kmcd->boxError);
   if (!kmcd || !bytes) return setError(TCHEM_ERR_INVALID, "tchem_atmchem_workspace_bytes: null argument");
   tchem_atmchem_options_t o;
@@ -590,7 +610,7 @@ int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBat
                                    const double* tol_time, double* fac, const tchem_time_advance_t* tadv,
                                    double* state, const double* photo_rates, double* t_out, double* dt_out,
                                    double* state_out, tchem_cell_stats_t* stats, tchem_kmcd_t kmcd, void* stream) {
-  if (kmcd && !kmcd->boxModel)
+  if (kmcd && !kmcd->cb05Kinetics)
     return setError(TCHEM_ERR_UNSUPPORTED, std::string("run_device_batch: ") + kmcd->boxError);
   (void)fac;
   if (!kmcd) return setError(TCHEM_ERR_INVALID, "run_device_batch: null kmcd");
@@ -622,6 +642,48 @@ int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBat
   return runIntegrator(kmcd, o, nullptr, 0, a, s, "run_device_batch");
 }
 
+int tchem_atmchem_e3sm_run_device_batch(const tchem_atmchem_options_t* opts, int nBatch, const double* tol_newton,
+                                        const double* tol_time, double* fac, const tchem_time_advance_t* tadv,
+                                        double* state, const double* photo_rates, int nPhotoRates,
+                                        const double* external_sources, double* t_out, double* dt_out,
+                                        double* state_out, tchem_cell_stats_t* stats, tchem_kmcd_t kmcd,
+                                        void* stream) {
+  (void)fac;
+  const char* who = "e3sm_run_device_batch";
+  if (!kmcd) return setError(TCHEM_ERR_INVALID, std::string(who) + ": null kmcd");
+  if (!kmcd->e3smBoxError.empty())
+    return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": " + kmcd->e3smBoxError);
+  if (nBatch < 0 || nPhotoRates < 0 || nPhotoRates > kmcd->e3.nReac)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": bad size");
+  tchem_atmchem_options_t o;
+  int rc = resolveOptions(opts, kmcd, &o, who);
+  if (rc) return rc;
+  if (nBatch == 0) return TCHEM_OK;
+  if (!tol_newton || !tol_time || !tadv || !state || !t_out || !dt_out || !state_out ||
+      (nPhotoRates > 0 && !photo_rates))
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": null device array");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if ((rc = ensureKf(kmcd, nBatch, s))) return rc;
+  // rate constants from the E3SM kinetics (constant over the interval: they
+  // depend on T, p, the photolysis rates and constant species only)
+  hipError_t e = launchE3smKinetics(kmcd->e3, 3, nBatch, state, nPhotoRates ? photo_rates : nullptr, nPhotoRates,
+                                    nullptr, kmcd->kf, kmcd->e3.nReac, s);
+  if (e != hipSuccess) return hipFail(e, "e3sm_kinetics_kernel launch");
+  IntegratorArgs a{};
+  a.nBatch = nBatch;
+  a.tol_newton = tol_newton;
+  a.tol_time = tol_time;
+  a.tadv = tadv;
+  a.state = state;
+  a.t_out = t_out;
+  a.dt_out = dt_out;
+  a.state_out = state_out;
+  a.stats = stats;
+  a.kf = kmcd->kf;
+  a.ext = external_sources;
+  return runIntegrator(kmcd, o, nullptr, 0, a, s, who, true);
+}
+
 int tchem_atmchem_update_time_advance(int nBatch, tchem_time_advance_t* tadv, const double* t_out,
                                       const double* dt_out, void* stream) {
   if (nBatch < 0 || (nBatch > 0 && (!tadv || !t_out || !dt_out)))
@@ -634,7 +696,7 @@ int tchem_atmchem_update_time_advance(int nBatch, tchem_time_advance_t* tadv, co
 int tchem_sourceterm_run_device_batch(int nBatch, const double* state, const double* photo_rates,
                                       const double* external_sources, double* omega, tchem_kmcd_t kmcd,
                                       void* stream) {
-  if (kmcd && !kmcd->boxModel)
+  if (kmcd && !kmcd->cb05Kinetics)
     return setError(TCHEM_ERR_UNSUPPORTED, std::string("sourceterm: ") + kmcd->boxError);
   if (!kmcd || nBatch < 0) return setError(TCHEM_ERR_INVALID, "sourceterm: bad argument");
   if (nBatch == 0) return TCHEM_OK;
@@ -648,7 +710,7 @@ int tchem_sourceterm_run_device_batch(int nBatch, const double* state, const dou
 
 int tchem_jacobian_run_device_batch(int nBatch, const double* state, const double* photo_rates, double* jac,
                                     tchem_kmcd_t kmcd, void* stream) {
-  if (kmcd && !kmcd->boxModel)
+  if (kmcd && !kmcd->cb05Kinetics)
     return setError(TCHEM_ERR_UNSUPPORTED, std::string("jacobian: ") + kmcd->boxError);
   if (!kmcd || nBatch < 0) return setError(TCHEM_ERR_INVALID, "jacobian: bad argument");
   if (nBatch == 0) return TCHEM_OK;
@@ -803,7 +865,7 @@ int tchem_atmchem_graph_create(const tchem_atmchem_options_t* opts, int nBatch, 
                                const double* photo_rates, double* t_out, double* dt_out, double* state_out,
                                tchem_cell_stats_t* stats, int nIntervals, double stopFraction, tchem_kmcd_t kmcd,
                                tchem_graph_t* out) {
-  if (kmcd && !kmcd->boxModel)
+  if (kmcd && !kmcd->cb05Kinetics)
     return setError(TCHEM_ERR_UNSUPPORTED, std::string("tchem_atmchem_graph_create: ") + kmcd->boxError);
   const char* who = "tchem_atmchem_graph_create";
   if (!kmcd || !out) return setError(TCHEM_ERR_INVALID, std::string(who) + ": null argument");
@@ -917,7 +979,7 @@ int tchem_atmchem_autotune(tchem_atmchem_options_t* opts, int nBatch, const doub
                            const double* tol_time, const tchem_time_advance_t* tadv, const double* state,
                            const double* photo_rates, int sampleCells, int sampleIterations, double* best_seconds,
                            tchem_kmcd_t kmcd, void* stream) {
-  if (kmcd && !kmcd->boxModel)
+  if (kmcd && !kmcd->cb05Kinetics)
     return setError(TCHEM_ERR_UNSUPPORTED, std::string("tchem_atmchem_autotune: ") + kmcd->boxError);
   const char* who = "tchem_atmchem_autotune";
   if (!kmcd || !opts) return setError(TCHEM_ERR_INVALID, std::string(who) + ": null argument");

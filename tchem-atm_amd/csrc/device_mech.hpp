@@ -112,6 +112,7 @@ struct IntegratorArgs {
   int cellBytes;                     // LDS bytes of one cell region
   AdjointRecord rec;                 // h == nullptr: no record
   const int* skip;                   // nullable: the launch does nothing when *skip != 0 (captured interval loops)
+  const double* ext;                 // nullable [nBatch][m]: per-cell external sources added to the source term
 };
 
 // LDS bytes of one cell: the iteration matrix in TA precision (bytesA per

@@ -130,6 +130,7 @@ struct CellLds {
   double* seg;  // [nSeg] source-term segment sums
   TA* rdiag;    // [m] reciprocal pivots 1 / u_kk
   int* swp;     // [2m] row interchanges (k, p) of the factorisation
+  const double* src;  // nullable: this cell's external sources [m], reference order
 };
 
 template <typename TA>
@@ -145,6 +146,7 @@ __device__ CellLds<TA> carve(char* base, const DevMech& M, bool withMatrix = tru
   L.seg = L.scr + nScr;
   L.rdiag = reinterpret_cast<TA*>(L.seg + M.nSeg);
   L.swp = reinterpret_cast<int*>(L.seg + M.nSeg + M.m);
+  L.src = nullptr;
   return L;
 }
 
@@ -192,6 +194,7 @@ __device__ void source_term_from_lds(const DevMech& M, const CellLds<TA>& L, int
     double s = 0.0;
     if (a < M.m) {
       s = M.emission()[a];
+      if (L.src) s += L.src[M.perm()[a]];
       const int s1 = M.rowSegPtr()[a + 1];
       for (int g = M.rowSegPtr()[a]; g < s1; ++g) s += L.seg[g];
     }
@@ -489,6 +492,7 @@ struct SpLds {
   double* kf;   // [nReac+1], kf[nReac] = 0
   double* rop;  // [nReac+1], rop[nReac] = 0
   double* seg;  // [nSeg+1], seg[nSeg] = trash
+  const double* src;  // nullable: this cell's external sources [m], reference order
   // discrete adjoint only (nullptr otherwise)
   double* mu;   // [m+1] adjoint vector, mu[m] = 0
   double* w;    // [nReac+2] nu_r . mu, w[nReac] = 0, w[nReac+1] = trash
@@ -531,6 +535,7 @@ __device__ SpLds<TA> carve_sparse(char* base, const DevMech& M, const DevSparse&
   L.rop = L.kf + M.nReac + 1;
   L.seg = L.rop + M.nReac + 1;
   L.mu = L.w = L.v = nullptr;
+  L.src = nullptr;
   return L;
 }
 
@@ -714,15 +719,18 @@ __device__ void sp_sums(const DevSparse& S, const SpLds<TA>& L, int lane, bool j
                      scale);
 }
 
-// per-species sums of the segment sums plus emission (register layout)
+// per-species sums of the segment sums plus emission and external sources
+// (register layout)
 template <int QM>
-__device__ __forceinline__ void row_sums(const DevMech& M, const double* seg, int lane, double (&f)[QM]) {
+__device__ __forceinline__ void row_sums(const DevMech& M, const double* seg, const double* src, int lane,
+                                         double (&f)[QM]) {
 #pragma unroll
   for (int q = 0; q < QM; ++q) {
     const int a = lane + kWave * q;
     double s = 0.0;
     if (a < M.m) {
       s = M.emission()[a];
+      if (src) s += src[M.perm()[a]];
       const int s1 = M.rowSegPtr()[a + 1];
       for (int g = M.rowSegPtr()[a]; g < s1; ++g) s += seg[g];
     }
@@ -921,7 +929,7 @@ struct SparseBackend {
     wave_sync();
     TCHEM_PT_END(sums, 12)
     TCHEM_PT_BEGIN(rows)
-    row_sums<QM>(a.M, L.seg, lane, f);
+    row_sums<QM>(a.M, L.seg, L.src, lane, f);
 #pragma unroll
     for (int q = 0; q < QM; ++q) {
       const int i = lane + kWave * q;
@@ -994,10 +1002,12 @@ __device__ __forceinline__ double uniform_d(double v) {
 //          after BDF2 estimate the local error and accept or reject.
 // REC: also record the accepted steps for the discrete adjoint (a.rec).
 template <class B, int QM, bool REC>
-__device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L, int cell, int lane) {
+__device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& Lw, int cell, int lane) {
   using TA = typename B::TA;
   const DevMech& M = a.M;
   const int m = M.m, nSpec = M.nSpec, dim = nSpec + 3;
+  typename B::Lds L = Lw;
+  L.src = a.ext ? a.ext + size_t(cell) * m : nullptr;
   const tchem_time_advance_t ta = a.tadv[cell];
   const double tend = ta._tend;
   double* row = a.state + size_t(cell) * dim;
@@ -1302,6 +1312,7 @@ __global__ __launch_bounds__(256) void rates_kernel(DevMech M, int nBatch, const
 //   mode 0: kf   (ReactionRates)
 //   mode 1: rop  (RateofProgress; photolysis rates replace kf of the first nPhoto reactions)
 //   mode 2: omega of the active species (NetProductionRates: external sources + nu . rop)
+//   mode 3: kf with the photolysis rates applied (the box model's rate constants)
 // Species concentrations are the state row's species block (active then constant).
 
 __device__ __forceinline__ double t300_pow(double T, double B) { return B == 0.0 ? 1.0 : pow(T / 300.0, B); }
@@ -1376,7 +1387,7 @@ __global__ __launch_bounds__(64) void e3sm_kinetics_kernel(DevE3sm E, int mode, 
   }
   wave_sync();
   double* o = out + size_t(cell) * ldOut;
-  if (mode == 0) {
+  if (mode == 0 || mode == 3) {
     for (int r = lane; r < E.nReac; r += kWave) o[r] = kf[r];
     return;
   }

@@ -16,15 +16,15 @@ MechTables compileMechanism(const KineticModel& km) {
   t.convPPM = km.CONV_PPM;
   const int nR = km.nReac, m = t.m;
 
-  for (int r = 0; r < nR; ++r) {
-    const int ty = km.reactionType[r];
-    if (ty != kArrhenius && ty != kTroe && ty != kCmaqH2O2)
-      throw std::runtime_error("reaction " + std::to_string(r) +
-                               " has a type the gas-phase box model does not evaluate (E3SM kinetics)");
-  }
-
   // rate-constant records
   t.rateKind.assign(nR, kRateArrhenius);
+  for (int r = 0; r < nR; ++r) {
+    const int ty = km.reactionType[r];
+    if (ty != kArrhenius && ty != kTroe && ty != kCmaqH2O2) {
+      t.rateKind[r] = kRateExternal;
+      t.externalRates = true;
+    }
+  }
   t.ratePar.assign(size_t(nR) * 8, 0.0);
   t.convExp = km.convExponent;
   t.photoSlot.assign(nR, -1);

@@ -24,7 +24,9 @@
 
 namespace tchem_amd {
 
-enum RateKind : int { kRateArrhenius = 0, kRateTroe = 1, kRateCmaqH2O2 = 2 };
+// kRateExternal: JPL / ratio-JPL / PHOTO_RATE reactions, whose rate constants
+// only the E3SM kinetics kernel computes (the box model then takes them from it)
+enum RateKind : int { kRateArrhenius = 0, kRateTroe = 1, kRateCmaqH2O2 = 2, kRateExternal = 3 };
 
 struct MechTables {
   int nSpec = 0, nConst = 0, m = 0, nReac = 0, nPhoto = 0;
@@ -39,6 +41,7 @@ struct MechTables {
   std::vector<int> perm, iperm;  // [m]
 
   // rate constants, per reaction
+  bool externalRates = false;    // some rate constants need the E3SM kinetics (kRateExternal)
   std::vector<int> rateKind;     // RateKind
   std::vector<double> ratePar;   // [nReac][8]
   std::vector<int> convExp;      // [nReac]
