@@ -15,6 +15,7 @@ import subprocess
 
 import numpy as np
 
+from . import amd as amd_mod
 from . import kmd as kmd_mod
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -34,6 +35,14 @@ class _KMD(ctypes.Structure):
         ("nEmis", ctypes.c_int), ("emisSpecies", ctypes.c_void_p), ("emisRate", ctypes.c_void_p),
         ("nPhoto", ctypes.c_int), ("photoReac", ctypes.c_void_p),
         ("CONV_PPM", ctypes.c_double),
+    ]
+
+
+class _AMD(ctypes.Structure):
+    _fields_ = [
+        ("nSpec", ctypes.c_int), ("nParticles", ctypes.c_int), ("nSimpol", ctypes.c_int),
+        ("mw", ctypes.c_void_p), ("density", ctypes.c_void_p), ("gas", ctypes.c_void_p),
+        ("aero", ctypes.c_void_p), ("par", ctypes.c_void_p),
     ]
 
 
@@ -85,6 +94,11 @@ def lib():
         _lib.oracle_record_steps.restype = i
         _lib.oracle_tlm_fixed_steps.argtypes = [vp, vp, i, i, vp, vp, vp, vp, i, i, vp, vp, vp]
         _lib.oracle_tlm_fixed_steps.restype = i
+        _lib.oracle_simpol_constants.argtypes = [vp, d, d, vp]
+        _lib.oracle_aerosol_omega.argtypes = [vp, vp, d, d, vp, vp, vp, vp, vp]
+        _lib.oracle_aerosol_jacobian.argtypes = [vp, vp, d, d, vp, vp, vp, vp, vp]
+        _lib.oracle_aerosol_rhs_batch.argtypes = [vp, vp, i, vp, vp, vp]
+        _lib.oracle_aerosol_run_batch.argtypes = [vp, vp, i, vp, vp, vp, vp, vp, vp, vp, vp, i]
     return _lib
 
 
@@ -187,6 +201,72 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("oracle_tlm_fixed_steps: Newton failure")
         return u, S[:, :len(sens)]
+
+
+class AerosolOracle:
+    """Binds a gas mechanism and an aerosol file to the C oracle's
+    single-particle SIMPOL.1 functions."""
+
+    def __init__(self, chem, aero):
+        self.g = chem if isinstance(chem, Oracle) else Oracle(chem)
+        self.a = aero if isinstance(aero, amd_mod.OracleAMD) else amd_mod.load(aero, list(self.g.k.names))
+        a = self.a
+        self._keep = [a.mw, a.density, a.gas, a.aero, a.par]
+        self.s = _AMD(a.nSpec, a.nParticles, a.nSimpol, _p(a.mw), _p(a.density), _p(a.gas), _p(a.aero),
+                      _p(a.par))
+        self.ptr = ctypes.addressof(self.s)
+
+    @property
+    def mg(self):
+        return self.g.m
+
+    @property
+    def mt(self):
+        return self.g.m + self.a.nSpec * self.a.nParticles
+
+    @property
+    def dim(self):
+        return 3 + self.g.k.nSpec + self.a.nSpec * self.a.nParticles
+
+    def simpol_constants(self, s, T, p):
+        out = np.zeros(4)
+        lib().oracle_simpol_constants(_p(np.ascontiguousarray(self.a.par[s])), float(T), float(p), _p(out))
+        return out
+
+    def omega(self, T, p, x, cconst, nc):
+        kfor = self.g.kforward(T, p)
+        out = np.zeros(self.mt)
+        lib().oracle_aerosol_omega(self.g.ptr, self.ptr, float(T), float(p), _p(kfor),
+                                   _p(np.ascontiguousarray(cconst, np.float64)),
+                                   _p(np.ascontiguousarray(nc, np.float64)),
+                                   _p(np.ascontiguousarray(x, np.float64)), _p(out))
+        return out
+
+    def jacobian(self, T, p, x, cconst, nc):
+        kfor = self.g.kforward(T, p)
+        out = np.zeros((self.mt, self.mt))
+        lib().oracle_aerosol_jacobian(self.g.ptr, self.ptr, float(T), float(p), _p(kfor),
+                                      _p(np.ascontiguousarray(cconst, np.float64)),
+                                      _p(np.ascontiguousarray(nc, np.float64)),
+                                      _p(np.ascontiguousarray(x, np.float64)), _p(out))
+        return out
+
+    def rhs_batch(self, state, nc):
+        state = np.ascontiguousarray(state, np.float64)
+        nc = np.ascontiguousarray(nc, np.float64)
+        out = np.zeros((state.shape[0], self.mt))
+        lib().oracle_aerosol_rhs_batch(self.g.ptr, self.ptr, state.shape[0], _p(state), _p(nc), _p(out))
+        return out
+
+    def run_batch(self, tol_newton, tol_time, tadv, state, nc, t_out, dt_out, stats=None, nthreads=0):
+        """One AerosolChemistry::runDeviceBatch interval, in place."""
+        assert state.dtype == np.float64 and state.flags.c_contiguous and state.shape[1] == self.dim
+        assert tadv.dtype == TADV_DTYPE
+        nc = np.ascontiguousarray(nc, np.float64)
+        lib().oracle_aerosol_run_batch(self.g.ptr, self.ptr, state.shape[0],
+                                       _p(np.ascontiguousarray(tol_newton, np.float64)),
+                                       _p(np.ascontiguousarray(tol_time, np.float64)), _p(tadv), _p(state),
+                                       _p(nc), _p(t_out), _p(dt_out), _p(stats), int(nthreads))
 
 
 def lu(A):

@@ -188,7 +188,9 @@ void oracle_trbdf2_constants(double* out6) {
  */
 typedef struct {
   const oracle_kmd* k;
-  int m, mode;
+  const oracle_amd* a; /* NULL: gas-phase box model */
+  const double* nc;    /* [nParticles] number concentrations (aerosol) */
+  int m, mg, mode;     /* unknowns, active gas species, Jacobian mode */
   double T, p;
   const double* photo;
   const double* lscale; /* [nReac] log-scale of the rate constants, or NULL */
@@ -210,9 +212,12 @@ static void cell_kforward(cell_ctx* x) {
 
 /* source term at u (active part); the constant part of c is fixed per cell */
 static void rhs(cell_ctx* x, const double* u, double* f) {
-  memcpy(x->c, u, sizeof(double) * x->m);
+  memcpy(x->c, u, sizeof(double) * x->mg);
   if (x->mode == 1) cell_kforward(x); /* reference: every call */
-  oracle_omega(x->k, x->kfor, x->c, f);
+  if (x->a)
+    oracle_aerosol_omega(x->k, x->a, x->T, x->p, x->kfor, x->c + x->mg, x->nc, u, f);
+  else
+    oracle_omega(x->k, x->kfor, x->c, f);
 }
 
 /* Forward-difference Jacobian with m+1 source-term evaluations — the
@@ -238,6 +243,9 @@ static void jac_fd(cell_ctx* x, const double* u, double* J) {
 static void jac(cell_ctx* x, const double* u, double* J) {
   if (x->mode == 1) {
     jac_fd(x, u, J);
+  } else if (x->a) {
+    memcpy(x->c, u, sizeof(double) * x->mg);
+    oracle_aerosol_jacobian(x->k, x->a, x->T, x->p, x->kfor, x->c + x->mg, x->nc, u, J);
   } else {
     memcpy(x->c, u, sizeof(double) * x->m);
     oracle_jacobian(x->k, x->kfor, x->c, J);
@@ -294,8 +302,11 @@ static int newton(cell_ctx* x, double* u, const double* b, double ch, const doub
 static void integrate_cell(const oracle_kmd* k, const double* tol_newton, const double* tol_time,
                            const oracle_time_advance* ta, double* row, const double* photo, double* t_out,
                            double* dt_out, oracle_stats* st, int mode, double* work, int* iwork,
-                           const double* lscale, double* h_rec, int max_rec, int* n_rec) {
-  const int nSpec = k->nSpec, m = nSpec - k->nConstSpec;
+                           const double* lscale, double* h_rec, int max_rec, int* n_rec, const oracle_amd* am,
+                           const double* nc) {
+  const int nSpec = k->nSpec, mg = nSpec - k->nConstSpec;
+  const int m = am ? mg + am->nSpec * am->nParticles : mg; /* unknowns */
+#define ROWCOL(i) ((i) < mg ? 3 + (i) : 3 + nSpec + ((i) - mg))
   const double tend = ta->tend;
   if (*t_out >= tend) return; /* .cpp:83 */
   const double dt_in = ta->dt;
@@ -307,6 +318,9 @@ static void integrate_cell(const oracle_kmd* k, const double* tol_newton, const 
 
   cell_ctx x;
   x.k = k;
+  x.a = am;
+  x.nc = nc;
+  x.mg = mg;
   x.m = m;
   x.mode = mode;
   x.p = row[1];
@@ -332,7 +346,7 @@ static void integrate_cell(const oracle_kmd* k, const double* tol_newton, const 
   double* b = w; w += m;
 
   for (int i = 0; i < nSpec; ++i) x.c[i] = row[3 + i];
-  for (int i = 0; i < m; ++i) un[i] = row[3 + i];
+  for (int i = 0; i < m; ++i) un[i] = row[ROWCOL(i)];
   if (mode == 0) cell_kforward(&x); /* constant per cell */
   int nrec = 0;
 
@@ -391,16 +405,17 @@ static void integrate_cell(const oracle_kmd* k, const double* tol_newton, const 
   if (n_rec) *n_rec = (failed || nrec > max_rec) ? -1 : nrec;
 
   if (failed) {
-    for (int i = 0; i < m; ++i) row[3 + i] = 0.0;
+    for (int i = 0; i < m; ++i) row[ROWCOL(i)] = 0.0;
     *t_out = tend;
     *dt_out = -1.0;
     if (st) st->failed = 1;
   } else {
-    for (int i = 0; i < m; ++i) row[3 + i] = un[i];
+    for (int i = 0; i < m; ++i) row[ROWCOL(i)] = un[i];
     *t_out = t;
     *dt_out = dt;
   }
 }
+#undef ROWCOL
 
 void oracle_run_batch(const oracle_kmd* k, int nBatch, const double* tol_newton, const double* tol_time,
                       const oracle_time_advance* tadv, double* state, const double* photo, double* t_out,
@@ -418,7 +433,7 @@ void oracle_run_batch(const oracle_kmd* k, int nBatch, const double* tol_newton,
     for (int c = 0; c < nBatch; ++c) {
       integrate_cell(k, tol_newton, tol_time, &tadv[c], state + (size_t)c * (nSpec + 3),
                      photo ? photo + (size_t)c * k->nPhoto : NULL, &t_out[c], &dt_out[c],
-                     stats ? &stats[c] : NULL, jacobian_mode, work, iwork, NULL, NULL, 0, NULL);
+                     stats ? &stats[c] : NULL, jacobian_mode, work, iwork, NULL, NULL, 0, NULL, NULL, NULL);
     }
     free(work);
     free(iwork);
@@ -480,7 +495,7 @@ int oracle_record_steps(const oracle_kmd* k, const double* tol_newton, const dou
   double t_out = ta->tbeg, dt_out = ta->dt;
   int n = -1;
   integrate_cell(k, tol_newton, tol_time, ta, row, photo, &t_out, &dt_out, NULL, 0, work, iwork, lscale, h_out,
-                 max_steps, &n);
+                 max_steps, &n, NULL, NULL);
   free(work);
   free(iwork);
   return n;
@@ -630,4 +645,215 @@ int oracle_tlm_fixed_steps(const oracle_kmd* k, const double* tol_newton, int ma
   free(work);
   free(iwork);
   return rc;
+}
+
+/* ---------------------------------------------------------------------------
+ * Single-particle SIMPOL.1 gas-particle partitioning
+ */
+#define SP_RUNIV 8.314472          /* src/core/TChem_Util.hpp:176 */
+#define SP_PI 3.141592653589793    /* TChem_Util.hpp:212 */
+#define SP_ATMPA 101325.0          /* TChem_Util.hpp:227-229 */
+/* src/core/impl/TChem_Impl_SingleParticleUtils.hpp:25-30 (long double
+ * literals there; double here, a difference below 1e-16 of a 1e-25 term) */
+#define SP_MIN_MASS 1.0e-25
+#define SP_MIN_MW 0.1
+#define SP_MIN_DENSITY 1800.0
+
+/* src/core/impl/TChem_Impl_SIMPOL_constant.hpp:60-128: mass accommodation
+ * (Ervens et al. 2003) when N* is given, else 0.1; mean free path
+ * 3 D / sqrt(8 R T / (pi MW)) (TChem_Util.hpp:1211-1238); SIMPOL.1 vapour
+ * pressure; kg m-3 -> ppm factor; partitioning constant.  Same expression
+ * order as the reference.  Pinned by the SIMPOL RHS golden
+ * (Simpol_RHS_HOST.txt) through oracle_aerosol_omega. */
+void oracle_simpol_constants(const double* par, double T, double p, double* out4) {
+  const double B1 = par[0], B2 = par[1], B3 = par[2], B4 = par[3], D = par[4], MW = par[5], Nstar = par[6];
+  double alpha = 0.1;
+  if (par[7] != 0.0) {
+    const double N23 = pow(Nstar, 2.0 / 3.0);
+    double dh = -10.0 * (Nstar - 1.0) + 7.53 * (N23 - 1.0) - 1.0;
+    double ds = -32.0 * (Nstar - 1.0) + 9.21 * (N23 - 1.0) - 1.3;
+    dh = dh * 4184.0;
+    ds = ds * 4.184;
+    const double dG = dh - T * ds;
+    alpha = exp(-dG / (SP_RUNIV * T));
+    alpha = alpha / (1.0 + alpha);
+  }
+  const double mfp = 3.0 * D / sqrt(8.0 * SP_RUNIV * T / (SP_PI * MW));
+  double vp = B1 / T + B2 + B3 * T + B4 * log(T);
+  vp = SP_ATMPA * pow(10, vp);
+  const double conv = SP_RUNIV / MW * 1.0e6;
+  out4[0] = alpha;
+  out4[1] = mfp;
+  out4[2] = conv * T / p;
+  out4[3] = vp / p / MW * 1.0e6;
+}
+
+/* per-particle sums of SingleParticleUtils.hpp:62-111: volume -> effective
+ * radius, total mass and moles */
+static void particle_sums(const oracle_amd* a, const double* y, double* radius, double* volume, double* mass,
+                          double* moles) {
+  double v = SP_MIN_MASS / SP_MIN_DENSITY;
+  for (int i = 0; i < a->nSpec; ++i) v += y[i] / a->density[i];
+  *volume = v;
+  *radius = pow((v * 3.0 / 4.0 / SP_PI), 1.0 / 3.0);
+  double ms = SP_MIN_MASS, mo = SP_MIN_MASS / SP_MIN_MW;
+  for (int i = 0; i < a->nSpec; ++i) {
+    ms += y[i];
+    mo += y[i] / a->mw[i];
+  }
+  *mass = ms;
+  *moles = mo;
+}
+
+/* Fuchs-Sutugin transition-regime factor (SingleParticleUtils.hpp:128-135) */
+static double fuchs_sutugin(double Kn, double alpha) {
+  return (0.75 * alpha * (1.0 + Kn)) / (Kn * Kn + (1.0 + 0.283 * alpha) * Kn + 0.75 * alpha);
+}
+
+/* src/core/impl/TChem_Impl_Aerosol_RHS.hpp:64-103: omega := 0, gas-phase
+ * source term over c = [x_gas, const] (ReactionRatesAerosol), then for each
+ * particle and each SIMPOL transfer (SIMPOL_phase_transfer.hpp:61-150):
+ *   k_c = 4 pi r D f(lambda / r, alpha),  k_e = k_c (K_eq MW_avg / mass)
+ *   diff = k_c x_gas - k_e y_aero
+ *   omega_gas -= N_p diff,  omega_aero += diff / (kg m-3 -> ppm)
+ * accumulated serially in (particle, transfer) order.
+ * Pinned by: the SIMPOL RHS golden, a mass-balance invariant and the
+ * equilibrium zero (tests/test_oracle_aerosol.py). */
+void oracle_aerosol_omega(const oracle_kmd* k, const oracle_amd* a, double T, double p, const double* kfor,
+                          const double* cconst, const double* nc, const double* x, double* omega) {
+  const int mg = k->nSpec - k->nConstSpec, mt = mg + a->nSpec * a->nParticles;
+  double* c = (double*)malloc(sizeof(double) * (size_t)(k->nSpec > 0 ? k->nSpec : 1));
+  for (int i = 0; i < mg; ++i) c[i] = x[i];
+  for (int i = mg; i < k->nSpec; ++i) c[i] = cconst[i - mg];
+  for (int i = 0; i < mt; ++i) omega[i] = 0.0;
+  oracle_omega(k, kfor, c, omega);
+  free(c);
+  for (int ip = 0; ip < a->nParticles; ++ip) {
+    const double* y = x + mg + (size_t)ip * a->nSpec;
+    double radius, volume, mass, moles;
+    particle_sums(a, y, &radius, &volume, &mass, &moles);
+    const double avgMW = mass / moles;
+    for (int s = 0; s < a->nSimpol; ++s) {
+      double c4[4];
+      oracle_simpol_constants(a->par + 8 * s, T, p, c4);
+      const double D = a->par[8 * s + 4];
+      const double Kn = c4[1] / radius;
+      double cond = 4.0 * SP_PI * radius * D * fuchs_sutugin(Kn, c4[0]);
+      double evap = cond * 1.0 * (c4[3] * avgMW / mass);
+      const int g = a->gas[s], ai = mg + ip * a->nSpec + a->aero[s];
+      cond *= x[g];
+      evap *= x[ai];
+      const double diff = -evap + cond;
+      omega[g] += -nc[ip] * diff;
+      omega[ai] += diff / c4[2];
+    }
+  }
+}
+
+/* Analytic Jacobian of oracle_aerosol_omega.  The reference uses a
+ * forward-difference Jacobian for this problem
+ * (TChem_Impl_AerosolChemistry_Problem.hpp:211-239); both solve the same
+ * Newton system.  With r = (3V / 4 pi)^(1/3), V = V0 + sum y_i / rho_i,
+ * Kn = lambda / r:
+ *   d k_c / d y_i = 4 pi D (f - Kn f'(Kn)) (r / 3V) / rho_i
+ *   k_e = k_c K_eq / moles, d k_e / d y_i = K_eq (d k_c / d y_i) / moles
+ *                                           - k_e / (moles MW_i)
+ *   d diff / d x_gas = k_c,  d diff / d y_i = x_gas d k_c - y_aero d k_e - [i = aero] k_e
+ * Pinned by: central differences of oracle_aerosol_omega
+ * (tests/test_oracle_aerosol.py). */
+void oracle_aerosol_jacobian(const oracle_kmd* k, const oracle_amd* a, double T, double p, const double* kfor,
+                             const double* cconst, const double* nc, const double* x, double* J) {
+  const int mg = k->nSpec - k->nConstSpec, mt = mg + a->nSpec * a->nParticles;
+  memset(J, 0, sizeof(double) * (size_t)mt * mt);
+  double* c = (double*)malloc(sizeof(double) * ((size_t)(k->nSpec > 0 ? k->nSpec : 1) + (size_t)mg * mg + 2 * (size_t)a->nSpec + 1));
+  double* Jg = c + (k->nSpec > 0 ? k->nSpec : 1);
+  double* dkc = Jg + (size_t)mg * mg;
+  double* dke = dkc + a->nSpec;
+  for (int i = 0; i < mg; ++i) c[i] = x[i];
+  for (int i = mg; i < k->nSpec; ++i) c[i] = cconst[i - mg];
+  if (mg > 0) oracle_jacobian(k, kfor, c, Jg);
+  for (int i = 0; i < mg; ++i)
+    for (int j = 0; j < mg; ++j) J[(size_t)i * mt + j] = Jg[(size_t)i * mg + j];
+  for (int ip = 0; ip < a->nParticles; ++ip) {
+    const int base = mg + ip * a->nSpec;
+    const double* y = x + base;
+    double radius, volume, mass, moles;
+    particle_sums(a, y, &radius, &volume, &mass, &moles);
+    const double avgMW = mass / moles;
+    for (int s = 0; s < a->nSimpol; ++s) {
+      double c4[4];
+      oracle_simpol_constants(a->par + 8 * s, T, p, c4);
+      const double D = a->par[8 * s + 4], alpha = c4[0], Keq = c4[3];
+      const double Kn = c4[1] / radius;
+      const double den = Kn * Kn + (1.0 + 0.283 * alpha) * Kn + 0.75 * alpha;
+      const double f = fuchs_sutugin(Kn, alpha);
+      const double fp = 0.75 * alpha * (den - (1.0 + Kn) * (2.0 * Kn + 1.0 + 0.283 * alpha)) / (den * den);
+      const double kc = 4.0 * SP_PI * radius * D * f;
+      const double ke = kc * 1.0 * (Keq * avgMW / mass);
+      const double dkc_dV = 4.0 * SP_PI * D * (f - Kn * fp) * (radius / (3.0 * volume));
+      for (int i = 0; i < a->nSpec; ++i) {
+        dkc[i] = dkc_dV / a->density[i];
+        dke[i] = Keq * dkc[i] / moles - ke / (moles * a->mw[i]);
+      }
+      const int g = a->gas[s], ai = base + a->aero[s];
+      const double xg = x[g], ya = x[ai];
+      /* d diff / d x_g = kc */
+      J[(size_t)g * mt + g] += -nc[ip] * kc;
+      J[(size_t)ai * mt + g] += kc / c4[2];
+      for (int i = 0; i < a->nSpec; ++i) {
+        double dd = xg * dkc[i] - ya * dke[i];
+        if (base + i == ai) dd -= ke;
+        J[(size_t)g * mt + base + i] += -nc[ip] * dd;
+        J[(size_t)ai * mt + base + i] += dd / c4[2];
+      }
+    }
+  }
+  free(c);
+}
+
+static void aerosol_split_row(const oracle_kmd* k, const oracle_amd* a, const double* row, double* x) {
+  const int mg = k->nSpec - k->nConstSpec;
+  for (int i = 0; i < mg; ++i) x[i] = row[3 + i];
+  for (int i = 0; i < a->nSpec * a->nParticles; ++i) x[mg + i] = row[3 + k->nSpec + i];
+}
+
+void oracle_aerosol_rhs_batch(const oracle_kmd* k, const oracle_amd* a, int nBatch, const double* state,
+                              const double* nc, double* omega) {
+  const int mg = k->nSpec - k->nConstSpec, mt = mg + a->nSpec * a->nParticles;
+  const int dim = 3 + k->nSpec + a->nSpec * a->nParticles;
+  double* kfor = (double*)malloc(sizeof(double) * ((size_t)k->nReac + 1));
+  double* x = (double*)malloc(sizeof(double) * ((size_t)mt + 1));
+  for (int c = 0; c < nBatch; ++c) {
+    const double* row = state + (size_t)c * dim;
+    oracle_kforward(k, row[2], row[1], NULL, kfor);
+    aerosol_split_row(k, a, row, x);
+    oracle_aerosol_omega(k, a, row[2], row[1], kfor, row + 3 + mg, nc + (size_t)c * a->nParticles, x,
+                         omega + (size_t)c * mt);
+  }
+  free(kfor);
+  free(x);
+}
+
+void oracle_aerosol_run_batch(const oracle_kmd* k, const oracle_amd* a, int nBatch, const double* tol_newton,
+                              const double* tol_time, const oracle_time_advance* tadv, double* state,
+                              const double* nc, double* t_out, double* dt_out, oracle_stats* stats, int nthreads) {
+  const int mg = k->nSpec - k->nConstSpec, mt = mg + a->nSpec * a->nParticles;
+  const int dim = 3 + k->nSpec + a->nSpec * a->nParticles;
+  const size_t wlen = (size_t)k->nReac + k->nSpec + 2 * (size_t)mt * mt + 12 * (size_t)mt + 1;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+  {
+    double* work = (double*)malloc(sizeof(double) * wlen);
+    int* iwork = (int*)malloc(sizeof(int) * ((size_t)mt + 1));
+#pragma omp for schedule(static)
+    for (int c = 0; c < nBatch; ++c) {
+      integrate_cell(k, tol_newton, tol_time, &tadv[c], state + (size_t)c * dim, NULL, &t_out[c], &dt_out[c],
+                     stats ? &stats[c] : NULL, 0, work, iwork, NULL, NULL, 0, NULL, a,
+                     nc + (size_t)c * a->nParticles);
+    }
+    free(work);
+    free(iwork);
+  }
 }

@@ -112,6 +112,39 @@ int oracle_tlm_fixed_steps(const oracle_kmd* k, const double* tol_newton, int ma
                            const double* row, const double* photo, const double* lscale, const double* h, int nsteps,
                            int nsens, const int* sens, double* u_out, double* S_out);
 
+/* ---- single-particle SIMPOL.1 aerosol model (AerosolModelData) ----
+ * Unknowns of a cell: [active gas species (mg), particle 0 species (nSpec),
+ * particle 1 species, ...]; state rows [nSpec_gas + 3 + nSpec * nParticles]:
+ * density, pressure, temperature, gas species, particle species. */
+typedef struct {
+  int nSpec, nParticles, nSimpol;
+  const double* mw;       /* [nSpec] kg mol-1 */
+  const double* density;  /* [nSpec] kg m-3 */
+  const int* gas;         /* [nSimpol] gas species (kinetic model index, active) */
+  const int* aero;        /* [nSimpol] aerosol species index within a particle */
+  const double* par;      /* [nSimpol][8] B1 B2 B3 B4 diffusion mw Nstar computeAlpha */
+} oracle_amd;
+
+/* alpha, mean free path [m], kg m-3 -> ppm factor and equilibrium constant
+ * of SIMPOL transfer s at T, p */
+void oracle_simpol_constants(const double* par8, double T, double p, double* out4);
+
+/* omega[mt] = gas-phase source term + SIMPOL transfer at unknowns x[mt];
+ * cconst: constant gas species; nc: particle number concentrations */
+void oracle_aerosol_omega(const oracle_kmd* k, const oracle_amd* a, double T, double p, const double* kfor,
+                          const double* cconst, const double* nc, const double* x, double* omega);
+/* analytic J[mt][mt] = d omega / d x */
+void oracle_aerosol_jacobian(const oracle_kmd* k, const oracle_amd* a, double T, double p, const double* kfor,
+                             const double* cconst, const double* nc, const double* x, double* J);
+/* batched omega over state rows; nc [nBatch][nParticles]; omega [nBatch][mt] */
+void oracle_aerosol_rhs_batch(const oracle_kmd* k, const oracle_amd* a, int nBatch, const double* state,
+                              const double* nc, double* omega);
+/* One AerosolChemistry::runDeviceBatch interval (analytic Jacobian);
+ * tol_time [mt][2]; other arguments as oracle_run_batch. */
+void oracle_aerosol_run_batch(const oracle_kmd* k, const oracle_amd* a, int nBatch, const double* tol_newton,
+                              const double* tol_time, const oracle_time_advance* tadv, double* state,
+                              const double* nc, double* t_out, double* dt_out, oracle_stats* stats, int nthreads);
+
 /* TR-BDF2 constants used by the oracle (for tests) */
 void oracle_trbdf2_constants(double* out6);
 

@@ -40,9 +40,30 @@ FILES = {
     "CMAQ_H2O2_results.txt": "src/examples/runs/atmospheric_chemistry/CMAQ_H2O2/CMAQ_H2O2_results.txt",
     "emission_results.txt": "src/examples/runs/atmospheric_chemistry/Emission/emission_results.txt",
     "emission_Emissions.dat": "src/examples/runs/atmospheric_chemistry/Emission/Emissions.dat",
+    "e3sm_reaction_rates.dat": "src/unit-tests/references/reaction_rates/reaction-rates.dat",
+    "e3sm_rate_of_progress.dat": "src/unit-tests/references/rate_of_progress/rate-of-progress.dat",
+    "e3sm_net_production_rates.dat": "src/unit-tests/references/net_production_rates/net-production-rates.dat",
+    "uci_col_net_production_rates.dat": "src/unit-tests/references/net_production_rates/uci_col/net_production_rates.dat",
+    "uci_col_input_conditions.yaml": "src/examples/runs/atmospheric_chemistry/uci_col/input_conditions_col.yaml",
+    # single-particle SIMPOL.1 aerosol (inputs and expected outputs)
+    "simpol_gas.yaml": "src/examples/runs/atmospheric_chemistry/Simpol/config_gas.yaml",
+    "simpol_aero.yaml": "src/examples/runs/atmospheric_chemistry/Simpol/test_SIMPOL_phase_transfer.yaml",
+    "simpol_particles.yaml": "src/examples/runs/atmospheric_chemistry/Simpol/scenario_conditions_particle.yaml",
+    "simpol_rhs_host.txt": "src/unit-tests/references/partmc_integration/simpol/Simpol_RHS_HOST.txt",
+    "simpol_camp_results.txt": "src/examples/runs/atmospheric_chemistry/Simpol/camp_out/SIMPOL_phase_transfer_results.txt",
+    "cb05_soa_gas.yaml": "src/examples/runs/atmospheric_chemistry/CB05CL_AE5_w_simpolSOA/config_full_gas.yaml",
+    "cb05_soa_aero.yaml": "src/examples/runs/atmospheric_chemistry/CB05CL_AE5_w_simpolSOA/mechanism_aero.yaml",
+    "cb05_soa_particles.yaml": "src/examples/runs/atmospheric_chemistry/CB05CL_AE5_w_simpolSOA/scenario_conditions_particle.yaml",
+    "driver_config_solver.yaml": "src/unit-tests/TChem_Driver/config_solver.yaml",
+    "driver_output_ref.txt": "src/unit-tests/TChem_Driver/references/output_ref.txt",
+    "driver_gas_species_names_ref.txt": "src/unit-tests/TChem_Driver/references/gas_species_names_ref.txt",
+    "driver_aero_species_props_ref.txt": "src/unit-tests/TChem_Driver/references/aero_species_props_ref.txt",
 }
 
-H5 = {"cb05cl_ae5_full_gas.npy": "src/unit-tests/CB05CL_AE5/references/full_gas.hdf5"}
+H5 = {"cb05cl_ae5_full_gas.npy": "src/unit-tests/CB05CL_AE5/references/full_gas.hdf5",
+      "simpol_full_gas.npy": "src/unit-tests/TinesODE/references/full_gas.hdf5",
+      "simpol_batched_full_gas.npy": "src/unit-tests/TinesODE/references/batched_full_gas.hdf5",
+      "cb05_soa_batched_full_gas.npy": "src/unit-tests/TinesODE/references/batched_cb_full_gas.hdf5"}
 
 
 def main():
