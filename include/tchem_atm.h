@@ -32,6 +32,8 @@ extern "C" {
 typedef struct tchem_kmd_s* tchem_kmd_t;   /* host kinetic model data */
 typedef struct tchem_kmcd_s* tchem_kmcd_t; /* device-resident constant data */
 typedef struct tchem_graph_s* tchem_graph_t; /* captured launch sequence (HIP graph) */
+typedef struct tchem_amd_s* tchem_amd_t;   /* host aerosol model data */
+typedef struct tchem_amcd_s* tchem_amcd_t; /* device-resident aerosol constant data */
 
 /* Byte layout of TChem::TimeAdvance (src/core/TChem_Util.hpp:469-476). */
 typedef struct {
@@ -303,6 +305,60 @@ int tchem_adjoint_backward(int nBatch, const double* lambda, double* lambda_out,
  * overflowed) and, when h is not NULL, step sizes h[nBatch][maxSteps]. */
 int tchem_adjoint_steps(int nBatch, const void* workspace, int maxSteps, int* nsteps, double* h, tchem_kmcd_t kmcd,
                         void* stream);
+
+
+/* ---- single-particle SIMPOL.1 aerosol model ------------------------------ */
+/* Unknowns of a cell: the active gas species, then nParticles particles of
+ * nSpec aerosol species each.  State rows [nSpec_gas + 3 + nSpec * nParticles]:
+ * density, pressure, temperature, gas species (active then constant),
+ * particle species (particle-major), kg m-3 per particle. */
+
+/* Replaces AerosolModelData(mechfile, kmd) (src/core/TChem_AerosolModelData.cpp:30-249):
+ * aerosol species, particle count and SIMPOL_PHASE_TRANSFER reactions of an
+ * NCAR-format aerosol file, against the gas model kmd. */
+int tchem_amd_load_yaml(const char* path, tchem_kmd_t kmd, tchem_amd_t* out);
+int tchem_amd_destroy(tchem_amd_t amd);
+/* nSpec_, nParticles_, nSimpol_tran_ (AerosolModelData.hpp:31-41); any may be NULL */
+int tchem_amd_info(tchem_amd_t amd, int* nSpec, int* nParticles, int* nSimpol);
+/* Replaces AerosolModelData::setNumberofParticles (.cpp:251-260) */
+int tchem_amd_set_number_of_particles(tchem_amd_t amd, int nParticles);
+/* aerosol species name and (density, molecular weight, kappa) of index i
+ * (TChem_Driver.cpp: getAerosolSpeciesName / Density / MW / Kappa) */
+int tchem_amd_species_name(tchem_amd_t amd, int i, char* buf, size_t len);
+int tchem_amd_species_props(tchem_amd_t amd, int i, double* density, double* mw, double* kappa);
+/* Replaces AerosolModelData::scenarioConditionParticles (.cpp:263-300):
+ * number_conc_host [nBatch][nParticles] written; particle species of
+ * state_host rows [nBatch][stateDim] set to initial_value / number
+ * concentration.  *found = 0 (nothing written) when the file has no
+ * `particles` block. */
+int tchem_scenario_particles(tchem_amd_t amd, const char* path, int nBatch, int stateDim, double* state_host,
+                             double* number_conc_host, int* found);
+
+/* Replaces create_AerosolModelConstData<device>(amd) (AerosolModelData.hpp:94-108)
+ * for use with the gas model's kmcd, which must outlive it.  At most 128
+ * unknowns per cell. */
+int tchem_amcd_create(tchem_amd_t amd, tchem_kmcd_t kmcd, tchem_amcd_t* out);
+int tchem_amcd_destroy(tchem_amcd_t amcd);
+
+/* Batched aerosol source term (src/core/impl/TChem_Impl_Aerosol_RHS.hpp:64-103;
+ * the reference's AerosolChemistry_RHSs example), FP64:
+ *   state [device] [nBatch][stateDim], number_conc [device] [nBatch][nParticles]
+ *   omega [device] [nBatch][mt] in the unknown order above. */
+int tchem_aerosol_rhs_run_device_batch(int nBatch, const double* state, const double* number_conc, double* omega,
+                                       tchem_amcd_t amcd, void* stream);
+
+/* Replaces AerosolChemistry::runDeviceBatch(policy, tol_newton, tol_time, fac,
+ * tadv, state, number_conc, t_out, dt_out, state_out, kmcd, amcd)
+ * (src/core/TChem_AerosolChemistry.hpp, .cpp:31-258): TR-BDF2 over the gas
+ * and particle unknowns of every cell, FP64, dense LU with partial pivoting,
+ * analytic Jacobian.  Arguments as tchem_atmchem_run_device_batch with
+ * tol_time [mt][2] and state rows of stateDim; opts NULL or FP64 precision
+ * (other fields ignored). */
+int tchem_aerosolchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBatch, const double* tol_newton,
+                                       const double* tol_time, double* fac, const tchem_time_advance_t* tadv,
+                                       double* state, const double* number_conc, double* t_out, double* dt_out,
+                                       double* state_out, tchem_cell_stats_t* stats, tchem_amcd_t amcd,
+                                       void* stream);
 
 #ifdef __cplusplus
 }

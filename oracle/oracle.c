@@ -317,6 +317,7 @@ static void integrate_cell(const oracle_kmd* k, const double* tol_newton, const 
   const double g = g6[0], c_tr = g6[1], a_g = g6[2], a_n = g6[3], c_bdf = g6[4], k_err = g6[5];
 
   cell_ctx x;
+  memset(&x, 0, sizeof x);
   x.k = k;
   x.a = am;
   x.nc = nc;
@@ -542,7 +543,9 @@ int oracle_tlm_fixed_steps(const oracle_kmd* k, const double* tol_newton, int ma
   oracle_trbdf2_constants(g6);
   const double c_tr = g6[1], a_g = g6[2], a_n = g6[3], c_bdf = g6[4];
   cell_ctx x;
+  memset(&x, 0, sizeof x); /* gas-phase cell: no aerosol model */
   x.k = k;
+  x.mg = m;
   x.m = m;
   x.mode = 0;
   x.p = row[1];

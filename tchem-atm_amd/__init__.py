@@ -68,7 +68,10 @@ EXPORTED_SYMBOLS = (
     "tchem_graph_launch", "tchem_graph_status", "tchem_graph_destroy", "tchem_atmchem_autotune",
     "tchem_reactionrates_run_device_batch", "tchem_rateofprogress_run_device_batch",
     "tchem_netproductionrates_run_device_batch", "tchem_scenario_photolysis_rates", "tchem_scenario_external_forcing",
-    "tchem_atmchem_e3sm_run_device_batch",
+    "tchem_atmchem_e3sm_run_device_batch", "tchem_amd_load_yaml", "tchem_amd_destroy", "tchem_amd_info",
+    "tchem_amd_set_number_of_particles", "tchem_amd_species_name", "tchem_amd_species_props",
+    "tchem_scenario_particles", "tchem_amcd_create", "tchem_amcd_destroy", "tchem_aerosol_rhs_run_device_batch",
+    "tchem_aerosolchem_run_device_batch",
 )
 
 
@@ -121,6 +124,19 @@ def _load():
         "tchem_scenario_external_forcing": [vp, cp, i, vp, ctypes.POINTER(i)],
         "tchem_atmchem_e3sm_run_device_batch": [ctypes.POINTER(_Options), i, vp, vp, vp, vp, vp, vp, i, vp, vp, vp,
                                                 vp, vp, vp, vp],
+        "tchem_amd_load_yaml": [cp, vp, ctypes.POINTER(vp)],
+        "tchem_amd_destroy": [vp],
+        "tchem_amd_info": [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)],
+        "tchem_amd_set_number_of_particles": [vp, i],
+        "tchem_amd_species_name": [vp, i, cp, ctypes.c_size_t],
+        "tchem_amd_species_props": [vp, i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double)],
+        "tchem_scenario_particles": [vp, cp, i, i, vp, vp, ctypes.POINTER(i)],
+        "tchem_amcd_create": [vp, vp, ctypes.POINTER(vp)],
+        "tchem_amcd_destroy": [vp],
+        "tchem_aerosol_rhs_run_device_batch": [i, vp, vp, vp, vp, vp],
+        "tchem_aerosolchem_run_device_batch": [ctypes.POINTER(_Options), i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                               vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -463,6 +479,115 @@ class Adjoint:
                                         _ptr(ns, None, "nsteps"), _ptr(h, "float64", "h") if with_h else None,
                                         kmcd._h, _stream_ptr(stream)))
         return ns[:nBatch], (h[:nBatch] if with_h else None)
+
+
+class AerosolModelData:
+    """Single-particle SIMPOL.1 aerosol model read against a gas model
+    (AerosolModelData, src/core/TChem_AerosolModelData.cpp:30-249)."""
+
+    def __init__(self, mechfile: str, kmd: KineticModelData):
+        h = ctypes.c_void_p()
+        _check(_lib.tchem_amd_load_yaml(mechfile.encode(), kmd._h, ctypes.byref(h)))
+        self._h = h
+        self.kmd = kmd
+        self._refresh()
+
+    def _refresh(self):
+        n = [ctypes.c_int() for _ in range(3)]
+        _check(_lib.tchem_amd_info(self._h, *[ctypes.byref(x) for x in n]))
+        self.nSpec, self.nParticles, self.nSimpol = (x.value for x in n)
+        buf = ctypes.create_string_buffer(128)
+        self.speciesNames, self.density, self.molecularWeights, self.kappa = [], [], [], []
+        d, w, k = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        for i in range(self.nSpec):
+            _check(_lib.tchem_amd_species_name(self._h, i, buf, 128))
+            self.speciesNames.append(buf.value.decode())
+            _check(_lib.tchem_amd_species_props(self._h, i, ctypes.byref(d), ctypes.byref(w), ctypes.byref(k)))
+            self.density.append(d.value)
+            self.molecularWeights.append(w.value)
+            self.kappa.append(k.value)
+
+    def setNumberofParticles(self, n: int):
+        _check(_lib.tchem_amd_set_number_of_particles(self._h, int(n)))
+        self._refresh()
+
+    @property
+    def nUnknowns(self) -> int:
+        """Active gas species + particle species (getNumberOfTimeODEs)."""
+        return self.kmd.nActive + self.nSpec * self.nParticles
+
+    @property
+    def stateDim(self) -> int:
+        return 3 + self.kmd.nSpec + self.nSpec * self.nParticles
+
+    def scenarioConditionParticles(self, filename: str, state: np.ndarray):
+        """Particle block of a scenario file: writes particle species into the
+        host state rows (in place) and returns number concentrations
+        [nBatch][nParticles], or None when the file has no particle block."""
+        assert state.dtype == np.float64 and state.flags.c_contiguous and state.shape[1] == self.stateDim
+        nc = np.zeros((state.shape[0], max(self.nParticles, 1)))
+        found = ctypes.c_int()
+        _check(_lib.tchem_scenario_particles(self._h, filename.encode(), state.shape[0], self.stateDim,
+                                             state.ctypes.data, nc.ctypes.data, ctypes.byref(found)))
+        return nc[:, : self.nParticles].copy() if found.value else None
+
+    def createConstData(self, kmcd: KineticModelConstData) -> "AerosolModelConstData":
+        return AerosolModelConstData(self, kmcd)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.tchem_amd_destroy(h)
+            self._h = None
+
+
+class AerosolModelConstData:
+    """Device tables of an aerosol model, bound to the gas model's kmcd
+    (create_AerosolModelConstData)."""
+
+    def __init__(self, amd: AerosolModelData, kmcd: KineticModelConstData):
+        h = ctypes.c_void_p()
+        _check(_lib.tchem_amcd_create(amd._h, kmcd._h, ctypes.byref(h)))
+        self._h = h
+        self.kmcd = kmcd  # must outlive this object
+        self.nUnknowns, self.stateDim, self.nParticles = amd.nUnknowns, amd.stateDim, amd.nParticles
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.tchem_amcd_destroy(h)
+            self._h = None
+
+
+def create_AerosolModelConstData(amd: AerosolModelData, kmcd: KineticModelConstData) -> AerosolModelConstData:
+    return AerosolModelConstData(amd, kmcd)
+
+
+class AerosolChemistry:
+    """Box model with single-particle SIMPOL.1 partitioning
+    (AerosolChemistry::runDeviceBatch, src/core/TChem_AerosolChemistry.cpp:31-258)."""
+
+    @staticmethod
+    def runDeviceBatch(tol_newton, tol_time, fac, tadv, state, number_conc, t_out, dt_out, state_out, amcd,
+                       stats=None, stream=None):
+        nBatch = state.shape[0]
+        opts = _Options(PRECISION_FP64, SCHEDULE_GRID, 0, LU_DENSE, 0, PROGRAMS_AUTO)
+        _check(_lib.tchem_aerosolchem_run_device_batch(
+            ctypes.byref(opts), nBatch, _ptr(tol_newton, what="tol_newton"), _ptr(tol_time, what="tol_time"),
+            _ptr(fac, None, "fac"), _ptr(tadv, None, "tadv"), _ptr(state, what="state"),
+            _ptr(number_conc, what="number_conc"), _ptr(t_out, what="t_out"), _ptr(dt_out, what="dt_out"),
+            _ptr(state_out, what="state_out"), _ptr(stats, None, "stats"), amcd._h, _stream_ptr(stream)))
+
+
+class AerosolRHS:
+    """Batched gas + SIMPOL source term (Aerosol_RHS; the reference's
+    AerosolChemistry_RHSs example)."""
+
+    @staticmethod
+    def runDeviceBatch(state, number_conc, omega, amcd, stream=None):
+        _check(_lib.tchem_aerosol_rhs_run_device_batch(
+            state.shape[0], _ptr(state, what="state"), _ptr(number_conc, what="number_conc"),
+            _ptr(omega, what="omega"), amcd._h, _stream_ptr(stream)))
 
 
 def time_advance_tensor(n, tbeg, tend, dt, dtmin, dtmax, max_newton, iters_per_interval, jacobian_interval=1,

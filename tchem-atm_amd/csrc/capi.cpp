@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/tchem_atm.h"
+#include "aerosol_model.hpp"
 #include "device_mech.hpp"
 #include "kinetic_model.hpp"
 #include "mechanism.hpp"
@@ -25,7 +26,10 @@ int sparseMaxCellsPerGroup(int precision, int m);
 size_t integratorStaticLds(int precision, int sparse, int m);
 hipError_t launchResetCounter(int* counter, hipStream_t stream);
 hipError_t launchRates(const DevMech& M, int nBatch, const double* state, const double* photo, const double* lscale,
-                       int lstride, double* kf, const int* skip, hipStream_t stream);
+                       int lstride, double* kf, const int* skip, hipStream_t stream, int rowDim = 0);
+int aerosolCellsPerGroup(const DevMech& M, const DevAerosol& X, size_t maxLds);
+hipError_t launchAerosol(const IntegratorArgs& a, int grid, size_t lds, hipStream_t stream);
+hipError_t launchAerosolRhs(const IntegratorArgs& a, size_t lds, hipStream_t stream);
 hipError_t launchLoopControl(int n, const double* t_out, const tchem_time_advance_t* tadv, double fraction, int* ctl,
                              bool reset, hipStream_t stream);
 size_t adjointCellBytes(const DevMech& M, const DevSparse& S);
@@ -68,6 +72,25 @@ struct tchem_kmcd_s {
   size_t kfCapacity = 0;  // cells
   ~tchem_kmcd_s() {
     if (kf) allocations.push_back(kf);
+    int cur = 0;
+    if (hipGetDevice(&cur) == hipSuccess && cur != device) (void)hipSetDevice(device);
+    for (void* p : allocations) (void)hipFree(p);
+    if (cur != device) (void)hipSetDevice(cur);
+  }
+};
+
+struct tchem_amd_s {
+  AerosolModel am;
+  KineticModel km;  // the gas model it was read against (names, state layout)
+};
+
+struct tchem_amcd_s {
+  tchem_kmcd_s* kmcd = nullptr;  // not owned
+  DevAerosol X{};
+  int stateDim = 0;
+  std::vector<void*> allocations;
+  int device = 0;
+  ~tchem_amcd_s() {
     int cur = 0;
     if (hipGetDevice(&cur) == hipSuccess && cur != device) (void)hipSetDevice(device);
     for (void* p : allocations) (void)hipFree(p);
@@ -1209,3 +1232,206 @@ int tchem_netproductionrates_run_device_batch(int nBatch, const double* state, c
 }
 
 }  // extern "C"
+
+// ---- single-particle SIMPOL.1 aerosol model ----------------------------------
+
+int tchem_amd_load_yaml(const char* path, tchem_kmd_t kmd, tchem_amd_t* out) {
+  if (!path || !kmd || !out) return setError(TCHEM_ERR_INVALID, "tchem_amd_load_yaml: null argument");
+  *out = nullptr;
+  try {
+    std::unique_ptr<tchem_amd_s> h(new tchem_amd_s);
+    h->am = loadAerosolModel(path, kmd->km);
+    h->km = kmd->km;
+    *out = h.release();
+    return TCHEM_OK;
+  } catch (const std::exception& e) {
+    return setError(TCHEM_ERR_PARSE, e.what());
+  }
+}
+
+int tchem_amd_destroy(tchem_amd_t amd) {
+  delete amd;
+  return TCHEM_OK;
+}
+
+int tchem_amd_info(tchem_amd_t amd, int* nSpec, int* nParticles, int* nSimpol) {
+  if (!amd) return setError(TCHEM_ERR_INVALID, "tchem_amd_info: null handle");
+  if (nSpec) *nSpec = amd->am.nSpec;
+  if (nParticles) *nParticles = amd->am.nParticles;
+  if (nSimpol) *nSimpol = int(amd->am.simpol.size());
+  return TCHEM_OK;
+}
+
+int tchem_amd_set_number_of_particles(tchem_amd_t amd, int nParticles) {
+  if (!amd || nParticles < 0) return setError(TCHEM_ERR_INVALID, "tchem_amd_set_number_of_particles: bad argument");
+  amd->am.nParticles = nParticles;
+  return TCHEM_OK;
+}
+
+int tchem_amd_species_name(tchem_amd_t amd, int i, char* buf, size_t len) {
+  if (!amd || !buf || len == 0) return setError(TCHEM_ERR_INVALID, "tchem_amd_species_name: bad argument");
+  if (i < 0 || i >= amd->am.nSpec) return setError(TCHEM_ERR_INVALID, "tchem_amd_species_name: index out of range");
+  std::strncpy(buf, amd->am.names[i].c_str(), len - 1);
+  buf[len - 1] = '\0';
+  return TCHEM_OK;
+}
+
+int tchem_amd_species_props(tchem_amd_t amd, int i, double* density, double* mw, double* kappa) {
+  if (!amd) return setError(TCHEM_ERR_INVALID, "tchem_amd_species_props: null handle");
+  if (i < 0 || i >= amd->am.nSpec) return setError(TCHEM_ERR_INVALID, "tchem_amd_species_props: index out of range");
+  if (density) *density = amd->am.density[i];
+  if (mw) *mw = amd->am.mw[i];
+  if (kappa) *kappa = amd->am.kappa[i];
+  return TCHEM_OK;
+}
+
+int tchem_scenario_particles(tchem_amd_t amd, const char* path, int nBatch, int stateDim, double* state_host,
+                             double* number_conc_host, int* found) {
+  const char* who = "tchem_scenario_particles";
+  if (!amd || !path || nBatch < 0 || !found) return setError(TCHEM_ERR_INVALID, std::string(who) + ": bad argument");
+  if (stateDim != amd->am.stateDim(amd->km))
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": stateDim differs from the model's state length");
+  if (nBatch > 0 && (!state_host || (amd->am.nParticles > 0 && !number_conc_host)))
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": null host array");
+  try {
+    std::vector<double> st(state_host, state_host + size_t(nBatch) * size_t(stateDim));
+    std::vector<double> nc;
+    *found = loadScenarioParticles(path, amd->am, amd->km, nBatch, st, nc) ? 1 : 0;
+    if (*found) {
+      std::copy(st.begin(), st.end(), state_host);
+      std::copy(nc.begin(), nc.end(), number_conc_host);
+    }
+    return TCHEM_OK;
+  } catch (const std::exception& e) {
+    return setError(TCHEM_ERR_PARSE, e.what());
+  }
+}
+
+int tchem_amcd_create(tchem_amd_t amd, tchem_kmcd_t kmcd, tchem_amcd_t* out) {
+  const char* who = "tchem_amcd_create";
+  if (!amd || !kmcd || !out) return setError(TCHEM_ERR_INVALID, std::string(who) + ": null argument");
+  *out = nullptr;
+  if (!kmcd->cb05Kinetics)
+    return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": gas mechanism: " + kmcd->boxError);
+  const AerosolModel& am = amd->am;
+  const MechTables& t = kmcd->tables;
+  if (am.nGasActive != t.m || amd->km.nSpec != t.nSpec)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": the aerosol model was read against another gas model");
+  const int mt = t.m + am.nSpec * am.nParticles;
+  if (mt > 128) return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": more than 128 unknowns per cell");
+  std::unique_ptr<tchem_amcd_s> h(new tchem_amcd_s);
+  h->kmcd = kmcd;
+  h->device = kmcd->device;
+  h->stateDim = am.stateDim(amd->km);
+  DevAerosol& X = h->X;
+  X.nAero = am.nSpec;
+  X.nPart = am.nParticles;
+  X.nSimpol = int(am.simpol.size());
+  X.mt = mt;
+  std::vector<int> ib;
+  std::vector<double> db(am.mw.begin(), am.mw.end());
+  X.oMw = 0;
+  X.oDens = int(db.size());
+  db.insert(db.end(), am.density.begin(), am.density.end());
+  X.oPar = int(db.size());
+  for (const SimpolParam& s : am.simpol) {
+    ib.push_back(t.iperm[s.gas]);  // gas unknowns are in the internal order
+    const double par[8] = {s.B1, s.B2, s.B3, s.B4, s.diffusion, s.mw, s.Nstar, double(s.computeAlpha)};
+    db.insert(db.end(), par, par + 8);
+  }
+  X.oGas = 0;
+  X.oAero = int(ib.size());
+  for (const SimpolParam& s : am.simpol) ib.push_back(s.aero);
+  ib.push_back(0);
+  db.push_back(0.0);
+  void* pi = nullptr;
+  void* pd = nullptr;
+  hipError_t e = hipMalloc(&pi, sizeof(int) * ib.size());
+  if (e != hipSuccess) return hipFail(e, "hipMalloc");
+  h->allocations.push_back(pi);
+  e = hipMalloc(&pd, sizeof(double) * db.size());
+  if (e != hipSuccess) return hipFail(e, "hipMalloc");
+  h->allocations.push_back(pd);
+  if ((e = hipMemcpy(pi, ib.data(), sizeof(int) * ib.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(pd, db.data(), sizeof(double) * db.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    return hipFail(e, "hipMemcpy");
+  X.ib = static_cast<const int*>(pi);
+  X.db = static_cast<const double*>(pd);
+  if (aerosolCellsPerGroup(kmcd->dev, X, kmcd->maxLds) < 1)
+    return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": one cell does not fit the LDS");
+  *out = h.release();
+  return TCHEM_OK;
+}
+
+int tchem_amcd_destroy(tchem_amcd_t amcd) {
+  delete amcd;
+  return TCHEM_OK;
+}
+
+namespace {
+IntegratorArgs aerosolArgs(tchem_amcd_s* h, int nBatch) {
+  IntegratorArgs a{};
+  a.M = h->kmcd->dev;
+  a.S = h->kmcd->sparse;
+  a.X = h->X;
+  a.nBatch = nBatch;
+  a.cellBytes = int((aerosolLdsBytes(a.M, a.X) + 15) & ~size_t(15));
+  return a;
+}
+}  // namespace
+
+int tchem_aerosol_rhs_run_device_batch(int nBatch, const double* state, const double* number_conc, double* omega,
+                                       tchem_amcd_t amcd, void* stream) {
+  const char* who = "aerosol_rhs_run_device_batch";
+  if (!amcd || nBatch < 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": bad argument");
+  if (nBatch == 0) return TCHEM_OK;
+  if (!state || !omega || (amcd->X.nPart > 0 && !number_conc))
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": null device array");
+  IntegratorArgs a = aerosolArgs(amcd, nBatch);
+  a.state = const_cast<double*>(state);
+  a.numberConc = number_conc;
+  a.state_out = omega;  // the kernel's output rows [nBatch][mt]
+  hipError_t e = launchAerosolRhs(a, size_t(a.cellBytes), static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hipFail(e, "aerosol_rhs_kernel launch");
+  return TCHEM_OK;
+}
+
+int tchem_aerosolchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBatch, const double* tol_newton,
+                                       const double* tol_time, double* fac, const tchem_time_advance_t* tadv,
+                                       double* state, const double* number_conc, double* t_out, double* dt_out,
+                                       double* state_out, tchem_cell_stats_t* stats, tchem_amcd_t amcd,
+                                       void* stream) {
+  (void)fac;
+  const char* who = "aerosolchem_run_device_batch";
+  if (!amcd || nBatch < 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": bad argument");
+  if (opts && opts->precision != TCHEM_PRECISION_FP64)
+    return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": the aerosol integrator is FP64 only");
+  if (nBatch == 0) return TCHEM_OK;
+  if (!tol_newton || !tol_time || !tadv || !state || !t_out || !dt_out || !state_out ||
+      (amcd->X.nPart > 0 && !number_conc))
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": null device array");
+  tchem_kmcd_s* k = amcd->kmcd;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int rc = ensureKf(k, nBatch, s);
+  if (rc) return rc;
+  hipError_t e = launchRates(k->dev, nBatch, state, nullptr, nullptr, 0, k->kf, nullptr, s, amcd->stateDim);
+  if (e != hipSuccess) return hipFail(e, "rates_kernel launch");
+  IntegratorArgs a = aerosolArgs(amcd, nBatch);
+  a.tol_newton = tol_newton;
+  a.tol_time = tol_time;
+  a.tadv = tadv;
+  a.state = state;
+  a.t_out = t_out;
+  a.dt_out = dt_out;
+  a.state_out = state_out;
+  a.stats = stats;
+  a.kf = k->kf;
+  a.numberConc = number_conc;
+  const int cells = aerosolCellsPerGroup(a.M, a.X, k->maxLds);
+  const int perGroup = cells < nBatch ? cells : nBatch;
+  a.cellsPerGroup = perGroup;
+  const int grid = (nBatch + perGroup - 1) / perGroup;
+  e = launchAerosol(a, grid, size_t(perGroup) * size_t(a.cellBytes), s);
+  if (e != hipSuccess) return hipFail(e, "aerosol integrator launch");
+  return TCHEM_OK;
+}

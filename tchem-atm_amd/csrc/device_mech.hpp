@@ -91,6 +91,18 @@ struct AdjointRecord {
   int maxSteps;
 };
 
+// Single-particle SIMPOL.1 aerosol tables (aerosol_model.hpp).  Unknowns of
+// a cell: the m active gas species (internal order), then nPart particles of
+// nAero species each; state rows [nSpec + 3 + nAero * nPart].
+struct DevAerosol {
+  int nAero, nPart, nSimpol, mt;  // mt = m + nAero * nPart
+  const int* ib;
+  const double* db;
+  int oMw, oDens;   // [nAero] kg mol-1, kg m-3
+  int oGas, oAero;  // [nSimpol] gas unknown (internal order), aerosol species within a particle
+  int oPar;         // [nSimpol][8] B1 B2 B3 B4 diffusion mw Nstar computeAlpha
+};
+
 struct IntegratorArgs {
   DevMech M;
   DevSparse S;
@@ -113,7 +125,23 @@ struct IntegratorArgs {
   AdjointRecord rec;                 // h == nullptr: no record
   const int* skip;                   // nullable: the launch does nothing when *skip != 0 (captured interval loops)
   const double* ext;                 // nullable [nBatch][m]: per-cell external sources added to the source term
+  DevAerosol X;                      // aerosol integrator only
+  const double* numberConc;          // aerosol integrator only: [nBatch][nPart] particles m-3
 };
+
+// LDS bytes of one cell of the aerosol integrator: the gas cell region
+// (integratorLdsBytes) with an mt x mt FP64 matrix, plus the unknown copy,
+// the source term, reciprocal pivots and interchanges over mt, SIMPOL
+// constants [nSimpol][4], particle sums [nPart][4] and transfer terms
+// [nPart * nSimpol][6].
+inline size_t aerosolLdsBytes(const DevMech& M, const DevAerosol& X) {
+  const size_t nScr = size_t(M.nReac > M.nPairs ? M.nReac : M.nPairs);
+  size_t a = sizeof(double) * size_t(X.mt) * size_t(X.mt);
+  a = (a + 15) & ~size_t(15);
+  const size_t d = size_t(M.nSpec) + size_t(M.nReac) + nScr + size_t(M.nSeg) + 3 * size_t(X.mt) +
+                   4 * size_t(X.nSimpol) + 4 * size_t(X.nPart) + 6 * size_t(X.nPart) * size_t(X.nSimpol);
+  return a + sizeof(double) * d + sizeof(int) * 2 * size_t(X.mt);
+}
 
 // LDS bytes of one cell: the iteration matrix in TA precision (bytesA per
 // entry, 0 for kernels without a matrix), then doubles for concentrations,

@@ -844,6 +844,15 @@ struct DenseBackend {
   using TA = TA_;
   using Lds = CellLds<TA>;
   struct Fact { int nswp = 0; };
+  // unknowns, state row length, state column and tolerance row of unknown ai
+  static __device__ __forceinline__ int unknowns(const IntegratorArgs& a) { return a.M.m; }
+  static __device__ __forceinline__ int rowDim(const IntegratorArgs& a) { return a.M.nSpec + 3; }
+  static __device__ __forceinline__ int column(const IntegratorArgs& a, int ai) { return 3 + a.M.perm()[ai]; }
+  static __device__ __forceinline__ int tolIndex(const IntegratorArgs& a, int ai) { return a.M.perm()[ai]; }
+  // per-cell bindings: this cell's external sources
+  static __device__ __forceinline__ void bind(const IntegratorArgs& a, Lds& L, int cell) {
+    L.src = a.ext ? a.ext + size_t(cell) * a.M.m : nullptr;
+  }
   static __device__ void stage(const IntegratorArgs&, char*) {}
   static __device__ Lds carve(char* base, const IntegratorArgs& a, int) { return tchem_amd::carve<TA>(base, a.M); }
   static __device__ void init(const IntegratorArgs&, const Lds&, int) {}
@@ -876,6 +885,13 @@ struct SparseBackend {
   using TA = TA_;
   using Lds = SpLds<TA>;
   struct Fact {};
+  static __device__ __forceinline__ int unknowns(const IntegratorArgs& a) { return a.M.m; }
+  static __device__ __forceinline__ int rowDim(const IntegratorArgs& a) { return a.M.nSpec + 3; }
+  static __device__ __forceinline__ int column(const IntegratorArgs& a, int ai) { return 3 + a.M.perm()[ai]; }
+  static __device__ __forceinline__ int tolIndex(const IntegratorArgs& a, int ai) { return a.M.perm()[ai]; }
+  static __device__ __forceinline__ void bind(const IntegratorArgs& a, Lds& L, int cell) {
+    L.src = a.ext ? a.ext + size_t(cell) * a.M.m : nullptr;
+  }
   // the mechanism program (LU and BACK sections) is copied into the
   // workgroup's LDS once; every wavefront of the workgroup reads it
   static __device__ void stage(const IntegratorArgs& a, char* base) {
@@ -1005,9 +1021,9 @@ template <class B, int QM, bool REC>
 __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& Lw, int cell, int lane) {
   using TA = typename B::TA;
   const DevMech& M = a.M;
-  const int m = M.m, nSpec = M.nSpec, dim = nSpec + 3;
+  const int m = B::unknowns(a), mg = M.m, nSpec = M.nSpec, dim = B::rowDim(a);
   typename B::Lds L = Lw;
-  L.src = a.ext ? a.ext + size_t(cell) * m : nullptr;
+  B::bind(a, L, cell);
   const tchem_time_advance_t ta = a.tadv[cell];
   const double tend = ta._tend;
   double* row = a.state + size_t(cell) * dim;
@@ -1038,14 +1054,14 @@ __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L
     const double* kf = a.kf + size_t(cell) * M.nReac;
     for (int r = lane; r < M.nReac; r += kWave) L.kf[r] = kf[r];
   }
-  for (int i = m + lane; i < nSpec; i += kWave) L.c[i] = row[3 + i];
+  for (int i = mg + lane; i < nSpec; i += kWave) L.c[i] = row[3 + i];
 
   // un (state) and fn (f at un) live across steps; u, bc, est per step
   double un[QM], u[QM], fn[QM], bc[QM], est[QM];
 #pragma unroll
   for (int q = 0; q < QM; ++q) {
     const int ai = lane + kWave * q;
-    un[q] = (ai < m) ? row[3 + M.perm()[ai]] : 0.0;
+    un[q] = (ai < m) ? row[B::column(a, ai)] : 0.0;
     u[q] = un[q];
     fn[q] = bc[q] = est[q] = 0.0;
   }
@@ -1185,7 +1201,7 @@ __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L
     for (int q = 0; q < QM; ++q) {
       const int ai = lane + kWave * q;
       if (ai < m) {
-        const int xs = M.perm()[ai];
+        const int xs = B::tolIndex(a, ai);
         const double lte = 2.0 * k_err * h * (est[q] + f[q] / (1.0 - gam));
         double w = fabs(lte) / (a.tol_time[2 * xs] + a.tol_time[2 * xs + 1] * fabs(u[q]));
         if (!(w == w)) w = INFINITY;
@@ -1235,14 +1251,14 @@ __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L
   for (int q = 0; q < QM; ++q) {
     const int ai = lane + kWave * q;
     if (ai < m) {
-      const int xs = M.perm()[ai];
+      const int col = B::column(a, ai);
       const double v = failed ? 0.0 : un[q];
-      row[3 + xs] = v;
-      if (rowo != row) rowo[3 + xs] = v;
+      row[col] = v;
+      if (rowo != row) rowo[col] = v;
     }
   }
   if (rowo != row)
-    for (int i = m + lane; i < nSpec; i += kWave) rowo[3 + i] = row[3 + i];
+    for (int i = mg + lane; i < nSpec; i += kWave) rowo[3 + i] = row[3 + i];
   if (lane == 0) {
     rowo[0] = row[0];
     rowo[1] = row[1];
@@ -1256,6 +1272,315 @@ __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L
                                          cnt.factorizations, cnt.failed, t_start,
                                          (unsigned long long)__builtin_amdgcn_s_memrealtime()};
     }
+  }
+}
+
+
+// ---- single-particle SIMPOL.1 aerosol integrator ---------------------------
+// AerosolChemistry::runDeviceBatch (src/core/TChem_AerosolChemistry.cpp:31-186):
+// the gas-phase box model extended by gas-particle partitioning
+// (src/core/impl/TChem_Impl_Aerosol_RHS.hpp:64-103,
+// TChem_Impl_SIMPOL_phase_transfer.hpp:61-150).  One cell per wavefront,
+// FP64 dense LU with partial pivoting over the mt gas + particle unknowns
+// (the reference factors a dense Jacobian too); the Jacobian is analytic.
+
+constexpr double kSpRuniv = 8.314472;          // src/core/TChem_Util.hpp:176
+constexpr double kSpPi = 3.141592653589793;    // TChem_Util.hpp:212
+constexpr double kSpAtmPa = 101325.0;          // TChem_Util.hpp:227-229
+constexpr double kSpMinMass = 1.0e-25;         // TChem_Impl_SingleParticleUtils.hpp:25-30
+constexpr double kSpMinMw = 0.1;
+constexpr double kSpMinDensity = 1800.0;
+
+struct AeroLds : CellLds<double> {
+  double* x;    // [mt] unknowns (gas internal order, then particles)
+  double* f;    // [mt] source term
+  double* sc;   // [nSimpol][4] alpha, mean free path, kg m-3 -> ppm, partitioning constant
+  double* ps;   // [nPart][4] effective radius, volume, mass, moles
+  double* pd;   // [nPart * nSimpol][6] k_c, k_e, diff, d k_c / dV, (unused) x2
+  const double* nc;  // this cell's particle number concentrations
+  double T, p;
+};
+
+__device__ AeroLds carve_aero(char* base, const DevMech& M, const DevAerosol& X) {
+  AeroLds L;
+  const int mt = X.mt;
+  size_t off = sizeof(double) * size_t(mt) * size_t(mt);
+  off = (off + 15) & ~size_t(15);
+  L.A = reinterpret_cast<double*>(base);
+  L.c = reinterpret_cast<double*>(base + off);
+  L.kf = L.c + M.nSpec;
+  L.scr = L.kf + M.nReac;
+  const int nScr = M.nReac > M.nPairs ? M.nReac : M.nPairs;
+  L.seg = L.scr + nScr;
+  L.x = L.seg + M.nSeg;
+  L.f = L.x + mt;
+  L.rdiag = L.f + mt;
+  L.sc = L.rdiag + mt;
+  L.ps = L.sc + 4 * X.nSimpol;
+  L.pd = L.ps + 4 * X.nPart;
+  L.swp = reinterpret_cast<int*>(L.pd + 6 * X.nPart * X.nSimpol);
+  L.src = nullptr;
+  L.nc = nullptr;
+  L.T = L.p = 0.0;
+  return L;
+}
+
+// SIMPOL_constant.hpp:60-128 (same expression order as the reference)
+__device__ void simpol_constants(const double* par, double T, double p, double* out) {
+  const double B1 = par[0], B2 = par[1], B3 = par[2], B4 = par[3], D = par[4], MW = par[5], Nstar = par[6];
+  double alpha = 0.1;
+  if (par[7] != 0.0) {
+    const double N23 = pow(Nstar, 2.0 / 3.0);
+    double dh = -10.0 * (Nstar - 1.0) + 7.53 * (N23 - 1.0) - 1.0;
+    double ds = -32.0 * (Nstar - 1.0) + 9.21 * (N23 - 1.0) - 1.3;
+    dh = dh * 4184.0;
+    ds = ds * 4.184;
+    const double dG = dh - T * ds;
+    alpha = exp(-dG / (kSpRuniv * T));
+    alpha = alpha / (1.0 + alpha);
+  }
+  const double mfp = 3.0 * D / sqrt(8.0 * kSpRuniv * T / (kSpPi * MW));
+  double vp = B1 / T + B2 + B3 * T + B4 * log(T);
+  vp = kSpAtmPa * pow(10.0, vp);
+  const double conv = kSpRuniv / MW * 1.0e6;
+  out[0] = alpha;
+  out[1] = mfp;
+  out[2] = conv * T / p;
+  out[3] = vp / p / MW * 1.0e6;
+}
+
+__device__ __forceinline__ double fuchs_sutugin(double Kn, double alpha) {
+  return (0.75 * alpha * (1.0 + Kn)) / (Kn * Kn + (1.0 + 0.283 * alpha) * Kn + 0.75 * alpha);
+}
+
+// Transfer terms of every (particle, transfer) pair at the unknowns in L.x:
+// per-particle sums (lanes over particles), then k_c, k_e, diff and
+// d k_c / dV (lanes over pairs).  SingleParticleUtils.hpp:62-160.
+__device__ void simpol_terms(const IntegratorArgs& a, const AeroLds& L, int lane) {
+  const DevAerosol& X = a.X;
+  const double* mw = X.db + X.oMw;
+  const double* dens = X.db + X.oDens;
+  for (int ip = lane; ip < X.nPart; ip += kWave) {
+    const double* y = L.x + a.M.m + ip * X.nAero;
+    double v = kSpMinMass / kSpMinDensity;
+    for (int i = 0; i < X.nAero; ++i) v += y[i] / dens[i];
+    double ms = kSpMinMass, mo = kSpMinMass / kSpMinMw;
+    for (int i = 0; i < X.nAero; ++i) {
+      ms += y[i];
+      mo += y[i] / mw[i];
+    }
+    L.ps[4 * ip + 0] = pow((v * 3.0 / 4.0 / kSpPi), 1.0 / 3.0);
+    L.ps[4 * ip + 1] = v;
+    L.ps[4 * ip + 2] = ms;
+    L.ps[4 * ip + 3] = mo;
+  }
+  wave_sync();
+  const int nPair = X.nPart * X.nSimpol;
+  for (int j = lane; j < nPair; j += kWave) {
+    const int ip = j / X.nSimpol, s = j - ip * X.nSimpol;
+    const double* c4 = L.sc + 4 * s;
+    const double alpha = c4[0], D = X.db[X.oPar + 8 * s + 4];
+    const double radius = L.ps[4 * ip], volume = L.ps[4 * ip + 1], mass = L.ps[4 * ip + 2];
+    const double moles = L.ps[4 * ip + 3];
+    const double Kn = c4[1] / radius;
+    const double den = Kn * Kn + (1.0 + 0.283 * alpha) * Kn + 0.75 * alpha;
+    const double fk = fuchs_sutugin(Kn, alpha);
+    const double fp = 0.75 * alpha * (den - (1.0 + Kn) * (2.0 * Kn + 1.0 + 0.283 * alpha)) / (den * den);
+    const double kc = 4.0 * kSpPi * radius * D * fk;
+    const double ke = kc * 1.0 * (c4[3] * (mass / moles) / mass);
+    const int g = X.ib[X.oGas + s], ai = a.M.m + ip * X.nAero + X.ib[X.oAero + s];
+    double* o = L.pd + 6 * j;
+    o[0] = kc;
+    o[1] = ke;
+    o[2] = -(ke * L.x[ai]) + kc * L.x[g];
+    o[3] = 4.0 * kSpPi * D * (fk - Kn * fp) * (radius / (3.0 * volume));
+  }
+  wave_sync();
+}
+
+template <int QM>
+struct AerosolBackend {
+  using TA = double;
+  using Lds = AeroLds;
+  struct Fact { int nswp = 0; };
+  static __device__ __forceinline__ int unknowns(const IntegratorArgs& a) { return a.X.mt; }
+  static __device__ __forceinline__ int rowDim(const IntegratorArgs& a) {
+    return a.M.nSpec + 3 + a.X.nAero * a.X.nPart;
+  }
+  static __device__ __forceinline__ int column(const IntegratorArgs& a, int ai) {
+    return ai < a.M.m ? 3 + a.M.perm()[ai] : 3 + a.M.nSpec + (ai - a.M.m);
+  }
+  static __device__ __forceinline__ int tolIndex(const IntegratorArgs& a, int ai) {
+    return ai < a.M.m ? a.M.perm()[ai] : ai;
+  }
+  // this cell's number concentrations, temperature and pressure, and the
+  // SIMPOL constants (functions of T and p only)
+  static __device__ void bind(const IntegratorArgs& a, Lds& L, int cell) {
+    const double* row = a.state + size_t(cell) * rowDim(a);
+    L.nc = a.numberConc + size_t(cell) * a.X.nPart;
+    L.T = row[2];
+    L.p = row[1];
+    const int lane = threadIdx.x & (kWave - 1);
+    for (int s = lane; s < a.X.nSimpol; s += kWave) simpol_constants(a.X.db + a.X.oPar + 8 * s, L.T, L.p, L.sc + 4 * s);
+    wave_sync();
+  }
+  static __device__ void stage(const IntegratorArgs&, char*) {}
+  static __device__ Lds carve(char* base, const IntegratorArgs& a, int wave) {
+    return carve_aero(base + size_t(wave) * size_t(a.cellBytes), a.M, a.X);
+  }
+  static __device__ void init(const IntegratorArgs&, const Lds&, int) {}
+
+  // f at u: gas source term, then every transfer in (particle, transfer)
+  // order as the reference's serial accumulation; for a Newton iteration
+  // also x = -(u - bc - ch f) and, when refreshing, I - ch J
+  static __device__ void evaluate(const IntegratorArgs& a, const Lds& L, int lane, const double (&u)[QM],
+                                  const double (&bc)[QM], double ch, bool newton, bool refresh, double (&f)[QM],
+                                  double (&x)[QM]) {
+    const DevMech& M = a.M;
+    const DevAerosol& X = a.X;
+    const int mt = X.mt;
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const int ai = lane + kWave * q;
+      if (ai < mt) L.x[ai] = u[q];
+      if (ai < M.m) L.c[ai] = u[q];
+    }
+    wave_sync();
+    double fg[QM];
+    source_term_from_lds<QM>(M, L, lane, fg);
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const int ai = lane + kWave * q;
+      if (ai < mt) L.f[ai] = ai < M.m ? fg[q] : 0.0;
+    }
+    simpol_terms(a, L, lane);  // syncs first
+    if (lane == 0) {
+      for (int ip = 0; ip < X.nPart; ++ip)
+        for (int s = 0; s < X.nSimpol; ++s) {
+          const double diff = L.pd[6 * (ip * X.nSimpol + s) + 2];
+          const int g = X.ib[X.oGas + s], ai = M.m + ip * X.nAero + X.ib[X.oAero + s];
+          L.f[g] += -L.nc[ip] * diff;
+          L.f[ai] += diff / L.sc[4 * s + 2];
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const int ai = lane + kWave * q;
+      f[q] = ai < mt ? L.f[ai] : 0.0;
+      x[q] = (newton && ai < mt) ? -(u[q] - bc[q] - ch * f[q]) : 0.0;
+    }
+    if (refresh) assemble(a, L, lane, -ch);
+  }
+
+  // A = I + scale * J: gas-phase derivative pairs and Jacobian entries as
+  // assemble_jacobian_from_lds (leading dimension mt), then the transfer
+  // derivatives of SIMPOL_phase_transfer.hpp differentiated analytically:
+  //   d diff / d x_gas = k_c
+  //   d diff / d y_i = x_gas d k_c - y_aero d k_e - [i = aero] k_e
+  //   d k_c = (d k_c / dV) / rho_i,  d k_e = K_eq d k_c / moles - k_e / (moles MW_i)
+  static __device__ void assemble(const IntegratorArgs& a, const Lds& L, int lane, double scale) {
+    const DevMech& M = a.M;
+    const DevAerosol& X = a.X;
+    const int mt = X.mt;
+    for (int idx = lane; idx < mt * mt; idx += kWave) L.A[idx] = 0.0;
+    wave_sync();
+    for (int i = lane; i < mt; i += kWave) L.A[i * mt + i] = 1.0;
+    for (int r = lane; r < M.nReac; r += kWave) {
+      const int p0 = M.pairPtr()[r], p1 = M.pairPtr()[r + 1];
+      if (p0 == p1) continue;
+      const int* sp = M.reacSpec() + r * M.maxR;
+      const int* pw = M.reacPow() + r * M.maxR;
+      for (int pp = p0; pp < p1; ++pp) {
+        const int js = M.pairSlot()[pp];
+        const int ns = pw[js];
+        double d = L.kf[r] * double(ns) * pow_small(L.c[sp[js]], ns - 1);
+        for (int j = 0; j < M.maxR; ++j) {
+          const int s2 = sp[j];
+          if (s2 < 0) break;
+          if (j == js) continue;
+          d *= pow_small(L.c[s2], pw[j]);
+        }
+        L.scr[pp] = d;
+      }
+    }
+    wave_sync();
+    for (int n = lane; n < M.nnzJ; n += kWave) {
+      double sum = 0.0;
+      const int e1 = M.jnzPtr()[n + 1];
+      for (int e = M.jnzPtr()[n]; e < e1; ++e) sum += M.jcCoef()[e] * L.scr[M.jcPair()[e]];
+      const int rc = M.jnzRowCol()[n];
+      const int row = rc >> 16, col = rc & 0xffff;
+      L.A[row * mt + col] = (row == col ? 1.0 : 0.0) + scale * sum;
+    }
+    wave_sync();
+    if (lane == 0) {
+      const double* mw = X.db + X.oMw;
+      const double* dens = X.db + X.oDens;
+      for (int ip = 0; ip < X.nPart; ++ip) {
+        const int base = M.m + ip * X.nAero;
+        const double moles = L.ps[4 * ip + 3];
+        for (int s = 0; s < X.nSimpol; ++s) {
+          const double* o = L.pd + 6 * (ip * X.nSimpol + s);
+          const double kc = o[0], ke = o[1], dkdv = o[3];
+          const double keq = L.sc[4 * s + 3], kg = L.sc[4 * s + 2], n = L.nc[ip];
+          const int g = X.ib[X.oGas + s], ai = base + X.ib[X.oAero + s];
+          const double xg = L.x[g], ya = L.x[ai];
+          L.A[g * mt + g] += scale * (-n * kc);
+          L.A[ai * mt + g] += scale * (kc / kg);
+          for (int i = 0; i < X.nAero; ++i) {
+            const double dkc = dkdv / dens[i];
+            const double dke = keq * dkc / moles - ke / (moles * mw[i]);
+            double dd = xg * dkc - ya * dke;
+            if (base + i == ai) dd -= ke;
+            L.A[g * mt + base + i] += scale * (-n * dd);
+            L.A[ai * mt + base + i] += scale * (dd / kg);
+          }
+        }
+      }
+    }
+    wave_sync();
+  }
+  static __device__ bool factor_solve(const IntegratorArgs& a, const Lds& L, int lane, double (&x)[QM], Fact& fs) {
+    if (!lu_factor_fused<QM>(L, a.X.mt, lane, x, fs.nswp)) return false;
+    lu_back<QM>(L, a.X.mt, lane, x);
+    return true;
+  }
+  static __device__ void solve(const IntegratorArgs& a, const Lds& L, int lane, double (&x)[QM], const Fact& fs) {
+    lu_forward<QM>(L, a.X.mt, lane, x, fs.nswp);
+    lu_back<QM>(L, a.X.mt, lane, x);
+  }
+};
+
+// Batched aerosol source term (gas + transfer) at the state rows: one
+// wavefront per cell; omega [nBatch][mt] in the reference's unknown order
+// (active gas species, then particle species).
+template <int QM>
+__global__ __launch_bounds__(64) void aerosol_rhs_kernel(IntegratorArgs a) {
+  extern __shared__ __align__(16) char lds_raw[];
+  const int lane = threadIdx.x, cell = blockIdx.x;
+  if (cell >= a.nBatch) return;
+  AeroLds L = carve_aero(lds_raw, a.M, a.X);
+  AerosolBackend<QM>::bind(a, L, cell);
+  const double* row = a.state + size_t(cell) * AerosolBackend<QM>::rowDim(a);
+  const DevMech& M = a.M;
+  const double t_1 = 1.0 / L.T;
+  const double conv = M.convPPM * L.p * t_1;
+  for (int r = lane; r < M.nReac; r += kWave) L.kf[r] = rate_constant(M, r, L.T, L.p, t_1, conv, nullptr);
+  for (int i = M.m + lane; i < M.nSpec; i += kWave) L.c[i] = row[3 + i];
+  double u[QM], bc[QM], f[QM], x[QM];
+#pragma unroll
+  for (int q = 0; q < QM; ++q) {
+    const int ai = lane + kWave * q;
+    u[q] = ai < a.X.mt ? row[AerosolBackend<QM>::column(a, ai)] : 0.0;
+    bc[q] = 0.0;
+  }
+  wave_sync();
+  AerosolBackend<QM>::evaluate(a, L, lane, u, bc, 0.0, false, false, f, x);
+#pragma unroll
+  for (int q = 0; q < QM; ++q) {
+    const int ai = lane + kWave * q;
+    if (ai < a.X.mt) a.state_out[size_t(cell) * a.X.mt + (ai < M.m ? M.perm()[ai] : ai)] = f[q];
   }
 }
 
@@ -1291,13 +1616,15 @@ __global__ __launch_bounds__(kThreads) void trbdf2_kernel(IntegratorArgs a) {
 // lscale (nullable): kf_r *= exp(lscale[cell * lstride + r]), the log-scale
 // parameters the discrete adjoint differentiates with respect to (lstride 0:
 // one set for every cell).
+// rowDim: state row length (0: nSpec + 3, the gas-phase box model's rows)
 __global__ __launch_bounds__(256) void rates_kernel(DevMech M, int nBatch, const double* state, const double* photo,
-                                                   const double* lscale, int lstride, double* kf, const int* skip) {
+                                                   const double* lscale, int lstride, double* kf, const int* skip,
+                                                   int rowDim) {
   if (skip && *skip) return;
   const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
   if (idx >= (long long)nBatch * M.nReac) return;
   const int cell = int(idx / M.nReac), r = int(idx - (long long)cell * M.nReac);
-  const double* row = state + size_t(cell) * (M.nSpec + 3);
+  const double* row = state + size_t(cell) * (rowDim > 0 ? rowDim : M.nSpec + 3);
   const double p = row[1], T = row[2];
   const double t_1 = 1.0 / T;
   const double conv = M.convPPM * p * t_1;
@@ -1890,12 +2217,45 @@ hipError_t integratorMaxBlocksPerCU(int precision, int sparse, int m, int thread
 }
 
 hipError_t launchRates(const DevMech& M, int nBatch, const double* state, const double* photo, const double* lscale,
-                       int lstride, double* kf, const int* skip, hipStream_t stream) {
+                       int lstride, double* kf, const int* skip, hipStream_t stream, int rowDim) {
   const long long n = (long long)nBatch * M.nReac;
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(rates_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, M, nBatch, state, photo,
-                     lscale, lstride, kf, skip);
+                     lscale, lstride, kf, skip, rowDim);
   return hipGetLastError();
+}
+
+// ---- aerosol integrator launches (FP64 dense, mt <= 128 unknowns) ----
+static const void* aerosolKernel(int mt) {
+  if (mt <= 64) return integratorFn<AerosolBackend<1>, 1, 256>();
+  if (mt <= 128) return integratorFn<AerosolBackend<2>, 2, 256>();
+  return nullptr;
+}
+
+// cells (wavefronts) per workgroup of the aerosol integrator: up to 4, as
+// many as the LDS holds (0 when one cell does not fit)
+int aerosolCellsPerGroup(const DevMech& M, const DevAerosol& X, size_t maxLds) {
+  const size_t cb = (aerosolLdsBytes(M, X) + 15) & ~size_t(15);
+  int c = int(maxLds / cb);
+  return c > 4 ? 4 : c;
+}
+
+hipError_t launchAerosol(const IntegratorArgs& a, int grid, size_t lds, hipStream_t stream) {
+  const void* fn = aerosolKernel(a.X.mt);
+  if (!fn || a.cellsPerGroup < 1 || a.cellsPerGroup > 4) return hipErrorInvalidValue;
+  IntegratorArgs arg = a;
+  void* params[] = {&arg};
+  return hipLaunchKernel(fn, dim3(grid), dim3(64 * a.cellsPerGroup), params, lds, stream);
+}
+
+hipError_t launchAerosolRhs(const IntegratorArgs& a, size_t lds, hipStream_t stream) {
+  if (a.nBatch <= 0) return hipSuccess;
+  IntegratorArgs arg = a;
+  void* params[] = {&arg};
+  const void* fn = a.X.mt <= 64 ? reinterpret_cast<const void*>(&aerosol_rhs_kernel<1>)
+                                : (a.X.mt <= 128 ? reinterpret_cast<const void*>(&aerosol_rhs_kernel<2>) : nullptr);
+  if (!fn) return hipErrorInvalidValue;
+  return hipLaunchKernel(fn, dim3(a.nBatch), dim3(64), params, lds, stream);
 }
 
 static const void* adjointKernel(int qm) {
