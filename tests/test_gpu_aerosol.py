@@ -232,3 +232,60 @@ def test_invalid_arguments_are_rejected(models, gpu, pkg):
     rc = pkg.lib().tchem_aerosolchem_run_device_batch(ctypes.byref(opts), 2, None, None, None, None, None, None,
                                                       None, None, None, None, amcd._h, None)
     assert rc == -3 and b"FP64 only" in pkg.lib().tchem_last_error()
+
+
+def test_tchem_driver_ctest_through_the_executable(tmp_path, gpu, pkg):
+    """The reference's DriverInterface ctest on TChem_Driver.x (the coupling
+    interface, libtchem_driver.so): gas species names and aerosol properties
+    line by line, the state after one 30 s step with np.isclose
+    (src/unit-tests/TChem_Driver/compare_files.py)."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(pkg.LIB_PATH), "TChem_Driver.x")
+    print("case: TChem_Driver.x cb05_soa", flush=True)
+    r = subprocess.run([exe, golden("cb05_soa_gas.yaml"), golden("cb05_soa_aero.yaml"),
+                        golden("driver_config_solver.yaml")], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    for mine, ref in (("gas_species_names.txt", "driver_gas_species_names_ref.txt"),
+                      ("aero_species_props.txt", "driver_aero_species_props_ref.txt")):
+        assert [ln.strip() for ln in open(tmp_path / mine)] == [ln.strip() for ln in open(golden(ref))]
+    out = np.loadtxt(tmp_path / "output.txt")
+    ref = np.loadtxt(golden("driver_output_ref.txt"))
+    assert out.shape == ref.shape and np.all(np.isclose(out, ref))
+
+
+def test_driver_interface_integrates_every_batch_entry(gpu, pkg):
+    """Two batch entries through libtchem_driver.so: entry 0 is the ctest
+    state (matches output_ref.txt), entry 1 the same at 280 K; both advance
+    (the reference copies back entry 0 only)."""
+    import ctypes
+    import os
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(pkg.LIB_PATH), "libtchem_driver.so"))
+    lib.TChem_getLastError.restype = ctypes.c_char_p
+    lib.initialize(golden("cb05_soa_gas.yaml").encode(), golden("cb05_soa_aero.yaml").encode(),
+                   golden("driver_config_solver.yaml").encode(), 2)
+    assert lib.TChem_getLastError() == b""
+    n = lib.TChem_getLengthOfStateVector()
+    n_gas = lib.TChem_getNumberOfSpecies()
+    rows = np.zeros((2, n))
+    rows[:, 1], rows[:, 2] = 81060.0, 270.5
+    rows[1, 2] = 280.0
+    rows[:, 50], rows[:, 51], rows[:, 36], rows[:, 81] = 0.06, 1e-06, 0.001, 1e8
+    rows[:, 3 + n_gas] = 1e-08
+    nc = np.full(lib.TChem_getNumberConcentrationVectorSize(), 10000.0)
+    for i in range(2):
+        r = np.ascontiguousarray(rows[i])
+        lib.TChem_setStateVector(r.ctypes.data_as(ctypes.c_void_p), i)
+        lib.TChem_setNumberConcentrationVector(nc.ctypes.data_as(ctypes.c_void_p), i)
+    dt = ctypes.c_double(30.0)
+    lib.TChem_doTimestep(ctypes.byref(dt))
+    assert lib.TChem_getLastError() == b""
+    out = np.zeros((2, n))
+    lib.TChem_getAllStateVectorHost(out.ctypes.data_as(ctypes.c_void_p))
+    lib.finalize()
+    ref = np.loadtxt(golden("driver_output_ref.txt"))
+    assert np.all(np.isclose(np.array([float(f"{v:e}") for v in out[0]]), ref))
+    moved = np.abs(out[1] - rows[1]) > 1e-6 * np.abs(rows[1])
+    assert moved[3:3 + n_gas].sum() > 10 and np.all(out[1, :3] == rows[1, :3])
+    assert np.abs(out[1] - out[0])[3:].max() > 0

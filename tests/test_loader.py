@@ -1,5 +1,6 @@
 """Mechanism loader (KineticModelData) and the C ABI surface — CPU only."""
 import ctypes
+import os
 import filecmp
 import subprocess
 
@@ -113,6 +114,23 @@ def test_c_abi_exports_every_declared_symbol(pkg):
     lib = ctypes.CDLL(pkg.LIB_PATH)
     for s in declared:
         assert hasattr(lib, s)
+
+
+def test_driver_library_exports_every_declared_symbol(pkg):
+    """libtchem_driver.so (include/tchem_driver.h): the reference's
+    TChem_Driver extern "C" names, loadable without a device."""
+    import re
+    header = open(golden("../../include/tchem_driver.h")).read()
+    declared = set(re.findall(r"^\w[\w\s\*]*?\b(initialize|finalize|TChem_\w+)\(", header, re.M))
+    assert {"initialize", "finalize", "TChem_doTimestep", "TChem_getStateVector", "TChem_setStateVector",
+            "TChem_getAerosolSpeciesKappa", "TChem_setNumberConcentrationVector"} <= declared
+    path = os.path.join(os.path.dirname(pkg.LIB_PATH), "libtchem_driver.so")
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    assert declared <= exported
+    lib = ctypes.CDLL(path)
+    lib.TChem_getLengthOfStateVector.restype = ctypes.c_int
+    assert lib.TChem_getLengthOfStateVector() == -1  # not initialized: no device call
 
 
 def test_c_abi_argument_errors_without_device(pkg):
