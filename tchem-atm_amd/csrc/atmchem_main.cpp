@@ -4,7 +4,10 @@
 // TCHEM_E3SM_DRIVER, of TChem_AtmosphericChemistryE3SM.x
 // (src/examples/TChem_AtmosphericChemistryE3SM.cpp: E3SM kinetics, scenario
 // from --inputfile with per-cell photolysis rates and external forcing,
-// --atol-time): load the mechanism
+// --atol-time) and, built with TCHEM_AEROSOL_DRIVER, of
+// TChem_AerosolChemistry.x (src/examples/TChem_AerosolChemistry.cpp:46-440:
+// --aerofile single-particle SIMPOL.1 aerosol model, particle scenario from
+// --inputfile_particles, --number_of_particles, FP64): load the mechanism
 // (writing kmod.echo), read the scenario, relaunch the batched integrator
 // interval by interval until the mean cell time exceeds 0.9999 tend, write
 // the state of every cell after every interval (iter, t, dt, state row) and
@@ -35,10 +38,18 @@ constexpr bool kE3sm = true;
 #else
 constexpr bool kE3sm = false;
 #endif
+#ifdef TCHEM_AEROSOL_DRIVER
+constexpr bool kAero = true;
+#else
+constexpr bool kAero = false;
+#endif
 
 struct Options {
   std::string chemfile = "chem.yaml", inputfile = "None", times = "wall_times.json";
-  std::string outputfile = kE3sm ? "AtmosphericChemistryE3SM.dat" : "AtmosphericChemistry.dat";
+  std::string aerofile = "aero.yaml", particles = "None";
+  int nParticles = -1;
+  std::string outputfile = kE3sm ? "AtmosphericChemistryE3SM.dat"
+                                 : (kAero ? "AerosolChemistry.dat" : "AtmosphericChemistry.dat");
   double tbeg = 0, tend = 1, dtmin = 1e-8, dtmax = 1e-1, rtol_time = 1e-4, atol_newton = 1e-10, rtol_newton = 1e-6;
   double atol_time = 1e-12;
   int per_interval = 10, max_iters = 1000, max_newton = 100, jac_interval = 1, batch = 1;
@@ -52,6 +63,7 @@ void usage(const char* argv0) {
   std::printf(
       "Usage: %s [options]\n  options (--name=value):\n"
       "  --chemfile --outputfile --outputfile_times --inputsPath (E3SM: --inputfile --atol-time)\n"
+      "  (aerosol: --aerofile --inputfile --inputfile_particles --number_of_particles --atol-time)\n"
       "  --tbeg --tend --dtmin --dtmax --tol-time --atol-newton --rtol-newton\n"
       "  --time-iterations-per-interval --max-time-iterations --max-newton-iterations --jacobian-interval\n"
       "  --batch_size --use_cloned_samples --write-time-profiles --verbose\n"
@@ -74,8 +86,12 @@ int parse(int argc, char** argv, Options& o) {
     const std::string val = eq == std::string::npos ? "true" : s.substr(eq + 1);
     try {
       if (key == "chemfile") o.chemfile = val;
-      else if (kE3sm && key == "inputfile") o.inputfile = val;
-      else if (kE3sm && key == "atol-time") o.atol_time = std::stod(val);
+      else if ((kE3sm || kAero) && key == "inputfile") o.inputfile = val;
+      else if ((kE3sm || kAero) && key == "atol-time") o.atol_time = std::stod(val);
+      else if (kAero && key == "aerofile") o.aerofile = val;
+      else if (kAero && key == "inputfile_particles") o.particles = val;
+      else if (kAero && key == "number_of_particles") o.nParticles = std::stoi(val);
+      else if (kAero && key == "use-cvode") ;
       else if (key == "outputfile") o.outputfile = val;
       else if (key == "outputfile_times") o.times = val;
       else if (key == "tbeg") o.tbeg = std::stod(val);
@@ -159,19 +175,47 @@ int main(int argc, char** argv) {
   CHECK_T(tchem_kmd_info(kmd, &nSpec, &nConst, &nReac, &nPhoto));
   std::printf("Number of Species %d \n", nSpec);
   std::printf("Number of Reactions %d \n", nReac);
-  const int dim = nSpec + 3, m = nSpec - nConst;
+  const int gasDim = nSpec + 3;
   std::vector<std::string> names(nSpec);
   for (int k = 0; k < nSpec; ++k) {
     char buf[64];
     CHECK_T(tchem_kmd_species_name(kmd, k, buf, sizeof(buf)));
     names[k] = buf;
   }
+  // aerosol model: particle species appended to the state rows and unknowns
+  tchem_amd_t amd = nullptr;
+  int nAero = 0, nPart = 0;
+  if (kAero) {
+    CHECK_T(tchem_amd_load_yaml(o.aerofile.c_str(), kmd, &amd));
+    if (o.nParticles > 0) CHECK_T(tchem_amd_set_number_of_particles(amd, o.nParticles));
+    int nSimpol = 0;
+    CHECK_T(tchem_amd_info(amd, &nAero, &nPart, &nSimpol));
+    std::printf("Number of Aerosol Species %d \n", nAero);
+    std::printf("Number of Aerosol Particles %d \n", nPart);
+    for (int ip = 0; ip < nPart; ++ip)
+      for (int k = 0; k < nAero; ++k) {
+        char buf[128];
+        CHECK_T(tchem_amd_species_name(amd, k, buf, sizeof(buf)));
+        names.push_back(std::string(buf) + "_p" + std::to_string(ip));
+      }
+  }
+  const int dim = gasDim + nAero * nPart, m = nSpec - nConst + nAero * nPart;
 
-  const std::string input = (kE3sm && o.inputfile != "None") ? o.inputfile : o.chemfile;
+  const std::string input = ((kE3sm || kAero) && o.inputfile != "None") ? o.inputfile : o.chemfile;
   int nFiles = 0;
   CHECK_T(tchem_scenario_load(kmd, input.c_str(), nullptr, 0, &nFiles));
   std::vector<double> scen(size_t(nFiles) * dim);
-  CHECK_T(tchem_scenario_load(kmd, input.c_str(), scen.data(), nFiles, &nFiles));
+  {
+    std::vector<double> gas(size_t(nFiles) * gasDim);
+    CHECK_T(tchem_scenario_load(kmd, input.c_str(), gas.data(), nFiles, &nFiles));
+    for (int i = 0; i < nFiles; ++i)
+      std::copy(gas.begin() + size_t(i) * gasDim, gas.begin() + size_t(i + 1) * gasDim, scen.begin() + size_t(i) * dim);
+  }
+  std::vector<double> ncScen(size_t(nFiles) * (nPart > 0 ? nPart : 1), 0.0);
+  if (kAero && o.particles != "None") {
+    int found = 0;
+    CHECK_T(tchem_scenario_particles(amd, o.particles.c_str(), nFiles, dim, scen.data(), ncScen.data(), &found));
+  }
   // E3SM: per-cell photolysis rates and external forcing of the scenario
   int nPhotoCols = 0;
   std::vector<double> photoScen, extScen;
@@ -185,7 +229,7 @@ int main(int argc, char** argv) {
     CHECK_T(tchem_scenario_external_forcing(kmd, input.c_str(), nFiles, extScen.data(), &nExt));
   }
   int n = nFiles;
-  std::vector<double> state, photo, ext;
+  std::vector<double> state, photo, ext, nc;
   auto clone = [](const std::vector<double>& row0, int width, int count) {
     std::vector<double> v(size_t(count) * width);
     for (int i = 0; i < count; ++i) std::copy(row0.begin(), row0.begin() + width, v.begin() + size_t(i) * width);
@@ -195,6 +239,7 @@ int main(int argc, char** argv) {
     std::printf("Using cloned samples ... only for numerical experiments\n");
     n = o.batch;
     state = clone(scen, dim, n);
+    if (kAero) nc = clone(ncScen, nPart > 0 ? nPart : 1, n);
     if (kE3sm) {
       photo = clone(photoScen, nPhotoCols, n);
       ext = clone(extScen, m, n);
@@ -203,12 +248,15 @@ int main(int argc, char** argv) {
     state = scen;
     photo = photoScen;
     ext = extScen;
+    nc = ncScen;
   }
 
   tchem_kmcd_t kmcd = nullptr;
   CHECK_T(tchem_kmcd_create(kmd, &kmcd));
+  tchem_amcd_t amcd = nullptr;
+  if (kAero) CHECK_T(tchem_amcd_create(amd, kmcd, &amcd));
 
-  DeviceArray<double> d_state, d_t, d_dt, d_tn, d_tt, d_photo, d_ext;
+  DeviceArray<double> d_state, d_t, d_dt, d_tn, d_tt, d_photo, d_ext, d_nc;
   DeviceArray<tchem_time_advance_t> d_tadv;
   CHECK_H(hipMalloc(&d_state.p, sizeof(double) * state.size()));
   CHECK_H(hipMalloc(&d_t.p, sizeof(double) * n));
@@ -216,11 +264,13 @@ int main(int argc, char** argv) {
   CHECK_H(hipMalloc(&d_tn.p, sizeof(double) * 2));
   CHECK_H(hipMalloc(&d_tt.p, sizeof(double) * 2 * (m > 0 ? m : 1)));
   CHECK_H(hipMalloc(&d_tadv.p, sizeof(tchem_time_advance_t) * n));
-  std::vector<double> t(n, o.tbeg), dt(n, o.dtmax);  // the reference fills dt with dtmax (examples/...:227)
+  // the gas driver fills dt with dtmax (examples/TChem_AtmosphericChemistry.cpp:227); the
+  // aerosol driver leaves it zero (examples/TChem_AerosolChemistry.cpp:299-302)
+  std::vector<double> t(n, o.tbeg), dt(n, kAero ? 0.0 : o.dtmax);
   const double tn[2] = {o.atol_newton, o.rtol_newton};
   std::vector<double> tt(size_t(2) * (m > 0 ? m : 1));
   for (int i = 0; i < m; ++i) {
-    tt[2 * i] = o.atol_time;  // 1e-12 (fixed in TChem_AtmosphericChemistry.x, --atol-time in the E3SM driver)
+    tt[2 * i] = o.atol_time;  // 1e-12 (fixed in TChem_AtmosphericChemistry.x, --atol-time in the E3SM/aerosol drivers)
     tt[2 * i + 1] = o.rtol_time;
   }
   if (kE3sm) {
@@ -230,6 +280,10 @@ int main(int argc, char** argv) {
     }
     CHECK_H(hipMalloc(&d_ext.p, sizeof(double) * (ext.empty() ? 1 : ext.size())));
     if (!ext.empty()) CHECK_H(hipMemcpy(d_ext.p, ext.data(), sizeof(double) * ext.size(), hipMemcpyHostToDevice));
+  }
+  if (kAero && nPart > 0) {
+    CHECK_H(hipMalloc(&d_nc.p, sizeof(double) * nc.size()));
+    CHECK_H(hipMemcpy(d_nc.p, nc.data(), sizeof(double) * nc.size(), hipMemcpyHostToDevice));
   }
   tchem_time_advance_t ta{};
   ta._tbeg = o.tbeg;
@@ -257,7 +311,7 @@ int main(int argc, char** argv) {
     }
     std::fprintf(fout, "%s \t %s \t %s \t ", "iter", "t", "dt");
     std::fprintf(fout, "%s \t %s \t %s \t", "Density[kg/m3]", "Pressure[Pascal]", "Temperature[K]");
-    for (int k = 0; k < nSpec; ++k) std::fprintf(fout, "%s \t", names[k].c_str());
+    for (const std::string& nm : names) std::fprintf(fout, "%s \t", nm.c_str());
     std::fprintf(fout, "\n");
     writeState(fout, -1, t, dt, state, n, dim);
   }
@@ -277,7 +331,10 @@ int main(int argc, char** argv) {
   double tsum = 0.0;
   for (; iter < o.max_iters && tsum <= o.tend * 0.9999; ++iter) {
     const auto w0 = std::chrono::steady_clock::now();
-    if (kE3sm)
+    if (kAero)
+      CHECK_T(tchem_aerosolchem_run_device_batch(nullptr, n, d_tn.p, d_tt.p, nullptr, d_tadv.p, d_state.p, d_nc.p,
+                                                 d_t.p, d_dt.p, d_state.p, nullptr, amcd, nullptr));
+    else if (kE3sm)
       CHECK_T(tchem_atmchem_e3sm_run_device_batch(&opts, n, d_tn.p, d_tt.p, nullptr, d_tadv.p, d_state.p, d_photo.p,
                                                   nPhotoCols, m > 0 ? d_ext.p : nullptr, d_t.p, d_dt.p, d_state.p,
                                                   nullptr, kmcd, nullptr));
@@ -311,6 +368,8 @@ int main(int argc, char** argv) {
   std::fprintf(ftimes, "}\n ");
   std::fclose(ftimes);
   if (fout) std::fclose(fout);
+  if (amcd) tchem_amcd_destroy(amcd);
+  if (amd) tchem_amd_destroy(amd);
   tchem_kmcd_destroy(kmcd);
   tchem_kmd_destroy(kmd);
   return 0;

@@ -96,3 +96,34 @@ def test_uci_column_e3sm_golden(tmp_path, gpu):
     print(f"cells identical to the golden: {exact.sum()} of 80; golden cells with pure doubling: {doubling.sum()}")
     assert exact.sum() >= 40 and exact[exact].size == (exact & doubling).sum()
     assert (d <= 10 * (1e-12 + 1e-6 * np.abs(fr[:, 6:]))).all()  # within 10x the integration tolerance
+
+
+EXE_AERO = os.path.join(ROOT, "tchem-atm_amd", "TChem_AerosolChemistry.x")
+
+
+@pytest.mark.parametrize("ref_name,chem,aero,part,extra", [
+    ("simpol_full_gas.npy", "simpol_gas.yaml", "simpol_aero.yaml", "simpol_particles.yaml", ()),
+    ("simpol_batched_full_gas.npy", "simpol_gas.yaml", "simpol_aero.yaml", "simpol_particles.yaml",
+     ("--use_cloned_samples=true", "--batch_size=3")),
+    ("cb05_soa_batched_full_gas.npy", "cb05_soa_gas.yaml", "cb05_soa_aero.yaml", "cb05_soa_particles.yaml",
+     ("--use_cloned_samples=true", "--batch_size=3")),
+])
+def test_tines_ode_aerosol_ctest_through_the_executable(tmp_path, gpu, ref_name, chem, aero, part, extra):
+    """The reference's TinesODE ctests (src/unit-tests/TinesODE/CMakeLists.txt)
+    on TChem_AerosolChemistry.x, checked as compare_tchem.py checks them."""
+    args = [EXE_AERO, f"--chemfile={golden(chem)}", f"--aerofile={golden(aero)}",
+            f"--inputfile_particles={golden(part)}", "--outputfile=full_gas.dat",
+            "--outputfile_times=AerosolChemistry_wall_times.json", "--dtmin=1e-20", "--dtmax=10", "--tend=600",
+            "--atol-newton=1e-18", "--rtol-newton=1e-8", "--max-time-iterations=2", *extra]
+    print("case:", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    test = np.loadtxt(tmp_path / "full_gas.dat", skiprows=1)
+    ref = np.load(golden(ref_name))
+    assert test.shape == ref.shape
+    assert compare_tchem_norms(test, ref) < 1e-8
+    header = [h.strip() for h in (tmp_path / "full_gas.dat").read_text().splitlines()[0].split("\t") if h.strip()]
+    assert len(header) == ref.shape[1]
+    assert header[-1].endswith("_p0")
+    body = json.loads((tmp_path / "AerosolChemistry_wall_times.json").read_text())
+    assert body["Aerosol Chemistry"]["number_of_time_iters"] == 2
