@@ -316,6 +316,93 @@ def config3_run(cells, out_path=None, max_steps=8192):
             f.write(text + "\n")
 
 
+def aerosol_run(cells, cpu_sample, threads, out_path=None):
+    """Single-particle SIMPOL.1 aerosol box model (AerosolChemistry): CB05CL_AE5
+    with the SOA scheme, one particle per cell, 3600 s, one FP64 launch to
+    t_end; cell-hours/s on the GPU and of the oracle on a sample of cells."""
+    import torch
+    T = importlib.import_module("tchem-atm_amd")
+    S = importlib.import_module("tchem-atm_amd.scenarios")
+    import oracle
+    g = os.path.join(ROOT, "tests", "golden")
+    chem, aero = os.path.join(g, "cb05_soa_gas.yaml"), os.path.join(g, "cb05_soa_aero.yaml")
+    kmd = T.KineticModelData(chem)
+    kmcd = kmd.createConstData()
+    amd = T.AerosolModelData(aero, kmd)
+    amcd = amd.createConstData(kmcd)
+    base = T.setScenarioConditions(chem, kmd)
+    w, nc = S.aerosol_soa(kmd, base[0], amd.nSpec, amd.nParticles, n=cells)
+    mt = amd.nUnknowns
+    dev = "cuda"
+    tn = torch.tensor([w.atol_newton, w.rtol_newton], dtype=torch.float64, device=dev)
+    tt = torch.tensor(np.tile([w.atol_time, w.rtol_time], (mt, 1)), device=dev)
+    st0 = torch.tensor(w.state, device=dev)
+    ncd = torch.tensor(nc, device=dev)
+    st = torch.empty_like(st0)
+    t = torch.empty(cells, dtype=torch.float64, device=dev)
+    dt = torch.empty_like(t)
+    stats = T.stats_tensor(cells)
+
+    def run():
+        st.copy_(st0)
+        t.fill_(0.0)
+        dt.zero_()
+        tadv = T.time_advance_tensor(cells, 0.0, w.tend, w.dt_in, w.dtmin, w.dtmax, w.max_newton, 1 << 30, 1, dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        T.AerosolChemistry.runDeviceBatch(tn, tt, None, tadv, st, ncd, t, dt, st, amcd, stats=stats)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    run()
+    wall = run()
+    s = T.stats_from_tensor(stats)
+    res = {"metric": "cell_hours_per_s", "workload": "cb05_soa_simpol_1particle", "cells": cells,
+           "unknowns_per_cell": mt, "t_end_s": w.tend, "value": w.cell_hours / wall, "wall_s": wall,
+           "accepted_steps_per_cell": float(s["accepted"].mean()),
+           "newton_iterations_per_step": float(s["newton_iterations"].sum() / max(1, s["accepted"].sum())),
+           "failed_cells": int(s["failed"].sum()), "all_reached_t_end": bool((t.cpu().numpy() == w.tend).all())}
+    if cpu_sample > 0:
+        o = oracle.AerosolOracle(chem, aero)
+        sample = np.ascontiguousarray(w.state[:cpu_sample])
+        ta = np.zeros(cpu_sample, oracle.TADV_DTYPE)
+        ta["tend"], ta["dt"], ta["dtmin"], ta["dtmax"] = w.tend, w.dt_in, w.dtmin, w.dtmax
+        ta["max_num_newton_iterations"], ta["num_time_iterations_per_interval"] = w.max_newton, 1 << 30
+        ta["jacobian_interval"] = 1
+        to, dto = np.zeros(cpu_sample), np.zeros(cpu_sample)
+        t0 = time.perf_counter()
+        o.run_batch(np.array([w.atol_newton, w.rtol_newton]), np.tile([w.atol_time, w.rtol_time], (mt, 1)), ta,
+                    sample, np.ascontiguousarray(nc[:cpu_sample]), to, dto, nthreads=threads)
+        cpu_wall = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": cpu_sample * w.tend / 3600.0 / cpu_wall, "unit": "cell_hours_per_s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"first {cpu_sample} cells of the workload, oracle, analytic Jacobian"}
+        g_state = st.cpu().numpy()[:cpu_sample]
+        cols = np.r_[3:3 + kmd.nActive, 3 + kmd.nSpec:amd.stateDim]
+        err = np.abs(g_state[:, cols] - sample[:, cols]) / np.maximum(np.abs(sample[:, cols]), 1e-12)
+        res["sample_max_rel_err_vs_oracle"] = float(err.max())
+    bad = np.nonzero(s["failed"])[0]
+    if len(bad):
+        # do the cells the GPU gave up on fail in the oracle too?
+        o = oracle.AerosolOracle(chem, aero)
+        ta = np.zeros(len(bad), oracle.TADV_DTYPE)
+        ta["tend"], ta["dt"], ta["dtmin"], ta["dtmax"] = w.tend, w.dt_in, w.dtmin, w.dtmax
+        ta["max_num_newton_iterations"], ta["num_time_iterations_per_interval"] = w.max_newton, 1 << 30
+        ta["jacobian_interval"] = 1
+        ost = np.ascontiguousarray(w.state[bad])
+        ostats = np.zeros(len(bad), oracle.STATS_DTYPE)
+        o.run_batch(np.array([w.atol_newton, w.rtol_newton]), np.tile([w.atol_time, w.rtol_time], (mt, 1)), ta,
+                    ost, np.ascontiguousarray(nc[bad]), np.zeros(len(bad)), np.zeros(len(bad)), stats=ostats,
+                    nthreads=threads)
+        res["failed_cell_indices"] = [int(c) for c in bad]
+        res["failed_cells_also_failed_by_oracle"] = int(ostats["failed"].sum())
+    text = json.dumps(res)
+    print(text, flush=True)
+    if out_path:
+        with open(out_path, "w") as f:
+            f.write(text + "\n")
+
+
 def copy_probe():
     import torch
     n = 1 << 27  # 1 GiB of float64
@@ -350,6 +437,10 @@ def main():
     ap.add_argument("--config3", metavar="OUT", nargs="?", const="",
                     help="time the discrete adjoint (forward, recording forward, backward) instead of the benchmark")
     ap.add_argument("--config3-cells", type=int, default=4096)
+    ap.add_argument("--aerosol", metavar="OUT", nargs="?", const="",
+                    help="time the SIMPOL.1 aerosol box model instead (JSON line, optionally to OUT)")
+    ap.add_argument("--aerosol-cells", type=int, default=4096)
+    ap.add_argument("--aerosol-cpu-sample", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the path shards cells across GPUs with no exchange; "
@@ -363,6 +454,10 @@ def main():
         return
     if args.config3 is not None:
         config3_run(args.config3_cells, args.config3 or None)
+        return
+    if args.aerosol is not None:
+        aerosol_run(args.aerosol_cells, args.aerosol_cpu_sample, args.cpu_threads or min(16, os.cpu_count() or 1),
+                    args.aerosol or None)
         return
     if args.precision_study is not None:
         precision_study(args.study_cells, args.cpu_threads or min(16, os.cpu_count() or 1),

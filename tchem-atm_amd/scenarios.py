@@ -86,6 +86,28 @@ def config1(kmd, base_state_row, n: int = 131072, seed: int = 1) -> Workload:
     return Workload("cb05_config1", st, ph, 0.0, 3600.0, 1e-20, 1e-20, 60.0, 1e-12, 1e-6)
 
 
+def aerosol_soa(kmd, base_state_row, n_aero: int, n_particles: int, n: int = 16384, seed: int = 4):
+    """CB05CL_AE5 with the two-product SOA scheme and single-particle
+    SIMPOL.1 partitioning: n cells with config-0 draws of pressure,
+    temperature and gas mixing ratios (Philox keyed by `seed`), then
+    particles per m3 N ~ 10^U[6,9] and primary organic aerosol (aerosol
+    species 0) 10^U[-9,-7] kg m-3 shared by the particles; other particle
+    species start at zero.  3600 s, dtmax 60 s, time tolerances (1e-12, 1e-6).
+    Returns the workload (state rows [n][nSpec + 3 + n_aero * n_particles])
+    and number concentrations [n][n_particles]."""
+    m = kmd.nSpec - kmd.nConstSpec
+    gas, _ = cb05_cells(n, seed, m, kmd.nSpec, 0, base_state_row)
+    g = np.random.Generator(np.random.Philox(key=seed + 1000))
+    nc = np.repeat((10.0 ** g.uniform(6.0, 9.0, n))[:, None], n_particles, axis=1)
+    poa = 10.0 ** g.uniform(-9.0, -7.0, n)
+    state = np.zeros((n, kmd.nSpec + 3 + n_aero * n_particles))
+    state[:, :kmd.nSpec + 3] = gas
+    for ip in range(n_particles):
+        state[:, kmd.nSpec + 3 + ip * n_aero] = poa / nc[:, ip]
+    w = Workload("cb05_soa_simpol", state, np.zeros((n, 0)), 0.0, 3600.0, 1e-20, 1e-20, 60.0, 1e-12, 1e-6)
+    return w, nc
+
+
 def tol_arrays(w: Workload, m: int):
     tol_newton = np.array([w.atol_newton, w.rtol_newton])
     tol_time = np.tile([w.atol_time, w.rtol_time], (m, 1))
