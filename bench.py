@@ -439,7 +439,7 @@ def main():
     ap.add_argument("--config3-cells", type=int, default=4096)
     ap.add_argument("--aerosol", metavar="OUT", nargs="?", const="",
                     help="time the SIMPOL.1 aerosol box model instead (JSON line, optionally to OUT)")
-    ap.add_argument("--aerosol-cells", type=int, default=4096)
+    ap.add_argument("--aerosol-cells", type=int, default=16384)
     ap.add_argument("--aerosol-cpu-sample", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:

@@ -569,9 +569,11 @@ class AerosolChemistry:
 
     @staticmethod
     def runDeviceBatch(tol_newton, tol_time, fac, tadv, state, number_conc, t_out, dt_out, state_out, amcd,
-                       stats=None, stream=None):
+                       stats=None, lu=LU_AUTO, stream=None):
+        """lu: AUTO / SPARSE (compiled programs over gas + particle unknowns)
+        or DENSE (dense LU with partial pivoting); FP64 either way."""
         nBatch = state.shape[0]
-        opts = _Options(PRECISION_FP64, SCHEDULE_GRID, 0, LU_DENSE, 0, PROGRAMS_AUTO)
+        opts = _Options(PRECISION_FP64, SCHEDULE_GRID, 0, lu, 0, PROGRAMS_AUTO)
         _check(_lib.tchem_aerosolchem_run_device_batch(
             ctypes.byref(opts), nBatch, _ptr(tol_newton, what="tol_newton"), _ptr(tol_time, what="tol_time"),
             _ptr(fac, None, "fac"), _ptr(tadv, None, "tadv"), _ptr(state, what="state"),

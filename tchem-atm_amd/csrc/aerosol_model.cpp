@@ -121,4 +121,30 @@ bool loadScenarioParticles(const std::string& path, const AerosolModel& am, cons
   return true;
 }
 
+MechTables aerosolTables(const MechTables& t, const AerosolModel& am, std::vector<std::array<int, 2>>* entries) {
+  const int mt = t.m + am.nSpec * am.nParticles;
+  MechTables ta = t;
+  ta.m = mt;
+  std::vector<char> have(size_t(mt) * mt, 0);
+  for (size_t q = 0; q < t.jnzRow.size(); ++q) have[size_t(t.jnzRow[q]) * mt + t.jnzCol[q]] = 1;
+  std::vector<std::array<int, 2>> ent;
+  for (int ip = 0; ip < am.nParticles; ++ip)
+    for (const SimpolParam& sp : am.simpol) {
+      const int g = t.iperm[sp.gas], base = t.m + ip * am.nSpec, ai = base + sp.aero;
+      ent.push_back({g, g});
+      ent.push_back({ai, g});
+      for (int i = 0; i < am.nSpec; ++i) ent.push_back({g, base + i});
+      for (int i = 0; i < am.nSpec; ++i) ent.push_back({ai, base + i});
+    }
+  for (const auto& e : ent) {
+    if (have[size_t(e[0]) * mt + e[1]]) continue;
+    have[size_t(e[0]) * mt + e[1]] = 1;
+    ta.jnzRow.push_back(e[0]);
+    ta.jnzCol.push_back(e[1]);
+    ta.jnzPtr.push_back(ta.jnzPtr.back());  // no gas-phase contributions
+  }
+  if (entries) *entries = ent;
+  return ta;
+}
+
 }  // namespace tchem_amd

@@ -16,11 +16,13 @@
 //   species (active then constant), particle species].
 #pragma once
 
+#include <array>
 #include <map>
 #include <string>
 #include <vector>
 
 #include "kinetic_model.hpp"
+#include "mechanism.hpp"
 
 namespace tchem_amd {
 
@@ -60,5 +62,15 @@ AerosolModel loadAerosolModel(const std::string& path, const KineticModel& km);
 // the file has no `particles` block (nothing written).
 bool loadScenarioParticles(const std::string& path, const AerosolModel& am, const KineticModel& km, int nBatch,
                            std::vector<double>& state, std::vector<double>& numberConc);
+
+// Mechanism tables of the aerosol problem for the sparse programs: `t` with
+// m = mt = t.m + nSpec * nParticles unknowns (particle unknowns after the gas
+// species) and the Jacobian pattern extended by every transfer derivative
+// (entries without gas-phase contributions).  `entries` receives the
+// (row, col) of each transfer derivative in the order the integrator adds
+// them: per (particle, transfer) pair (g,g), (a,g), (g, species i of the
+// particle) for every i, (a, species i) for every i, where g is the gas
+// species (internal order) and a the particle species of the transfer.
+MechTables aerosolTables(const MechTables& t, const AerosolModel& am, std::vector<std::array<int, 2>>* entries);
 
 }  // namespace tchem_amd

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
@@ -87,6 +88,8 @@ struct tchem_amd_s {
 struct tchem_amcd_s {
   tchem_kmcd_s* kmcd = nullptr;  // not owned
   DevAerosol X{};
+  DevSparse S{};                 // programs over the mt unknowns (S.enabled = 0: dense form only)
+  SparseSchedule sched;
   int stateDim = 0;
   std::vector<void*> allocations;
   int device = 0;
@@ -1342,6 +1345,59 @@ int tchem_amcd_create(tchem_amd_t amd, tchem_kmcd_t kmcd, tchem_amcd_t* out) {
   X.oGas = 0;
   X.oAero = int(ib.size());
   for (const SimpolParam& s : am.simpol) ib.push_back(s.aero);
+  // sparse form: the gas Jacobian pattern plus every transfer derivative
+  // over mt unknowns, particle unknowns after the gas species
+  std::vector<int> pib;
+  std::vector<double> pdb;
+  try {
+    std::vector<std::array<int, 2>> ent;  // (row, col) of every transfer derivative, table order
+    const MechTables ta = aerosolTables(t, am, &ent);
+    h->sched = buildSparseSchedule(ta, false);
+    const SparseSchedule& sc = h->sched;
+    auto position = [&](int r, int c) {
+      for (int q = sc.rowPtr[r]; q < sc.rowPtr[r + 1]; ++q)
+        if (sc.colIdx[q] == c) return q;
+      throw std::runtime_error("transfer derivative outside the factor pattern");
+    };
+    X.oJpos = int(ib.size());
+    for (const auto& e : ent) ib.push_back(position(e[0], e[1]));
+    auto addI = [&](const std::vector<unsigned>& v) {
+      const int off = int(pib.size());
+      pib.insert(pib.end(), v.begin(), v.end());
+      while (pib.size() % 4) pib.push_back(0);
+      return off;
+    };
+    DevSparse& sp = h->S;
+    sp.enabled = 1;
+    sp.nnzF = sc.nnzF;
+    sp.zeroSlot = sc.zeroSlot;
+    sp.trashSlot = sc.trashSlot;
+    sp.pairSlots = sc.pairSlots;
+    sp.oEvalHdr = addI(sc.evalHdr);
+    sp.oEvalBody = addI(sc.evalBody);
+    sp.evalBodyWords = int(sc.evalBody.size());
+    sp.oCoef = 0;
+    pdb = sc.coefTable;
+    sp.nCoef = int(sc.coefTable.size());
+    sp.ropRounds = sc.ropRounds;
+    sp.pairRounds = sc.pairRounds;
+    sp.segRounds = sc.segRounds;
+    sp.jacRounds = sc.jacRounds;
+    sp.pairBody = sc.pairBody;
+    sp.segBody = sc.segBody;
+    sp.jacBody = sc.jacBody;
+    sp.oProgHdr = addI(sc.progHdr);
+    sp.oProgBody = addI(sc.progBody);
+    sp.luRounds = sc.luRounds;
+    sp.backRounds = sc.backRounds;
+    sp.fwdRounds = sc.fwdRounds;
+    sp.backBody = sc.backBody;
+    sp.fwdBody = sc.fwdBody;
+  } catch (const std::exception&) {
+    h->S = DevSparse{};  // the dense form still runs
+    pib.clear();
+    pdb.clear();
+  }
   ib.push_back(0);
   db.push_back(0.0);
   void* pi = nullptr;
@@ -1357,6 +1413,19 @@ int tchem_amcd_create(tchem_amd_t amd, tchem_kmcd_t kmcd, tchem_amcd_t* out) {
     return hipFail(e, "hipMemcpy");
   X.ib = static_cast<const int*>(pi);
   X.db = static_cast<const double*>(pd);
+  if (h->S.enabled) {
+    void* qi = nullptr;
+    void* qd = nullptr;
+    if ((e = hipMalloc(&qi, sizeof(int) * pib.size())) != hipSuccess) return hipFail(e, "hipMalloc");
+    h->allocations.push_back(qi);
+    if ((e = hipMalloc(&qd, sizeof(double) * pdb.size())) != hipSuccess) return hipFail(e, "hipMalloc");
+    h->allocations.push_back(qd);
+    if ((e = hipMemcpy(qi, pib.data(), sizeof(int) * pib.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(qd, pdb.data(), sizeof(double) * pdb.size(), hipMemcpyHostToDevice)) != hipSuccess)
+      return hipFail(e, "hipMemcpy");
+    X.pib = static_cast<const int*>(qi);
+    X.pdb = static_cast<const double*>(qd);
+  }
   if (aerosolCellsPerGroup(kmcd->dev, X, kmcd->maxLds) < 1)
     return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": one cell does not fit the LDS");
   *out = h.release();
@@ -1406,6 +1475,9 @@ int tchem_aerosolchem_run_device_batch(const tchem_atmchem_options_t* opts, int 
   if (!amcd || nBatch < 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": bad argument");
   if (opts && opts->precision != TCHEM_PRECISION_FP64)
     return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": the aerosol integrator is FP64 only");
+  if (opts && opts->lu == TCHEM_LU_SPARSE && !amcd->S.enabled)
+    return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": no sparse programs for this model");
+  const bool sparse = amcd->S.enabled && !(opts && opts->lu == TCHEM_LU_DENSE);
   if (nBatch == 0) return TCHEM_OK;
   if (!tol_newton || !tol_time || !tadv || !state || !t_out || !dt_out || !state_out ||
       (amcd->X.nPart > 0 && !number_conc))
@@ -1427,7 +1499,15 @@ int tchem_aerosolchem_run_device_batch(const tchem_atmchem_options_t* opts, int 
   a.stats = stats;
   a.kf = k->kf;
   a.numberConc = number_conc;
-  const int cells = aerosolCellsPerGroup(a.M, a.X, k->maxLds);
+  int cells = aerosolCellsPerGroup(a.M, a.X, k->maxLds);
+  if (sparse) {
+    a.sparse = 1;
+    a.S = amcd->S;
+    a.cellBytes = int(aerosolSparseCellBytes(a.M, a.S, a.X));
+    cells = int(k->maxLds / size_t(a.cellBytes));
+    if (cells > 8) cells = 8;
+  }
+  if (cells < 1) return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": one cell does not fit the LDS");
   const int perGroup = cells < nBatch ? cells : nBatch;
   a.cellsPerGroup = perGroup;
   const int grid = (nBatch + perGroup - 1) / perGroup;

@@ -101,6 +101,13 @@ struct DevAerosol {
   int oMw, oDens;   // [nAero] kg mol-1, kg m-3
   int oGas, oAero;  // [nSimpol] gas unknown (internal order), aerosol species within a particle
   int oPar;         // [nSimpol][8] B1 B2 B3 B4 diffusion mw Nstar computeAlpha
+  // sparse form: the mechanism's programs compiled over the mt unknowns
+  // (IntegratorArgs::S) live in these buffers; oJpos indexes X.ib:
+  // [nPart * nSimpol][2 + 2 nAero] factor positions of the transfer
+  // derivatives (g,g), (a,g), (g, particle species i), (a, particle species i)
+  const int* pib;
+  const double* pdb;
+  int oJpos;
 };
 
 struct IntegratorArgs {
@@ -134,6 +141,17 @@ struct IntegratorArgs {
 // the source term, reciprocal pivots and interchanges over mt, SIMPOL
 // constants [nSimpol][4], particle sums [nPart][4] and transfer terms
 // [nPart * nSimpol][6].
+// LDS bytes of one cell of the sparse aerosol integrator: the sparse cell
+// region (sparseCellBytes, reciprocal pivots over mt) plus the unknown copy,
+// source term and SIMPOL scratch of aerosolLdsBytes.
+inline size_t aerosolSparseCellBytes(const DevMech& M, const DevSparse& S, const DevAerosol& X) {
+  auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
+  return al(8 * size_t(S.trashSlot + 1)) + al(8 * size_t(X.mt + 2)) + al(8 * size_t(S.pairSlots)) +
+         al(8 * (size_t(M.nSpec) + 2 * size_t(M.nReac + 1) + size_t(M.nSeg + 1))) +
+         al(8 * (2 * size_t(X.mt) + 4 * size_t(X.nSimpol) + 4 * size_t(X.nPart) +
+                 6 * size_t(X.nPart) * size_t(X.nSimpol)));
+}
+
 inline size_t aerosolLdsBytes(const DevMech& M, const DevAerosol& X) {
   const size_t nScr = size_t(M.nReac > M.nPairs ? M.nReac : M.nPairs);
   size_t a = sizeof(double) * size_t(X.mt) * size_t(X.mt);

@@ -1356,7 +1356,8 @@ __device__ __forceinline__ double fuchs_sutugin(double Kn, double alpha) {
 // Transfer terms of every (particle, transfer) pair at the unknowns in L.x:
 // per-particle sums (lanes over particles), then k_c, k_e, diff and
 // d k_c / dV (lanes over pairs).  SingleParticleUtils.hpp:62-160.
-__device__ void simpol_terms(const IntegratorArgs& a, const AeroLds& L, int lane) {
+template <class LdsT>
+__device__ void simpol_terms(const IntegratorArgs& a, const LdsT& L, int lane) {
   const DevAerosol& X = a.X;
   const double* mw = X.db + X.oMw;
   const double* dens = X.db + X.oDens;
@@ -1398,11 +1399,8 @@ __device__ void simpol_terms(const IntegratorArgs& a, const AeroLds& L, int lane
   wave_sync();
 }
 
-template <int QM>
-struct AerosolBackend {
-  using TA = double;
-  using Lds = AeroLds;
-  struct Fact { int nswp = 0; };
+// unknown layout and per-cell bindings shared by the aerosol backends
+struct AerosolLayout {
   static __device__ __forceinline__ int unknowns(const IntegratorArgs& a) { return a.X.mt; }
   static __device__ __forceinline__ int rowDim(const IntegratorArgs& a) {
     return a.M.nSpec + 3 + a.X.nAero * a.X.nPart;
@@ -1415,7 +1413,8 @@ struct AerosolBackend {
   }
   // this cell's number concentrations, temperature and pressure, and the
   // SIMPOL constants (functions of T and p only)
-  static __device__ void bind(const IntegratorArgs& a, Lds& L, int cell) {
+  template <class LdsT>
+  static __device__ void bind(const IntegratorArgs& a, LdsT& L, int cell) {
     const double* row = a.state + size_t(cell) * rowDim(a);
     L.nc = a.numberConc + size_t(cell) * a.X.nPart;
     L.T = row[2];
@@ -1424,6 +1423,13 @@ struct AerosolBackend {
     for (int s = lane; s < a.X.nSimpol; s += kWave) simpol_constants(a.X.db + a.X.oPar + 8 * s, L.T, L.p, L.sc + 4 * s);
     wave_sync();
   }
+};
+
+template <int QM>
+struct AerosolBackend : AerosolLayout {
+  using TA = double;
+  using Lds = AeroLds;
+  struct Fact { int nswp = 0; };
   static __device__ void stage(const IntegratorArgs&, char*) {}
   static __device__ Lds carve(char* base, const IntegratorArgs& a, int wave) {
     return carve_aero(base + size_t(wave) * size_t(a.cellBytes), a.M, a.X);
@@ -1549,6 +1555,169 @@ struct AerosolBackend {
   static __device__ void solve(const IntegratorArgs& a, const Lds& L, int lane, double (&x)[QM], const Fact& fs) {
     lu_forward<QM>(L, a.X.mt, lane, x, fs.nswp);
     lu_back<QM>(L, a.X.mt, lane, x);
+  }
+};
+
+
+// Sparse form of the aerosol integrator: the mechanism's compiled programs
+// built over the mt unknowns (gas Jacobian pattern + the transfer
+// derivatives' pattern; particle unknowns after the gas species), read from
+// device memory; the transfer terms and their derivatives are added to the
+// source term and to the factor entries by one lane, in the dense form's
+// order.
+struct AeroSpLds : SpLds<double> {
+  double* x;    // [mt] unknowns
+  double* f;    // [mt] source term
+  double* sc;   // [nSimpol][4]
+  double* ps;   // [nPart][4]
+  double* pd;   // [nPart * nSimpol][6]
+  const double* nc;
+  double T, p;
+};
+
+__device__ AeroSpLds carve_aero_sparse(char* base, const IntegratorArgs& a, int wave) {
+  auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
+  const DevMech& M = a.M;
+  const DevSparse& S = a.S;
+  const DevAerosol& X = a.X;
+  AeroSpLds L;
+  L.hdr = reinterpret_cast<const unsigned*>(X.pib + S.oProgHdr);
+  L.body = reinterpret_cast<const unsigned*>(X.pib + S.oProgBody);
+  L.ehdr = reinterpret_cast<const unsigned*>(X.pib + S.oEvalHdr);
+  L.ebody = reinterpret_cast<const unsigned*>(X.pib + S.oEvalBody);
+  L.coef = X.pdb + S.oCoef;
+  char* cell = base + size_t(wave) * size_t(a.cellBytes);
+  size_t off = 0;
+  L.A = reinterpret_cast<double*>(cell);
+  off += al(8 * size_t(S.trashSlot + 1));
+  L.rdiag = reinterpret_cast<double*>(cell + off);
+  off += al(8 * size_t(X.mt + 2));
+  L.dpr = reinterpret_cast<double*>(cell + off);
+  off += al(8 * size_t(S.pairSlots));
+  L.c = reinterpret_cast<double*>(cell + off);
+  L.kf = L.c + M.nSpec;
+  L.rop = L.kf + M.nReac + 1;
+  L.seg = L.rop + M.nReac + 1;
+  off += al(8 * (size_t(M.nSpec) + 2 * size_t(M.nReac + 1) + size_t(M.nSeg + 1)));
+  L.x = reinterpret_cast<double*>(cell + off);
+  L.f = L.x + X.mt;
+  L.sc = L.f + X.mt;
+  L.ps = L.sc + 4 * X.nSimpol;
+  L.pd = L.ps + 4 * X.nPart;
+  L.mu = L.w = L.v = nullptr;
+  L.src = nullptr;
+  L.nc = nullptr;
+  L.T = L.p = 0.0;
+  return L;
+}
+
+template <int QM>
+struct AerosolSparseBackend : AerosolLayout {
+  using TA = double;
+  using Lds = AeroSpLds;
+  struct Fact {};
+  static __device__ void stage(const IntegratorArgs&, char*) {}
+  static __device__ Lds carve(char* base, const IntegratorArgs& a, int wave) { return carve_aero_sparse(base, a, wave); }
+  static __device__ void init(const IntegratorArgs& a, const Lds& L, int lane) {
+    if (lane == 0) {
+      L.kf[a.M.nReac] = 0.0;
+      L.rop[a.M.nReac] = 0.0;
+      L.dpr[a.M.nPairs] = 0.0;
+      L.A[a.S.zeroSlot] = 0.0;
+      L.rdiag[a.X.mt] = 0.0;
+    }
+    wave_sync();
+  }
+  static __device__ void evaluate(const IntegratorArgs& a, const Lds& L, int lane, const double (&u)[QM],
+                                  const double (&bc)[QM], double ch, bool newton, bool refresh, double (&f)[QM],
+                                  double (&x)[QM]) {
+    const DevMech& M = a.M;
+    const DevAerosol& X = a.X;
+    const int mt = X.mt;
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const int i = lane + kWave * q;
+      if (i < mt) L.x[i] = u[q];
+      if (i < M.m) L.c[i] = u[q];
+    }
+    wave_sync();
+    sp_rates(a.S, L, lane, refresh);
+    wave_sync();
+    sp_sums(a.S, L, lane, refresh, 1.0, -ch);
+    wave_sync();
+    double fg[QM];
+    row_sums<QM>(M, L.seg, nullptr, lane, fg);
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const int i = lane + kWave * q;
+      if (i < mt) L.f[i] = i < M.m ? fg[q] : 0.0;
+    }
+    simpol_terms(a, L, lane);  // syncs first
+    if (lane == 0) {
+      const double* mw = X.db + X.oMw;
+      const double* dens = X.db + X.oDens;
+      const int nj = 2 + 2 * X.nAero;
+      const double scale = -ch;
+      for (int ip = 0; ip < X.nPart; ++ip) {
+        const int base = M.m + ip * X.nAero;
+        const double moles = L.ps[4 * ip + 3];
+        for (int s = 0; s < X.nSimpol; ++s) {
+          const int j = ip * X.nSimpol + s;
+          const double* o = L.pd + 6 * j;
+          const double kc = o[0], ke = o[1], diff = o[2], dkdv = o[3];
+          const double keq = L.sc[4 * s + 3], kg = L.sc[4 * s + 2], n = L.nc[ip];
+          const int g = X.ib[X.oGas + s], ai = base + X.ib[X.oAero + s];
+          L.f[g] += -n * diff;
+          L.f[ai] += diff / kg;
+          if (refresh) {
+            const int* jp = X.ib + X.oJpos + j * nj;
+            const double xg = L.x[g], ya = L.x[ai];
+            L.A[jp[0]] += scale * (-n * kc);
+            L.A[jp[1]] += scale * (kc / kg);
+            for (int i = 0; i < X.nAero; ++i) {
+              const double dkc = dkdv / dens[i];
+              const double dke = keq * dkc / moles - ke / (moles * mw[i]);
+              double dd = xg * dkc - ya * dke;
+              if (base + i == ai) dd -= ke;
+              L.A[jp[2 + i]] += scale * (-n * dd);
+              L.A[jp[2 + X.nAero + i]] += scale * (dd / kg);
+            }
+          }
+        }
+      }
+    }
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const int i = lane + kWave * q;
+      f[q] = i < mt ? L.f[i] : 0.0;
+      x[q] = (newton && i < mt) ? -(u[q] - bc[q] - ch * f[q]) : 0.0;
+      if (newton && i < mt) L.A[a.S.nnzF + i] = x[q];
+    }
+    if (newton) wave_sync();
+  }
+  static __device__ __forceinline__ void get_x(const IntegratorArgs& a, const Lds& L, int lane, double (&x)[QM]) {
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const int i = lane + kWave * q;
+      x[q] = (i < a.X.mt) ? L.A[a.S.nnzF + i] * L.rdiag[i] : 0.0;
+    }
+  }
+  static __device__ void back(const IntegratorArgs& a, const Lds& L, int lane) {
+    run_program(L.hdr + a.S.luRounds, L.body + a.S.backBody, a.S.backRounds, L.A, L.rdiag, a.X.mt, a.S.nnzF,
+                a.S.trashSlot, lane);
+  }
+  static __device__ bool factor_solve(const IntegratorArgs& a, const Lds& L, int lane, double (&x)[QM], Fact&) {
+    if (!run_program(L.hdr, L.body, a.S.luRounds, L.A, L.rdiag, a.X.mt, a.S.nnzF, a.S.trashSlot, lane)) return false;
+    back(a, L, lane);
+    get_x(a, L, lane, x);
+    return true;
+  }
+  static __device__ void solve(const IntegratorArgs& a, const Lds& L, int lane, double (&x)[QM], const Fact&) {
+    run_program(L.hdr + a.S.luRounds + a.S.backRounds, L.body + a.S.fwdBody, a.S.fwdRounds, L.A, L.rdiag, a.X.mt,
+                a.S.nnzF, a.S.trashSlot, lane);
+    back(a, L, lane);
+    get_x(a, L, lane, x);
   }
 };
 
@@ -2226,7 +2395,12 @@ hipError_t launchRates(const DevMech& M, int nBatch, const double* state, const 
 }
 
 // ---- aerosol integrator launches (FP64 dense, mt <= 128 unknowns) ----
-static const void* aerosolKernel(int mt) {
+static const void* aerosolKernel(int mt, bool sparse) {
+  if (sparse) {
+    if (mt <= 64) return integratorFn<AerosolSparseBackend<1>, 1, 512>();
+    if (mt <= 128) return integratorFn<AerosolSparseBackend<2>, 2, 512>();
+    return nullptr;
+  }
   if (mt <= 64) return integratorFn<AerosolBackend<1>, 1, 256>();
   if (mt <= 128) return integratorFn<AerosolBackend<2>, 2, 256>();
   return nullptr;
@@ -2241,8 +2415,9 @@ int aerosolCellsPerGroup(const DevMech& M, const DevAerosol& X, size_t maxLds) {
 }
 
 hipError_t launchAerosol(const IntegratorArgs& a, int grid, size_t lds, hipStream_t stream) {
-  const void* fn = aerosolKernel(a.X.mt);
-  if (!fn || a.cellsPerGroup < 1 || a.cellsPerGroup > 4) return hipErrorInvalidValue;
+  const bool sparse = a.sparse != 0;
+  const void* fn = aerosolKernel(a.X.mt, sparse);
+  if (!fn || a.cellsPerGroup < 1 || a.cellsPerGroup > (sparse ? 8 : 4)) return hipErrorInvalidValue;
   IntegratorArgs arg = a;
   void* params[] = {&arg};
   return hipLaunchKernel(fn, dim3(grid), dim3(64 * a.cellsPerGroup), params, lds, stream);

@@ -9,12 +9,15 @@
 // (fresh factorisation, and forward + back substitution with stored factors)
 // with a long-double dense partial-pivoting solve to 1e-6 relative.
 //
-// usage: schedule_check <mechanism.yaml>   (exit status 0 when every check passes)
+// usage: schedule_check <mechanism.yaml> [<aerosol.yaml>]   (exit status 0
+// when every check passes).  With an aerosol file, the LU program of the
+// aerosol problem (gas + particle unknowns, aerosolTables) is checked too.
 #include <cmath>
 #include <cstdio>
 #include <map>
 #include <random>
 #include <vector>
+#include "aerosol_model.hpp"
 #include "kinetic_model.hpp"
 #include "mechanism.hpp"
 #include "schedule.hpp"
@@ -324,6 +327,19 @@ int main(int argc, char** argv) {
     std::printf("-- %s programs\n", tr ? "transposed" : "forward");
     check_eval_program(t, s);
     check_lu_program(t, s);
+  }
+  if (argc > 2) {
+    AerosolModel am = loadAerosolModel(argv[2], km);
+    std::vector<std::array<int, 2>> ent;
+    MechTables ta = aerosolTables(t, am, &ent);
+    SparseSchedule s = buildSparseSchedule(ta, false);
+    std::printf("-- aerosol LU program (%d transfer derivatives)\n", int(ent.size()));
+    for (const auto& e : ent) {
+      bool found = false;
+      for (int q = s.rowPtr[e[0]]; q < s.rowPtr[e[0] + 1]; ++q) found = found || s.colIdx[q] == e[1];
+      CHECK(found, "transfer derivative (%d, %d) outside the factor pattern", e[0], e[1]);
+    }
+    check_lu_program(ta, s);
   }
   std::printf("%s (%d failures)\n", fails ? "FAILED" : "OK", fails);
   return fails ? 1 : 0;

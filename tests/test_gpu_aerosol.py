@@ -97,7 +97,7 @@ def test_rhs_matches_oracle(models, gpu, pkg, name):
 
 
 def gpu_intervals(pkg, torch, amcd, amd, state, nc, tend, dtmax, tol_newton, atol_time, rtol_time, steps,
-                  max_intervals, dtmin=1e-20, max_newton=100, rows=False):
+                  max_intervals, dtmin=1e-20, max_newton=100, rows=False, lu=None):
     """The TChem_AerosolChemistry.x interval loop on the GPU: per interval one
     runDeviceBatch, then tadv <- (t_out, dt_out)."""
     n = state.shape[0]
@@ -112,7 +112,8 @@ def gpu_intervals(pkg, torch, amcd, amd, state, nc, tend, dtmax, tol_newton, ato
     total = np.zeros((n, 6), np.int64)
     out = [np.c_[np.full(n, -1.0), np.zeros(n), np.zeros(n), state]]
     for it in range(max_intervals):
-        pkg.AerosolChemistry.runDeviceBatch(tn, tt, None, tadv, st, ncd, t, dt, st, amcd, stats=stats)
+        pkg.AerosolChemistry.runDeviceBatch(tn, tt, None, tadv, st, ncd, t, dt, st, amcd, stats=stats,
+                                            lu=pkg.LU_AUTO if lu is None else lu)
         pkg.AtmosphericChemistry.updateTimeAdvance(tadv, t, dt)
         s = pkg.stats_from_tensor(stats)
         for j, f in enumerate(pkg.COUNTERS):
@@ -148,20 +149,21 @@ def oracle_intervals(oracle_mod, o, state, nc, tend, dtmax, tol_newton, atol_tim
     return st, t, dt, total
 
 
+@pytest.mark.parametrize("lu", ["sparse", "dense"])
 @pytest.mark.parametrize("ref_name,name,cells", [
     ("simpol_full_gas.npy", "simpol", 1),
     ("simpol_batched_full_gas.npy", "simpol", 3),
     ("cb05_soa_batched_full_gas.npy", "soa", 3),
 ])
-def test_tines_ode_goldens_on_gpu(models, gpu, pkg, ref_name, name, cells):
+def test_tines_ode_goldens_on_gpu(models, gpu, pkg, ref_name, name, cells, lu):
     """TinesODE ctest (dtmin 1e-20, dtmax 10, tend 600, Newton (1e-18, 1e-8),
     two intervals of 10 steps, time tolerances (1e-12, 1e-4)) with
     compare_tchem.py's norm criterion at 1e-8."""
     kmd, kmcd, amd, amcd, o = models[name]
     state, nc = scenario(pkg, kmd, amd, name, cells)
-    print("case:", ref_name, flush=True)
+    print("case:", ref_name, lu, flush=True)
     *_, rows = gpu_intervals(pkg, gpu, amcd, amd, state, nc, 600.0, 10.0, (1e-18, 1e-8), 1e-12, 1e-4, 10, 2,
-                             rows=True)
+                             rows=True, lu=pkg.LU_SPARSE if lu == "sparse" else pkg.LU_DENSE)
     # rows are interval-major like the .dat writer
     ref = np.load(golden(ref_name))
     assert rows.shape == ref.shape
@@ -172,17 +174,28 @@ def test_tines_ode_goldens_on_gpu(models, gpu, pkg, ref_name, name, cells):
     assert worst <= 1e-8, worst
 
 
-def test_integration_matches_oracle(models, gpu, pkg, oracle_mod):
+@pytest.fixture(scope="module")
+def soa_oracle_run(models, pkg, oracle_mod):
+    kmd, kmcd, amd, amcd, o = models["soa"]
+    state, nc = random_cells(pkg, kmd, amd, "soa", 48, seed=5)
+    return state, nc, oracle_intervals(oracle_mod, o, state, nc, *SOA_ARGS)
+
+
+SOA_ARGS = (600.0, 10.0, (1e-12, 1e-8), 1e-12, 1e-6, 100, 1000)
+
+
+@pytest.mark.parametrize("lu", ["sparse", "dense"])
+def test_integration_matches_oracle(models, gpu, pkg, soa_oracle_run, lu):
     """CB05CL_AE5 + SOA partitioning, 48 perturbed cells, 600 s: the FP64 GPU
-    form against the oracle.  Cells with identical step sequences agree to
-    1e-9; any other cell stays within the integration tolerance."""
+    forms (sparse programs, dense LU) against the oracle.  Cells with
+    identical step sequences agree to 1e-9; any other cell stays within the
+    integration tolerance."""
     kmd, kmcd, amd, amcd, o = models["soa"]
     n = 48
-    state, nc = random_cells(pkg, kmd, amd, "soa", n, seed=5)
-    args = (600.0, 10.0, (1e-12, 1e-8), 1e-12, 1e-6, 100, 1000)
-    print("case: soa 48 cells seed 5", flush=True)
-    g_state, g_t, g_dt, g_stats, _ = gpu_intervals(pkg, gpu, amcd, amd, state, nc, *args)
-    o_state, o_t, o_dt, o_stats = oracle_intervals(oracle_mod, o, state, nc, *args)
+    state, nc, (o_state, o_t, o_dt, o_stats) = soa_oracle_run
+    print("case: soa 48 cells seed 5", lu, flush=True)
+    g_state, g_t, g_dt, g_stats, _ = gpu_intervals(pkg, gpu, amcd, amd, state, nc, *SOA_ARGS,
+                                                   lu=pkg.LU_SPARSE if lu == "sparse" else pkg.LU_DENSE)
     assert np.all(g_t == 600.0) and np.all(o_t == 600.0)
     assert g_stats[:, 5].sum() == 0 and o_stats["failed"].sum() == 0
     cols = np.r_[3:3 + kmd.nActive, 3 + kmd.nSpec:amd.stateDim]

@@ -17,7 +17,8 @@ CSRC = os.path.join(ROOT, "tchem-atm_amd", "csrc")
 def checker(tmp_path_factory):
     exe = str(tmp_path_factory.mktemp("native") / "schedule_check")
     srcs = [os.path.join(ROOT, "tests", "native", "schedule_check.cpp")] + [
-        os.path.join(CSRC, f) for f in ("yaml_lite.cpp", "kinetic_model.cpp", "mechanism.cpp", "schedule.cpp")]
+        os.path.join(CSRC, f) for f in ("yaml_lite.cpp", "kinetic_model.cpp", "mechanism.cpp", "schedule.cpp",
+                                        "aerosol_model.cpp")]
     subprocess.run(["g++", "-O2", "-std=c++17", "-I" + CSRC, *srcs, "-o", exe], check=True)
     return exe
 
@@ -28,3 +29,13 @@ def test_sparse_programs(checker, mech):
     r = subprocess.run([checker, golden(mech)], capture_output=True, text=True)
     print(r.stdout[-2000:])
     assert r.returncode == 0, r.stdout[-3000:]
+
+
+@pytest.mark.parametrize("chem,aero", [("cb05_soa_gas.yaml", "cb05_soa_aero.yaml"),
+                                       ("simpol_gas.yaml", "simpol_aero.yaml")])
+def test_aerosol_lu_program(checker, chem, aero):
+    """The sparse aerosol form's LU program over gas + particle unknowns."""
+    r = subprocess.run([checker, golden(chem), golden(aero)], capture_output=True, text=True)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout[-3000:]
+    assert "aerosol LU program" in r.stdout
