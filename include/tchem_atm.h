@@ -335,8 +335,8 @@ int tchem_scenario_particles(tchem_amd_t amd, const char* path, int nBatch, int 
                              double* number_conc_host, int* found);
 
 /* Replaces create_AerosolModelConstData<device>(amd) (AerosolModelData.hpp:94-108)
- * for use with the gas model's kmcd, which must outlive it.  At most 128
- * unknowns per cell. */
+ * for use with the gas model's kmcd, which must outlive it.  At most 256
+ * unknowns per cell (the dense form: 128). */
 int tchem_amcd_create(tchem_amd_t amd, tchem_kmcd_t kmcd, tchem_amcd_t* out);
 int tchem_amcd_destroy(tchem_amcd_t amcd);
 
@@ -350,10 +350,12 @@ int tchem_aerosol_rhs_run_device_batch(int nBatch, const double* state, const do
 /* Replaces AerosolChemistry::runDeviceBatch(policy, tol_newton, tol_time, fac,
  * tadv, state, number_conc, t_out, dt_out, state_out, kmcd, amcd)
  * (src/core/TChem_AerosolChemistry.hpp, .cpp:31-258): TR-BDF2 over the gas
- * and particle unknowns of every cell, FP64, dense LU with partial pivoting,
- * analytic Jacobian.  Arguments as tchem_atmchem_run_device_batch with
- * tol_time [mt][2] and state rows of stateDim; opts NULL or FP64 precision
- * (other fields ignored). */
+ * and particle unknowns of every cell, FP64, analytic Jacobian; opts->lu
+ * selects the sparse form (AUTO, SPARSE: the mechanism's compiled programs
+ * over all unknowns) or DENSE (LU with partial pivoting, <= 128 unknowns).
+ * Arguments as tchem_atmchem_run_device_batch with tol_time [mt][2] and
+ * state rows of stateDim; opts NULL or FP64 precision (other fields
+ * ignored). */
 int tchem_aerosolchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBatch, const double* tol_newton,
                                        const double* tol_time, double* fac, const tchem_time_advance_t* tadv,
                                        double* state, const double* number_conc, double* t_out, double* dt_out,

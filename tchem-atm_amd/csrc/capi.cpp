@@ -1321,7 +1321,7 @@ int tchem_amcd_create(tchem_amd_t amd, tchem_kmcd_t kmcd, tchem_amcd_t* out) {
   if (am.nGasActive != t.m || amd->km.nSpec != t.nSpec)
     return setError(TCHEM_ERR_INVALID, std::string(who) + ": the aerosol model was read against another gas model");
   const int mt = t.m + am.nSpec * am.nParticles;
-  if (mt > 128) return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": more than 128 unknowns per cell");
+  if (mt > 256) return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": more than 256 unknowns per cell");
   std::unique_ptr<tchem_amcd_s> h(new tchem_amcd_s);
   h->kmcd = kmcd;
   h->device = kmcd->device;
@@ -1426,7 +1426,10 @@ int tchem_amcd_create(tchem_amd_t amd, tchem_kmcd_t kmcd, tchem_amcd_t* out) {
     X.pib = static_cast<const int*>(qi);
     X.pdb = static_cast<const double*>(qd);
   }
-  if (aerosolCellsPerGroup(kmcd->dev, X, kmcd->maxLds) < 1)
+  const bool denseFits = aerosolCellsPerGroup(kmcd->dev, X, kmcd->maxLds) >= 1;
+  const bool sparseFits = h->S.enabled && aerosolSparseCellBytes(kmcd->dev, h->S, X) <= kmcd->maxLds;
+  if (!sparseFits) h->S = DevSparse{};
+  if (!denseFits && !sparseFits)
     return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": one cell does not fit the LDS");
   *out = h.release();
   return TCHEM_OK;
@@ -1460,7 +1463,8 @@ int tchem_aerosol_rhs_run_device_batch(int nBatch, const double* state, const do
   a.state = const_cast<double*>(state);
   a.numberConc = number_conc;
   a.state_out = omega;  // the kernel's output rows [nBatch][mt]
-  hipError_t e = launchAerosolRhs(a, size_t(a.cellBytes), static_cast<hipStream_t>(stream));
+  const size_t lds = (aerosolLdsBytes(a.M, a.X, false) + 15) & ~size_t(15);
+  hipError_t e = launchAerosolRhs(a, lds, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hipFail(e, "aerosol_rhs_kernel launch");
   return TCHEM_OK;
 }
@@ -1478,6 +1482,8 @@ int tchem_aerosolchem_run_device_batch(const tchem_atmchem_options_t* opts, int 
   if (opts && opts->lu == TCHEM_LU_SPARSE && !amcd->S.enabled)
     return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": no sparse programs for this model");
   const bool sparse = amcd->S.enabled && !(opts && opts->lu == TCHEM_LU_DENSE);
+  if (!sparse && aerosolCellsPerGroup(amcd->kmcd->dev, amcd->X, amcd->kmcd->maxLds) < 1)
+    return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": the dense form needs at most 128 unknowns");
   if (nBatch == 0) return TCHEM_OK;
   if (!tol_newton || !tol_time || !tadv || !state || !t_out || !dt_out || !state_out ||
       (amcd->X.nPart > 0 && !number_conc))

@@ -152,9 +152,9 @@ inline size_t aerosolSparseCellBytes(const DevMech& M, const DevSparse& S, const
                  6 * size_t(X.nPart) * size_t(X.nSimpol)));
 }
 
-inline size_t aerosolLdsBytes(const DevMech& M, const DevAerosol& X) {
+inline size_t aerosolLdsBytes(const DevMech& M, const DevAerosol& X, bool withMatrix = true) {
   const size_t nScr = size_t(M.nReac > M.nPairs ? M.nReac : M.nPairs);
-  size_t a = sizeof(double) * size_t(X.mt) * size_t(X.mt);
+  size_t a = withMatrix ? sizeof(double) * size_t(X.mt) * size_t(X.mt) : 0;
   a = (a + 15) & ~size_t(15);
   const size_t d = size_t(M.nSpec) + size_t(M.nReac) + nScr + size_t(M.nSeg) + 3 * size_t(X.mt) +
                    4 * size_t(X.nSimpol) + 4 * size_t(X.nPart) + 6 * size_t(X.nPart) * size_t(X.nSimpol);

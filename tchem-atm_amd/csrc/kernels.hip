@@ -1301,10 +1301,10 @@ struct AeroLds : CellLds<double> {
   double T, p;
 };
 
-__device__ AeroLds carve_aero(char* base, const DevMech& M, const DevAerosol& X) {
+__device__ AeroLds carve_aero(char* base, const DevMech& M, const DevAerosol& X, bool withMatrix = true) {
   AeroLds L;
   const int mt = X.mt;
-  size_t off = sizeof(double) * size_t(mt) * size_t(mt);
+  size_t off = withMatrix ? sizeof(double) * size_t(mt) * size_t(mt) : 0;
   off = (off + 15) & ~size_t(15);
   L.A = reinterpret_cast<double*>(base);
   L.c = reinterpret_cast<double*>(base + off);
@@ -1729,7 +1729,7 @@ __global__ __launch_bounds__(64) void aerosol_rhs_kernel(IntegratorArgs a) {
   extern __shared__ __align__(16) char lds_raw[];
   const int lane = threadIdx.x, cell = blockIdx.x;
   if (cell >= a.nBatch) return;
-  AeroLds L = carve_aero(lds_raw, a.M, a.X);
+  AeroLds L = carve_aero(lds_raw, a.M, a.X, false);  // no iteration matrix
   AerosolBackend<QM>::bind(a, L, cell);
   const double* row = a.state + size_t(cell) * AerosolBackend<QM>::rowDim(a);
   const DevMech& M = a.M;
@@ -2399,6 +2399,7 @@ static const void* aerosolKernel(int mt, bool sparse) {
   if (sparse) {
     if (mt <= 64) return integratorFn<AerosolSparseBackend<1>, 1, 512>();
     if (mt <= 128) return integratorFn<AerosolSparseBackend<2>, 2, 512>();
+    if (mt <= 256) return integratorFn<AerosolSparseBackend<4>, 4, 512>();
     return nullptr;
   }
   if (mt <= 64) return integratorFn<AerosolBackend<1>, 1, 256>();
@@ -2406,9 +2407,10 @@ static const void* aerosolKernel(int mt, bool sparse) {
   return nullptr;
 }
 
-// cells (wavefronts) per workgroup of the aerosol integrator: up to 4, as
-// many as the LDS holds (0 when one cell does not fit)
+// cells (wavefronts) per workgroup of the dense aerosol integrator: up to
+// 4, as many as the LDS holds (0 when one cell does not fit or mt > 128)
 int aerosolCellsPerGroup(const DevMech& M, const DevAerosol& X, size_t maxLds) {
+  if (X.mt > 128) return 0;
   const size_t cb = (aerosolLdsBytes(M, X) + 15) & ~size_t(15);
   int c = int(maxLds / cb);
   return c > 4 ? 4 : c;
@@ -2428,7 +2430,8 @@ hipError_t launchAerosolRhs(const IntegratorArgs& a, size_t lds, hipStream_t str
   IntegratorArgs arg = a;
   void* params[] = {&arg};
   const void* fn = a.X.mt <= 64 ? reinterpret_cast<const void*>(&aerosol_rhs_kernel<1>)
-                                : (a.X.mt <= 128 ? reinterpret_cast<const void*>(&aerosol_rhs_kernel<2>) : nullptr);
+                  : (a.X.mt <= 128 ? reinterpret_cast<const void*>(&aerosol_rhs_kernel<2>)
+                                   : (a.X.mt <= 256 ? reinterpret_cast<const void*>(&aerosol_rhs_kernel<4>) : nullptr));
   if (!fn) return hipErrorInvalidValue;
   return hipLaunchKernel(fn, dim3(a.nBatch), dim3(64), params, lds, stream);
 }

@@ -302,3 +302,52 @@ def test_driver_interface_integrates_every_batch_entry(gpu, pkg):
     moved = np.abs(out[1] - rows[1]) > 1e-6 * np.abs(rows[1])
     assert moved[3:3 + n_gas].sum() > 10 and np.all(out[1, :3] == rows[1, :3])
     assert np.abs(out[1] - out[0])[3:].max() > 0
+
+
+def test_many_particles_match_oracle(gpu, pkg, oracle_mod):
+    """20 particles per cell (178 unknowns: the sparse form's 4-slot
+    instance; the dense form is limited to 128 unknowns): 8 perturbed cells
+    with different particle loads per particle, 300 s, against the oracle."""
+    from oracle import amd as amd_mod
+    kmd = pkg.KineticModelData(golden("cb05_soa_gas.yaml"))
+    kmcd = kmd.createConstData()
+    amd = pkg.AerosolModelData(golden("cb05_soa_aero.yaml"), kmd)
+    amd.setNumberofParticles(20)
+    amcd = amd.createConstData(kmcd)
+    assert amd.nUnknowns == kmd.nActive + 5 * 20
+    o = oracle_mod.AerosolOracle(golden("cb05_soa_gas.yaml"), golden("cb05_soa_aero.yaml"))
+    a = o.a
+    a.nParticles = 20
+    o = oracle_mod.AerosolOracle(o.g, a)
+    rng = np.random.default_rng(21)
+    n = 8
+    g = pkg.setScenarioConditions(golden("cb05_soa_gas.yaml"), kmd)
+    state = np.zeros((n, amd.stateDim))
+    state[:, :g.shape[1]] = g[0]
+    state[:, 2] = rng.uniform(265, 295, n)
+    ma = kmd.nActive
+    state[:, 3:3 + ma] += 10.0 ** rng.uniform(-6, -3, (n, ma))
+    base = 3 + kmd.nSpec
+    state[:, base:] = 10.0 ** rng.uniform(-17, -13, (n, 100))
+    nc = 10.0 ** rng.uniform(4, 6, (n, 20))
+    args = (300.0, 10.0, (1e-12, 1e-8), 1e-12, 1e-6, 100, 1000)
+    print("case: soa 20 particles 8 cells", flush=True)
+    g_state, g_t, _, g_stats, _ = gpu_intervals(pkg, gpu, amcd, amd, state, nc, *args)
+    o_state, o_t, _, o_stats = oracle_intervals(oracle_mod, o, state, nc, *args)
+    assert np.all(g_t == 300.0) and g_stats[:, 5].sum() == 0 and o_stats["failed"].sum() == 0
+    same = (g_stats[:, 0] == o_stats["accepted"]) & (g_stats[:, 1] == o_stats["rejected"])
+    gas = np.abs(g_state[:, 3:3 + ma] - o_state[:, 3:3 + ma]) / np.maximum(np.abs(o_state[:, 3:3 + ma]), 1e-12)
+    part = np.abs(g_state[:, base:] - o_state[:, base:]) / np.maximum(np.abs(o_state[:, base:]), 1e-30)
+    err = np.maximum(gas.max(axis=1), part.max(axis=1))
+    print("identical step sequences:", same.sum(), "of", n, "max err there:", err[same].max())
+    assert same.sum() >= 6 and err[same].max() < 1e-9 and err.max() < 1e-3
+    # the dense form refuses more than 128 unknowns
+    st = gpu.tensor(state, device="cuda")
+    with pytest.raises(pkg.TChemError, match="at most 128"):
+        pkg.AerosolChemistry.runDeviceBatch(gpu.zeros(2, dtype=gpu.float64, device="cuda"),
+                                            gpu.zeros((amd.nUnknowns, 2), dtype=gpu.float64, device="cuda"),
+                                            None, pkg.time_advance_tensor(n, 0, 1, 1e-6, 1e-6, 1, 10, 1), st,
+                                            gpu.tensor(nc, device="cuda"), gpu.zeros(n, dtype=gpu.float64,
+                                                                                     device="cuda"),
+                                            gpu.zeros(n, dtype=gpu.float64, device="cuda"), st, amcd,
+                                            lu=pkg.LU_DENSE)
