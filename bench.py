@@ -403,6 +403,73 @@ def aerosol_run(cells, cpu_sample, threads, out_path=None):
             f.write(text + "\n")
 
 
+def e3sm_run(tiles, out_path=None):
+    """E3SM box model (AtmosphericChemistryE3SM, UCI mechanism): the
+    reference's 80-cell column scenario (per-cell photolysis rates and
+    external forcing) tiled `tiles` times, 1800 s, one persistent launch to
+    t_end per form (mixed and FP64 sparse).  There is no CPU restatement of
+    the E3SM kinetics (its parity is pinned by the reference's goldens), so
+    this mode has no CPU leg."""
+    import torch
+    T = importlib.import_module("tchem-atm_amd")
+    g = os.path.join(ROOT, "tests", "golden")
+    chem, scen = os.path.join(g, "uci_v2_test3.yaml"), os.path.join(g, "uci_col_input_conditions.yaml")
+    kmd = T.KineticModelData(chem)
+    kmcd = kmd.createConstData()
+    base = T.setScenarioConditions(scen, kmd)
+    n0 = base.shape[0]
+    photo = T.setScenarioConditionsPhotolysisReactions(scen, kmd, n0)
+    ext = T.setScenarioConditionsExternalForcing(scen, kmd, n0)
+    cells = n0 * tiles
+    m = kmd.nActive
+    dev = "cuda"
+    tn = torch.tensor([1e-18, 1e-8], dtype=torch.float64, device=dev)
+    tt = torch.tensor(np.tile([1e-12, 1e-6], (m, 1)), device=dev)
+    st0 = torch.tensor(np.tile(base, (tiles, 1)), device=dev)
+    ph = torch.tensor(np.tile(photo, (tiles, 1)), device=dev)
+    ex = torch.tensor(np.tile(ext, (tiles, 1)), device=dev)
+    st = torch.empty_like(st0)
+    t = torch.empty(cells, dtype=torch.float64, device=dev)
+    dt = torch.empty_like(t)
+    stats = T.stats_tensor(cells)
+    res = {"metric": "cell_hours_per_s", "workload": "uci_e3sm_column_tiled", "cells": cells, "t_end_s": 1800.0,
+           "mechanism": f"UCI v2 ({m} active species, {kmd.nReac} reactions)"}
+
+    def run(prec):
+        st.copy_(st0)
+        t.fill_(0.0)
+        dt.zero_()
+        tadv = T.time_advance_tensor(cells, 0.0, 1800.0, 1e-20, 1e-20, 3600.0, 20, 1 << 30, 1, dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        T.AtmosphericChemistryE3SM.runDeviceBatch(tn, tt, None, tadv, st, ph, ex, t, dt, st, kmcd, stats=stats,
+                                                  precision=prec, schedule=T.SCHEDULE_PERSISTENT)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    for name, prec in (("mixed", T.PRECISION_MIXED), ("fp64", T.PRECISION_FP64)):
+        run(prec)
+        wall = run(prec)
+        s = T.stats_from_tensor(stats)
+        res[name] = {"value": cells * 1800.0 / 3600.0 / wall, "wall_s": wall,
+                     "accepted_steps_per_cell": float(s["accepted"].mean()),
+                     "failed_cells": int(s["failed"].sum()),
+                     "all_reached_t_end": bool((t.cpu().numpy() == 1800.0).all())}
+        if name == "fp64":
+            fp64_state = st.cpu().numpy()
+        else:
+            mixed_state = st.cpu().numpy()
+    c = slice(3, 3 + m)
+    rel = np.abs(mixed_state[:, c] - fp64_state[:, c]) / np.maximum(np.abs(fp64_state[:, c]), 1e-12)
+    res["mixed_vs_fp64_relerr"] = {"max": float(rel.max()), "median": float(np.median(rel))}
+    res["value"] = res["mixed"]["value"]
+    text = json.dumps(res)
+    print(text, flush=True)
+    if out_path:
+        with open(out_path, "w") as f:
+            f.write(text + "\n")
+
+
 def copy_probe():
     import torch
     n = 1 << 27  # 1 GiB of float64
@@ -440,6 +507,9 @@ def main():
     ap.add_argument("--aerosol", metavar="OUT", nargs="?", const="",
                     help="time the SIMPOL.1 aerosol box model instead (JSON line, optionally to OUT)")
     ap.add_argument("--aerosol-cells", type=int, default=16384)
+    ap.add_argument("--e3sm", metavar="OUT", nargs="?", const="",
+                    help="time the E3SM box model on the tiled UCI column instead (JSON line, optionally to OUT)")
+    ap.add_argument("--e3sm-tiles", type=int, default=8192)
     ap.add_argument("--aerosol-cpu-sample", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -454,6 +524,9 @@ def main():
         return
     if args.config3 is not None:
         config3_run(args.config3_cells, args.config3 or None)
+        return
+    if args.e3sm is not None:
+        e3sm_run(args.e3sm_tiles, args.e3sm or None)
         return
     if args.aerosol is not None:
         aerosol_run(args.aerosol_cells, args.aerosol_cpu_sample, args.cpu_threads or min(16, os.cpu_count() or 1),
