@@ -316,10 +316,11 @@ def config3_run(cells, out_path=None, max_steps=8192):
             f.write(text + "\n")
 
 
-def aerosol_run(cells, cpu_sample, threads, out_path=None):
+def aerosol_run(cells, cpu_sample, threads, out_path=None, particles=1):
     """Single-particle SIMPOL.1 aerosol box model (AerosolChemistry): CB05CL_AE5
-    with the SOA scheme, one particle per cell, 3600 s, one FP64 launch to
-    t_end; cell-hours/s on the GPU and of the oracle on a sample of cells."""
+    with the SOA scheme, `particles` particles per cell (the file's 1 by
+    default), 3600 s, one FP64 launch to t_end; cell-hours/s on the GPU and
+    of the oracle on a sample of cells."""
     import torch
     T = importlib.import_module("tchem-atm_amd")
     S = importlib.import_module("tchem-atm_amd.scenarios")
@@ -329,6 +330,8 @@ def aerosol_run(cells, cpu_sample, threads, out_path=None):
     kmd = T.KineticModelData(chem)
     kmcd = kmd.createConstData()
     amd = T.AerosolModelData(aero, kmd)
+    if particles != amd.nParticles:
+        amd.setNumberofParticles(particles)
     amcd = amd.createConstData(kmcd)
     base = T.setScenarioConditions(chem, kmd)
     w, nc = S.aerosol_soa(kmd, base[0], amd.nSpec, amd.nParticles, n=cells)
@@ -354,16 +357,26 @@ def aerosol_run(cells, cpu_sample, threads, out_path=None):
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
-    run()
+    log(f"aerosol: {cells} cells, {amd.nParticles} particle(s), {mt} unknowns per cell")
+    log(f"aerosol: warm-up run {run():.2f} s")
     wall = run()
+    log(f"aerosol: timed run {wall:.2f} s")
     s = T.stats_from_tensor(stats)
-    res = {"metric": "cell_hours_per_s", "workload": "cb05_soa_simpol_1particle", "cells": cells,
+    res = {"metric": "cell_hours_per_s", "workload": f"cb05_soa_simpol_{particles}particle", "cells": cells,
            "unknowns_per_cell": mt, "t_end_s": w.tend, "value": w.cell_hours / wall, "wall_s": wall,
            "accepted_steps_per_cell": float(s["accepted"].mean()),
            "newton_iterations_per_step": float(s["newton_iterations"].sum() / max(1, s["accepted"].sum())),
            "failed_cells": int(s["failed"].sum()), "all_reached_t_end": bool((t.cpu().numpy() == w.tend).all())}
-    if cpu_sample > 0:
+    def aerosol_oracle():
         o = oracle.AerosolOracle(chem, aero)
+        if particles != o.a.nParticles:
+            a = o.a
+            a.nParticles = particles
+            o = oracle.AerosolOracle(o.g, a)
+        return o
+
+    if cpu_sample > 0:
+        o = aerosol_oracle()
         sample = np.ascontiguousarray(w.state[:cpu_sample])
         ta = np.zeros(cpu_sample, oracle.TADV_DTYPE)
         ta["tend"], ta["dt"], ta["dtmin"], ta["dtmax"] = w.tend, w.dt_in, w.dtmin, w.dtmax
@@ -374,6 +387,7 @@ def aerosol_run(cells, cpu_sample, threads, out_path=None):
         o.run_batch(np.array([w.atol_newton, w.rtol_newton]), np.tile([w.atol_time, w.rtol_time], (mt, 1)), ta,
                     sample, np.ascontiguousarray(nc[:cpu_sample]), to, dto, nthreads=threads)
         cpu_wall = time.perf_counter() - t0
+        log(f"aerosol: oracle sample of {cpu_sample} cells {cpu_wall:.2f} s")
         res["cpu_baseline"] = {"value": cpu_sample * w.tend / 3600.0 / cpu_wall, "unit": "cell_hours_per_s",
                                "cores": threads, "kind": "port",
                                "sample": f"first {cpu_sample} cells of the workload, oracle, analytic Jacobian"}
@@ -384,7 +398,7 @@ def aerosol_run(cells, cpu_sample, threads, out_path=None):
     bad = np.nonzero(s["failed"])[0]
     if len(bad):
         # do the cells the GPU gave up on fail in the oracle too?
-        o = oracle.AerosolOracle(chem, aero)
+        o = aerosol_oracle()
         ta = np.zeros(len(bad), oracle.TADV_DTYPE)
         ta["tend"], ta["dt"], ta["dtmin"], ta["dtmax"] = w.tend, w.dt_in, w.dtmin, w.dtmax
         ta["max_num_newton_iterations"], ta["num_time_iterations_per_interval"] = w.max_newton, 1 << 30
@@ -507,6 +521,7 @@ def main():
     ap.add_argument("--aerosol", metavar="OUT", nargs="?", const="",
                     help="time the SIMPOL.1 aerosol box model instead (JSON line, optionally to OUT)")
     ap.add_argument("--aerosol-cells", type=int, default=16384)
+    ap.add_argument("--aerosol-particles", type=int, default=1)
     ap.add_argument("--e3sm", metavar="OUT", nargs="?", const="",
                     help="time the E3SM box model on the tiled UCI column instead (JSON line, optionally to OUT)")
     ap.add_argument("--e3sm-tiles", type=int, default=8192)
@@ -530,7 +545,7 @@ def main():
         return
     if args.aerosol is not None:
         aerosol_run(args.aerosol_cells, args.aerosol_cpu_sample, args.cpu_threads or min(16, os.cpu_count() or 1),
-                    args.aerosol or None)
+                    args.aerosol or None, args.aerosol_particles)
         return
     if args.precision_study is not None:
         precision_study(args.study_cells, args.cpu_threads or min(16, os.cpu_count() or 1),

@@ -122,15 +122,20 @@ bool loadScenarioParticles(const std::string& path, const AerosolModel& am, cons
 }
 
 MechTables aerosolTables(const MechTables& t, const AerosolModel& am, std::vector<std::array<int, 2>>* entries) {
-  const int mt = t.m + am.nSpec * am.nParticles;
+  const int np = am.nSpec * am.nParticles, mt = t.m + np;
+  auto piv = [&](int u) { return aerosolPivot(u, t.m, np); };
   MechTables ta = t;
   ta.m = mt;
   std::vector<char> have(size_t(mt) * mt, 0);
-  for (size_t q = 0; q < t.jnzRow.size(); ++q) have[size_t(t.jnzRow[q]) * mt + t.jnzCol[q]] = 1;
+  for (size_t q = 0; q < t.jnzRow.size(); ++q) {
+    ta.jnzRow[q] = piv(t.jnzRow[q]);
+    ta.jnzCol[q] = piv(t.jnzCol[q]);
+    have[size_t(ta.jnzRow[q]) * mt + ta.jnzCol[q]] = 1;
+  }
   std::vector<std::array<int, 2>> ent;
   for (int ip = 0; ip < am.nParticles; ++ip)
     for (const SimpolParam& sp : am.simpol) {
-      const int g = t.iperm[sp.gas], base = t.m + ip * am.nSpec, ai = base + sp.aero;
+      const int g = piv(t.iperm[sp.gas]), base = piv(t.m + ip * am.nSpec), ai = base + sp.aero;
       ent.push_back({g, g});
       ent.push_back({ai, g});
       for (int i = 0; i < am.nSpec; ++i) ent.push_back({g, base + i});

@@ -63,14 +63,23 @@ AerosolModel loadAerosolModel(const std::string& path, const KineticModel& km);
 bool loadScenarioParticles(const std::string& path, const AerosolModel& am, const KineticModel& km, int nBatch,
                            std::vector<double>& state, std::vector<double>& numberConc);
 
+// Elimination order of the aerosol problem's unknowns (gas species in the
+// mechanism's internal order 0..m-1, then particle species m..mt-1): the
+// particle species are eliminated first, then the gas species in their
+// fill-reducing order.  Eliminating the gas species first would fill the
+// particle rows with every particle's columns (all particles couple through
+// the same gas species); this way the fill stays among the gas species that
+// share a particle.
+inline int aerosolPivot(int u, int m, int nParticleUnknowns) { return u < m ? u + nParticleUnknowns : u - m; }
+
 // Mechanism tables of the aerosol problem for the sparse programs: `t` with
-// m = mt = t.m + nSpec * nParticles unknowns (particle unknowns after the gas
-// species) and the Jacobian pattern extended by every transfer derivative
+// m = mt = t.m + nSpec * nParticles and the Jacobian pattern, in
+// elimination order (aerosolPivot), extended by every transfer derivative
 // (entries without gas-phase contributions).  `entries` receives the
-// (row, col) of each transfer derivative in the order the integrator adds
-// them: per (particle, transfer) pair (g,g), (a,g), (g, species i of the
-// particle) for every i, (a, species i) for every i, where g is the gas
-// species (internal order) and a the particle species of the transfer.
+// (row, col) of each transfer derivative, in elimination order, in the order
+// the integrator adds them: per (particle, transfer) pair (g,g), (a,g),
+// (g, species i of the particle) for every i, (a, species i) for every i,
+// where g is the gas species and a the particle species of the transfer.
 MechTables aerosolTables(const MechTables& t, const AerosolModel& am, std::vector<std::array<int, 2>>* entries);
 
 }  // namespace tchem_amd

@@ -1561,8 +1561,8 @@ struct AerosolBackend : AerosolLayout {
 
 // Sparse form of the aerosol integrator: the mechanism's compiled programs
 // built over the mt unknowns (gas Jacobian pattern + the transfer
-// derivatives' pattern; particle unknowns after the gas species), read from
-// device memory; the transfer terms and their derivatives are added to the
+// derivatives' pattern, in elimination order: particle species first, then
+// the gas species), read from device memory; the transfer terms and their derivatives are added to the
 // source term and to the factor entries by one lane, in the dense form's
 // order.
 struct AeroSpLds : SpLds<double> {
@@ -1653,38 +1653,48 @@ struct AerosolSparseBackend : AerosolLayout {
       if (i < mt) L.f[i] = i < M.m ? fg[q] : 0.0;
     }
     simpol_terms(a, L, lane);  // syncs first
-    if (lane == 0) {
+    // Transfer terms and derivatives.  Targets in a particle's own rows or
+    // columns are written by that particle's lane, the shared gas-species
+    // targets (f_g and the (g, g) entries) by lane 0 afterwards; each target
+    // receives its contributions in (particle, transfer) order either way.
+    const int nj = 2 + 2 * X.nAero;
+    const double scale = -ch;
+    for (int ip = lane; ip < X.nPart; ip += kWave) {
       const double* mw = X.db + X.oMw;
       const double* dens = X.db + X.oDens;
-      const int nj = 2 + 2 * X.nAero;
-      const double scale = -ch;
-      for (int ip = 0; ip < X.nPart; ++ip) {
-        const int base = M.m + ip * X.nAero;
-        const double moles = L.ps[4 * ip + 3];
-        for (int s = 0; s < X.nSimpol; ++s) {
-          const int j = ip * X.nSimpol + s;
-          const double* o = L.pd + 6 * j;
-          const double kc = o[0], ke = o[1], diff = o[2], dkdv = o[3];
-          const double keq = L.sc[4 * s + 3], kg = L.sc[4 * s + 2], n = L.nc[ip];
-          const int g = X.ib[X.oGas + s], ai = base + X.ib[X.oAero + s];
-          L.f[g] += -n * diff;
-          L.f[ai] += diff / kg;
-          if (refresh) {
-            const int* jp = X.ib + X.oJpos + j * nj;
-            const double xg = L.x[g], ya = L.x[ai];
-            L.A[jp[0]] += scale * (-n * kc);
-            L.A[jp[1]] += scale * (kc / kg);
-            for (int i = 0; i < X.nAero; ++i) {
-              const double dkc = dkdv / dens[i];
-              const double dke = keq * dkc / moles - ke / (moles * mw[i]);
-              double dd = xg * dkc - ya * dke;
-              if (base + i == ai) dd -= ke;
-              L.A[jp[2 + i]] += scale * (-n * dd);
-              L.A[jp[2 + X.nAero + i]] += scale * (dd / kg);
-            }
+      const int base = M.m + ip * X.nAero;
+      const double moles = L.ps[4 * ip + 3];
+      for (int s = 0; s < X.nSimpol; ++s) {
+        const int j = ip * X.nSimpol + s;
+        const double* o = L.pd + 6 * j;
+        const double kc = o[0], ke = o[1], diff = o[2], dkdv = o[3];
+        const double keq = L.sc[4 * s + 3], kg = L.sc[4 * s + 2], n = L.nc[ip];
+        const int g = X.ib[X.oGas + s], ai = base + X.ib[X.oAero + s];
+        L.f[ai] += diff / kg;
+        if (refresh) {
+          const int* jp = X.ib + X.oJpos + j * nj;
+          const double xg = L.x[g], ya = L.x[ai];
+          L.A[jp[1]] += scale * (kc / kg);
+          for (int i = 0; i < X.nAero; ++i) {
+            const double dkc = dkdv / dens[i];
+            const double dke = keq * dkc / moles - ke / (moles * mw[i]);
+            double dd = xg * dkc - ya * dke;
+            if (base + i == ai) dd -= ke;
+            L.A[jp[2 + i]] += scale * (-n * dd);
+            L.A[jp[2 + X.nAero + i]] += scale * (dd / kg);
           }
         }
       }
+    }
+    wave_sync();
+    if (lane == 0) {
+      for (int ip = 0; ip < X.nPart; ++ip)
+        for (int s = 0; s < X.nSimpol; ++s) {
+          const int j = ip * X.nSimpol + s;
+          const double n = L.nc[ip];
+          L.f[X.ib[X.oGas + s]] += -n * L.pd[6 * j + 2];
+          if (refresh) L.A[X.ib[X.oJpos + j * nj]] += scale * (-n * L.pd[6 * j]);
+        }
     }
     wave_sync();
 #pragma unroll
@@ -1692,15 +1702,21 @@ struct AerosolSparseBackend : AerosolLayout {
       const int i = lane + kWave * q;
       f[q] = i < mt ? L.f[i] : 0.0;
       x[q] = (newton && i < mt) ? -(u[q] - bc[q] - ch * f[q]) : 0.0;
-      if (newton && i < mt) L.A[a.S.nnzF + i] = x[q];
+      if (newton && i < mt) L.A[a.S.nnzF + pivot(a, i)] = x[q];
     }
     if (newton) wave_sync();
+  }
+  // elimination index of unknown i: particle species first, then the gas
+  // species (aerosolPivot)
+  static __device__ __forceinline__ int pivot(const IntegratorArgs& a, int i) {
+    return i < a.M.m ? i + (a.X.mt - a.M.m) : i - a.M.m;
   }
   static __device__ __forceinline__ void get_x(const IntegratorArgs& a, const Lds& L, int lane, double (&x)[QM]) {
 #pragma unroll
     for (int q = 0; q < QM; ++q) {
       const int i = lane + kWave * q;
-      x[q] = (i < a.X.mt) ? L.A[a.S.nnzF + i] * L.rdiag[i] : 0.0;
+      const int k = pivot(a, i);
+      x[q] = (i < a.X.mt) ? L.A[a.S.nnzF + k] * L.rdiag[k] : 0.0;
     }
   }
   static __device__ void back(const IntegratorArgs& a, const Lds& L, int lane) {
