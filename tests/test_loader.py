@@ -164,3 +164,40 @@ def test_scenario_photolysis_rates_and_external_forcing(pkg):
     np.testing.assert_array_equal(ext, expect)
     assert pkg.setScenarioConditionsPhotolysisReactions(golden("cb05cl_ae5.yaml"),
                                                         pkg.KineticModelData(golden("cb05cl_ae5.yaml")), 1).shape == (1, 0)
+
+
+def test_aerosol_loader_matches_oracle_reader(pkg, oracle_mod):
+    """AerosolModelData (C++ loader, host only) against the oracle's
+    independent PyYAML reader: species order, properties, particle count,
+    transfer parameters; the particle scenario reader against the oracle's."""
+    from oracle import amd as amd_mod
+    for chem, aero, part in (("simpol_gas.yaml", "simpol_aero.yaml", "simpol_particles.yaml"),
+                             ("cb05_soa_gas.yaml", "cb05_soa_aero.yaml", "cb05_soa_particles.yaml")):
+        kmd = pkg.KineticModelData(golden(chem))
+        amd = pkg.AerosolModelData(golden(aero), kmd)
+        ref = amd_mod.load(golden(aero), list(kmd.speciesNames))
+        assert amd.speciesNames == ref.names and amd.nParticles == ref.nParticles and amd.nSimpol == len(ref.gas)
+        np.testing.assert_array_equal(amd.density, ref.density)
+        np.testing.assert_array_equal(amd.molecularWeights, ref.mw)
+        np.testing.assert_array_equal(amd.kappa, ref.kappa)
+        state = np.zeros((1, amd.stateDim))
+        nc = amd.scenarioConditionParticles(golden(part), state)
+        state_ref = np.zeros((1, amd.stateDim))
+        nc_ref = amd_mod.load_particles(golden(part), ref, kmd.nSpec, state_ref)
+        np.testing.assert_array_equal(nc, nc_ref)
+        np.testing.assert_array_equal(state, state_ref)
+    amd.setNumberofParticles(3)
+    assert amd.nParticles == 3 and amd.stateDim == 3 + kmd.nSpec + 3 * amd.nSpec
+    # no particle block: nothing written
+    st = np.zeros((1, amd.stateDim))
+    assert amd.scenarioConditionParticles(golden("cb05_soa_gas.yaml"), st) is None and not st.any()
+
+
+def test_aerosol_loader_errors(pkg, tmp_path):
+    kmd = pkg.KineticModelData(golden("simpol_gas.yaml"))
+    bad = open(golden("simpol_aero.yaml")).read().replace("gas-phase species: ethanol", "gas-phase species: nope")
+    (tmp_path / "bad.yaml").write_text(bad)
+    with pytest.raises(pkg.TChemError, match="nope"):
+        pkg.AerosolModelData(str(tmp_path / "bad.yaml"), kmd)
+    with pytest.raises(pkg.TChemError):
+        pkg.AerosolModelData(str(tmp_path / "missing.yaml"), kmd)
