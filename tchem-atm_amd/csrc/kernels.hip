@@ -771,7 +771,8 @@ __device__ __forceinline__ void back_round(const unsigned (&w)[1 + SparseSchedul
 }
 
 // Runs rounds of the LU / substitution program (schedule.hpp): multiplier
-// and pivot records (kMul), chained updates A[t] -= sum A[lo] * A[hi] (kUpd)
+// and pivot records (kMul), chained updates A[t] -= sum A[lo] * A[hi] (kUpd),
+// single-pivot levels' multipliers fused with their updates (kMulUpd)
 // and column-oriented back-substitution updates
 // A[t] -= sum A[lo] * (A[rhs0 + k] * rdiag[k]) (kBack).  Lanes of a round
 // touch disjoint targets; rounds run in order.  Branch free per lane
@@ -815,6 +816,20 @@ __device__ bool run_program(const unsigned* hdr, const unsigned* body, int nRoun
         case 3: upd_round<3>(w, A); break;
         default: upd_round<4>(w, A); break;
       }
+    } else if (kind == SparseSchedule::kMulUpd) {
+      const int t = int(w[0]);
+      const int ik = int(w[1] & 0xffff), kj = int(w[1] >> 16);
+      const int kk = int(w[2] & 0x7fff), ri = int(w[2] >> 16);
+      const bool mul = (w[2] & 0x8000u) != 0;
+      const TA p = A[kk];
+      const TA a = A[ik];
+      const TA b = A[kj];
+      const TA c = A[t];
+      const TA inv = TA(1) / p;
+      const TA l = a * inv;
+      A[t] = mul ? l : c - l * b;
+      rdiag[ri] = inv;
+      bad |= ri != m + 1 && !(p != TA(0) && isfinite(p) && isfinite(inv));
     } else {
       switch (len) {
         case 1: back_round<1>(w, A, rdiag, rhs0); break;

@@ -10,6 +10,7 @@
 #include "schedule.hpp"
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <stdexcept>
 
@@ -258,6 +259,61 @@ SparseSchedule buildSparseSchedule(const MechTables& t, bool transposed) {
   s.nLevels = m ? *std::max_element(lev.begin(), lev.end()) : 0;
   const unsigned zz = pack16(Z, Z);
   for (int L = 1; L <= s.nLevels; ++L) {
+    // A level with a single pivot k (the tail of the elimination) takes one
+    // kMulUpd round for its multiplier records and as many of its updates
+    // as fit: those lanes scale A[ik] themselves (reads see the unscaled
+    // value), later update rounds read the stored multipliers.
+    int single = -1, nPiv = 0;
+    for (int k = 0; k < m; ++k)
+      if (lev[k] == L) {
+        single = k;
+        ++nPiv;
+      }
+    if (nPiv == 1) {
+      const int k = single;
+      std::vector<int> rows;
+      for (int i = k + 1; i < m; ++i)
+        if (Fz(i, k)) rows.push_back(i);
+      if (!rows.empty() && rows.size() < size_t(kLanes)) {
+        const unsigned kk = P(k, k), trashDiag = unsigned(m + 1) << 16;
+        std::vector<std::array<unsigned, 3>> lanes;  // (target, ik | kj << 16, kk | mul << 15 | rdiag index << 16)
+        for (size_t q = 0; q < rows.size(); ++q) {
+          const unsigned ik = P(rows[q], k);
+          lanes.push_back({ik, pack16(ik, Z), kk | 0x8000u | (q == 0 ? unsigned(k) << 16 : trashDiag)});
+          ++s.nMultipliers;
+        }
+        std::vector<std::array<unsigned, 3>> upd;  // every target of this level: exactly one contribution
+        std::map<unsigned, unsigned> byTarget;
+        for (int i : rows) {
+          for (int j = k + 1; j < m; ++j)
+            if (Fz(k, j)) {
+              byTarget[P(i, j)] = pack16(P(i, k), P(k, j));
+              ++s.nUpdates;
+            }
+          byTarget[rhs0 + unsigned(i)] = pack16(P(i, k), rhs0 + unsigned(k));
+        }
+        std::vector<Chain> rest;
+        for (const auto& e : byTarget) {
+          if (lanes.size() < size_t(kLanes)) {
+            lanes.push_back({e.first, e.second, kk | trashDiag});
+          } else {
+            Chain c;
+            c.target = e.first;
+            c.a.push_back(e.second);
+            rest.push_back(c);
+          }
+        }
+        s.progHdr.push_back(2u | (unsigned(SparseSchedule::kMulUpd) << 16));
+        for (int slot = 0; slot < 3; ++slot)
+          for (int l = 0; l < kLanes; ++l) {
+            if (size_t(l) < lanes.size()) s.progBody.push_back(lanes[l][slot]);
+            else s.progBody.push_back(slot == 0 ? T : (slot == 1 ? zz : (Z | trashDiag)));
+          }
+        ++s.luRounds;
+        s.luRounds += emitChains(rest, SparseSchedule::kUpd, T, zz, false, s.progHdr, s.progBody);
+        continue;
+      }
+    }
     // multiplier and pivot records
     std::vector<unsigned> recs;
     for (int k = 0; k < m; ++k) {

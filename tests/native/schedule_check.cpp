@@ -63,6 +63,45 @@ struct Emu {
           else { CHECK(dst < Z || dst == T, "mul dst %d", dst); if (dst != T) { CHECK(!wr.count(dst), "dup mul dst"); wr[dst] = l; rdset.push_back({dst, l}); } outA.push_back({dst, A[dst] * inv}); }
         }
         off += 64;
+      } else if (kind == SparseSchedule::kMulUpd) {
+        CHECK(len == 2, "mulupd len");
+        std::map<long, int> mulw;  // multiplier slots written by mul lanes: update lanes may read them (old value)
+        for (int l = 0; l < 64; ++l) {
+          const unsigned w2 = s.progBody[off + 128 + l];
+          if (w2 & 0x8000u) mulw[long(s.progBody[off + l])] = l;
+        }
+        for (int l = 0; l < 64; ++l) {
+          const int t = int(s.progBody[off + l]);
+          const unsigned w1 = s.progBody[off + 64 + l], w2 = s.progBody[off + 128 + l];
+          const int ik = w1 & 0xffff, kj = w1 >> 16, kk = w2 & 0x7fff, ri = w2 >> 16;
+          const bool mul = (w2 & 0x8000u) != 0;
+          CHECK(t < Z || t == T, "mulupd target %d", t);
+          CHECK(ik <= Z && kj <= Z && kk <= Z, "mulupd index");
+          CHECK(ri < m || ri == m + 1, "mulupd rdiag %d", ri);
+          if (mul) CHECK(t == ik && kj == Z, "mul lane layout");
+          const double p = A[kk], inv = 1.0 / p, lv = A[ik] * inv;
+          const double val = mul ? lv : A[t] - lv * A[kj];
+          if (ri != m + 1) {
+            CHECK(!wr.count(-1 - ri), "dup rdiag");
+            wr[-1 - ri] = l;
+            outR.push_back({ri, inv});
+            if (!(p != 0 && std::isfinite(p) && std::isfinite(inv))) bad = true;
+          }
+          rdset.push_back({kk, l});
+          if (!mul) {
+            auto it = mulw.find(ik);
+            CHECK(it == mulw.end() || t != ik, "update lane writes a multiplier");
+            if (it == mulw.end()) rdset.push_back({ik, l});  // reading the unscaled value is intended
+            rdset.push_back({kj, l});
+          }
+          if (t != T) {
+            CHECK(!wr.count(t), "dup target %d", t);
+            wr[t] = l;
+            if (!mul) rdset.push_back({t, l});
+          }
+          outA.push_back({t, val});
+        }
+        off += 64 * 3;
       } else {
         for (int l = 0; l < 64; ++l) {
           int t = int(s.progBody[off + l]);
