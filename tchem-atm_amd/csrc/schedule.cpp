@@ -259,49 +259,61 @@ SparseSchedule buildSparseSchedule(const MechTables& t, bool transposed) {
   s.nLevels = m ? *std::max_element(lev.begin(), lev.end()) : 0;
   const unsigned zz = pack16(Z, Z);
   for (int L = 1; L <= s.nLevels; ++L) {
-    // A level with a single pivot k (the tail of the elimination) takes one
-    // kMulUpd round for its multiplier records and as many of its updates
-    // as fit: those lanes scale A[ik] themselves (reads see the unscaled
-    // value), later update rounds read the stored multipliers.
-    int single = -1, nPiv = 0;
-    for (int k = 0; k < m; ++k)
-      if (lev[k] == L) {
-        single = k;
+    // A level whose multiplier records (and the pivot records of pivots
+    // without any) fit one round takes one kMulUpd round: it carries those
+    // records and, on the remaining lanes, the first contribution of as many
+    // targets as fit; those lanes scale A[ik] themselves (reads see the
+    // unscaled value).  Later update rounds read the stored multipliers.
+    {
+      std::vector<std::array<unsigned, 3>> lanes;  // (target, ik | kj << 16, kk | mul << 15 | rdiag index << 16)
+      const unsigned trashDiag = unsigned(m + 1) << 16;
+      int nPiv = 0;
+      bool anyMul = false;
+      for (int k = 0; k < m; ++k) {
+        if (lev[k] != L) continue;
         ++nPiv;
+        const unsigned kk = P(k, k);
+        bool first = true;
+        for (int i = k + 1; i < m; ++i)
+          if (Fz(i, k)) {
+            const unsigned ik = P(i, k);
+            lanes.push_back({ik, pack16(ik, Z), kk | 0x8000u | (first ? unsigned(k) << 16 : trashDiag)});
+            first = false;
+            anyMul = true;
+          }
+        if (first) lanes.push_back({T, pack16(Z, Z), kk | (unsigned(k) << 16)});  // pivot record only
       }
-    if (nPiv == 1) {
-      const int k = single;
-      std::vector<int> rows;
-      for (int i = k + 1; i < m; ++i)
-        if (Fz(i, k)) rows.push_back(i);
-      if (!rows.empty() && rows.size() < size_t(kLanes)) {
-        const unsigned kk = P(k, k), trashDiag = unsigned(m + 1) << 16;
-        std::vector<std::array<unsigned, 3>> lanes;  // (target, ik | kj << 16, kk | mul << 15 | rdiag index << 16)
-        for (size_t q = 0; q < rows.size(); ++q) {
-          const unsigned ik = P(rows[q], k);
-          lanes.push_back({ik, pack16(ik, Z), kk | 0x8000u | (q == 0 ? unsigned(k) << 16 : trashDiag)});
-          ++s.nMultipliers;
-        }
-        std::vector<std::array<unsigned, 3>> upd;  // every target of this level: exactly one contribution
-        std::map<unsigned, unsigned> byTarget;
-        for (int i : rows) {
-          for (int j = k + 1; j < m; ++j)
-            if (Fz(k, j)) {
-              byTarget[P(i, j)] = pack16(P(i, k), P(k, j));
+      if (anyMul && lanes.size() < size_t(kLanes)) {
+        for (const auto& l : lanes)
+          if (l[2] & 0x8000u) ++s.nMultipliers;
+        std::map<unsigned, Chain> byTarget;  // contributions in pivot order, each with its pivot entry
+        std::map<unsigned, std::vector<unsigned>> pivotOf;
+        for (int k = 0; k < m; ++k) {
+          if (lev[k] != L) continue;
+          for (int i = k + 1; i < m; ++i) {
+            if (!Fz(i, k)) continue;
+            for (int j = k + 1; j < m; ++j) {
+              if (!Fz(k, j)) continue;
+              Chain& c = byTarget[P(i, j)];
+              c.target = P(i, j);
+              c.a.push_back(pack16(P(i, k), P(k, j)));
+              pivotOf[P(i, j)].push_back(P(k, k));
               ++s.nUpdates;
             }
-          byTarget[rhs0 + unsigned(i)] = pack16(P(i, k), rhs0 + unsigned(k));
+            Chain& c = byTarget[rhs0 + unsigned(i)];
+            c.target = rhs0 + unsigned(i);
+            c.a.push_back(pack16(P(i, k), rhs0 + unsigned(k)));
+            pivotOf[rhs0 + unsigned(i)].push_back(P(k, k));
+          }
         }
         std::vector<Chain> rest;
-        for (const auto& e : byTarget) {
+        for (auto& e : byTarget) {
+          Chain c = e.second;
           if (lanes.size() < size_t(kLanes)) {
-            lanes.push_back({e.first, e.second, kk | trashDiag});
-          } else {
-            Chain c;
-            c.target = e.first;
-            c.a.push_back(e.second);
-            rest.push_back(c);
+            lanes.push_back({c.target, c.a[0], pivotOf[e.first][0] | trashDiag});
+            c.a.erase(c.a.begin());
           }
+          if (!c.a.empty()) rest.push_back(c);
         }
         s.progHdr.push_back(2u | (unsigned(SparseSchedule::kMulUpd) << 16));
         for (int slot = 0; slot < 3; ++slot)
@@ -313,6 +325,7 @@ SparseSchedule buildSparseSchedule(const MechTables& t, bool transposed) {
         s.luRounds += emitChains(rest, SparseSchedule::kUpd, T, zz, false, s.progHdr, s.progBody);
         continue;
       }
+      (void)nPiv;
     }
     // multiplier and pivot records
     std::vector<unsigned> recs;
