@@ -203,6 +203,53 @@ def test_persistent_schedule_is_bitwise_identical(fp64_vs_oracle, gpu, pkg):
     assert np.array_equal(run.t.cpu().numpy(), g_t)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_finished_and_failed_cells_are_skipped(fp64_vs_oracle, precision, gpu, pkg):
+    """runDeviceBatch skips a cell whose t_out already reaches t_end
+    (src/core/TChem_AtmosphericChemistry.cpp:83) and, as the reference's
+    failure convention implies, one whose dt_in is negative: state, t_out and
+    dt_out stay as they were, the failure counter marks the failed ones.  The
+    other cells of the batch give the same bits as in a launch without
+    skipped cells; an empty batch is a no-op."""
+    w, run, *_ = fp64_vs_oracle
+    torch = gpu
+    prec = pkg.PRECISION_FP64 if precision == "fp64" else pkg.PRECISION_MIXED
+    n = w.n
+    print("case: config0 seed 0, 64 cells, one interval,", precision, "lu", run.lu, flush=True)
+
+    def one_interval(finished=(), failed=()):
+        run.reset()
+        ta = pkg.time_advance_from_tensor(run.tadv).copy()
+        ta["_dt"][list(failed)] = -1.0
+        run.tadv.copy_(torch.from_numpy(ta.view(np.uint8).copy()).to(run.tadv.device))
+        run.t[list(finished)] = w.tend
+        run.dt[list(finished) + list(failed)] = 5.0
+        run.stats.zero_()
+        pkg.AtmosphericChemistry.runDeviceBatch(run.tol_newton, run.tol_time, None, run.tadv, run.state, run.t,
+                                                run.dt, run.state, run.kmcd, run.photo, run.stats, precision=prec,
+                                                lu=run.lu)
+        torch.cuda.synchronize()
+        return (run.state.cpu().numpy(), run.t.cpu().numpy(), run.dt.cpu().numpy(),
+                pkg.stats_from_tensor(run.stats))
+
+    s0, t0, d0, _ = one_interval()
+    finished, failed = list(range(0, 8)), list(range(8, 16))
+    s1, t1, d1, st1 = one_interval(finished, failed)
+    skipped = finished + failed
+    live = list(range(16, n))
+    np.testing.assert_array_equal(s1[skipped], w.state[skipped])
+    assert np.all(t1[finished] == w.tend) and np.all(t1[failed] == w.tbeg)
+    assert np.all(d1[skipped] == 5.0)
+    assert np.all(st1["failed"][failed] == 1) and np.all(st1["failed"][finished] == 0)
+    assert np.all(st1["accepted"][skipped] == 0) and np.all(st1["accepted"][live] > 0)
+    np.testing.assert_array_equal(s1[live], s0[live])
+    np.testing.assert_array_equal(t1[live], t0[live])
+    np.testing.assert_array_equal(d1[live], d0[live])
+    lib = pkg.lib()
+    assert lib.tchem_atmchem_run_device_batch(pkg._Options(prec), 0, None, None, None, None, None, None, None, None,
+                                              None, None, run.kmcd._h, None) == 0
+
+
 @pytest.mark.parametrize("schedule", ["grid", "persistent"])
 def test_captured_interval_loop_matches_eager_loop(fp64_vs_oracle, cb05, schedule, gpu, pkg):
     """The driver loop captured into a HIP graph (7 intervals per launch, the
