@@ -753,6 +753,33 @@ __device__ __forceinline__ void upd_round(const unsigned (&w)[1 + SparseSchedule
   for (int j = 0; j < LEN; ++j) acc -= l[j] * u[j];
   A[w[0]] = acc;
 }
+// A kMulUpd round with LEN chain entries per lane (w[3..3+LEN)); returns
+// true for a zero or non-finite pivot on a lane that stores one.
+template <int LEN, typename TA>
+__device__ __forceinline__ bool mulupd_round(const unsigned (&w)[1 + SparseSchedule::kMaxChain], TA* A, TA* rdiag,
+                                             int m) {
+  const int t = int(w[0]);
+  const int ik = int(w[1] & 0xffff), kj = int(w[1] >> 16);
+  const int kk = int(w[2] & 0x7fff), ri = int(w[2] >> 16);
+  const bool mul = (w[2] & 0x8000u) != 0;
+  const TA p = A[kk];
+  const TA a = A[ik];
+  const TA b = A[kj];
+  TA acc = A[t];
+  TA lo[LEN > 0 ? LEN : 1], hi[LEN > 0 ? LEN : 1];
+#pragma unroll
+  for (int j = 0; j < LEN; ++j) {
+    lo[j] = A[w[3 + j] & 0xffff];
+    hi[j] = A[w[3 + j] >> 16];
+  }
+  const TA inv = TA(1) / p;
+  const TA l = a == TA(0) ? TA(0) : a * inv;
+#pragma unroll
+  for (int j = 0; j < LEN; ++j) acc -= lo[j] * hi[j];
+  A[t] = mul ? l : acc - l * b;
+  rdiag[ri] = inv;
+  return ri != m + 1 && !(p != TA(0) && isfinite(p) && isfinite(inv));
+}
 template <int LEN, typename TA>
 __device__ __forceinline__ void back_round(const unsigned (&w)[1 + SparseSchedule::kMaxChain], TA* A,
                                            const TA* rdiag, int rhs0) {
@@ -772,7 +799,8 @@ __device__ __forceinline__ void back_round(const unsigned (&w)[1 + SparseSchedul
 
 // Runs rounds of the LU / substitution program (schedule.hpp): multiplier
 // and pivot records (kMul), chained updates A[t] -= sum A[lo] * A[hi] (kUpd),
-// single-pivot levels' multipliers fused with their updates (kMulUpd)
+// multipliers fused with updates that scale A[ik] themselves, plus chained
+// updates (kMulUpd)
 // and column-oriented back-substitution updates
 // A[t] -= sum A[lo] * (A[rhs0 + k] * rdiag[k]) (kBack).  Lanes of a round
 // touch disjoint targets; rounds run in order.  Branch free per lane
@@ -817,19 +845,12 @@ __device__ bool run_program(const unsigned* hdr, const unsigned* body, int nRoun
         default: upd_round<4>(w, A); break;
       }
     } else if (kind == SparseSchedule::kMulUpd) {
-      const int t = int(w[0]);
-      const int ik = int(w[1] & 0xffff), kj = int(w[1] >> 16);
-      const int kk = int(w[2] & 0x7fff), ri = int(w[2] >> 16);
-      const bool mul = (w[2] & 0x8000u) != 0;
-      const TA p = A[kk];
-      const TA a = A[ik];
-      const TA b = A[kj];
-      const TA c = A[t];
-      const TA inv = TA(1) / p;
-      const TA l = a * inv;
-      A[t] = mul ? l : c - l * b;
-      rdiag[ri] = inv;
-      bad |= ri != m + 1 && !(p != TA(0) && isfinite(p) && isfinite(inv));
+      static_assert(SparseSchedule::kMaxMulUpdChain == 2, "kMulUpd chain cases");
+      switch (len) {
+        case 2: bad |= mulupd_round<0>(w, A, rdiag, m); break;
+        case 3: bad |= mulupd_round<1>(w, A, rdiag, m); break;
+        default: bad |= mulupd_round<2>(w, A, rdiag, m); break;
+      }
     } else {
       switch (len) {
         case 1: back_round<1>(w, A, rdiag, rhs0); break;

@@ -258,112 +258,197 @@ SparseSchedule buildSparseSchedule(const MechTables& t, bool transposed) {
   }
   s.nLevels = m ? *std::max_element(lev.begin(), lev.end()) : 0;
   const unsigned zz = pack16(Z, Z);
-  for (int L = 1; L <= s.nLevels; ++L) {
-    // A level whose multiplier records (and the pivot records of pivots
-    // without any) fit one round takes one kMulUpd round: it carries those
-    // records and, on the remaining lanes, the first contribution of as many
-    // targets as fit; those lanes scale A[ik] themselves (reads see the
-    // unscaled value).  Later update rounds read the stored multipliers.
-    {
-      std::vector<std::array<unsigned, 3>> lanes;  // (target, ik | kj << 16, kk | mul << 15 | rdiag index << 16)
-      const unsigned trashDiag = unsigned(m + 1) << 16;
-      int nPiv = 0;
-      bool anyMul = false;
-      for (int k = 0; k < m; ++k) {
-        if (lev[k] != L) continue;
-        ++nPiv;
-        const unsigned kk = P(k, k);
-        bool first = true;
-        for (int i = k + 1; i < m; ++i)
-          if (Fz(i, k)) {
-            const unsigned ik = P(i, k);
-            lanes.push_back({ik, pack16(ik, Z), kk | 0x8000u | (first ? unsigned(k) << 16 : trashDiag)});
-            first = false;
-            anyMul = true;
-          }
-        if (first) lanes.push_back({T, pack16(Z, Z), kk | (unsigned(k) << 16)});  // pivot record only
-      }
-      if (anyMul && lanes.size() < size_t(kLanes)) {
-        for (const auto& l : lanes)
-          if (l[2] & 0x8000u) ++s.nMultipliers;
-        std::map<unsigned, Chain> byTarget;  // contributions in pivot order, each with its pivot entry
-        std::map<unsigned, std::vector<unsigned>> pivotOf;
-        for (int k = 0; k < m; ++k) {
-          if (lev[k] != L) continue;
-          for (int i = k + 1; i < m; ++i) {
-            if (!Fz(i, k)) continue;
-            for (int j = k + 1; j < m; ++j) {
-              if (!Fz(k, j)) continue;
-              Chain& c = byTarget[P(i, j)];
-              c.target = P(i, j);
-              c.a.push_back(pack16(P(i, k), P(k, j)));
-              pivotOf[P(i, j)].push_back(P(k, k));
-              ++s.nUpdates;
-            }
-            Chain& c = byTarget[rhs0 + unsigned(i)];
-            c.target = rhs0 + unsigned(i);
-            c.a.push_back(pack16(P(i, k), rhs0 + unsigned(k)));
-            pivotOf[rhs0 + unsigned(i)].push_back(P(k, k));
-          }
-        }
-        std::vector<Chain> rest;
-        for (auto& e : byTarget) {
-          Chain c = e.second;
-          if (lanes.size() < size_t(kLanes)) {
-            lanes.push_back({c.target, c.a[0], pivotOf[e.first][0] | trashDiag});
-            c.a.erase(c.a.begin());
-          }
-          if (!c.a.empty()) rest.push_back(c);
-        }
-        s.progHdr.push_back(2u | (unsigned(SparseSchedule::kMulUpd) << 16));
-        for (int slot = 0; slot < 3; ++slot)
-          for (int l = 0; l < kLanes; ++l) {
-            if (size_t(l) < lanes.size()) s.progBody.push_back(lanes[l][slot]);
-            else s.progBody.push_back(slot == 0 ? T : (slot == 1 ? zz : (Z | trashDiag)));
-          }
-        ++s.luRounds;
-        s.luRounds += emitChains(rest, SparseSchedule::kUpd, T, zz, false, s.progHdr, s.progBody);
-        continue;
-      }
-      (void)nPiv;
-    }
-    // multiplier and pivot records
-    std::vector<unsigned> recs;
-    for (int k = 0; k < m; ++k) {
-      if (lev[k] != L) continue;
-      recs.push_back((0x8000u | unsigned(k)) | (P(k, k) << 16));
-      for (int i = k + 1; i < m; ++i)
-        if (Fz(i, k)) {
-          recs.push_back(P(i, k) | (P(k, k) << 16));
-          ++s.nMultipliers;
-        }
-    }
-    for (size_t base = 0; base < recs.size(); base += kLanes) {
-      s.progHdr.push_back(unsigned(SparseSchedule::kMul) << 16);
-      for (int l = 0; l < kLanes; ++l) s.progBody.push_back(base + l < recs.size() ? recs[base + l] : pack16(T, Z));
-      ++s.luRounds;
-    }
-    // updates grouped by target, contributions in pivot order; the RHS rides along as column m
-    std::map<unsigned, Chain> byTarget;
-    for (int k = 0; k < m; ++k) {
-      if (lev[k] != L) continue;
+  // Rounds by readiness (list scheduling).  Pivot k factors (multipliers
+  // l_ik = a_ik / a_kk, reciprocal pivot) once its diagonal and column are
+  // final; the contribution of pivot p to (i, j) -- a_ij -= l_ip u_pj, the
+  // RHS riding along as column m -- is ready once p has factored and u_pj is
+  // final.  Each round takes what is ready, most urgent targets first (the
+  // level of the pivot that reads them first), so the trailing updates of a
+  // dense block wait and then go in chains of up to kMaxChain contributions
+  // per lane instead of one contribution per lane and round.  Pivots that
+  // are ready factor in a kMulUpd round when their multiplier and pivot
+  // records fit, its other lanes carrying one contribution of those pivots
+  // each (those lanes scale a_ip themselves: they read the unscaled value),
+  // else in kMul rounds.  Later rounds read the stored multipliers.
+  {
+    const unsigned trashDiag = unsigned(m + 1) << 16;
+    const int nT = s.nnzF + m;  // targets: factor positions, then the RHS entry of each row
+    struct Contribution {
+      int pivot;
+      unsigned ik, kj;
+      bool done;
+    };
+    std::vector<std::vector<Contribution>> contrib(static_cast<size_t>(nT));
+    std::vector<int> rem(size_t(nT), 0);
+    for (int k = 0; k < m; ++k)
       for (int i = k + 1; i < m; ++i) {
         if (!Fz(i, k)) continue;
-        for (int j = k + 1; j < m; ++j) {
-          if (!Fz(k, j)) continue;
-          Chain& c = byTarget[P(i, j)];
-          c.target = P(i, j);
-          c.a.push_back(pack16(P(i, k), P(k, j)));
-          ++s.nUpdates;
+        for (int j = k + 1; j <= m; ++j) {
+          if (j < m && !Fz(k, j)) continue;
+          const unsigned tgt = j == m ? rhs0 + unsigned(i) : P(i, j);
+          contrib[tgt].push_back({k, P(i, k), j == m ? rhs0 + unsigned(k) : P(k, j), false});
+          ++rem[tgt];
+          if (j < m) ++s.nUpdates;
         }
-        Chain& c = byTarget[rhs0 + unsigned(i)];
-        c.target = rhs0 + unsigned(i);
-        c.a.push_back(pack16(P(i, k), rhs0 + unsigned(k)));
       }
+    std::vector<int> urg(size_t(nT), 0);
+    for (int i = 0; i < m; ++i) {
+      for (int q = s.rowPtr[i]; q < s.rowPtr[i + 1]; ++q) urg[size_t(q)] = lev[std::min(i, s.colIdx[size_t(q)])];
+      urg[rhs0 + unsigned(i)] = lev[i];
     }
-    std::vector<Chain> ch;
-    for (auto& e : byTarget) ch.push_back(e.second);
-    s.luRounds += emitChains(ch, SparseSchedule::kUpd, T, zz, false, s.progHdr, s.progBody);
+    std::vector<char> factored(m, 0);
+    int nFactored = 0;
+    long remaining = 0;
+    for (int v : rem) remaining += v;
+    auto isFinal = [&](unsigned e) { return e == Z || rem[e] == 0; };
+    // most urgent first: the level that reads the target first, brought
+    // forward by the contributions still to apply (a long backlog must
+    // start early: a lane applies at most kMaxChain per round)
+    auto byUrgency = [&](std::vector<unsigned>& tg) {
+      std::stable_sort(tg.begin(), tg.end(), [&](unsigned x, unsigned y) {
+        const int kx = 3 * urg[x] - rem[x], ky = 3 * urg[y] - rem[y];
+        if (kx != ky) return kx < ky;
+        return rem[x] > rem[y];
+      });
+    };
+    while (nFactored < m || remaining > 0) {
+      std::vector<int> R;
+      for (int k = 0; k < m; ++k) {
+        if (factored[k] || !isFinal(P(k, k))) continue;
+        bool ok = true;
+        for (int i = k + 1; i < m && ok; ++i)
+          if (Fz(i, k) && !isFinal(P(i, k))) ok = false;
+        if (ok) R.push_back(k);
+      }
+      if (!R.empty()) {
+        constexpr int kC = SparseSchedule::kMaxMulUpdChain;
+        // (target, ik | kj << 16, kk | mul << 15 | rdiag index << 16, chain entries)
+        std::vector<std::array<unsigned, 3 + kC>> lanes;
+        auto lane = [&](unsigned t, unsigned w1, unsigned w2) {
+          std::array<unsigned, 3 + kC> x;
+          x.fill(zz);
+          x[0] = t;
+          x[1] = w1;
+          x[2] = w2;
+          lanes.push_back(x);
+        };
+        for (int k : R) {
+          const unsigned kk = P(k, k);
+          bool first = true;
+          for (int i = k + 1; i < m; ++i)
+            if (Fz(i, k)) {
+              lane(P(i, k), pack16(P(i, k), Z), kk | 0x8000u | (first ? unsigned(k) << 16 : trashDiag));
+              first = false;
+              ++s.nMultipliers;
+            }
+          if (first) lane(T, zz, kk | (unsigned(k) << 16));  // pivot record only
+        }
+        if (lanes.size() <= size_t(kLanes)) {
+          std::vector<char> inR(m, 0);
+          for (int k : R) inR[k] = 1;
+          // from the state before the round: one contribution of a pivot of
+          // R (scaled in the lane) and up to kC of factored pivots per target
+          std::vector<int> head(static_cast<size_t>(nT), -1);
+          std::vector<std::vector<size_t>> chainOf(static_cast<size_t>(nT));
+          std::vector<unsigned> tg;
+          for (int e = 0; e < nT; ++e) {
+            if (rem[e] == 0) continue;
+            for (size_t c = 0; c < contrib[e].size(); ++c) {
+              const Contribution& x = contrib[e][c];
+              if (x.done || !isFinal(x.kj)) continue;
+              if (inR[x.pivot]) {
+                if (head[e] < 0) head[e] = int(c);
+              } else if (factored[x.pivot] && chainOf[e].size() < size_t(kC)) {
+                chainOf[e].push_back(c);
+              }
+            }
+            if (head[e] >= 0 || !chainOf[e].empty()) tg.push_back(unsigned(e));
+          }
+          byUrgency(tg);
+          int chain = 0;
+          for (unsigned e : tg) {
+            if (lanes.size() >= size_t(kLanes)) break;
+            unsigned w1 = zz, w2 = Z | trashDiag;
+            if (head[e] >= 0) {
+              Contribution& x = contrib[e][size_t(head[e])];
+              w1 = pack16(x.ik, x.kj);
+              w2 = P(x.pivot, x.pivot) | trashDiag;
+              x.done = true;
+              --rem[e];
+              --remaining;
+            }
+            lane(e, w1, w2);
+            for (size_t j = 0; j < chainOf[e].size(); ++j) {
+              Contribution& x = contrib[e][chainOf[e][j]];
+              lanes.back()[3 + j] = pack16(x.ik, x.kj);
+              x.done = true;
+              --rem[e];
+              --remaining;
+            }
+            chain = std::max(chain, int(chainOf[e].size()));
+          }
+          s.progHdr.push_back(unsigned(2 + chain) | (unsigned(SparseSchedule::kMulUpd) << 16));
+          for (int slot = 0; slot < 3 + chain; ++slot)
+            for (int l = 0; l < kLanes; ++l) {
+              if (size_t(l) < lanes.size()) s.progBody.push_back(lanes[l][slot]);
+              else s.progBody.push_back(slot == 0 ? T : (slot == 2 ? (Z | trashDiag) : zz));
+            }
+          ++s.luRounds;
+        } else {
+          std::vector<unsigned> recs;
+          for (int k : R) {
+            recs.push_back((0x8000u | unsigned(k)) | (P(k, k) << 16));
+            for (int i = k + 1; i < m; ++i)
+              if (Fz(i, k)) recs.push_back(P(i, k) | (P(k, k) << 16));
+          }
+          for (size_t base = 0; base < recs.size(); base += kLanes) {
+            s.progHdr.push_back(unsigned(SparseSchedule::kMul) << 16);
+            for (int l = 0; l < kLanes; ++l) s.progBody.push_back(base + l < recs.size() ? recs[base + l] : pack16(T, Z));
+            ++s.luRounds;
+          }
+        }
+        for (int k : R) factored[k] = 1;
+        nFactored += int(R.size());
+        continue;
+      }
+      // no pivot ready: chained updates with stored multipliers
+      // readiness from the state before the round: an entry finished in
+      // this round is not read in it
+      std::vector<unsigned> tg;
+      std::vector<std::vector<size_t>> readyOf(static_cast<size_t>(nT));
+      for (int e = 0; e < nT; ++e) {
+        if (rem[e] == 0) continue;
+        for (size_t c = 0; c < contrib[e].size(); ++c) {
+          const Contribution& x = contrib[e][c];
+          if (!x.done && factored[x.pivot] && isFinal(x.kj)) readyOf[e].push_back(c);
+        }
+        if (!readyOf[e].empty()) tg.push_back(unsigned(e));
+      }
+      if (tg.empty()) throw std::runtime_error("sparse schedule: LU program stalled");
+      byUrgency(tg);
+      if (tg.size() > size_t(kLanes)) tg.resize(kLanes);
+      int len = 0;
+      for (unsigned e : tg) len = std::max(len, std::min(int(readyOf[e].size()), SparseSchedule::kMaxChain));
+      std::vector<std::vector<unsigned>> words(size_t(len), std::vector<unsigned>(kLanes, zz));
+      for (size_t l = 0; l < tg.size(); ++l) {
+        const unsigned e = tg[l];
+        int taken = 0;
+        for (size_t c : readyOf[e]) {
+          if (taken == len) break;
+          Contribution& x = contrib[e][c];
+          words[size_t(taken)][l] = pack16(x.ik, x.kj);
+          x.done = true;
+          ++taken;
+        }
+        rem[e] -= taken;
+        remaining -= taken;
+      }
+      s.progHdr.push_back(unsigned(len) | (unsigned(SparseSchedule::kUpd) << 16));
+      for (int l = 0; l < kLanes; ++l) s.progBody.push_back(size_t(l) < tg.size() ? tg[l] : T);
+      for (int j = 0; j < len; ++j)
+        for (int l = 0; l < kLanes; ++l) s.progBody.push_back(words[size_t(j)][size_t(l)]);
+      ++s.luRounds;
+    }
   }
 
   // back substitution, column oriented: when column k is final

@@ -30,6 +30,7 @@ struct SparseSchedule {
   enum : int { kMul = 0, kUpd = 1, kBack = 2, kRop = 3, kPair = 4, kSeg = 5, kJac = 6, kNuT = 7, kJtv = 8,
                kMulUpd = 9 };
   static constexpr int kMaxChain = 4;            // chain entries per lane per round
+  static constexpr int kMaxMulUpdChain = kMaxChain - 2;  // chain entries of a kMulUpd lane
   static constexpr int kSlack = 1 + kMaxChain;   // slots read past a section's last round
   static constexpr unsigned kCont = 0x80000000u;  // target flag: continues a chain split over rounds
   static constexpr unsigned kNone = 0xffffu;      // packed reactant field: no reactant
@@ -72,13 +73,16 @@ struct SparseSchedule {
   //          dst = 0x8000 | k: pivot record, rdiag[k] = 1 / A[src] (a zero or
   //          non-finite pivot fails the factorisation); padding (trash | zero)
   //   kUpd:  target, len entries (lo | hi << 16): A[t] -= sum A[lo] * A[hi]
-  //   kMulUpd (len 2, a level with one pivot k): target t, (ik | kj << 16),
-  //          (kk | mul << 15 | d << 16): l = A[ik] * (1 / A[kk]); multiplier
-  //          lanes (mul) store A[t = ik] = l, update lanes A[t] -= l * A[kj];
-  //          rdiag[d] = 1 / A[kk] (d = k on one lane, m + 1 on the others).
+  //   kMulUpd (len 2 + c, c <= kMaxMulUpdChain): target t, (ik | kj << 16),
+  //          (kk | mul << 15 | d << 16), then c entries (lo | hi << 16):
+  //          l = A[ik] * (1 / A[kk]) (0 when A[ik] = 0); multiplier lanes
+  //          (mul) store A[t = ik] = l, update lanes
+  //          A[t] -= sum A[lo] * A[hi] + l * A[kj]; rdiag[d] = 1 / A[kk]
+  //          (d = k on one lane per factored pivot, m + 1 on the others).
   //          Update lanes read A[ik] before the multiplier lane's store (all
-  //          lanes of a round read before any writes).  Padding (trash,
-  //          zero | zero, zero | (m + 1) << 16).
+  //          lanes of a round read before any writes).  A lane with chain
+  //          entries only has ik = kj = kk = zero.  Padding (trash, zero |
+  //          zero, zero | (m + 1) << 16, zero | zero).
   //   kBack: target, len entries (U position | k << 16):
   //          A[t] -= sum A[lo] * (A[rhs0 + k] * rdiag[k]); padding (zero | m << 16)
   // Sections in order: factorisation with the RHS eliminated alongside (LU),

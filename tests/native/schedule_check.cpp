@@ -64,7 +64,7 @@ struct Emu {
         }
         off += 64;
       } else if (kind == SparseSchedule::kMulUpd) {
-        CHECK(len == 2, "mulupd len");
+        CHECK(len >= 2 && len <= 2 + SparseSchedule::kMaxMulUpdChain, "mulupd len");
         std::map<long, int> mulw;  // multiplier slots written by mul lanes: update lanes may read them (old value)
         for (int l = 0; l < 64; ++l) {
           const unsigned w2 = s.progBody[off + 128 + l];
@@ -79,8 +79,20 @@ struct Emu {
           CHECK(ik <= Z && kj <= Z && kk <= Z, "mulupd index");
           CHECK(ri < m || ri == m + 1, "mulupd rdiag %d", ri);
           if (mul) CHECK(t == ik && kj == Z, "mul lane layout");
-          const double p = A[kk], inv = 1.0 / p, lv = A[ik] * inv;
-          const double val = mul ? lv : A[t] - lv * A[kj];
+          const double p = A[kk], inv = 1.0 / p, lv = A[ik] == 0.0 ? 0.0 : A[ik] * inv;
+          double acc = A[t];
+          for (int j = 0; j < len - 2; ++j) {
+            const unsigned e = s.progBody[off + 64 * (3 + j) + l];
+            const int lo = e & 0xffff, hi = e >> 16;
+            CHECK(lo <= Z && hi <= Z, "mulupd chain index");
+            if (t != T) CHECK(lo != T && hi != T, "reads trash");
+            CHECK(!mul || (lo == Z && hi == Z), "chain on a multiplier lane");
+            CHECK(!mulw.count(lo) && !mulw.count(hi), "chain reads a multiplier stored in the same round");
+            acc -= A[lo] * A[hi];
+            rdset.push_back({lo, l});
+            rdset.push_back({hi, l});
+          }
+          const double val = mul ? lv : acc - lv * A[kj];
           if (ri != m + 1) {
             CHECK(!wr.count(-1 - ri), "dup rdiag");
             wr[-1 - ri] = l;
@@ -101,7 +113,7 @@ struct Emu {
           }
           outA.push_back({t, val});
         }
-        off += 64 * 3;
+        off += size_t(64) * (1 + len);
       } else {
         for (int l = 0; l < 64; ++l) {
           int t = int(s.progBody[off + l]);
