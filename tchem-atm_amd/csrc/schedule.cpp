@@ -4,9 +4,10 @@
 // Markowitz) species order; F = structure of I - J plus fill.  Pivot k
 // depends on every earlier pivot k' < k whose row or column it touches
 // (F(k,k') or F(k',k) nonzero); its level is one more than the highest
-// level among those, so pivots of one level touch disjoint entries: the
-// multipliers of a level read only final pivots, and its updates read only
-// final multipliers and final rows of U.
+// level among those, so pivots of one level touch disjoint entries.  The
+// forward-substitution section runs level by level; the factorisation is
+// list-scheduled (entries become final as their contributions are applied,
+// rounds take whatever is ready), with the levels as its urgency measure.
 #include "schedule.hpp"
 
 #include <algorithm>
