@@ -268,9 +268,12 @@ SparseSchedule buildSparseSchedule(const MechTables& t, bool transposed) {
   // dense block wait and then go in chains of up to kMaxChain contributions
   // per lane instead of one contribution per lane and round.  Pivots that
   // are ready factor in a kMulUpd round when their multiplier and pivot
-  // records fit, its other lanes carrying one contribution of those pivots
-  // each (those lanes scale a_ip themselves: they read the unscaled value),
-  // else in kMul rounds.  Later rounds read the stored multipliers.
+  // records fit; its other lanes each carry one contribution of those
+  // pivots (scaling a_ip themselves: they read the unscaled value) and up to
+  // kMaxMulUpdChain contributions of pivots factored earlier.  Otherwise
+  // they factor in kMul rounds.  Later rounds read the stored multipliers.
+  // Readiness is taken from the state before each round, so no round reads
+  // an entry it writes.
   {
     const unsigned trashDiag = unsigned(m + 1) << 16;
     const int nT = s.nnzF + m;  // targets: factor positions, then the RHS entry of each row
