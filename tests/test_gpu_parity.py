@@ -203,6 +203,25 @@ def test_persistent_schedule_is_bitwise_identical(fp64_vs_oracle, gpu, pkg):
     assert np.array_equal(run.t.cpu().numpy(), g_t)
 
 
+@pytest.mark.parametrize("programs", ["lds", "memory"])
+def test_program_placement_is_bitwise_identical(fp64_vs_oracle, programs, gpu, pkg):
+    """The sparse form's compiled mechanism programs staged into LDS once per
+    workgroup or read by every wavefront from device memory, with 1 and 4
+    cells per workgroup: the same per-cell arithmetic, identical bits."""
+    w, run, g_state, g_t, *_ = fp64_vs_oracle
+    if run.lu != pkg.LU_SPARSE:
+        pytest.skip("program placement applies to the sparse form")
+    code = pkg.PROGRAMS_LDS if programs == "lds" else pkg.PROGRAMS_MEMORY
+    for cpb in (1, 4):
+        print("case: config0 seed 0, 64 cells, fp64 sparse, programs", programs, "cells per block", cpb, flush=True)
+        run.reset()
+        run.run_intervals(precision=pkg.PRECISION_FP64, schedule=pkg.SCHEDULE_GRID, lu=run.lu, cells_per_block=cpb,
+                          programs=code)
+        gpu.cuda.synchronize()
+        assert np.array_equal(run.state.cpu().numpy(), g_state)
+        assert np.array_equal(run.t.cpu().numpy(), g_t)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_finished_and_failed_cells_are_skipped(fp64_vs_oracle, precision, gpu, pkg):
     """runDeviceBatch skips a cell whose t_out already reaches t_end
