@@ -475,8 +475,8 @@ __device__ void lu_back(const CellLds<TA>& L, int m, int lane, TA (&b)[QM]) {
 // ---- sparse form ------------------------------------------------------------
 // Compiled schedules (schedule.hpp): the iteration matrix is held as the
 // nonzeros of its symbolic LU pattern, factorised without interchanges in
-// the fill-reducing order, level by level; the RHS is stored after the
-// matrix entries and eliminated alongside.
+// the fill-reducing order by list-scheduled rounds; the RHS is stored after
+// the matrix entries and eliminated alongside.
 
 template <typename TA>
 struct SpLds {
