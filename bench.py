@@ -1,27 +1,42 @@
 """Throughput of the batched gas-phase kinetics path on one MI355X.
 
-Workload (one "step"): CB05CL_AE5, 131072 independent cells (1820 columns x
-72 levels), seed 1, T ~ U[220,320] K, half the cells at night (photolysis 0),
-every cell advanced 3600 s with TR-BDF2 (dtmin = dt_in = 1e-20 s, dtmax 60 s,
-atol 1e-12, rtol 1e-6; Newton atol 1e-18, rtol 1e-8, at most 20 iterations,
+Workload: CB05CL_AE5, 131072 independent cells (1820 columns x 72 levels),
+seed 1, T ~ U[220,320] K, half the cells at night (photolysis 0), every cell
+advanced one hour with TR-BDF2 (dtmin = dt_in = 1e-20 s, dtmax 60 s, atol
+1e-12, rtol 1e-6; Newton atol 1e-18, rtol 1e-8, at most 20 iterations,
 Jacobian refreshed every iteration).  Inputs are generated on the host and
-copied to HBM before timing; a step restores the initial state (device copy)
-and integrates every cell to t_end.
+copied to HBM before timing.
+
+One step = one slice of the hour: --steps K cuts [0, 3600 s] into K equal
+slices, and a step advances every cell from its own time to the slice end in
+one persistent work-queue launch (the reference's host interval loop with the
+interval as the slice: t_out / dt_out carried into the next slice's
+TimeAdvance records, src/examples/TChem_AtmosphericChemistry.cpp:340-350).
+The K timed steps together integrate the whole hour of all 131072 cells, so
+value = 131072 cell-hours / (K x seconds per step).  --warmup W runs W
+slices of a 4096-cell sample first.
 
 Forms:
   mixed (the metric): FP32 Jacobian assembly + sparse LU, FP64 state,
-         residual, error estimate and step control; persistent work-queue
-         launch, one kernel for the whole hour.
-  fp64:  the same launch, all FP64 (measured in the same run unless
-         --no-compare), with the max/median relative difference of the mixed
-         result against it.
+         residual, error estimate and step control; persistent launch,
+         autotuned launch shape.
+  fp64:  the same launch, all FP64, measured on the same K slices, with the
+         relative difference of the mixed result against it.
   fp64-plain: dense LU with partial pivoting (the oracle's arithmetic), one
          wavefront per cell, the reference's host interval loop (10 TR-BDF2
-         iterations per launch); timed on --plain-cells cells.
+         iterations per launch), timed on a strided sample of the cells
+         (--plain-cells), plus the same loop captured in a HIP graph.
+  CPU:   the oracle on a strided sample of the cells, on the host's cores:
+         the reference algorithm restated (forward-difference Jacobian, rate
+         constants in every source-term call: cpu_baseline) and the
+         analytic-Jacobian oracle (Newton iterations per step for the
+         roofline, and the error of both GPU forms on the sample cells).
+The side legs run under a wall-clock budget (--budget, seconds from start).
 
 Other modes (one JSON line each, instead of the benchmark): --config3 times
 the discrete adjoint against the forward solve, --config4 the uneven-batch
-synthetic-mechanism workload, --precision-study the error study.
+synthetic-mechanism workload, --precision-study the error study,
+--cpu-study the CPU baseline on config 0 and a config-1 subset.
 
 Prints one JSON line on stdout; diagnostics go to stderr.  --copy-probe runs
 only a 1 GiB device-to-device copy (a known-size kernel for calibrating
@@ -78,44 +93,64 @@ def mechanism_flops(o):
     return reactant_entries + 2 * scatter, 2 * pairs + 2 * contrib
 
 
-def cpu_baseline(w, sample, threads):
-    """The reference algorithm restated on the CPU (forward-difference
-    Jacobian with m+1 source-term evaluations, all rate constants in every
-    evaluation, a fresh factorisation every Newton iteration, one cell per
-    thread, static schedule), timed on the first `sample` cells."""
+def host_info():
+    """CPU model and the number of cores this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return model, usable
+
+
+def default_threads():
+    """Host threads for the CPU legs: the cores this process may use, at most
+    16 (one GPU's share of the host)."""
+    return max(1, min(16, host_info()[1]))
+
+
+def cpu_leg(o, w, n_slices, mode, threads):
+    """The oracle on the rows of workload w (the hour cut into n_slices
+    intervals, as the GPU run): mode 1 = the reference algorithm restated
+    (forward-difference Jacobian with m+1 source-term evaluations, all rate
+    constants in every evaluation, a factorisation every Newton iteration,
+    one cell per thread, static schedule, src/core/TChem_Util.hpp:71-73);
+    mode 0 = analytic Jacobian.  Returns timing, final rows and counters."""
     import oracle
-    o = oracle.Oracle(MECH)
-    st = w.state[:sample].copy()
-    ph = w.photo[:sample].copy()
-    n = st.shape[0]
-    t = np.full(n, w.tbeg)
-    dt = np.full(n, w.dtmax)
-    ta = np.zeros(n, oracle.TADV_DTYPE)
-    ta["tbeg"], ta["tend"], ta["dt"], ta["dtmin"], ta["dtmax"] = w.tbeg, w.tend, w.dt_in, w.dtmin, w.dtmax
-    ta["max_num_newton_iterations"] = w.max_newton
-    ta["num_time_iterations_per_interval"] = w.iters_per_interval
-    ta["jacobian_interval"] = w.jacobian_interval
-    tn = np.array([w.atol_newton, w.rtol_newton])
-    tt = np.tile([w.atol_time, w.rtol_time], (o.m, 1))
     t0 = time.perf_counter()
-    for _ in range(1_000_000):
-        o.run_batch(tn, tt, ta, st, t, dt, photo=ph, jacobian_mode=1, nthreads=threads)
-        ta["tbeg"], ta["dt"] = t, dt
-        if t.mean() > w.tend * 0.9999:
-            break
+    st, t, dt, stats = oracle.run_sliced(o, w, w.state, w.photo, n_slices=n_slices, jacobian_mode=mode,
+                                         nthreads=threads)
     el = time.perf_counter() - t0
-    return n * (w.tend - w.tbeg) / 3600.0 / el, el
+    acc = stats["accepted"].astype(np.float64)
+    nit = stats["newton_iterations"] / np.maximum(acc, 1.0)
+    return dict(wall_s=el, value=w.cell_hours / el, state=st, t=t, stats=stats,
+                steps_p50=float(np.percentile(acc, 50)), steps_p99=float(np.percentile(acc, 99)),
+                newton_per_step=float(stats["newton_iterations"].sum() / max(1.0, acc.sum())),
+                newton_per_step_p50=float(np.percentile(nit, 50)), newton_per_step_p99=float(np.percentile(nit, 99)),
+                failed=int(stats["failed"].sum()))
 
 
 def measured_traffic(workload, form):
-    """HBM bytes per launch of the integrator from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary of the same workload and form (None when
-    there is none)."""
-    path = os.path.join(ROOT, "profiles", "trbdf2_kernel_cb05_131072_cells_hbm_traffic.json")
-    if form != "mixed" or workload != "cb05_131072_cells_1h" or not os.path.exists(path):
-        return None
+    """HBM bytes per integrator launch from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload and form (None when
+    there is none): rocprof counters cannot be read inside this process."""
+    path = os.path.join(ROOT, "profiles", f"trbdf2_kernel_{workload}_{form}_hbm_traffic.json")
+    if not os.path.exists(path):
+        return None, None
     with open(path) as f:
-        return json.load(f)["traffic_bytes_per_launch"]
+        return json.load(f)["traffic_bytes_per_launch"], os.path.relpath(path, ROOT)
+
+
+def rel_err(a, b, floor):
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)
 
 
 def precision_study(cells, threads, out_path=None):
@@ -498,16 +533,22 @@ def copy_probe():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=10, help="timed steps = slices of the simulated hour")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed slices on a 4096-cell sample first")
     ap.add_argument("--cells", type=int, default=131072)
-    ap.add_argument("--form", choices=["mixed", "fp64", "fp64-plain", "fp64-plain-graph"], default="mixed")
+    ap.add_argument("--form", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-autotune", action="store_true",
                     help="persistent forms: default launch shape instead of the autotuner's")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 run of the same workload")
-    ap.add_argument("--plain-cells", type=int, default=8192, help="cells timed in the fp64-plain form (0: skip)")
-    ap.add_argument("--cpu-sample", type=int, default=16, help="cells in the CPU baseline sample (0: skip)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, os.cpu_count())")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="only the timed leg of --form (profiling runs): no FP64, plain or CPU legs")
+    ap.add_argument("--plain-cells", type=int, default=16384, help="strided sample for the fp64-plain form (0: skip)")
+    ap.add_argument("--graph-cells", type=int, default=4096, help="strided sample for the captured plain loop (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="cells in the reference-algorithm CPU sample (0: one per host thread)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: the usable host cores, at most 16")
+    ap.add_argument("--budget", type=float, default=420.0,
+                    help="wall seconds from start after which optional legs are skipped")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--precision-study", metavar="OUT", nargs="?", const="",
                     help="run the mixed/FP64-vs-oracle error study on config-0 inputs instead of the benchmark")
@@ -529,8 +570,9 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the path shards cells across GPUs with no exchange; "
-              "multi-GPU runs are not part of this build", file=sys.stderr)
+              "runs on more than one GPU are not part of this build", file=sys.stderr)
         sys.exit(2)
+    threads = args.cpu_threads or default_threads()
     if args.copy_probe:
         copy_probe()
         return
@@ -544,158 +586,243 @@ def main():
         e3sm_run(args.e3sm_tiles, args.e3sm or None)
         return
     if args.aerosol is not None:
-        aerosol_run(args.aerosol_cells, args.aerosol_cpu_sample, args.cpu_threads or min(16, os.cpu_count() or 1),
-                    args.aerosol or None, args.aerosol_particles)
+        aerosol_run(args.aerosol_cells, args.aerosol_cpu_sample, threads, args.aerosol or None,
+                    args.aerosol_particles)
         return
     if args.precision_study is not None:
-        precision_study(args.study_cells, args.cpu_threads or min(16, os.cpu_count() or 1),
-                        args.precision_study or None)
+        precision_study(args.study_cells, threads, args.precision_study or None)
         return
+
+    T0 = time.perf_counter()
+
+    def left():
+        return args.budget - (time.perf_counter() - T0)
 
     import torch
     T = importlib.import_module("tchem-atm_amd")
     S = importlib.import_module("tchem-atm_amd.scenarios")
     D = importlib.import_module("tchem-atm_amd.driver")
-    import oracle
 
     kmd = T.KineticModelData(MECH)
     kmcd = kmd.createConstData()
     base = T.setScenarioConditions(MECH, kmd)
     m = kmd.nActive
-    forms = {
-        "mixed": dict(precision=T.PRECISION_MIXED, lu=T.LU_AUTO, persistent=True),
-        "fp64": dict(precision=T.PRECISION_FP64, lu=T.LU_AUTO, persistent=True),
-        "fp64-plain": dict(precision=T.PRECISION_FP64, lu=T.LU_DENSE, persistent=False),
-        "fp64-plain-graph": dict(precision=T.PRECISION_FP64, lu=T.LU_DENSE, persistent=False, graph=True),
-    }
-    tuned_shapes = {}
+    n, K = args.cells, max(1, args.steps)
+    w = S.config1(kmd, base[0], n=n, seed=1)
+    night = S.night_flags(n, 1, m, kmd.nPhoto)
+    precisions = {"mixed": T.PRECISION_MIXED, "fp64": T.PRECISION_FP64}
+    log(f"workload: {n} CB05 cells x 3600 s in {K} slices of {3600.0 / K:g} s; {int(night.sum())} night cells")
 
-    def timed(form, n, steps, warmup):
-        """Runs `steps` timed steps of the workload on n cells; returns
-        (wall s/step, kernel-region s/step, launches/step, run)."""
-        f = forms[form]
-        w = S.config1(kmd, base[0], n=n, seed=1)
-        run = D.DeviceRun(kmcd, w)
-        shape = dict(lu=f["lu"], cells_per_block=0, blocks_per_cu=0)
-        if f["persistent"] and not args.no_autotune:
-            # launch shape from the autotuner on this batch (outside the timed region)
-            run.reset(10_000_000)
-            t0 = time.perf_counter()
-            tuned = T.AtmosphericChemistry.autotune(run.tol_newton, run.tol_time, run.tadv, run.state, kmcd,
-                                                    photo_rates=run.photo, precision=f["precision"],
-                                                    schedule=T.SCHEDULE_PERSISTENT, lu=f["lu"])
-            shape = dict(lu=tuned["lu"], cells_per_block=tuned["cells_per_block"],
-                         blocks_per_cu=tuned["blocks_per_cu"])
-            tuned_shapes[form] = dict(shape, tuning_s=time.perf_counter() - t0)
-            log(f"autotuned {form}: {tuned_shapes[form]}")
+    def tune(run, precision):
+        if args.no_autotune:
+            return dict(lu=T.LU_AUTO, cells_per_block=0, blocks_per_cu=0, programs=T.PROGRAMS_AUTO)
+        run.reset(10_000_000)
+        t0 = time.perf_counter()
+        r = T.AtmosphericChemistry.autotune(run.tol_newton, run.tol_time, run.tadv, run.state, kmcd,
+                                            photo_rates=run.photo, precision=precision,
+                                            schedule=T.SCHEDULE_PERSISTENT)
+        shape = dict(lu=r["lu"], cells_per_block=r["cells_per_block"], blocks_per_cu=r["blocks_per_cu"],
+                     programs=r["programs"])
+        log(f"autotuned: {shape} ({time.perf_counter() - t0:.1f} s)")
+        return shape
 
-        def step(collect):
-            run.reset()
-            if f["persistent"]:
-                return run.run_to_end(precision=f["precision"], schedule=T.SCHEDULE_PERSISTENT,
-                                      collect_stats=collect, **shape)
-            if f.get("graph"):
-                n_int = run.run_intervals_graph(precision=f["precision"], schedule=T.SCHEDULE_GRID, lu=f["lu"],
-                                                intervals_per_launch=64)
-                return (n_int + 63) // 64
-            return run.run_intervals(precision=f["precision"], schedule=T.SCHEDULE_GRID, lu=f["lu"],
-                                     collect_stats=collect)
-
-        for _ in range(warmup):
-            step(False)
-        torch.cuda.synchronize()
+    def sliced(run, precision, shape, slices):
+        """The hour in `slices` persistent launches; (wall s, [s per slice])."""
+        run.start_slices(slices)
         stream = torch.cuda.current_stream()
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        kernel_ms, launches = 0.0, 0
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(slices + 1)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(steps):
-            run.reset()
-            ev0.record(stream)
-            launches += step(True)  # per-cell counters are written once per cell: no extra cost
-            ev1.record(stream)
-            ev1.synchronize()
-            kernel_ms += ev0.elapsed_time(ev1)
+        ev[0].record(stream)
+        for k in range(slices):
+            run.run_slice(k, precision=precision, schedule=T.SCHEDULE_PERSISTENT, **shape)
+            ev[k + 1].record(stream)
         torch.cuda.synchronize()
-        wall = (time.perf_counter() - t0) / steps
-        return wall, kernel_ms / 1e3 / steps, launches / steps, run, w
+        wall = time.perf_counter() - t0
+        return wall, [ev[k].elapsed_time(ev[k + 1]) / 1e3 for k in range(slices)]
 
-    wall, kernel_s, launches, run, w = timed(args.form, args.cells, args.steps, args.warmup)
+    # ---- headline form: warm-up on a sample, then K timed slices -----------------
+    form = args.form
+    prec = precisions[form]
+    run = D.DeviceRun(kmcd, w)
+    shape = tune(run, prec)
+    if args.warmup > 0:
+        wu = D.DeviceRun(kmcd, S.config1(kmd, base[0], seed=1, cells=np.arange(0, n, max(1, n // 4096))))
+        wu.start_slices(K)
+        for k in range(args.warmup):
+            wu.run_slice(k % K, precision=prec, schedule=T.SCHEDULE_PERSISTENT, **shape)
+        torch.cuda.synchronize()
+        del wu
+    wall, per_slice = sliced(run, prec, shape, K)
     value = w.cell_hours / wall
-    st = run.total_stats
+    kernel_s = float(sum(per_slice))
+    acc = run.acc_stats.cpu().numpy()
     t_out = run.t.cpu().numpy()
-    if not np.all(t_out == w.tend):
-        log("warning: not every cell reached t_end", int(np.sum(t_out != w.tend)))
-    failed = int(st[:, 5].sum())
-    acc = float(st[:, 0].sum())
-    n_it = float(st[:, 3].sum()) / max(acc, 1.0)
+    failed_mask = acc[:, 5] > 0
+    steps_total = float(acc[:, 0].sum())
+    n_it_gpu = float(acc[:, 3].sum()) / max(steps_total, 1.0)
+    log(f"{form}: {value:.1f} cell-hours/s ({wall:.2f} s wall, kernel region {kernel_s:.2f} s, "
+        f"slice max/mean {max(per_slice) / (kernel_s / K):.2f}); steps/cell {steps_total / n:.0f}, "
+        f"Newton its/step {n_it_gpu:.3f}, failed cells {int(failed_mask.sum())}")
+    if not np.all((t_out == w.tend) | failed_mask):
+        log("warning: cells short of t_end:", int(np.sum((t_out != w.tend) & ~failed_mask)))
+    state_main = run.state.cpu().numpy()
+    config = {"workload": "cb05_131072_cells_1h" if n == 131072 else f"cb05_{n}_cells_1h",
+              "mechanism": "CB05CL_AE5 (74 active species, 187 reactions)", "cells": n, "seed": 1,
+              "night_cells": int(night.sum()), "t_end_s": w.tend, "form": form, "slices": K,
+              "slice_s": w.tend / K, "launch_shape": shape, "accepted_steps_per_cell": steps_total / n,
+              "accepted_steps_p50_p99": [float(np.percentile(acc[:, 0], 50)), float(np.percentile(acc[:, 0], 99))],
+              "newton_iterations_per_step": n_it_gpu, "failed_cells": int(failed_mask.sum()),
+              "failed_cell_indices": [int(i) for i in np.nonzero(failed_mask)[0][:64]],
+              "slice_kernel_s": [round(x, 4) for x in per_slice]}
+    del run
+    torch.cuda.empty_cache()
+
+    # ---- FP64 on the same launch and slices --------------------------------------
+    if form == "mixed" and not args.no_compare and not args.kernel_only:
+        run64 = D.DeviceRun(kmcd, w)
+        shape64 = tune(run64, T.PRECISION_FP64)
+        wall64, per64 = sliced(run64, T.PRECISION_FP64, shape64, K)
+        v64 = w.cell_hours / wall64
+        b = run64.state.cpu().numpy()[:, 3:3 + m]
+        a = state_main[:, 3:3 + m]
+        fail64 = run64.acc_stats[:, 5].cpu().numpy() > 0
+        both = ~failed_mask & ~fail64
+        e12, e8 = rel_err(a[both], b[both], 1e-12), rel_err(a[both], b[both], 1e-8)
+        config.update({"fp64_value": v64, "fp64_launch_shape": shape64, "mixed_over_fp64": value / v64,
+                       "fp64_failed_cells": int(fail64.sum()),
+                       "failed_sets_equal": bool(np.array_equal(failed_mask, fail64)),
+                       "mixed_vs_fp64_relerr": {
+                           "floor_1e-12": {"max": float(e12.max()), "q99": float(np.quantile(e12, 0.99)),
+                                           "median": float(np.median(e12))},
+                           "floor_1e-8": {"max": float(e8.max()), "q99": float(np.quantile(e8, 0.99)),
+                                          "median": float(np.median(e8))},
+                           "cells_max_over_1e-4_floor_1e-12": int(np.sum(e12.max(axis=1) > 1e-4))}})
+        log(f"fp64 same launch: {v64:.1f} cell-hours/s; mixed/fp64 {value / v64:.2f}; mixed vs fp64 max "
+            f"{e12.max():.2e} (floor 1e-12) {e8.max():.2e} (floor 1e-8), median {np.median(e12):.1e}")
+        state64 = run64.state.cpu().numpy()
+        del run64
+        torch.cuda.empty_cache()
+    else:
+        state64 = None
+
+    # ---- plain FP64: dense LU, one wavefront per cell, host interval loop --------
+    if form == "mixed" and args.plain_cells > 0 and not args.kernel_only and left() > 120:
+        idx = np.arange(0, n, max(1, n // args.plain_cells))[: args.plain_cells]
+        wp = S.config1(kmd, base[0], seed=1, cells=idx)
+        runp = D.DeviceRun(kmcd, wp)
+        runp.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        launches = runp.run_intervals(precision=T.PRECISION_FP64, schedule=T.SCHEDULE_GRID, lu=T.LU_DENSE)
+        torch.cuda.synchronize()
+        wallp = time.perf_counter() - t0
+        tp = runp.t.cpu().numpy()
+        vp = float(np.sum(np.minimum(tp, wp.tend) - wp.tbeg)) / 3600.0 / wallp
+        config.update({"fp64_plain_value": vp, "fp64_plain_cells": int(idx.size), "fp64_plain_sample": f"every "
+                       f"{max(1, n // args.plain_cells)}th cell", "fp64_plain_launches": launches,
+                       "mixed_over_fp64_plain": value / vp})
+        log(f"fp64-plain on {idx.size} cells: {vp:.1f} cell-hours/s ({launches} interval launches, {wallp:.1f} s)")
+        del runp
+        if args.graph_cells > 0 and left() > 90:
+            gidx = np.arange(0, n, max(1, n // args.graph_cells))[: args.graph_cells]
+            wg = S.config1(kmd, base[0], seed=1, cells=gidx)
+            runs = []
+            for graph in (False, True):
+                r = D.DeviceRun(kmcd, wg)
+                r.reset()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                if graph:
+                    nint = r.run_intervals_graph(precision=T.PRECISION_FP64, schedule=T.SCHEDULE_GRID,
+                                                 lu=T.LU_DENSE, intervals_per_launch=64)
+                else:
+                    nint = r.run_intervals(precision=T.PRECISION_FP64, schedule=T.SCHEDULE_GRID, lu=T.LU_DENSE)
+                torch.cuda.synchronize()
+                runs.append((time.perf_counter() - t0, nint, r.state.cpu().numpy()))
+            config.update({"fp64_plain_eager_vs_graph_cells": int(gidx.size),
+                           "fp64_plain_eager_value": wg.cell_hours / runs[0][0],
+                           "fp64_plain_graph_value": wg.cell_hours / runs[1][0],
+                           "fp64_plain_graph_intervals": runs[1][1],
+                           "fp64_plain_graph_identical": bool(np.array_equal(runs[0][2], runs[1][2]))})
+            log(f"captured plain loop on {gidx.size} cells: eager {wg.cell_hours / runs[0][0]:.1f}, graph "
+                f"{wg.cell_hours / runs[1][0]:.1f} cell-hours/s, identical {config['fp64_plain_graph_identical']}")
+
+    # ---- CPU: the oracle on strided samples of the same cells ---------------------
+    import oracle
     o = oracle.Oracle(MECH)
     f_rhs, f_jac = mechanism_flops(o)
-    lu, rest = flops_per_step(m, n_it, f_rhs, f_jac)
-    flop = acc * (lu + rest)
-    mixed = forms[args.form]["precision"] == T.PRECISION_MIXED
-    t_peak = acc * (lu / PEAK_FP32 + rest / PEAK_FP64) if mixed else flop / PEAK_FP64
-    peak_eff = flop / t_peak
+    cpu = None
+    n_it = n_it_gpu
+    n_it_source = "GPU run (no CPU leg)"
+    if not args.kernel_only and left() > 60:
+        model, usable = host_info()
+        ns = args.cpu_sample or threads
+        idx = np.arange(0, n, max(1, n // ns))[:ns]
+        ws = S.config1(kmd, base[0], seed=1, cells=idx)
+        fd = cpu_leg(o, ws, K, 1, threads)
+        log(f"cpu reference algorithm (FD Jacobian) on {ns} cells, {threads} threads: {fd['value']:.3f} "
+            f"cell-hours/s ({fd['wall_s']:.1f} s)")
+        na = 4 * threads
+        aidx = np.arange(0, n, max(1, n // na))[:na]
+        wa = S.config1(kmd, base[0], seed=1, cells=aidx)
+        an = cpu_leg(o, wa, K, 0, threads)
+        n_it, n_it_source = an["newton_per_step"], f"analytic oracle, {na}-cell strided sample"
+        log(f"cpu analytic oracle on {na} cells: {an['value']:.3f} cell-hours/s; Newton its/step {n_it:.3f}")
+        ok = (an["stats"]["failed"] == 0) & ~failed_mask[aidx]
+        ea = rel_err(state_main[aidx][ok, 3:3 + m], an["state"][ok, 3:3 + m], 1e-12)
+        sample_err = {f"{form}_max": float(ea.max()) if ea.size else None,
+                      f"{form}_median": float(np.median(ea)) if ea.size else None}
+        if state64 is not None:
+            e64 = rel_err(state64[aidx][ok, 3:3 + m], an["state"][ok, 3:3 + m], 1e-12)
+            sample_err.update({"fp64_max": float(e64.max()), "fp64_median": float(np.median(e64)),
+                               "fp64_cells_over_1e-9": int(np.sum(e64.max(axis=1) > 1e-9))})
+        cpu = {"value": fd["value"], "unit": "cell-hours/s", "cores": threads, "kind": "port",
+               "sample": f"{ns} cells (every {max(1, n // ns)}th) of the same workload, 3600 s each in {K} "
+                         f"slices, {fd['wall_s']:.1f} s wall; the reference algorithm restated in C "
+                         f"(forward-difference Jacobian, rate constants in every source-term call)",
+               "cpu_model": model, "host_cores_usable": usable,
+               "accepted_steps_p50_p99": [fd["steps_p50"], fd["steps_p99"]],
+               "newton_per_step_p50_p99": [fd["newton_per_step_p50"], fd["newton_per_step_p99"]],
+               "failed_cells": fd["failed"],
+               "analytic_oracle": {"value": an["value"], "cells": na, "wall_s": an["wall_s"],
+                                   "accepted_steps_p50_p99": [an["steps_p50"], an["steps_p99"]],
+                                   "newton_per_step": an["newton_per_step"], "failed_cells": an["failed"],
+                                   "gpu_relerr_on_sample_floor_1e-12": sample_err}}
+
+    # ---- roofline of the integrator launch ----------------------------------------
+    lu_f, rest_f = flops_per_step(m, n_it, f_rhs, f_jac)
+    flop = steps_total * (lu_f + rest_f)
+    mixed = prec == T.PRECISION_MIXED
+    t_peak = steps_total * (lu_f / PEAK_FP32 + rest_f / PEAK_FP64) if mixed else flop / PEAK_FP64
+    peak = flop / t_peak
     achieved = flop / kernel_s
-    log(f"form {args.form}: {wall:.2f} s wall per step, kernel region {kernel_s:.2f} s, {launches:.0f} launches; "
-        f"steps/cell {acc / w.n:.0f}, Newton its/step {n_it:.2f}, failed cells {failed}")
-    config = {"workload": "cb05_131072_cells_1h" if args.cells == 131072 else f"cb05_{args.cells}_cells_1h",
-              "mechanism": "CB05CL_AE5 (74 active species, 187 reactions)", "cells": w.n, "seed": 1,
-              "t_end_s": w.tend, "form": args.form, "accepted_steps_per_cell": acc / w.n,
-              "newton_iterations_per_step": n_it, "failed_cells": failed}
-
-    if args.form == "mixed" and not args.no_compare:
-        wall64, _, _, run64, _ = timed("fp64", args.cells, 1, 0)
-        v64 = w.cell_hours / wall64
-        a = run.state.cpu().numpy()[:, 3:3 + m]
-        b = run64.state.cpu().numpy()[:, 3:3 + m]
-        err = np.abs(a - b) / np.maximum(np.abs(b), 1e-12)
-        config.update({"fp64_value": v64, "mixed_over_fp64": value / v64,
-                       "mixed_vs_fp64_relerr": {"max": float(err.max()), "median": float(np.median(err)),
-                                                "q99": float(np.quantile(err, 0.99)), "floor": 1e-12}})
-        log(f"fp64 same launch: {v64:.1f} cell-hours/s; mixed/fp64 {value / v64:.2f}; "
-            f"rel diff max {err.max():.2e} median {np.median(err):.2e}")
-    if args.form == "mixed" and args.plain_cells > 0:
-        wallp, _, launchesp, runp, wp = timed("fp64-plain", args.plain_cells, 1, 0)
-        vp = wp.cell_hours / wallp
-        wallg, _, launchesg, rung, _ = timed("fp64-plain-graph", args.plain_cells, 1, 1)
-        vg = wp.cell_hours / wallg
-        same = bool(np.array_equal(runp.state.cpu().numpy(), rung.state.cpu().numpy()))
-        config.update({"fp64_plain_value": vp, "fp64_plain_cells": args.plain_cells,
-                       "fp64_plain_launches": launchesp, "fp64_plain_graph_value": vg,
-                       "fp64_plain_graph_launches": launchesg, "fp64_plain_graph_identical": same,
-                       "mixed_over_fp64_plain": value / max(vp, vg)})
-        log(f"fp64-plain on {args.plain_cells} cells: {vp:.1f} cell-hours/s ({launchesp:.0f} interval launches); "
-            f"captured loop {vg:.1f} cell-hours/s ({launchesg:.0f} graph launches, identical {same})")
-    if tuned_shapes:
-        config["autotuned"] = tuned_shapes
-
+    traffic, traffic_src = measured_traffic(config["workload"], form)
     out = {
         "metric": METRIC,
         "value": value,
         "unit": "cell-hours/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3,
+        "ms_per_step": wall / K * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64+f32" if mixed else "f64",
         "data": "synthetic",
         "config": config,
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak_eff / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / peak_eff, "traffic": measured_traffic(config["workload"], args.form),
-                     "flop_per_launch": flop / max(1.0, launches), "kernel": KERNEL,
-                     "kernel_s_per_step": kernel_s},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": KERNEL, "launches": K, "flop_per_launch": flop / K,
+                     "avg_launch_s": kernel_s / K, "newton_iterations_per_step": n_it,
+                     "newton_iterations_source": n_it_source,
+                     "peak_note": ("precision-weighted vector peak: LU flops at the FP32 rate, the rest at FP64"
+                                   if mixed else "FP64 vector peak")},
     }
-    if args.cpu_sample > 0:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        v, el = cpu_baseline(w, args.cpu_sample, threads)
-        out["cpu_baseline"] = {"value": v, "unit": "cell-hours/s", "cores": threads, "kind": "port",
-                               "sample": f"first {args.cpu_sample} cells of the same workload, 3600 s each, "
-                                         f"{el:.1f} s wall"}
-        log(f"cpu baseline: {v:.3f} cell-hours/s on {threads} threads ({el:.1f} s)")
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    out["bench_wall_s"] = time.perf_counter() - T0
     print(json.dumps(out), flush=True)
 
 

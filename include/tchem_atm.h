@@ -97,6 +97,12 @@ int tchem_kmd_species_name(tchem_kmd_t kmd, int i, char* buf, size_t len);
 /* Reaction indices of the photolysis slots, in slot order (nPhoto entries). */
 int tchem_kmd_photolysis_reactions(tchem_kmd_t kmd, int* reactions, int capacity);
 
+/* No counterpart in the reference (introspection of the compiled tables):
+ * the elimination order the device LU forms factor the Newton systems in,
+ * order[position] = active species index (m = nSpec - nConstSpec entries),
+ * a greedy minimum-Markowitz order of the structure of I - J.  Host only. */
+int tchem_kmd_elimination_order(tchem_kmd_t kmd, int* order, int capacity);
+
 /* Replaces AtmChemistry::setScenarioConditions (src/core/TChem_AtmosphericChemistry.cpp:429-504):
  * reads a scenario YAML into host state rows [nBatch][nSpec+3].  Call with
  * state_host == NULL to query nBatch. */

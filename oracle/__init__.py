@@ -34,7 +34,7 @@ class _KMD(ctypes.Structure):
         ("nCmaq", ctypes.c_int), ("cmaqIdx", ctypes.c_void_p), ("cmaqPar", ctypes.c_void_p),
         ("nEmis", ctypes.c_int), ("emisSpecies", ctypes.c_void_p), ("emisRate", ctypes.c_void_p),
         ("nPhoto", ctypes.c_int), ("photoReac", ctypes.c_void_p),
-        ("CONV_PPM", ctypes.c_double),
+        ("CONV_PPM", ctypes.c_double), ("lu_order", ctypes.c_void_p),
     ]
 
 
@@ -99,6 +99,7 @@ def lib():
         _lib.oracle_aerosol_jacobian.argtypes = [vp, vp, d, d, vp, vp, vp, vp, vp]
         _lib.oracle_aerosol_rhs_batch.argtypes = [vp, vp, i, vp, vp, vp]
         _lib.oracle_aerosol_run_batch.argtypes = [vp, vp, i, vp, vp, vp, vp, vp, vp, vp, vp, i]
+        _lib.oracle_detmath_batch.argtypes = [i, i, vp, vp, vp]
     return _lib
 
 
@@ -109,18 +110,28 @@ def _p(a):
 class Oracle:
     """Binds one mechanism's tables to the C oracle."""
 
-    def __init__(self, mech):
+    def __init__(self, mech, lu_order=True):
+        """lu_order: True = factor the Newton systems in the fill-reducing
+        elimination order (kmd.elimination_order), False = species order, or
+        an explicit order [m]."""
         self.k = mech if isinstance(mech, kmd_mod.OracleKMD) else kmd_mod.load(mech)
         k = self.k
+        if lu_order is True:
+            self.lu_order = kmd_mod.elimination_order(k)
+        elif lu_order is False or lu_order is None:
+            self.lu_order = None
+        else:
+            self.lu_order = np.ascontiguousarray(lu_order, np.int32)
         self._keep = [k.reacNreac, k.reacNprod, k.reacSidx, k.reacNuki, k.convExponent, k.arrIdx, k.arrPar,
-                      k.troeIdx, k.troePar, k.cmaqIdx, k.cmaqPar, k.emisSpecies, k.emisRate, k.photoReac]
+                      k.troeIdx, k.troePar, k.cmaqIdx, k.cmaqPar, k.emisSpecies, k.emisRate, k.photoReac,
+                      self.lu_order]
         self.s = _KMD(k.nSpec, k.nConstSpec, k.nReac, k.maxSpecInReac,
                       _p(k.reacNreac), _p(k.reacNprod), _p(k.reacSidx), _p(k.reacNuki), _p(k.convExponent),
                       len(k.arrIdx), _p(k.arrIdx), _p(k.arrPar),
                       len(k.troeIdx), _p(k.troeIdx), _p(k.troePar),
                       len(k.cmaqIdx), _p(k.cmaqIdx), _p(k.cmaqPar),
                       len(k.emisSpecies), _p(k.emisSpecies), _p(k.emisRate),
-                      len(k.photoReac), _p(k.photoReac), kmd_mod.CONV_PPM)
+                      len(k.photoReac), _p(k.photoReac), kmd_mod.CONV_PPM, _p(self.lu_order))
         self.ptr = ctypes.addressof(self.s)
 
     @property
@@ -269,6 +280,16 @@ class AerosolOracle:
                                        _p(nc), _p(t_out), _p(dt_out), _p(stats), int(nthreads))
 
 
+def detmath(fn: str, x, y=None):
+    """The portable exp / log / log10 / pow the rate constants use, on arrays."""
+    code = {"exp": 0, "log": 1, "log10": 2, "pow": 3}[fn]
+    x = np.ascontiguousarray(x, np.float64).ravel()
+    yy = np.ascontiguousarray(np.broadcast_to(0.0 if y is None else y, x.shape), np.float64).ravel()
+    out = np.zeros_like(x)
+    lib().oracle_detmath_batch(code, x.size, _p(x), _p(yy), _p(out))
+    return out
+
+
 def lu(A):
     A = np.array(A, np.float64, order="C")
     n = A.shape[0]
@@ -287,3 +308,32 @@ def trbdf2_constants():
     out = np.zeros(6)
     lib().oracle_trbdf2_constants(_p(out))
     return out
+
+
+def run_sliced(o: Oracle, workload, state, photo, n_slices=1, jacobian_mode=0, nthreads=0):
+    """The workload's hour [tbeg, tend] on the given rows, cut into n_slices
+    equal runDeviceBatch intervals (each run to its end; t_out / dt_out
+    carried into the next interval's TimeAdvance).  Returns (state, t_out,
+    dt_out, stats summed over the slices; failed = failed in any)."""
+    w = workload
+    st = np.ascontiguousarray(state, np.float64).copy()
+    ph = None if photo is None or photo.shape[1] == 0 else np.ascontiguousarray(photo, np.float64)
+    n = st.shape[0]
+    t = np.full(n, w.tbeg)
+    dt = np.full(n, w.dt_in)
+    tn = np.array([w.atol_newton, w.rtol_newton])
+    tt = np.tile([w.atol_time, w.rtol_time], (o.m, 1))
+    total = np.zeros(n, STATS_DTYPE)
+    for k in range(n_slices):
+        tend = w.tend if k == n_slices - 1 else w.tbeg + (w.tend - w.tbeg) * (k + 1) / n_slices
+        ta = np.zeros(n, TADV_DTYPE)
+        ta["tbeg"], ta["tend"], ta["dt"], ta["dtmin"], ta["dtmax"] = t, tend, dt, w.dtmin, w.dtmax
+        ta["max_num_newton_iterations"] = w.max_newton
+        ta["num_time_iterations_per_interval"] = 10_000_000
+        ta["jacobian_interval"] = w.jacobian_interval
+        s = np.zeros(n, STATS_DTYPE)
+        o.run_batch(tn, tt, ta, st, t, dt, photo=ph, stats=s, jacobian_mode=jacobian_mode, nthreads=nthreads)
+        for f in ("accepted", "rejected", "newton_failures", "newton_iterations", "factorizations"):
+            total[f] += s[f]
+        total["failed"] = np.maximum(total["failed"], s["failed"])
+    return st, t, dt, total

@@ -174,6 +174,51 @@ def load(path: str) -> OracleKMD:
     )
 
 
+def elimination_order(k: OracleKMD) -> np.ndarray:
+    """Fill-reducing order of the active species for the dense Newton solves:
+    greedy minimum Markowitz count on the structure of I - J (entry (i, s)
+    where species i has a nonzero net coefficient in a reaction with active
+    reactant s; the diagonal always).  Each step takes the remaining species
+    with the smallest (row count - 1) * (column count - 1) over the remaining
+    species (ties: lowest index) and adds its fill.  The product's mechanism
+    compiler uses the same rule (tests/test_oracle.py checks the two orders
+    agree); the order only changes the rounding of the linear solves."""
+    m = k.m
+    half = k.maxSpecInReac // 2
+    pat = np.eye(m, dtype=bool)
+    for r in range(k.nReac):
+        net = {}
+        for j in range(k.reacNreac[r]):
+            s = int(k.reacSidx[r, j])
+            if s < m:
+                net[s] = net.get(s, 0.0) + float(k.reacNuki[r, j])
+        for j in range(k.reacNprod[r]):
+            s, nu = int(k.reacSidx[r, half + j]), float(k.reacNuki[r, half + j])
+            if s < m and nu != 0.0:
+                net[s] = net.get(s, 0.0) + nu
+        rows = [e for e, v in net.items() if v != 0.0]
+        for j in range(k.reacNreac[r]):
+            s = int(k.reacSidx[r, j])
+            if s < m:
+                pat[rows, s] = True
+    done = np.zeros(m, bool)
+    order = []
+    for _ in range(m):
+        live = ~done
+        rc = pat[:, live].sum(axis=1).astype(np.int64)
+        cc = pat[live, :].sum(axis=0).astype(np.int64)
+        score = np.where(live, (rc - 1) * (cc - 1), np.iinfo(np.int64).max)
+        b = int(np.argmin(score))  # first minimum: lowest index on ties
+        order.append(b)
+        done[b] = True
+        live = ~done
+        rows = np.nonzero(live & pat[:, b])[0]
+        cols = np.nonzero(live & pat[b, :])[0]
+        if rows.size and cols.size:
+            pat[np.ix_(rows, cols)] = True
+    return np.array(order, np.int32)
+
+
 def load_scenario(path: str, k: OracleKMD) -> np.ndarray:
     """State rows [nBatch][nSpec+3] from a scenario/mechanism YAML, restating
     AtmChemistry::setScenarioConditions (src/core/TChem_AtmosphericChemistry.cpp:429-504)."""

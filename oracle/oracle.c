@@ -7,6 +7,11 @@
 #include "oracle.h"
 
 #include <math.h>
+
+/* exp / log / pow from IEEE operations only (the rate constants are then the
+ * same bits on the host and on the GPU); pinned against the C library in
+ * tests/test_detmath.py. */
+#include "../tchem-atm_amd/csrc/detmath.h"
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -16,7 +21,10 @@
 /* TR-BDF2 with gamma = 2 - sqrt(2) (Bank et al. 1985). */
 #define TRB_GAMMA (2.0 - 1.41421356237309504880)
 
-static double pow_int(double x, int n) { return pow(x, (double)n); }
+/* pow(double, int) of the reference (ats<real_type>::pow(c, niup)) for the
+ * small integer exponents of mechanisms: repeated multiplication, which for
+ * n <= 2 is the correctly rounded power. */
+static double pow_int(double x, int n) { return tchem_ipow(x, n); }
 
 /* Rate constants: src/core/impl/TChem_Impl_KForwardAerosol.hpp:54-131.
  * Same expression order as the reference; photolysis overrides replace A of
@@ -35,7 +43,7 @@ void oracle_kforward(const oracle_kmd* k, double t, double p, const double* phot
       for (int q = 0; q < k->nPhoto; ++q)
         if (k->photoReac[q] == r) A = photo[q];
     const double B = P[1], C = P[2], D = P[3], E = P[4];
-    kfor[r] = A * exp(C * t_1) * (B == 0.0 ? one : pow(t / D, B)) * (E == 0.0 ? one : (one + E * p)) *
+    kfor[r] = A * tchem_exp(C * t_1) * (B == 0.0 ? one : tchem_pow(t / D, B)) * (E == 0.0 ? one : (one + E * p)) *
               pow_int(conv, k->convExponent[r]);
   }
   for (int i = 0; i < k->nTroe; ++i) {
@@ -44,17 +52,18 @@ void oracle_kforward(const oracle_kmd* k, double t, double p, const double* phot
     const double k0_A = rp[0], k0_B = rp[1], k0_C = rp[2];
     const double kinf_A = rp[3], kinf_B = rp[4], kinf_C = rp[5], Fc = rp[6], N = rp[7];
     const double k0 =
-        k0_A * (k0_C == 0.0 ? one : exp(k0_C * t_1)) * (k0_B == 0.0 ? one : pow(t / 300.0, k0_B)) * conv;
+        k0_A * (k0_C == 0.0 ? one : tchem_exp(k0_C * t_1)) * (k0_B == 0.0 ? one : tchem_pow(t / 300.0, k0_B)) * conv;
     const double k0_kinf =
-        k0 / (kinf_A * (kinf_C == 0.0 ? one : exp(kinf_C * t_1)) * (kinf_B == 0.0 ? one : pow(t / 300.0, kinf_B)));
-    kfor[r] = k0 / (1.0 + k0_kinf) * pow(Fc, 1.0 / (1.0 + pow(log10(k0_kinf) / N, 2.0))) *
+        k0 / (kinf_A * (kinf_C == 0.0 ? one : tchem_exp(kinf_C * t_1)) *
+              (kinf_B == 0.0 ? one : tchem_pow(t / 300.0, kinf_B)));
+    kfor[r] = k0 / (1.0 + k0_kinf) * tchem_pow(Fc, 1.0 / (1.0 + pow_int(tchem_log10(k0_kinf) / N, 2))) *
               pow_int(conv, k->convExponent[r]);
   }
   for (int i = 0; i < k->nCmaq; ++i) {
     const double* P = k->cmaqPar + 6 * i;
     const int r = k->cmaqIdx[i];
-    kfor[r] = (P[0] * exp(P[2] * t_1) * (P[1] == 0.0 ? one : pow(t / 300.0, P[1])) +
-               P[3] * exp(P[5] * t_1) * (P[4] == 0.0 ? one : pow(t / 300.0, P[4])) * conv) *
+    kfor[r] = (P[0] * tchem_exp(P[2] * t_1) * (P[1] == 0.0 ? one : tchem_pow(t / 300.0, P[1])) +
+               P[3] * tchem_exp(P[5] * t_1) * (P[4] == 0.0 ? one : tchem_pow(t / 300.0, P[4])) * conv) *
               pow_int(conv, k->convExponent[r]);
   }
 }
@@ -124,7 +133,9 @@ void oracle_jacobian(const oracle_kmd* k, const double* kfor, const double* c, d
 }
 
 /* Dense LU with partial pivoting (row interchanges), Doolittle form;
- * multipliers use the reciprocal of the pivot, l = a_ik * (1 / a_kk).
+ * multipliers use the reciprocal of the pivot, l = a_ik * (1 / a_kk); every
+ * update a_ij - l u_kj is one fused multiply-add (one rounding), as are the
+ * substitution updates of oracle_lu_solve.
  * The reference factors with Tines' UTV; both solve the same Newton system.
  * Pinned by: scipy.linalg.lu_factor and the backward error on random and
  * CB05 iteration matrices. */
@@ -149,7 +160,7 @@ int oracle_lu(int n, double* A, int lda, int* piv) {
       const double l = A[(size_t)i * lda + k] * inv;
       A[(size_t)i * lda + k] = l;
       if (l != 0.0)
-        for (int j = k + 1; j < n; ++j) A[(size_t)i * lda + j] -= l * A[(size_t)k * lda + j];
+        for (int j = k + 1; j < n; ++j) A[(size_t)i * lda + j] = fma(-l, A[(size_t)k * lda + j], A[(size_t)i * lda + j]);
     }
   }
   return 0;
@@ -163,14 +174,21 @@ void oracle_lu_solve(int n, const double* A, int lda, const int* piv, double* b)
   for (int k = 0; k < n; ++k) {
     const double yk = b[k];
     if (yk != 0.0)
-      for (int i = k + 1; i < n; ++i) b[i] -= A[(size_t)i * lda + k] * yk;
+      for (int i = k + 1; i < n; ++i) b[i] = fma(-A[(size_t)i * lda + k], yk, b[i]);
   }
   for (int k = n - 1; k >= 0; --k) {
     const double xk = b[k] * (1.0 / A[(size_t)k * lda + k]);
     b[k] = xk;
     if (xk != 0.0)
-      for (int i = 0; i < k; ++i) b[i] -= A[(size_t)i * lda + k] * xk;
+      for (int i = 0; i < k; ++i) b[i] = fma(-A[(size_t)i * lda + k], xk, b[i]);
   }
+}
+
+/* the portable elementary functions of the rate constants, for tests:
+ * fn 0 exp(x), 1 log(x), 2 log10(x), 3 pow(x, y) */
+void oracle_detmath_batch(int fn, int n, const double* x, const double* y, double* out) {
+  for (int i = 0; i < n; ++i)
+    out[i] = fn == 0 ? tchem_exp(x[i]) : fn == 1 ? tchem_log(x[i]) : fn == 2 ? tchem_log10(x[i]) : tchem_pow(x[i], y[i]);
 }
 
 void oracle_trbdf2_constants(double* out6) {
@@ -207,7 +225,7 @@ typedef struct {
 static void cell_kforward(cell_ctx* x) {
   oracle_kforward(x->k, x->T, x->p, x->photo, x->kfor);
   if (x->lscale)
-    for (int r = 0; r < x->k->nReac; ++r) x->kfor[r] *= exp(x->lscale[r]);
+    for (int r = 0; r < x->k->nReac; ++r) x->kfor[r] *= tchem_exp(x->lscale[r]);
 }
 
 /* source term at u (active part); the constant part of c is fixed per cell */
@@ -253,26 +271,41 @@ static void jac(cell_ctx* x, const double* u, double* J) {
 }
 
 /* Newton iteration for one TR-BDF2 stage (Tines NewtonSolver, restated):
- * G(u) = u - b - ch * f(u), iteration matrix I - ch*J.  The Jacobian and its
+ * G(u) = u - b - ch * f(u), iteration matrix I - ch*J, factorised in the
+ * kinetic model's lu_order when it has one (a symmetric permutation of the
+ * same system: the order changes only the rounding of the solve).  The Jacobian and its
  * factorisation are refreshed when the iteration count within the stage is a
  * multiple of jacobian_interval (reference default 1: every iteration).
  * Converged when max_i |dx_i| / (atol + rtol |u_i|) <= 1. */
 static int newton(cell_ctx* x, double* u, const double* b, double ch, const double* tol_newton, int max_it,
                   int jac_interval, double* f, double* dx, oracle_stats* st) {
   const int m = x->m;
+  /* elimination order of the gas-phase solves (NULL: species order) */
+  const int* ord = x->a ? NULL : x->k->lu_order;
   if (jac_interval < 1) jac_interval = 1;
   for (int it = 0; it < max_it; ++it) {
     if (it % jac_interval == 0) {
       jac(x, u, x->J);
-      for (int i = 0; i < m; ++i)
-        for (int j = 0; j < m; ++j)
-          x->A[(size_t)i * m + j] = (i == j ? 1.0 : 0.0) - ch * x->J[(size_t)i * m + j];
+      for (int i = 0; i < m; ++i) {
+        const int ri = ord ? ord[i] : i;
+        for (int j = 0; j < m; ++j) {
+          const int cj = ord ? ord[j] : j;
+          x->A[(size_t)i * m + j] = (ri == cj ? 1.0 : 0.0) - ch * x->J[(size_t)ri * m + cj];
+        }
+      }
       if (oracle_lu(m, x->A, m, x->piv) != 0) return 0;
       if (st) st->factorizations++;
     }
     rhs(x, u, f);
     for (int i = 0; i < m; ++i) dx[i] = -(u[i] - b[i] - ch * f[i]);
-    oracle_lu_solve(m, x->A, m, x->piv, dx);
+    if (ord) {
+      double* y = x->fh; /* FD scratch, free once the Jacobian is built */
+      for (int i = 0; i < m; ++i) y[i] = dx[ord[i]];
+      oracle_lu_solve(m, x->A, m, x->piv, y);
+      for (int i = 0; i < m; ++i) dx[ord[i]] = y[i];
+    } else {
+      oracle_lu_solve(m, x->A, m, x->piv, dx);
+    }
     double nrm = 0.0;
     for (int i = 0; i < m; ++i) {
       u[i] += dx[i];

@@ -48,6 +48,9 @@ typedef struct {
   int nPhoto;
   const int* photoReac;    /* [nPhoto] reactions whose A is a per-cell photolysis rate */
   double CONV_PPM;
+  /* nullable [m]: elimination order of the dense Newton solves (position ->
+   * active species), a fill-reducing order; NULL = species order */
+  const int* lu_order;
 } oracle_kmd;
 
 typedef struct { /* byte layout of TChem::TimeAdvance (src/core/TChem_Util.hpp:469-476) */
@@ -144,6 +147,9 @@ void oracle_aerosol_rhs_batch(const oracle_kmd* k, const oracle_amd* a, int nBat
 void oracle_aerosol_run_batch(const oracle_kmd* k, const oracle_amd* a, int nBatch, const double* tol_newton,
                               const double* tol_time, const oracle_time_advance* tadv, double* state,
                               const double* nc, double* t_out, double* dt_out, oracle_stats* stats, int nthreads);
+
+/* exp / log / log10 / pow of the rate constants over arrays (for tests) */
+void oracle_detmath_batch(int fn, int n, const double* x, const double* y, double* out);
 
 /* TR-BDF2 constants used by the oracle (for tests) */
 void oracle_trbdf2_constants(double* out6);

@@ -61,7 +61,7 @@ REALTIME_HZ = 100e6  # device real-time counter of t_start / t_end
 # every symbol include/tchem_atm.h declares
 EXPORTED_SYMBOLS = (
     "tchem_last_error", "tchem_kmd_load_yaml", "tchem_kmd_destroy", "tchem_kmd_info", "tchem_kmd_species_name",
-    "tchem_kmd_photolysis_reactions", "tchem_scenario_load", "tchem_kmcd_create", "tchem_kmcd_destroy",
+    "tchem_kmd_photolysis_reactions", "tchem_kmd_elimination_order", "tchem_scenario_load", "tchem_kmcd_create", "tchem_kmcd_destroy",
     "tchem_atmchem_workspace_bytes", "tchem_atmchem_run_device_batch", "tchem_atmchem_update_time_advance",
     "tchem_sourceterm_run_device_batch", "tchem_jacobian_run_device_batch", "tchem_adjoint_workspace_bytes",
     "tchem_adjoint_forward", "tchem_adjoint_backward", "tchem_adjoint_steps", "tchem_atmchem_graph_create",
@@ -97,6 +97,7 @@ def _load():
         "tchem_kmd_info": [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)],
         "tchem_kmd_species_name": [vp, i, cp, ctypes.c_size_t],
         "tchem_kmd_photolysis_reactions": [vp, vp, i],
+        "tchem_kmd_elimination_order": [vp, vp, i],
         "tchem_scenario_load": [vp, cp, vp, i, ctypes.POINTER(i)],
         "tchem_kmcd_create": [vp, ctypes.POINTER(vp)],
         "tchem_kmcd_destroy": [vp],
@@ -202,6 +203,13 @@ class KineticModelData:
     @property
     def nActive(self) -> int:
         return self.nSpec - self.nConstSpec
+
+    def eliminationOrder(self) -> np.ndarray:
+        """Order in which the device LU forms eliminate the active species
+        (position -> species index); computed on the host."""
+        out = np.zeros(max(self.nActive, 1), np.int32)
+        _check(_lib.tchem_kmd_elimination_order(self._h, out.ctypes.data, out.size))
+        return out[: self.nActive].copy()
 
     def createConstData(self) -> "KineticModelConstData":
         return KineticModelConstData(self)

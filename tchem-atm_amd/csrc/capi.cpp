@@ -377,6 +377,18 @@ int tchem_kmd_photolysis_reactions(tchem_kmd_t kmd, int* reactions, int capacity
   return TCHEM_OK;
 }
 
+int tchem_kmd_elimination_order(tchem_kmd_t kmd, int* order, int capacity) {
+  if (!kmd || !order) return setError(TCHEM_ERR_INVALID, "tchem_kmd_elimination_order: bad argument");
+  try {
+    const MechTables t = compileMechanism(kmd->km);
+    if (capacity < t.m) return setError(TCHEM_ERR_INVALID, "tchem_kmd_elimination_order: capacity too small");
+    for (int a = 0; a < t.m; ++a) order[a] = t.perm[a];
+    return TCHEM_OK;
+  } catch (const std::exception& e) {
+    return setError(TCHEM_ERR_UNSUPPORTED, std::string("tchem_kmd_elimination_order: ") + e.what());
+  }
+}
+
 int tchem_scenario_load(tchem_kmd_t kmd, const char* path, double* state_host, int capacity_rows, int* nBatch) {
   if (!kmd || !path || !nBatch) return setError(TCHEM_ERR_INVALID, "tchem_scenario_load: null argument");
   try {
@@ -432,9 +444,12 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
   d.nPairs = t.nPairs();
   d.nnzJ = t.nnzJ();
   d.nSeg = t.nSeg();
+  d.nSerJ = t.nSerJ();
   d.convPPM = t.convPPM;
   std::vector<int> rowcol(t.jnzRow.size());
   for (size_t n = 0; n < rowcol.size(); ++n) rowcol[n] = (t.jnzRow[n] << 16) | t.jnzCol[n];
+  std::vector<int> serRowcol(t.serJRow.size());
+  for (size_t n = 0; n < serRowcol.size(); ++n) serRowcol[n] = (t.serJRow[n] << 16) | t.serJCol[n];
   // pack every table into one int and one double buffer
   std::vector<int> ib;
   std::vector<double> db;
@@ -469,6 +484,13 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
   d.oOmegaCoef = addD(t.omegaCoef);
   d.oEmission = addD(t.emission);
   d.oJcCoef = addD(t.jcCoef);
+  d.oSerOmegaPtr = addI(t.serOmegaPtr);
+  d.oSerOmegaReac = addI(t.serOmegaReac);
+  d.oSerOmegaCoef = addD(t.serOmegaCoef);
+  d.oSerJRowCol = addI(serRowcol);
+  d.oSerJPtr = addI(t.serJPtr);
+  d.oSerJPair = addI(t.serJPair);
+  d.oSerJCoef = addD(t.serJCoef);
   // sparse schedules (mechanisms they cannot express run the dense form only)
   auto words = [](const std::vector<unsigned>& v) { return std::vector<int>(v.begin(), v.end()); };
   auto pack = [&](const SparseSchedule& sc, DevSparse& sp) {

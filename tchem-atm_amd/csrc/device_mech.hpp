@@ -10,7 +10,7 @@
 namespace tchem_amd {
 
 struct DevMech {
-  int nSpec, nConst, m, nReac, nPhoto, maxR, nPairs, nnzJ, nSeg;
+  int nSpec, nConst, m, nReac, nPhoto, maxR, nPairs, nnzJ, nSeg, nSerJ;
   double convPPM;
   // Every table lives in one int buffer and one double buffer; the kernels
   // address them by 32-bit offsets so a wavefront keeps two base pointers
@@ -18,8 +18,9 @@ struct DevMech {
   const int* ib;
   const double* db;
   int oPerm, oRateKind, oConvExp, oPhotoSlot, oReacSpec, oReacPow, oOmegaReac, oSegBegin, oSegEnd, oRowSegPtr,
-      oPairPtr, oPairSlot, oJnzRowCol, oJnzPtr, oJcPair;
-  int oRatePar, oOmegaCoef, oEmission, oJcCoef;
+      oPairPtr, oPairSlot, oJnzRowCol, oJnzPtr, oJcPair, oSerOmegaPtr, oSerOmegaReac, oSerJRowCol, oSerJPtr,
+      oSerJPair;
+  int oRatePar, oOmegaCoef, oEmission, oJcCoef, oSerOmegaCoef, oSerJCoef;
 
   __host__ __device__ const int* perm() const { return ib + oPerm; }             // [m] internal -> reference species
   __host__ __device__ const int* rateKind() const { return ib + oRateKind; }     // [nReac]
@@ -40,6 +41,14 @@ struct DevMech {
   __host__ __device__ const double* omegaCoef() const { return db + oOmegaCoef; }// [nnzOmega]
   __host__ __device__ const double* emission() const { return db + oEmission; }  // [m]
   __host__ __device__ const double* jcCoef() const { return db + oJcCoef; }      // [nContrib]
+  // serial gathers of the dense form (mechanism.hpp)
+  __host__ __device__ const int* serOmegaPtr() const { return ib + oSerOmegaPtr; }   // [m+1]
+  __host__ __device__ const int* serOmegaReac() const { return ib + oSerOmegaReac; } // [nSerOmega]
+  __host__ __device__ const double* serOmegaCoef() const { return db + oSerOmegaCoef; }
+  __host__ __device__ const int* serJRowCol() const { return ib + oSerJRowCol; }     // [nSerJ] row * 65536 + col
+  __host__ __device__ const int* serJPtr() const { return ib + oSerJPtr; }           // [nSerJ+1]
+  __host__ __device__ const int* serJPair() const { return ib + oSerJPair; }         // [nSerContrib]
+  __host__ __device__ const double* serJCoef() const { return db + oSerJCoef; }
 };
 
 // Device view of the compiled sparse programs (schedule.hpp); arrays live

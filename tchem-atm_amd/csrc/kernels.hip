@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdint>
 
+#include "detmath.h"
 #include "device_mech.hpp"
 #include "schedule.hpp"
 
@@ -92,13 +93,17 @@ __device__ __forceinline__ TA bcast(const TA (&v)[QM], int idx) {
   return out;
 }
 
-__device__ __forceinline__ double pow_small(double x, int n) {
-  return n == 1 ? x : (n == 2 ? x * x : pow(x, double(n)));
-}
+// pow(double, int) of the reference for the small integer powers of
+// mechanisms: repeated multiplication (x^2 is the correctly rounded square)
+__device__ __forceinline__ double pow_small(double x, int n) { return tchem_ipow(x, n); }
 
-// Rate constant of reaction r: src/core/impl/TChem_Impl_KForwardAerosol.hpp:54-131.
+// Rate constant of reaction r: src/core/impl/TChem_Impl_KForwardAerosol.hpp:54-131,
+// in the reference's expression order, with the portable exp / log / pow of
+// detmath.h and no fused operations, so every device and the host give the
+// same bits.
 __device__ double rate_constant(const DevMech& M, int r, double t, double p, double t_1, double conv,
                                 const double* photo) {
+#pragma clang fp contract(off)
   const double* P = M.ratePar() + 8 * r;
   const double one = 1.0;
   const int kind = M.rateKind()[r];
@@ -107,18 +112,20 @@ __device__ double rate_constant(const DevMech& M, int r, double t, double p, dou
     double A = P[0];
     const int slot = M.photoSlot()[r];
     if (photo != nullptr && slot >= 0) A = photo[slot];
-    k = A * exp(P[2] * t_1) * (P[1] == 0.0 ? one : pow(t / P[3], P[1])) * (P[4] == 0.0 ? one : (one + P[4] * p));
+    k = A * tchem_exp(P[2] * t_1) * (P[1] == 0.0 ? one : tchem_pow(t / P[3], P[1])) *
+        (P[4] == 0.0 ? one : (one + P[4] * p));
   } else if (kind == 1) {  // Troe
-    const double k0 = P[0] * (P[2] == 0.0 ? one : exp(P[2] * t_1)) * (P[1] == 0.0 ? one : pow(t / 300.0, P[1])) * conv;
-    const double k0_kinf =
-        k0 / (P[3] * (P[5] == 0.0 ? one : exp(P[5] * t_1)) * (P[4] == 0.0 ? one : pow(t / 300.0, P[4])));
-    const double lg = log10(k0_kinf) / P[7];
-    k = k0 / (1.0 + k0_kinf) * pow(P[6], 1.0 / (1.0 + lg * lg));
+    const double k0 = P[0] * (P[2] == 0.0 ? one : tchem_exp(P[2] * t_1)) *
+                      (P[1] == 0.0 ? one : tchem_pow(t / 300.0, P[1])) * conv;
+    const double k0_kinf = k0 / (P[3] * (P[5] == 0.0 ? one : tchem_exp(P[5] * t_1)) *
+                                 (P[4] == 0.0 ? one : tchem_pow(t / 300.0, P[4])));
+    const double lg = tchem_log10(k0_kinf) / P[7];
+    k = k0 / (1.0 + k0_kinf) * tchem_pow(P[6], 1.0 / (1.0 + lg * lg));
   } else {  // CMAQ_H2O2
-    k = P[0] * exp(P[2] * t_1) * (P[1] == 0.0 ? one : pow(t / 300.0, P[1])) +
-        P[3] * exp(P[5] * t_1) * (P[4] == 0.0 ? one : pow(t / 300.0, P[4])) * conv;
+    k = P[0] * tchem_exp(P[2] * t_1) * (P[1] == 0.0 ? one : tchem_pow(t / 300.0, P[1])) +
+        P[3] * tchem_exp(P[5] * t_1) * (P[4] == 0.0 ? one : tchem_pow(t / 300.0, P[4])) * conv;
   }
-  return k * pow(conv, double(M.convExp()[r]));
+  return k * tchem_ipow(conv, M.convExp()[r]);
 }
 
 template <typename TA>
@@ -172,21 +179,16 @@ __device__ __forceinline__ double rate_of_progress(const DevMech& M, const doubl
 }
 
 // omega (internal order) at the concentrations in LDS; result in registers.
-// src/core/impl/TChem_Impl_ReactionRatesAerosol.hpp:78-138, as a gather:
-// segment sums over (reaction, net coefficient) entries, then per species
-// the emission rate plus its segment sums in order.
+// src/core/impl/TChem_Impl_ReactionRatesAerosol.hpp:78-138 as a gather: one
+// lane per species adds the emission rate, then nu * rop of every reactant
+// and product entry of that species one at a time, in the order of the
+// reference's loops (the serial order of its atomic adds), each product
+// rounded before the add: the same sum, bit for bit.
 template <int QM, typename TA>
 __device__ void source_term_from_lds(const DevMech& M, const CellLds<TA>& L, int lane, double (&f)[QM]) {
+#pragma clang fp contract(off)
 #pragma unroll 2
   for (int r = lane; r < M.nReac; r += kWave) L.scr[r] = rate_of_progress(M, L.c, L.kf, r);
-  wave_sync();
-  for (int s = lane; s < M.nSeg; s += kWave) {
-    const int e1 = M.segEnd()[s];
-    double acc = 0.0;
-#pragma unroll 4
-    for (int e = M.segBegin()[s]; e < e1; ++e) acc += M.omegaCoef()[e] * L.scr[M.omegaReac()[e]];
-    L.seg[s] = acc;
-  }
   wave_sync();
 #pragma unroll
   for (int q = 0; q < QM; ++q) {
@@ -195,8 +197,11 @@ __device__ void source_term_from_lds(const DevMech& M, const CellLds<TA>& L, int
     if (a < M.m) {
       s = M.emission()[a];
       if (L.src) s += L.src[M.perm()[a]];
-      const int s1 = M.rowSegPtr()[a + 1];
-      for (int g = M.rowSegPtr()[a]; g < s1; ++g) s += L.seg[g];
+      const int e1 = M.serOmegaPtr()[a + 1];
+      for (int e = M.serOmegaPtr()[a]; e < e1; ++e) {
+        const double v = M.serOmegaCoef()[e] * L.scr[M.serOmegaReac()[e]];
+        s = s + v;
+      }
     }
     f[q] = s;
   }
@@ -214,11 +219,17 @@ __device__ void eval_rhs(const DevMech& M, const CellLds<TA>& L, int lane, const
   source_term_from_lds<QM>(M, L, lane, f);
 }
 
-// A = diag * I + scale * J at the concentrations in LDS.  With external_order
-// the entries are placed at the reference's species indices.
+// A = diag * I + scale * J at the concentrations in LDS.  J is the analytic
+// Jacobian of source_term_from_lds: derivative pairs d rop_r / d c_s, then
+// per structural entry the contributions nu * d one at a time in the
+// reference loop order (reaction, reactant slot of the derivative, then the
+// reactants and products of the reaction), each product rounded before the
+// add.  With external_order the entries are placed at the reference's
+// species indices.
 template <typename TA>
 __device__ void assemble_jacobian_from_lds(const DevMech& M, const CellLds<TA>& L, int lane, double diag, double scale,
                                            bool external_order) {
+#pragma clang fp contract(off)
   const int m = M.m;
   {
     const size_t bytes = sizeof(TA) * size_t(m) * size_t(m);
@@ -247,12 +258,14 @@ __device__ void assemble_jacobian_from_lds(const DevMech& M, const CellLds<TA>& 
     }
   }
   wave_sync();
-  for (int n = lane; n < M.nnzJ; n += kWave) {
+  for (int n = lane; n < M.nSerJ; n += kWave) {
     double s = 0.0;
-    const int e1 = M.jnzPtr()[n + 1];
-#pragma unroll 4
-    for (int e = M.jnzPtr()[n]; e < e1; ++e) s += M.jcCoef()[e] * L.scr[M.jcPair()[e]];
-    const int rc = M.jnzRowCol()[n];
+    const int e1 = M.serJPtr()[n + 1];
+    for (int e = M.serJPtr()[n]; e < e1; ++e) {
+      const double v = M.serJCoef()[e] * L.scr[M.serJPair()[e]];
+      s = s + v;
+    }
+    const int rc = M.serJRowCol()[n];
     int row = rc >> 16, col = rc & 0xffff;
     const double v = (row == col ? diag : 0.0) + scale * s;
     if (external_order) {
@@ -268,11 +281,12 @@ __device__ void assemble_jacobian_from_lds(const DevMech& M, const CellLds<TA>& 
 // interchanges recorded in L.swp, count in nswp), eliminating the right-hand
 // side b (row a on lane a % 64, slot a / 64) alongside.  Same arithmetic as
 // oracle_lu + the forward half of oracle_lu_solve: pivot = first row of
-// largest |a_ik|, l = a_ik * (1 / a_kk), a_ij -= l a_kj, b_i -= l b_k.
-// The reciprocal pivots are kept for the back substitution.
-// Returns false for a zero or non-finite pivot.
+// largest |a_ik|, l = a_ik * (1 / a_kk), a_ij = fma(-l, a_kj, a_ij),
+// b_i = fma(-l, b_k, b_i).  The reciprocal pivots are kept for the back
+// substitution.  Returns false for a zero or non-finite pivot.
 template <int QM, typename TA>
 __device__ bool lu_factor_fused(const CellLds<TA>& L, int m, int lane, TA (&b)[QM], int& nswp) {
+#pragma clang fp contract(off)
   TA* A = L.A;
   nswp = 0;
   for (int k = 0; k < m; ++k) {
@@ -353,7 +367,7 @@ __device__ bool lu_factor_fused(const CellLds<TA>& L, int m, int lane, TA (&b)[Q
       if (i > k && i < m && a[q] != TA(0)) {
         l[q] = a[q] * inv;
         A[i * m + k] = l[q];
-        b[q] -= l[q] * bk;
+        b[q] = fma(-l[q], bk, b[q]);
       }
     }
     // trailing update: rows with a nonzero multiplier, two at a time
@@ -376,15 +390,15 @@ __device__ bool lu_factor_fused(const CellLds<TA>& L, int m, int lane, TA (&b)[Q
             if (j < m && uk[c] != TA(0)) {
               const TA x0 = A[i0 * m + j];
               const TA x1 = A[i1 * m + j];
-              A[i0 * m + j] = x0 - l0 * uk[c];
-              A[i1 * m + j] = x1 - l1 * uk[c];
+              A[i0 * m + j] = fma(-l0, uk[c], x0);
+              A[i1 * m + j] = fma(-l1, uk[c], x1);
             }
           }
         } else {
 #pragma unroll
           for (int c = 0; c < QM; ++c) {
             const int j = k + 1 + lane + kWave * c;
-            if (j < m && uk[c] != TA(0)) A[i0 * m + j] -= l0 * uk[c];
+            if (j < m && uk[c] != TA(0)) A[i0 * m + j] = fma(-l0, uk[c], A[i0 * m + j]);
           }
         }
       }
@@ -397,6 +411,7 @@ __device__ bool lu_factor_fused(const CellLds<TA>& L, int m, int lane, TA (&b)[Q
 // Forward elimination with stored factors: interchanges, then unit-lower L.
 template <int QM, typename TA>
 __device__ void lu_forward(const CellLds<TA>& L, int m, int lane, TA (&b)[QM], int nswp) {
+#pragma clang fp contract(off)
   const TA* A = L.A;
   for (int s = 0; s < nswp; ++s) {
     const int k = L.swp[2 * s], p = L.swp[2 * s + 1];
@@ -424,7 +439,7 @@ __device__ void lu_forward(const CellLds<TA>& L, int m, int lane, TA (&b)[QM], i
     const TA yk = bcast<QM>(b, k);
     if (yk != TA(0)) {
 #pragma unroll
-      for (int q = 0; q < QM; ++q) b[q] -= col[q] * yk;
+      for (int q = 0; q < QM; ++q) b[q] = fma(-col[q], yk, b[q]);
     }
 #pragma unroll
     for (int q = 0; q < QM; ++q) col[q] = nxt[q];
@@ -435,6 +450,7 @@ __device__ void lu_forward(const CellLds<TA>& L, int m, int lane, TA (&b)[QM], i
 // the next column is fetched from LDS while the current one is applied.
 template <int QM, typename TA>
 __device__ void lu_back(const CellLds<TA>& L, int m, int lane, TA (&b)[QM]) {
+#pragma clang fp contract(off)
   const TA* A = L.A;
   TA col[QM];
   TA ukk = L.rdiag[m - 1];
@@ -464,7 +480,7 @@ __device__ void lu_back(const CellLds<TA>& L, int m, int lane, TA (&b)[QM]) {
     }
     if (xk != TA(0)) {
 #pragma unroll
-      for (int q = 0; q < QM; ++q) b[q] -= col[q] * xk;
+      for (int q = 0; q < QM; ++q) b[q] = fma(-col[q], xk, b[q]);
     }
     ukk = nkk;
 #pragma unroll
@@ -897,6 +913,7 @@ struct DenseBackend {
   static __device__ void evaluate(const IntegratorArgs& a, const Lds& L, int lane, const double (&u)[QM],
                                   const double (&bc)[QM], double ch, bool newton, bool refresh, double (&f)[QM],
                                   TA (&x)[QM]) {
+#pragma clang fp contract(off)
     eval_rhs<QM>(a.M, L, lane, u, f);
 #pragma unroll
     for (int q = 0; q < QM; ++q) {
@@ -1055,6 +1072,9 @@ __device__ __forceinline__ double uniform_d(double v) {
 // REC: also record the accepted steps for the discrete adjoint (a.rec).
 template <class B, int QM, bool REC>
 __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& Lw, int cell, int lane) {
+  // no implicit fused multiply-adds: the stage, residual and error
+  // arithmetic rounds as oracle_run_batch does
+#pragma clang fp contract(off)
   using TA = typename B::TA;
   const DevMech& M = a.M;
   const int m = B::unknowns(a), mg = M.m, nSpec = M.nSpec, dim = B::rowDim(a);
@@ -1850,7 +1870,7 @@ __global__ __launch_bounds__(256) void rates_kernel(DevMech M, int nBatch, const
   const double t_1 = 1.0 / T;
   const double conv = M.convPPM * p * t_1;
   const double k = rate_constant(M, r, T, p, t_1, conv, photo ? photo + size_t(cell) * M.nPhoto : nullptr);
-  kf[idx] = lscale ? k * exp(lscale[size_t(cell) * lstride + r]) : k;
+  kf[idx] = lscale ? k * tchem_exp(lscale[size_t(cell) * lstride + r]) : k;
 }
 
 // ---- E3SM / UCI kinetics batch functions --------------------------------------

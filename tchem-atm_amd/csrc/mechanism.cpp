@@ -148,6 +148,50 @@ MechTables compileMechanism(const KineticModel& km) {
     }
     t.jnzPtr.push_back(int(t.jcPair.size()));
   }
+
+  // serial gathers (dense form): uncombined entries in the reference's order
+  {
+    const int joff = km.maxSpecInReac / 2;
+    std::vector<std::vector<std::pair<int, double>>> srows(m);
+    for (int r = 0; r < nR; ++r) {
+      for (int j = 0; j < km.reacNreac[r]; ++j)
+        if (km.sidx(r, j) < m) srows[t.iperm[km.sidx(r, j)]].push_back({r, km.nuki(r, j)});
+      for (int j = 0; j < km.reacNprod[r]; ++j)
+        if (km.sidx(r, j + joff) < m) srows[t.iperm[km.sidx(r, j + joff)]].push_back({r, km.nuki(r, j + joff)});
+    }
+    t.serOmegaPtr.assign(m + 1, 0);
+    for (int a = 0; a < m; ++a) {
+      t.serOmegaPtr[a + 1] = t.serOmegaPtr[a] + int(srows[a].size());
+      for (const auto& e : srows[a]) {
+        t.serOmegaReac.push_back(e.first);
+        t.serOmegaCoef.push_back(e.second);
+      }
+    }
+    std::map<std::pair<int, int>, std::vector<std::pair<int, double>>> smap;
+    for (int i = 0; i < m; ++i) smap[{i, i}];
+    int p = 0;  // pairs are numbered in (reaction, active reactant slot) order
+    for (int r = 0; r < nR; ++r)
+      for (int js = 0; js < km.reacNreac[r]; ++js) {
+        const int ks = km.sidx(r, js);
+        if (ks >= m) continue;
+        const int col = t.iperm[ks];
+        for (int j = 0; j < km.reacNreac[r]; ++j)
+          if (km.sidx(r, j) < m) smap[{t.iperm[km.sidx(r, j)], col}].push_back({p, km.nuki(r, j)});
+        for (int j = 0; j < km.reacNprod[r]; ++j)
+          if (km.sidx(r, j + joff) < m) smap[{t.iperm[km.sidx(r, j + joff)], col}].push_back({p, km.nuki(r, j + joff)});
+        ++p;
+      }
+    t.serJPtr.push_back(0);
+    for (const auto& e : smap) {
+      t.serJRow.push_back(e.first.first);
+      t.serJCol.push_back(e.first.second);
+      for (const auto& c : e.second) {
+        t.serJPair.push_back(c.first);
+        t.serJCoef.push_back(c.second);
+      }
+      t.serJPtr.push_back(int(t.serJPair.size()));
+    }
+  }
   return t;
 }
 

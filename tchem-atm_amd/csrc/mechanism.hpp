@@ -72,7 +72,24 @@ struct MechTables {
   std::vector<int> jcPair;       // [nContrib]
   std::vector<double> jcCoef;    // [nContrib]
 
+  // Serial forms of the source-term gather and of the Jacobian for the
+  // dense LU form: every reactant and product entry of the reference's
+  // tables, uncombined (zero yields included), in the order the reference's
+  // loops add them (reaction order; within a reaction the reactants, then
+  // the products; for J, the reactant slot of the derivative before the
+  // target entry), src/core/impl/TChem_Impl_ReactionRatesAerosol.hpp:116-138.
+  // Summed one entry at a time, they give the reference's serial sums bit
+  // for bit.  Rows and columns in the internal numbering.
+  std::vector<int> serOmegaPtr;     // [m+1]
+  std::vector<int> serOmegaReac;    // [nSerOmega]
+  std::vector<double> serOmegaCoef; // [nSerOmega]
+  std::vector<int> serJRow, serJCol; // [nSerJ] structural entries, row-major
+  std::vector<int> serJPtr;         // [nSerJ+1]
+  std::vector<int> serJPair;        // [nSerContrib]
+  std::vector<double> serJCoef;     // [nSerContrib]
+
   int nPairs() const { return int(pairSlot.size()); }
+  int nSerJ() const { return int(serJRow.size()); }
   int nSeg() const { return int(segBegin.size()); }
   int nnzJ() const { return int(jnzRow.size()); }
 };

@@ -103,6 +103,42 @@ class DeviceRun:
                 break
         return total
 
+    def start_slices(self, n_slices):
+        """Resets the run for run_slice: every cell at tbeg with dt_in, the
+        hour [tbeg, tend] cut into n_slices equal slices."""
+        w = self.w
+        self.reset()
+        self.slice_ends = [w.tbeg + (w.tend - w.tbeg) * (k + 1) / n_slices for k in range(n_slices)]
+        self.slice_ends[-1] = w.tend
+        self.tadv = T.time_advance_tensor(self.n, w.tbeg, self.slice_ends[0], w.dt_in, w.dtmin, w.dtmax,
+                                          w.max_newton, 10_000_000, w.jacobian_interval, self.device)
+        self._tadv64 = self.tadv.view(self.torch.float64).view(self.n, 7)
+        self._graph_key = None
+        self.acc_stats = self.torch.zeros((self.n, 6), dtype=self.torch.int64, device=self.device)
+
+    def run_slice(self, k, precision=T.PRECISION_MIXED, schedule=T.SCHEDULE_PERSISTENT, blocks_per_cu=0,
+                  lu=T.LU_AUTO, cells_per_block=0, programs=T.PROGRAMS_AUTO, collect_stats=True, stream=None):
+        """Slice k of the hour: every cell from its own t_out to the slice end
+        in one launch (persistent: one work-queue launch), then t_out / dt_out
+        carried into the next slice's TimeAdvance records.  Cells that
+        failed keep dt = -1 and stay failed.  Per-cell counters accumulate
+        in acc_stats (accepted, rejected, Newton failures, Newton
+        iterations, factorisations, failed)."""
+        torch = self.torch
+        if k > 0:
+            self._tadv64[:, 0].copy_(self.t)
+            self._tadv64[:, 1].fill_(self.slice_ends[k])
+            self._tadv64[:, 2].copy_(self.dt)
+        T.AtmosphericChemistry.runDeviceBatch(self.tol_newton, self.tol_time, None, self.tadv, self.state, self.t,
+                                              self.dt, self.state, self.kmcd, self.photo,
+                                              self.stats if collect_stats else None, precision=precision,
+                                              schedule=schedule, blocks_per_cu=blocks_per_cu, lu=lu,
+                                              cells_per_block=cells_per_block, programs=programs, stream=stream)
+        if collect_stats:
+            s = self.stats.view(torch.int32).view(self.n, 10)
+            self.acc_stats[:, :5] += s[:, :5]
+            self.acc_stats[:, 5] = torch.maximum(self.acc_stats[:, 5], s[:, 5].to(torch.int64))
+
     def run_to_end(self, precision=T.PRECISION_MIXED, schedule=T.SCHEDULE_PERSISTENT, blocks_per_cu=0,
                    collect_stats=False, stream=None, lu=T.LU_AUTO, cells_per_block=0, programs=T.PROGRAMS_AUTO):
         """Every cell to tend in one launch (iteration cap 10^7 per cell)."""

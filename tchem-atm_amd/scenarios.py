@@ -1,9 +1,13 @@
 """Seeded synthetic cell states for the CB05 box-model workloads.
 
-All draws use numpy's Philox4x32 counter-based generator keyed by the seed,
-one array per quantity, in this order: pressure [n], temperature [n],
-active-species mixing ratios [n][m] (ppb, log-uniform), photolysis rates
-[n][nPhoto], and for the day/night workload the night flags [n].
+Every random number is keyed by (seed, cell, slot): slot s of cell c is the
+Philox4x32-10 block of counter (s, c, 0, 0) under key (seed, 0), turned into
+a double in [0, 1) from its first two words (27 + 26 bits).  A cell's state
+therefore depends only on the seed and its own index, so any subset of a
+workload (the first cells, a strided sample, a list of indices) holds exactly
+the rows the full workload holds.  Slots per cell: 0 pressure, 1 temperature,
+2 .. 2+m-1 active-species mixing ratios (ppb, log-uniform, mechanism order),
+then nPhoto photolysis rates, then the night flag.
 
 State rows follow the reference's layout [density, pressure(Pa),
 temperature(K), species...] (src/core/TChem_Util.hpp:635-676) in its ppm
@@ -52,18 +56,56 @@ def _const_species(base_state_row: np.ndarray, m: int, n_spec: int) -> np.ndarra
     return base_state_row[3 + m: 3 + n_spec]
 
 
+_PHILOX_M0, _PHILOX_M1 = np.uint64(0xD2511F53), np.uint64(0xCD9E8D57)
+_PHILOX_W0, _PHILOX_W1 = np.uint64(0x9E3779B9), np.uint64(0xBB67AE85)
+_MASK32 = np.uint64(0xFFFFFFFF)
+
+
+def philox4x32(c0, c1, c2, c3, k0, k1, rounds=10):
+    """Philox4x32 (Salmon et al., SC'11) on uint64 arrays holding 32-bit
+    words; returns the four output words."""
+    c0, c1, c2, c3 = (np.asarray(x, np.uint64) & _MASK32 for x in (c0, c1, c2, c3))
+    k0, k1 = np.uint64(k0) & _MASK32, np.uint64(k1) & _MASK32
+    for _ in range(rounds):
+        p0 = _PHILOX_M0 * c0
+        p1 = _PHILOX_M1 * c2
+        hi0, lo0 = p0 >> np.uint64(32), p0 & _MASK32
+        hi1, lo1 = p1 >> np.uint64(32), p1 & _MASK32
+        c0, c1, c2, c3 = hi1 ^ c1 ^ k0, lo1, hi0 ^ c3 ^ k1, lo0
+        k0 = (k0 + _PHILOX_W0) & _MASK32
+        k1 = (k1 + _PHILOX_W1) & _MASK32
+    return c0, c1, c2, c3
+
+
+def keyed_uniform(seed: int, cells, slots) -> np.ndarray:
+    """U[0,1) doubles [len(cells)][len(slots)] keyed by (seed, cell, slot)."""
+    cells = np.asarray(cells, np.uint64).reshape(-1, 1)
+    slots = np.asarray(slots, np.uint64).reshape(1, -1)
+    w0, w1, _, _ = philox4x32(np.broadcast_to(slots, (cells.shape[0], slots.shape[1])), cells, 0, 0, seed, 0)
+    return ((w0 >> np.uint64(5)).astype(np.float64) * 67108864.0 + (w1 >> np.uint64(6)).astype(np.float64)) \
+        / 9007199254740992.0
+
+
+def _cell_index(n: int, cells) -> np.ndarray:
+    return np.arange(n, dtype=np.int64) if cells is None else np.asarray(cells, np.int64)
+
+
 def cb05_cells(n: int, seed: int, m: int, n_spec: int, n_photo: int, base_state_row: np.ndarray,
-               t_range=(270.0, 310.0), night_fraction: float = 0.0) -> tuple[np.ndarray, np.ndarray]:
-    """State rows and photolysis rates for n independent cells."""
-    g = np.random.Generator(np.random.Philox(key=seed))
-    p = g.uniform(5.0e4, 1.013e5, n)
-    T = g.uniform(t_range[0], t_range[1], n)
-    ppb = 10.0 ** g.uniform(-3.0, 2.0, (n, m))
-    photo = g.uniform(0.0, 1.0e-3, (n, n_photo))
+               t_range=(270.0, 310.0), night_fraction: float = 0.0, cells=None) -> tuple[np.ndarray, np.ndarray]:
+    """State rows and photolysis rates of cells 0..n-1 (or of the given cell
+    indices): p ~ U[5e4, 1.013e5] Pa, T ~ U[t_range] K, active species
+    10^U[-3, 2] ppb, photolysis U[0, 1e-3] s^-1, all photolysis 0 where the
+    night flag draw is below night_fraction."""
+    idx = _cell_index(n, cells)
+    u = keyed_uniform(seed, idx, np.arange(3 + m + n_photo))
+    p = 5.0e4 + (1.013e5 - 5.0e4) * u[:, 0]
+    T = t_range[0] + (t_range[1] - t_range[0]) * u[:, 1]
+    ppb = 10.0 ** (-3.0 + 5.0 * u[:, 2:2 + m])
+    photo = 1.0e-3 * u[:, 2 + m:2 + m + n_photo]
     if night_fraction > 0.0:
-        night = g.uniform(0.0, 1.0, n) < night_fraction
+        night = u[:, 2 + m + n_photo] < night_fraction
         photo[night, :] = 0.0
-    state = np.zeros((n, n_spec + 3))
+    state = np.zeros((idx.size, n_spec + 3))
     state[:, 1] = p
     state[:, 2] = T
     state[:, 3:3 + m] = ppb * 1e-3
@@ -71,18 +113,26 @@ def cb05_cells(n: int, seed: int, m: int, n_spec: int, n_photo: int, base_state_
     return state, photo
 
 
-def config0(kmd, base_state_row, n: int = 1024, seed: int = 0) -> Workload:
-    """1024 cells, T ~ U[270,310] K, 3600 s, dtmax 60 s, atol 1e-12, rtol 1e-6."""
+def night_flags(n: int, seed: int, m: int, n_photo: int, night_fraction: float = 0.5, cells=None) -> np.ndarray:
+    """The night flags cb05_cells draws for these cells."""
+    idx = _cell_index(n, cells)
+    return keyed_uniform(seed, idx, [2 + m + n_photo])[:, 0] < night_fraction
+
+
+def config0(kmd, base_state_row, n: int = 1024, seed: int = 0, cells=None) -> Workload:
+    """1024 cells, T ~ U[270,310] K, 3600 s, dtmax 60 s, atol 1e-12, rtol 1e-6
+    (cells: a subset of cell indices instead of 0..n-1)."""
     m = kmd.nSpec - kmd.nConstSpec
-    st, ph = cb05_cells(n, seed, m, kmd.nSpec, kmd.nPhoto, base_state_row)
+    st, ph = cb05_cells(n, seed, m, kmd.nSpec, kmd.nPhoto, base_state_row, cells=cells)
     return Workload("cb05_config0", st, ph, 0.0, 3600.0, 1e-20, 1e-20, 60.0, 1e-12, 1e-6)
 
 
-def config1(kmd, base_state_row, n: int = 131072, seed: int = 1) -> Workload:
-    """131072 cells (1820 columns x 72 levels), T ~ U[220,320] K, half the cells at night."""
+def config1(kmd, base_state_row, n: int = 131072, seed: int = 1, cells=None) -> Workload:
+    """131072 cells (1820 columns x 72 levels), T ~ U[220,320] K, half the
+    cells at night (cells: a subset of cell indices instead of 0..n-1)."""
     m = kmd.nSpec - kmd.nConstSpec
     st, ph = cb05_cells(n, seed, m, kmd.nSpec, kmd.nPhoto, base_state_row, t_range=(220.0, 320.0),
-                        night_fraction=0.5)
+                        night_fraction=0.5, cells=cells)
     return Workload("cb05_config1", st, ph, 0.0, 3600.0, 1e-20, 1e-20, 60.0, 1e-12, 1e-6)
 
 

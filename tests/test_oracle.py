@@ -281,3 +281,44 @@ def test_reference_algorithm_mode_agrees(cb05_oracle, oracle_mod):
         out.append(st[0, 3:3 + m])
     rel = np.abs(out[0] - out[1]) / np.maximum(np.abs(out[0]), 1e-12)
     assert rel.max() < 1e-4
+
+
+@pytest.mark.parametrize("mech", ["cb05cl_ae5.yaml", "carbonbond05.yaml", "cb05_soa_gas.yaml"])
+def test_elimination_order_matches_the_library(pkg, oracle_mod, mech):
+    """The oracle factors the Newton systems in the fill-reducing order it
+    computes itself (oracle/kmd.py: elimination_order); it is the order the
+    library's mechanism compiler picks (queried on the host)."""
+    k = oracle_mod.kmd_mod.load(golden(mech))
+    order = oracle_mod.kmd_mod.elimination_order(k)
+    assert sorted(order.tolist()) == list(range(k.m))
+    kmd = pkg.KineticModelData(golden(mech))
+    np.testing.assert_array_equal(order, kmd.eliminationOrder())
+
+
+def test_elimination_order_matches_on_the_synthetic_mechanism(pkg, oracle_mod, tmp_path):
+    import importlib
+    S = importlib.import_module("tchem-atm_amd.scenarios")
+    path = S.synthetic_mechanism_yaml(str(tmp_path / "synthetic250.yaml"))
+    k = oracle_mod.kmd_mod.load(path)
+    np.testing.assert_array_equal(oracle_mod.kmd_mod.elimination_order(k),
+                                  pkg.KineticModelData(path).eliminationOrder())
+
+
+def test_dense_lu_order_changes_only_rounding(cb05_oracle, oracle_mod):
+    """The same Newton system factored in species order and in the
+    elimination order: the solutions agree to rounding."""
+    k = cb05_oracle.k
+    rng = np.random.default_rng(5)
+    c = np.concatenate([10.0 ** rng.uniform(-6, -1, k.m), np.ones(k.nConstSpec)])
+    kf = cb05_oracle.kforward(290.0, 9e4, rng.uniform(0, 1e-3, len(k.photoReac)))
+    J = cb05_oracle.jacobian(kf, c)
+    A = np.eye(k.m) - 10.0 * J
+    b = rng.standard_normal(k.m)
+    order = cb05_oracle.lu_order
+    LU, piv, info = oracle_mod.lu(A)
+    x0 = oracle_mod.lu_solve(LU, piv, b)
+    LUp, pivp, infop = oracle_mod.lu(A[np.ix_(order, order)])
+    xp = np.empty(k.m)
+    xp[order] = oracle_mod.lu_solve(LUp, pivp, b[order])
+    assert info == 0 and infop == 0
+    assert np.abs(x0 - xp).max() <= 1e-12 * np.abs(x0).max()
