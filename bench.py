@@ -519,6 +519,49 @@ def e3sm_run(tiles, out_path=None):
             f.write(text + "\n")
 
 
+def cpu_study(threads, out_path=None, fd_cells=64, an_cells0=1024, an_cells1=2048):
+    """The CPU baseline on the host's cores: the reference algorithm restated
+    (forward-difference Jacobian) and the analytic oracle on config 0
+    (CB05CL_AE5, seed 0, T ~ U[270,310] K) and on a strided subset of
+    config 1 (seed 1, 131072 cells, T ~ U[220,320] K, half at night), each
+    cell advanced 3600 s in one interval; cell-hours/s, accepted steps and
+    Newton iterations per step (p50 / p99), failed cells."""
+    import oracle
+    T = importlib.import_module("tchem-atm_amd")
+    S = importlib.import_module("tchem-atm_amd.scenarios")
+    kmd = T.KineticModelData(MECH)
+    base = T.setScenarioConditions(MECH, kmd)
+    o = oracle.Oracle(MECH)
+    model, usable = host_info()
+    rows = []
+    for name, make, n_total in (("config0", S.config0, 1024), ("config1", S.config1, 131072)):
+        for mode, label, ncells in ((1, "reference algorithm (FD Jacobian)", fd_cells),
+                                    (0, "analytic oracle", an_cells0 if name == "config0" else an_cells1)):
+            idx = np.arange(0, n_total, max(1, n_total // ncells))[:ncells]
+            w = make(kmd, base[0], seed=0 if name == "config0" else 1, cells=idx)
+            r = cpu_leg(o, w, 1, mode, threads)
+            rows.append(dict(workload=name, form=label, cells=int(idx.size), stride=int(max(1, n_total // ncells)),
+                             value=r["value"], wall_s=r["wall_s"], steps_p50=r["steps_p50"], steps_p99=r["steps_p99"],
+                             newton_per_step=r["newton_per_step"], newton_p50=r["newton_per_step_p50"],
+                             newton_p99=r["newton_per_step_p99"], failed=r["failed"]))
+            log(f"{name} {label}: {rows[-1]}")
+    lines = [f"CPU baseline study: CB05CL_AE5, 3600 s per cell, one interval; host {model}, "
+             f"{usable} usable cores, {threads} threads (OpenMP, static schedule, one cell per thread)", "",
+             f"{'workload':8} {'form':34} {'cells':>6} {'cell-h/s':>9} {'wall s':>7} {'steps p50':>9} "
+             f"{'p99':>7} {'Newton/step':>11} {'p50':>5} {'p99':>5} {'failed':>6}"]
+    for r in rows:
+        lines.append(f"{r['workload']:8} {r['form']:34} {r['cells']:>6} {r['value']:>9.3f} {r['wall_s']:>7.1f} "
+                     f"{r['steps_p50']:>9.0f} {r['steps_p99']:>7.0f} {r['newton_per_step']:>11.3f} "
+                     f"{r['newton_p50']:>5.2f} {r['newton_p99']:>5.2f} {r['failed']:>6}")
+    text = "\n".join(lines)
+    print(json.dumps({"cpu_study": rows, "cpu_model": model, "host_cores_usable": usable, "threads": threads}),
+          flush=True)
+    log(text)
+    if out_path:
+        with open(out_path, "w") as f:
+            f.write(text + "\n")
+
+
 def copy_probe():
     import torch
     n = 1 << 27  # 1 GiB of float64
@@ -567,6 +610,8 @@ def main():
                     help="time the E3SM box model on the tiled UCI column instead (JSON line, optionally to OUT)")
     ap.add_argument("--e3sm-tiles", type=int, default=8192)
     ap.add_argument("--aerosol-cpu-sample", type=int, default=128)
+    ap.add_argument("--cpu-study", metavar="OUT", nargs="?", const="",
+                    help="time the CPU baseline on config 0 and a config-1 subset instead of the benchmark")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the path shards cells across GPUs with no exchange; "
@@ -575,6 +620,9 @@ def main():
     threads = args.cpu_threads or default_threads()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.cpu_study is not None:
+        cpu_study(threads, args.cpu_study or None)
         return
     if args.config4 is not None:
         config4_run(args.config4_scale, args.config4 or None)
@@ -627,19 +675,27 @@ def main():
         return shape
 
     def sliced(run, precision, shape, slices):
-        """The hour in `slices` persistent launches; (wall s, [s per slice])."""
+        """The hour in `slices` persistent launches; (wall s, [s per slice],
+        [(longest cell time s, its index) per slice])."""
         run.start_slices(slices)
         stream = torch.cuda.current_stream()
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(slices + 1)]
+        longest = torch.zeros((slices, 2), dtype=torch.int64, device="cuda")
+        st64 = run.stats.view(torch.int64).view(run.n, 5)  # [3] t_start, [4] t_end (device clock)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ev[0].record(stream)
         for k in range(slices):
             run.run_slice(k, precision=precision, schedule=T.SCHEDULE_PERSISTENT, **shape)
+            el = st64[:, 4] - st64[:, 3]
+            longest[k, 0] = el.max()
+            longest[k, 1] = el.argmax()
             ev[k + 1].record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        return wall, [ev[k].elapsed_time(ev[k + 1]) / 1e3 for k in range(slices)]
+        lg = longest.cpu().numpy()
+        return wall, [ev[k].elapsed_time(ev[k + 1]) / 1e3 for k in range(slices)], \
+            [(float(x) / T.REALTIME_HZ, int(c)) for x, c in lg]
 
     # ---- headline form: warm-up on a sample, then K timed slices -----------------
     form = args.form
@@ -653,7 +709,7 @@ def main():
             wu.run_slice(k % K, precision=prec, schedule=T.SCHEDULE_PERSISTENT, **shape)
         torch.cuda.synchronize()
         del wu
-    wall, per_slice = sliced(run, prec, shape, K)
+    wall, per_slice, longest_cells = sliced(run, prec, shape, K)
     value = w.cell_hours / wall
     kernel_s = float(sum(per_slice))
     acc = run.acc_stats.cpu().numpy()
@@ -674,7 +730,13 @@ def main():
               "accepted_steps_p50_p99": [float(np.percentile(acc[:, 0], 50)), float(np.percentile(acc[:, 0], 99))],
               "newton_iterations_per_step": n_it_gpu, "failed_cells": int(failed_mask.sum()),
               "failed_cell_indices": [int(i) for i in np.nonzero(failed_mask)[0][:64]],
-              "slice_kernel_s": [round(x, 4) for x in per_slice]}
+              "slice_kernel_s": [round(x, 4) for x in per_slice],
+              "slice_longest_cell_s": [round(x[0], 4) for x in longest_cells],
+              "slice_longest_cell_index": [x[1] for x in longest_cells]}
+    fidx = np.nonzero(failed_mask)[0]
+    if fidx.size:
+        log("failed cells (index: accepted, rejected, Newton failures):",
+            ", ".join(f"{c}: {acc[c, 0]}/{acc[c, 1]}/{acc[c, 2]}" for c in fidx[:16]))
     del run
     torch.cuda.empty_cache()
 
@@ -682,7 +744,7 @@ def main():
     if form == "mixed" and not args.no_compare and not args.kernel_only:
         run64 = D.DeviceRun(kmcd, w)
         shape64 = tune(run64, T.PRECISION_FP64)
-        wall64, per64 = sliced(run64, T.PRECISION_FP64, shape64, K)
+        wall64, per64, _ = sliced(run64, T.PRECISION_FP64, shape64, K)
         v64 = w.cell_hours / wall64
         b = run64.state.cpu().numpy()[:, 3:3 + m]
         a = state_main[:, 3:3 + m]
@@ -704,25 +766,52 @@ def main():
         del run64
         torch.cuda.empty_cache()
     else:
-        state64 = None
+        state64, fail64 = None, None
 
     # ---- plain FP64: dense LU, one wavefront per cell, host interval loop --------
-    if form == "mixed" and args.plain_cells > 0 and not args.kernel_only and left() > 120:
-        idx = np.arange(0, n, max(1, n // args.plain_cells))[: args.plain_cells]
+    # The dense FP64 form computes in the oracle's arithmetic (bit for bit,
+    # tests/test_gpu_parity.py), on the same K slices: its rows are the FP64
+    # oracle's rows for the sampled cells, the reference for the error of the
+    # mixed and sparse FP64 forms.
+    if form == "mixed" and args.plain_cells > 0 and not args.kernel_only and left() > 150:
+        stride = max(1, n // args.plain_cells)
+        idx = np.arange(0, n, stride)[: args.plain_cells]
         wp = S.config1(kmd, base[0], seed=1, cells=idx)
         runp = D.DeviceRun(kmcd, wp)
-        runp.reset()
+        runp.start_slices(K)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        launches = runp.run_intervals(precision=T.PRECISION_FP64, schedule=T.SCHEDULE_GRID, lu=T.LU_DENSE)
+        launches = 0
+        for k in range(K):
+            launches += runp.run_slice_intervals(k, iters=wp.iters_per_interval)
         torch.cuda.synchronize()
         wallp = time.perf_counter() - t0
-        tp = runp.t.cpu().numpy()
-        vp = float(np.sum(np.minimum(tp, wp.tend) - wp.tbeg)) / 3600.0 / wallp
-        config.update({"fp64_plain_value": vp, "fp64_plain_cells": int(idx.size), "fp64_plain_sample": f"every "
-                       f"{max(1, n // args.plain_cells)}th cell", "fp64_plain_launches": launches,
+        vp = wp.cell_hours / wallp
+        oracle_rows = runp.state.cpu().numpy()[:, 3:3 + m]
+        ofail = runp.acc_stats[:, 5].cpu().numpy() > 0
+        config.update({"fp64_plain_value": vp, "fp64_plain_cells": int(idx.size),
+                       "fp64_plain_sample": f"every {stride}th cell", "fp64_plain_launches": launches,
                        "mixed_over_fp64_plain": value / vp})
         log(f"fp64-plain on {idx.size} cells: {vp:.1f} cell-hours/s ({launches} interval launches, {wallp:.1f} s)")
+
+        def against_oracle(rows, fail):
+            ok = ~ofail & ~fail[idx]
+            a, b = rows[idx][ok][:, 3:3 + m], oracle_rows[ok]
+            e12, e8 = rel_err(a, b, 1e-12), rel_err(a, b, 1e-8)
+            tn = np.abs(a - b) / (w.atol_time + w.rtol_time * np.abs(b))
+            return {"cells": int(ok.sum()), "failed_sets_equal": bool(np.array_equal(fail[idx], ofail)),
+                    "max_floor_1e-8": float(e8.max()), "max_floor_1e-12": float(e12.max()),
+                    "median_floor_1e-12": float(np.median(e12)), "q99_floor_1e-12": float(np.quantile(e12, 0.99)),
+                    "tolerance_normalised_max": float(tn.max()),
+                    "cells_over_1e-9_floor_1e-12": int(np.sum(e12.max(axis=1) > 1e-9))}
+
+        acc_err = {form: against_oracle(state_main, failed_mask)}
+        if state64 is not None:
+            acc_err["fp64"] = against_oracle(state64, fail64)
+        config["error_vs_fp64_oracle_on_plain_sample"] = acc_err
+        log("vs FP64 oracle on the plain sample: " + "; ".join(
+            f"{k}: max {v['max_floor_1e-8']:.2e} (floor 1e-8) {v['max_floor_1e-12']:.2e} (floor 1e-12), "
+            f"tolerance-normalised {v['tolerance_normalised_max']:.3f}" for k, v in acc_err.items()))
         del runp
         if args.graph_cells > 0 and left() > 90:
             gidx = np.arange(0, n, max(1, n // args.graph_cells))[: args.graph_cells]

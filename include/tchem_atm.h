@@ -113,7 +113,9 @@ int tchem_scenario_load(tchem_kmd_t kmd, const char* path, double* state_host, i
 /* Replaces createNCAR_KineticModelConstData<device>(kmd)
  * (src/core/TChem_KineticModelNCAR_ConstData.hpp:115-148): compiles the
  * mechanism into device tables on the current HIP device. The kmcd is
- * independent of the kmd afterwards. */
+ * independent of the kmd afterwards.  tchem_kmcd_destroy returns
+ * TCHEM_ERR_INVALID (and destroys nothing) while captured graphs made from
+ * the kmcd exist: they read its tables. */
 int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out);
 int tchem_kmcd_destroy(tchem_kmcd_t kmcd);
 
@@ -177,7 +179,11 @@ int tchem_atmchem_e3sm_run_device_batch(const tchem_atmchem_options_t* opts, int
  * runs all nIntervals.  Options are resolved once at capture (blocks_per_cu 0
  * -> the occupancy limit).  graph_create is a setup call (synchronises the
  * device, allocates); tchem_graph_status reads the intervals run and the stop
- * flag of the last launch (synchronous). */
+ * flag of the last launch (synchronous).  Lifetime: the graph owns its rate
+ * constants, work-queue counter and loop control, so later calls on the same
+ * kmcd (any batch size) leave it valid; it reads the kmcd's tables and the
+ * caller's arrays, which must outlive it (the kmcd refuses destruction until
+ * its graphs are destroyed).  Launches of one graph must not overlap. */
 int tchem_atmchem_graph_create(const tchem_atmchem_options_t* opts, int nBatch, const double* tol_newton,
                                const double* tol_time, tchem_time_advance_t* tadv, double* state,
                                const double* photo_rates, double* t_out, double* dt_out, double* state_out,

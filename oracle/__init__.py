@@ -310,11 +310,14 @@ def trbdf2_constants():
     return out
 
 
-def run_sliced(o: Oracle, workload, state, photo, n_slices=1, jacobian_mode=0, nthreads=0):
+def run_sliced(o: Oracle, workload, state, photo, n_slices=1, jacobian_mode=0, nthreads=0, ends=None,
+               checkpoints=None):
     """The workload's hour [tbeg, tend] on the given rows, cut into n_slices
-    equal runDeviceBatch intervals (each run to its end; t_out / dt_out
-    carried into the next interval's TimeAdvance).  Returns (state, t_out,
-    dt_out, stats summed over the slices; failed = failed in any)."""
+    equal runDeviceBatch intervals, or intervals ending at `ends` (each run
+    to its end; t_out / dt_out carried into the next interval's
+    TimeAdvance).  Returns (state, t_out, dt_out, stats summed over the
+    slices; failed = failed in any); with a list as `checkpoints`, the state
+    after every slice is appended to it."""
     w = workload
     st = np.ascontiguousarray(state, np.float64).copy()
     ph = None if photo is None or photo.shape[1] == 0 else np.ascontiguousarray(photo, np.float64)
@@ -324,8 +327,10 @@ def run_sliced(o: Oracle, workload, state, photo, n_slices=1, jacobian_mode=0, n
     tn = np.array([w.atol_newton, w.rtol_newton])
     tt = np.tile([w.atol_time, w.rtol_time], (o.m, 1))
     total = np.zeros(n, STATS_DTYPE)
-    for k in range(n_slices):
-        tend = w.tend if k == n_slices - 1 else w.tbeg + (w.tend - w.tbeg) * (k + 1) / n_slices
+    if ends is None:
+        ends = [w.tend if k == n_slices - 1 else w.tbeg + (w.tend - w.tbeg) * (k + 1) / n_slices
+                for k in range(n_slices)]
+    for tend in ends:
         ta = np.zeros(n, TADV_DTYPE)
         ta["tbeg"], ta["tend"], ta["dt"], ta["dtmin"], ta["dtmax"] = t, tend, dt, w.dtmin, w.dtmax
         ta["max_num_newton_iterations"] = w.max_newton
@@ -336,4 +341,6 @@ def run_sliced(o: Oracle, workload, state, photo, n_slices=1, jacobian_mode=0, n
         for f in ("accepted", "rejected", "newton_failures", "newton_iterations", "factorizations"):
             total[f] += s[f]
         total["failed"] = np.maximum(total["failed"], s["failed"])
+        if checkpoints is not None:
+            checkpoints.append(st.copy())
     return st, t, dt, total

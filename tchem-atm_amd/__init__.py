@@ -88,6 +88,14 @@ class _Options(ctypes.Structure):
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C tchem-atm_amd/csrc)")
+    # torch brings its own HIP runtime under another file name; loaded first,
+    # it is the one the library binds to.  Loaded after the library, a second
+    # runtime would share the process with the library's, and the one that
+    # initialises second finds no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     vp, i, cp = ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p
     lib.tchem_last_error.restype = cp
@@ -296,7 +304,7 @@ class AtmosphericChemistry:
             _ptr(tadv, None, "tadv"), _ptr(state, what="state"), _ptr(photo_rates, what="photo_rates"),
             _ptr(t_out, what="t_out"), _ptr(dt_out, what="dt_out"), _ptr(state_out, what="state_out"),
             _ptr(stats, None, "stats"), int(n_intervals), float(stop_fraction), kmcd._h, ctypes.byref(h)))
-        return Graph(h, (tol_newton, tol_time, tadv, state, photo_rates, t_out, dt_out, state_out, stats))
+        return Graph(h, (tol_newton, tol_time, tadv, state, photo_rates, t_out, dt_out, state_out, stats, kmcd))
 
     @staticmethod
     def autotune(tol_newton, tol_time, tadv, state, kmcd, photo_rates=None, precision=PRECISION_MIXED,
@@ -323,7 +331,8 @@ class AtmosphericChemistry:
 
 
 class Graph:
-    """A captured interval loop (tchem_graph_t); keeps its arrays alive."""
+    """A captured interval loop (tchem_graph_t); keeps its arrays and the
+    kmcd whose tables it reads alive."""
 
     def __init__(self, h, keep):
         self._h = h

@@ -1,9 +1,11 @@
 // C ABI (include/tchem_atm.h): handles, validation, device table upload and
 // kernel launches.  No exception crosses this boundary; every entry point
 // returns a status code and records a message for tchem_last_error().
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <array>
 #include <cstdint>
 #include <cstring>
@@ -50,6 +52,47 @@ hipError_t launchJacobian(const DevMech& M, int nBatch, const double* state, con
 
 using namespace tchem_amd;
 
+namespace {
+// roctx ranges around the batch calls: the reference's Kokkos::Profiling
+// regions (src/core/TChem_AtmosphericChemistry.cpp:61,166,193,207), seen by
+// rocprofv3 --marker-trace.  The ROCm roctx library is not linked: loading a
+// second ROCm's profiler registration next to the HIP runtime a host
+// framework (torch) brings makes that runtime find no device.  It is used
+// when the process already holds it (a profiler preloaded it) or when
+// TCHEM_ROCTX=1 asks for it; otherwise a range costs one null check.
+struct RoctxApi {
+  int (*push)(const char*) = nullptr;
+  int (*pop)() = nullptr;
+};
+const RoctxApi& roctxApi() {
+  static const RoctxApi api = [] {
+    RoctxApi a;
+    const char* lib = "librocprofiler-sdk-roctx.so.1";
+    void* h = dlopen(lib, RTLD_NOW | RTLD_NOLOAD);
+    const char* want = std::getenv("TCHEM_ROCTX");
+    if (!h && want && want[0] == '1') h = dlopen(lib, RTLD_NOW | RTLD_LOCAL);
+    if (h) {
+      a.push = reinterpret_cast<int (*)(const char*)>(dlsym(h, "roctxRangePushA"));
+      a.pop = reinterpret_cast<int (*)()>(dlsym(h, "roctxRangePop"));
+      if (!a.push || !a.pop) a = RoctxApi{};
+    }
+    return a;
+  }();
+  return api;
+}
+struct ProfileRange {
+  explicit ProfileRange(const char* name) : on(roctxApi().push != nullptr) {
+    if (on) roctxApi().push(name);
+  }
+  ~ProfileRange() {
+    if (on) roctxApi().pop();
+  }
+  ProfileRange(const ProfileRange&) = delete;
+  ProfileRange& operator=(const ProfileRange&) = delete;
+  bool on;
+};
+}  // namespace
+
 struct tchem_kmd_s {
   KineticModel km;
 };
@@ -71,6 +114,7 @@ struct tchem_kmcd_s {
   int* queue = nullptr;
   double* kf = nullptr;  // [capacity][nReac] rate constants of the last batch
   size_t kfCapacity = 0;  // cells
+  int graphs = 0;         // live captured graphs that read this kmcd's tables
   ~tchem_kmcd_s() {
     if (kf) allocations.push_back(kf);
     int cur = 0;
@@ -101,16 +145,24 @@ struct tchem_amcd_s {
   }
 };
 
+// A captured interval loop owns the buffers its kernels write (rate
+// constants, work-queue counter, loop control), so later calls on the same
+// kmcd with other batch sizes cannot move them; it reads the kmcd's
+// mechanism tables, so the kmcd refuses destruction while graphs exist.
 struct tchem_graph_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int* ctl = nullptr;  // [0] stopped, [1] intervals run (device)
+  int* ctl = nullptr;    // [0] stopped, [1] intervals run (device); [64] work-queue counter
+  double* kf = nullptr;  // [nBatch][nReac] rate constants of the captured launches
+  tchem_kmcd_s* kmcd = nullptr;
   int device = 0;
   int intervals = 0;
   ~tchem_graph_s() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (ctl) (void)hipFree(ctl);
+    if (kf) (void)hipFree(kf);
+    if (kmcd) --kmcd->graphs;
   }
 };
 
@@ -260,7 +312,7 @@ int runIntegrator(tchem_kmcd_s* kmcd, const tchem_atmchem_options_t& o, const do
   a.M = kmcd->dev;
   a.S = kmcd->sparse;
   a.sparse = o.lu == TCHEM_LU_SPARSE ? (sparseStaged(o) ? 1 : 2) : 0;
-  a.queue = kmcd->queue;
+  if (!a.queue) a.queue = kmcd->queue;
   a.persistent = (o.schedule == TCHEM_SCHEDULE_PERSISTENT) ? 1 : 0;
   a.cellsPerGroup = o.cells_per_block;
   a.cellBytes = a.sparse ? int(sparseCellBytes(kmcd->dev, kmcd->sparse,
@@ -281,7 +333,7 @@ int runIntegrator(tchem_kmcd_s* kmcd, const tchem_atmchem_options_t& o, const do
     if (rc) return rc;
     const long long g = (long long)bpc * kmcd->numCUs;
     grid = int(g < groups ? g : groups);
-    hipError_t e = launchResetCounter(kmcd->queue, s);
+    hipError_t e = launchResetCounter(a.queue, s);
     if (e != hipSuccess) return hipFail(e, "reset_counter_kernel");
   }
   hipError_t e = launchIntegrator(a, o.precision, grid, lds, s);
@@ -639,6 +691,9 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
 }
 
 int tchem_kmcd_destroy(tchem_kmcd_t kmcd) {
+  if (kmcd && kmcd->graphs > 0)
+    return setError(TCHEM_ERR_INVALID, "tchem_kmcd_destroy: " + std::to_string(kmcd->graphs) +
+                                           " captured graph(s) still read this kmcd; destroy them first");
   delete kmcd;
   return TCHEM_OK;
 }
@@ -658,6 +713,7 @@ int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBat
                                    const double* tol_time, double* fac, const tchem_time_advance_t* tadv,
                                    double* state, const double* photo_rates, double* t_out, double* dt_out,
                                    double* state_out, tchem_cell_stats_t* stats, tchem_kmcd_t kmcd, void* stream) {
+  ProfileRange profile_range("TChem::AtmosphericChemistry::runDeviceBatch::kmcd");
   if (kmcd && !kmcd->cb05Kinetics)
     return setError(TCHEM_ERR_UNSUPPORTED, std::string("run_device_batch: ") + kmcd->boxError);
   (void)fac;
@@ -696,6 +752,7 @@ int tchem_atmchem_e3sm_run_device_batch(const tchem_atmchem_options_t* opts, int
                                         const double* external_sources, double* t_out, double* dt_out,
                                         double* state_out, tchem_cell_stats_t* stats, tchem_kmcd_t kmcd,
                                         void* stream) {
+  ProfileRange profile_range("TChem::AtmosphericChemistryE3SM::runDeviceBatch");
   (void)fac;
   const char* who = "e3sm_run_device_batch";
   if (!kmcd) return setError(TCHEM_ERR_INVALID, std::string(who) + ": null kmcd");
@@ -744,6 +801,7 @@ int tchem_atmchem_update_time_advance(int nBatch, tchem_time_advance_t* tadv, co
 int tchem_sourceterm_run_device_batch(int nBatch, const double* state, const double* photo_rates,
                                       const double* external_sources, double* omega, tchem_kmcd_t kmcd,
                                       void* stream) {
+  ProfileRange profile_range("TChem::SourceTerm::runDeviceBatch");
   if (kmcd && !kmcd->cb05Kinetics)
     return setError(TCHEM_ERR_UNSUPPORTED, std::string("sourceterm: ") + kmcd->boxError);
   if (!kmcd || nBatch < 0) return setError(TCHEM_ERR_INVALID, "sourceterm: bad argument");
@@ -758,6 +816,7 @@ int tchem_sourceterm_run_device_batch(int nBatch, const double* state, const dou
 
 int tchem_jacobian_run_device_batch(int nBatch, const double* state, const double* photo_rates, double* jac,
                                     tchem_kmcd_t kmcd, void* stream) {
+  ProfileRange profile_range("TChem::Jacobian::runDeviceBatch");
   if (kmcd && !kmcd->cb05Kinetics)
     return setError(TCHEM_ERR_UNSUPPORTED, std::string("jacobian: ") + kmcd->boxError);
   if (!kmcd || nBatch < 0) return setError(TCHEM_ERR_INVALID, "jacobian: bad argument");
@@ -784,6 +843,7 @@ int tchem_adjoint_forward(int nBatch, const double* tol_newton, const double* to
                           const double* log_rate_scale, int log_rate_scale_stride, double* t_out, double* dt_out,
                           tchem_cell_stats_t* stats, void* workspace, int maxSteps, tchem_kmcd_t kmcd,
                           void* stream) {
+  ProfileRange profile_range("TChem::Adjoint::forward");
   const char* who = "tchem_adjoint_forward";
   int rc = adjointCheck(kmcd, nBatch, maxSteps, who);
   if (rc) return rc;
@@ -830,6 +890,7 @@ int tchem_adjoint_forward(int nBatch, const double* tol_newton, const double* to
 
 int tchem_adjoint_backward(int nBatch, const double* lambda, double* lambda_out, double* grad, void* workspace,
                            int maxSteps, tchem_kmcd_t kmcd, void* stream) {
+  ProfileRange profile_range("TChem::Adjoint::backward");
   const char* who = "tchem_adjoint_backward";
   int rc = adjointCheck(kmcd, nBatch, maxSteps, who);
   if (rc) return rc;
@@ -926,7 +987,6 @@ int tchem_atmchem_graph_create(const tchem_atmchem_options_t* opts, int nBatch, 
   if (rc) return rc;
   if (kmcd->dev.nPhoto == 0) photo_rates = nullptr;
   // everything that queries or allocates happens before the capture
-  if ((rc = ensureKf(kmcd, nBatch, nullptr))) return rc;
   if (o.schedule == TCHEM_SCHEDULE_PERSISTENT) {
     int bpc = 0;
     if ((rc = resolveBlocksPerCU(kmcd, o, false, &bpc))) return rc;
@@ -934,27 +994,24 @@ int tchem_atmchem_graph_create(const tchem_atmchem_options_t* opts, int nBatch, 
   }
   hipError_t e = hipDeviceSynchronize();
   if (e != hipSuccess) return hipFail(e, "hipDeviceSynchronize");
-  hipStream_t cs = nullptr;
-  e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
-  if (e != hipSuccess) return hipFail(e, "hipStreamCreateWithFlags");
   std::unique_ptr<tchem_graph_s> g(new tchem_graph_s);
   g->device = kmcd->device;
   g->intervals = nIntervals;
   {
     void* p = nullptr;
-    e = hipMalloc(&p, 256);
-    if (e != hipSuccess) {
-      (void)hipStreamDestroy(cs);
-      return hipFail(e, "hipMalloc");
-    }
+    e = hipMalloc(&p, 512);
+    if (e != hipSuccess) return hipFail(e, "hipMalloc");
     g->ctl = static_cast<int*>(p);
-    e = hipMemset(g->ctl, 0, 256);  // outside the capture
+    e = hipMalloc(&p, sizeof(double) * size_t(nBatch) * size_t(kmcd->dev.nReac > 0 ? kmcd->dev.nReac : 1));
+    if (e != hipSuccess) return hipFail(e, "hipMalloc (rate constants)");
+    g->kf = static_cast<double*>(p);
+    e = hipMemset(g->ctl, 0, 512);  // outside the capture
     if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e != hipSuccess) {
-      (void)hipStreamDestroy(cs);
-      return hipFail(e, "hipMemset");
-    }
+    if (e != hipSuccess) return hipFail(e, "hipMemset");
   }
+  hipStream_t cs = nullptr;
+  e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e != hipSuccess) return hipFail(e, "hipStreamCreateWithFlags");
   const bool stop = stopFraction > 0.0;
   e = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
   if (e != hipSuccess) {
@@ -976,7 +1033,8 @@ int tchem_atmchem_graph_create(const tchem_atmchem_options_t* opts, int nBatch, 
     a.dt_out = dt_out;
     a.state_out = state_out;
     a.stats = stats;
-    a.kf = kmcd->kf;
+    a.kf = g->kf;
+    a.queue = g->ctl + 64;
     rc = runIntegrator(kmcd, o, nullptr, 0, a, cs, who);
     if (rc == TCHEM_OK) {
       e = launchUpdateTadv(nBatch, tadv, t_out, dt_out, g->ctl, cs);
@@ -993,11 +1051,14 @@ int tchem_atmchem_graph_create(const tchem_atmchem_options_t* opts, int nBatch, 
   if (ee != hipSuccess) return hipFail(ee, "hipStreamEndCapture");
   e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
   if (e != hipSuccess) return hipFail(e, "hipGraphInstantiate");
+  g->kmcd = kmcd;
+  ++kmcd->graphs;
   *out = g.release();
   return TCHEM_OK;
 }
 
 int tchem_graph_launch(tchem_graph_t graph, void* stream) {
+  ProfileRange profile_range("TChem::AtmosphericChemistry::graphLaunch");
   if (!graph || !graph->exec) return setError(TCHEM_ERR_INVALID, "tchem_graph_launch: null graph");
   int cur = 0;
   if (hipGetDevice(&cur) != hipSuccess || cur != graph->device)
@@ -1027,6 +1088,7 @@ int tchem_atmchem_autotune(tchem_atmchem_options_t* opts, int nBatch, const doub
                            const double* tol_time, const tchem_time_advance_t* tadv, const double* state,
                            const double* photo_rates, int sampleCells, int sampleIterations, double* best_seconds,
                            tchem_kmcd_t kmcd, void* stream) {
+  ProfileRange profile_range("TChem::AtmosphericChemistry::autotune");
   if (kmcd && !kmcd->cb05Kinetics)
     return setError(TCHEM_ERR_UNSUPPORTED, std::string("tchem_atmchem_autotune: ") + kmcd->boxError);
   const char* who = "tchem_atmchem_autotune";
@@ -1476,6 +1538,7 @@ IntegratorArgs aerosolArgs(tchem_amcd_s* h, int nBatch) {
 
 int tchem_aerosol_rhs_run_device_batch(int nBatch, const double* state, const double* number_conc, double* omega,
                                        tchem_amcd_t amcd, void* stream) {
+  ProfileRange profile_range("TChem::AerosolRHS::runDeviceBatch");
   const char* who = "aerosol_rhs_run_device_batch";
   if (!amcd || nBatch < 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": bad argument");
   if (nBatch == 0) return TCHEM_OK;
@@ -1496,6 +1559,7 @@ int tchem_aerosolchem_run_device_batch(const tchem_atmchem_options_t* opts, int 
                                        double* state, const double* number_conc, double* t_out, double* dt_out,
                                        double* state_out, tchem_cell_stats_t* stats, tchem_amcd_t amcd,
                                        void* stream) {
+  ProfileRange profile_range("TChem::AerosolChemistry::runDeviceBatch");
   (void)fac;
   const char* who = "aerosolchem_run_device_batch";
   if (!amcd || nBatch < 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": bad argument");

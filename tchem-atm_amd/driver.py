@@ -103,13 +103,16 @@ class DeviceRun:
                 break
         return total
 
-    def start_slices(self, n_slices):
+    def start_slices(self, n_slices=1, ends=None):
         """Resets the run for run_slice: every cell at tbeg with dt_in, the
-        hour [tbeg, tend] cut into n_slices equal slices."""
+        interval [tbeg, tend] cut into n_slices equal slices, or into slices
+        ending at the given times (the last one tend)."""
         w = self.w
         self.reset()
-        self.slice_ends = [w.tbeg + (w.tend - w.tbeg) * (k + 1) / n_slices for k in range(n_slices)]
-        self.slice_ends[-1] = w.tend
+        if ends is None:
+            ends = [w.tbeg + (w.tend - w.tbeg) * (k + 1) / n_slices for k in range(n_slices)]
+            ends[-1] = w.tend
+        self.slice_ends = [float(e) for e in ends]
         self.tadv = T.time_advance_tensor(self.n, w.tbeg, self.slice_ends[0], w.dt_in, w.dtmin, w.dtmax,
                                           w.max_newton, 10_000_000, w.jacobian_interval, self.device)
         self._tadv64 = self.tadv.view(self.torch.float64).view(self.n, 7)
@@ -156,3 +159,36 @@ class DeviceRun:
             self.total_stats = np.stack([s[f] for f in T.COUNTERS], 1).astype(np.int64)
             self.t_start, self.t_end = s["t_start"].copy(), s["t_end"].copy()
         return 1
+
+    def run_slice_intervals(self, k, iters=10, precision=T.PRECISION_FP64, schedule=T.SCHEDULE_GRID, lu=T.LU_DENSE,
+                            collect_stats=True, stream=None, max_intervals=1_000_000):
+        """Slice k through the reference's host interval loop: launches of at
+        most `iters` TR-BDF2 iterations, t_out / dt_out carried into tadv
+        after each, until every cell reached the slice end (a failed cell
+        counts as done: its t_out is the slice end).  The iteration cap does
+        not change a cell's step sequence, so the result equals run_slice's.
+        Returns the number of launches."""
+        torch = self.torch
+        end = self.slice_ends[k]
+        if k > 0:
+            self._tadv64[:, 0].copy_(self.t)
+            self._tadv64[:, 1].fill_(end)
+            self._tadv64[:, 2].copy_(self.dt)
+        ta32 = self.tadv.view(torch.int32).view(self.n, 14)
+        ta32[:, 11].fill_(iters)  # _num_time_iterations_per_interval
+        launches = 0
+        while launches < max_intervals:
+            T.AtmosphericChemistry.runDeviceBatch(self.tol_newton, self.tol_time, None, self.tadv, self.state,
+                                                  self.t, self.dt, self.state, self.kmcd, self.photo,
+                                                  self.stats if collect_stats else None, precision=precision,
+                                                  schedule=schedule, lu=lu, stream=stream)
+            if collect_stats:
+                s = self.stats.view(torch.int32).view(self.n, 10)
+                self.acc_stats[:, :5] += s[:, :5]
+                self.acc_stats[:, 5] = torch.maximum(self.acc_stats[:, 5], s[:, 5].to(torch.int64))
+            T.AtmosphericChemistry.updateTimeAdvance(self.tadv, self.t, self.dt, stream)
+            launches += 1
+            if bool((self.t >= end).all()):
+                break
+        ta32[:, 11].fill_(10_000_000)
+        return launches

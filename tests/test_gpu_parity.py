@@ -373,6 +373,47 @@ def test_captured_interval_loop_matches_eager_loop(fp64_vs_oracle, cb05, schedul
     np.testing.assert_array_equal(run.t.cpu().numpy(), g_t)
 
 
+def test_captured_graph_survives_kmcd_reuse(cb05, scen, drv, gpu, pkg):
+    """A captured loop owns its rate constants and work-queue counter: a
+    larger batch run on the same kmcd between capture and launch (which
+    grows the kmcd's own rate-constant buffer) leaves the graph's results
+    equal to the eager loop's, bit for bit; the kmcd refuses destruction
+    while the graph exists and accepts it afterwards."""
+    torch = gpu
+    kmd, _, base = cb05
+    kmcd = kmd.createConstData()  # a fresh kmcd: its rate-constant buffer starts small
+    w = scen.config0(kmd, base[0], n=16, seed=4)
+    print("case: captured graph, 16 cells, kmcd reused with 4096 cells before the launch", flush=True)
+    run = drv.DeviceRun(kmcd, w)
+    run.reset()
+    run.run_intervals(precision=pkg.PRECISION_FP64, schedule=pkg.SCHEDULE_PERSISTENT, lu=pkg.LU_SPARSE)
+    torch.cuda.synchronize()
+    eager = run.state.cpu().numpy().copy()
+    run.reset()
+    graph = pkg.AtmosphericChemistry.captureGraph(run.tol_newton, run.tol_time, run.tadv, run.state, run.t, run.dt,
+                                                  run.state, kmcd, 64, 0.9999, photo_rates=run.photo,
+                                                  precision=pkg.PRECISION_FP64, schedule=pkg.SCHEDULE_PERSISTENT,
+                                                  lu=pkg.LU_SPARSE)
+    big = drv.DeviceRun(kmcd, scen.config0(kmd, base[0], n=4096, seed=5))
+    big.reset()
+    pkg.AtmosphericChemistry.runDeviceBatch(big.tol_newton, big.tol_time, None, big.tadv, big.state, big.t, big.dt,
+                                            big.state, kmcd, big.photo, precision=pkg.PRECISION_FP64,
+                                            schedule=pkg.SCHEDULE_PERSISTENT, lu=pkg.LU_SPARSE)
+    torch.cuda.synchronize()
+    for _ in range(1000):
+        graph.launch()
+        n, stopped = graph.status()
+        if stopped or n < 64:
+            break
+    np.testing.assert_array_equal(run.state.cpu().numpy(), eager)
+    lib = pkg.lib()
+    assert lib.tchem_kmcd_destroy(kmcd._h) == -1  # TCHEM_ERR_INVALID
+    assert "captured graph" in lib.tchem_last_error().decode()
+    del graph
+    assert lib.tchem_kmcd_destroy(kmcd._h) == 0
+    kmcd._h = None
+
+
 def test_autotuned_launch_shape_gives_identical_results(fp64_vs_oracle, cb05, gpu, pkg):
     """The autotuner returns a valid launch shape for the batch; the per-cell
     arithmetic does not depend on it (bitwise equal to the grid launch)."""

@@ -322,3 +322,185 @@ def test_dense_lu_order_changes_only_rounding(cb05_oracle, oracle_mod):
     xp[order] = oracle_mod.lu_solve(LUp, pivp, b[order])
     assert info == 0 and infop == 0
     assert np.abs(x0 - xp).max() <= 1e-12 * np.abs(x0).max()
+
+
+def test_carbonbond05_long_trajectory_golden(oracle_mod):
+    """src/examples/runs/atmospheric_chemistry/CarbonBond05/full_gas.dat: one
+    cell, 12 intervals of 100 TR-BDF2 iterations to 10 800 s (runModel.sh:
+    dtmin 1e-20, dtmax 10 s, rtol 1e-6, Newton 1e-18 / 1e-8, 20 iterations;
+    100 iterations per interval reproduce the file's interval times).
+
+    The file comes from an older mechanism file (its species are in another
+    order and M has other units), so it pins the step sequence: from the
+    file's own initial row, 70 doublings from 1e-20 s, then the 10 s cap
+    with no rejection, every interval ending where the file's does, and the
+    last step clamped to 10 800 s.  Its species columns agree only to the
+    older mechanism's difference (compare_tchem norm 0.17); the bar below
+    guards against gross errors."""
+    mech = golden("carbonbond05.yaml")
+    o = oracle_mod.Oracle(mech)
+    k = o.k
+    path = golden("carbonbond05_full_gas.dat")
+    names = open(path).readline().split()[6:]
+    gold = np.loadtxt(path, skiprows=1)
+    col = np.array([names.index(nm) for nm in k.names])
+    G = np.concatenate([gold[:, :6], gold[:, 6:][:, col]], axis=1)
+    st = np.ascontiguousarray(G[:1, 3:])
+    tn, tt = np.array([1e-18, 1e-8]), np.tile([1e-12, 1e-6], (k.m, 1))
+    ta = _tadv(oracle_mod, 1, 10800.0, 1e-20, 1e-20, 10.0, 100)
+    t, dt = np.zeros(1), np.zeros(1)
+    rows = []
+    for it in range(12):
+        s = np.zeros(1, oracle_mod.STATS_DTYPE)
+        o.run_batch(tn, tt, ta, st, t, dt, stats=s)
+        assert s["rejected"][0] == 0 and s["failed"][0] == 0
+        rows.append(np.concatenate([[it, t[0], dt[0]], st[0]]))
+        ta["tbeg"], ta["dt"] = t, dt
+    rows = np.array(rows)
+    np.testing.assert_allclose(rows[:, 1], G[1:, 1], rtol=1e-9)  # the file prints 11 digits
+    assert np.all(rows[:-1, 2] == 10.0) and rows[0, 1] == pytest.approx(1e-20 * (2.0 ** 70 - 1) + 300.0, rel=1e-15)
+    assert rows[-1, 1] == 10800.0
+    assert compare_tchem_norms(rows[:, 3:], G[1:, 3:]) < 0.25
+
+
+_TINY_MECH = """NCAR-version: v1.0
+species:
+- name: A
+- name: B
+- name: C
+- name: D
+constant_species:
+- name: M
+sources:
+- coefficients:
+    emission_rate: 0.25
+  species: B
+  type: EMISSION
+reactions:
+- reactants: {A: 1, B: 1}
+  products: {C: 1}
+  type: ARRHENIUS
+  coefficients: {A: 1.0}
+- reactants: {A: 2}
+  products: {B: 1, D: 0.5}
+  type: ARRHENIUS
+  coefficients: {A: 1.0}
+- reactants: {C: 1, M: 1}
+  products: {A: 1, M: 1}
+  type: ARRHENIUS
+  coefficients: {A: 1.0}
+- reactants: {D: 1}
+  products: {D: 1, B: 2.0}
+  type: ARRHENIUS
+  coefficients: {A: 1.0}
+- reactants: {A: 1, B: 1, C: 1}
+  products: {C: 0.3, A: 1.7}
+  type: ARRHENIUS
+  coefficients: {A: 1.0}
+"""
+# (reactant powers, product yields) per reaction over species A B C D M
+_TINY_R = [({0: 1, 1: 1}, {2: 1.0}), ({0: 2}, {1: 1.0, 3: 0.5}), ({2: 1, 4: 1}, {0: 1.0, 4: 1.0}),
+           ({3: 1}, {3: 1.0, 1: 2.0}), ({0: 1, 1: 1, 2: 1}, {2: 0.3, 0: 1.7})]
+
+
+def test_source_term_and_jacobian_by_brute_force(oracle_mod, tmp_path):
+    """A hand-written mechanism with a second-order reactant (2 A), a
+    constant species as reactant and product, a species that is both
+    reactant and product (catalyst, and net yields 0.3 - 1 and 1.7 - 1), a
+    three-body reaction and an emission: the oracle's source term and
+    analytic Jacobian against the mass-action formulas evaluated term by term
+    in extended precision (long double), at random concentrations and rate
+    constants."""
+    f = tmp_path / "tiny.yaml"
+    f.write_text(_TINY_MECH)
+    o = oracle_mod.Oracle(str(f))
+    assert o.k.nSpec == 5 and o.m == 4
+    rng = np.random.default_rng(11)
+    ld = np.longdouble
+    for _ in range(50):
+        c = 10.0 ** rng.uniform(-3, 1, 5)
+        kf = 10.0 ** rng.uniform(-2, 2, 5)
+        cl, kl = c.astype(ld), kf.astype(ld)
+        om = np.zeros(4, ld)
+        om[1] += ld(0.25)
+        J = np.zeros((4, 4), ld)
+        mag = np.zeros(4, ld)
+        for r, (reac, prod) in enumerate(_TINY_R):
+            rop = kl[r] * np.prod([cl[s] ** n for s, n in reac.items()])
+            net = {}
+            for s, n in reac.items():
+                net[s] = net.get(s, ld(0)) - n
+            for s, y in prod.items():
+                net[s] = net.get(s, ld(0)) + ld(y)
+            for s, nu in net.items():
+                if s < 4:
+                    om[s] += nu * rop
+            # the oracle adds reactant and product entries one by one
+            for s, nu in [(s, -n) for s, n in reac.items()] + list(prod.items()):
+                if s < 4:
+                    mag[s] += abs(nu * rop)
+            for j, nj in reac.items():
+                if j >= 4:
+                    continue
+                d = kl[r] * nj * cl[j] ** (nj - 1) * np.prod([cl[s] ** n for s, n in reac.items() if s != j])
+                for s, nu in net.items():
+                    if s < 4:
+                        J[s, j] += nu * d
+        got = o.omega(kf, c)
+        assert np.all(np.abs(got - om.astype(float)) <= 1e-14 * (mag + 0.25).astype(float))
+        Jg = o.jacobian(kf, c)
+        np.testing.assert_allclose(Jg, J.astype(float), rtol=1e-13, atol=1e-13 * float(np.abs(J).max()))
+
+
+@pytest.mark.parametrize("case", ["linear", "quadratic"])
+def test_trbdf2_error_estimate_is_asymptotically_exact(oracle_mod, tmp_path, case):
+    """The step controller accepts a step when max |est| / (atol + rtol |u|)
+    <= 1.  On y' = -k y (A -> B) and on y' = -2 k y^2 (2 A -> B), whose exact
+    solutions are closed forms, the largest accepted step h* found by
+    bisection on the accept / reject decision equals, to 1 %, the step at
+    which the TRUE one-step error |y_1 - y(h)| reaches the tolerance: the
+    estimate 2 k_err h (f_n / g - f_g / (g (1 - g)) + f_1 / (1 - g)) is the
+    local error to leading order (a dropped term or a wrong constant moves
+    h* by 25 % or more).  Also pinned: the growth rule (dt_out = 2 h when
+    the estimate is below 1/8 of the tolerance, h otherwise)."""
+    k = 0.7
+    reac = "A: 1" if case == "linear" else "A: 2"
+    f = tmp_path / f"{case}.yaml"
+    f.write_text("NCAR-version: v1.0\nspecies:\n- name: A\n- name: B\nconstant_species:\n- name: M\n"
+                 f"reactions:\n- reactants:\n    {reac}\n  products:\n    B: 1\n  type: ARRHENIUS\n"
+                 f"  coefficients:\n    A: {k}\n")
+    o = oracle_mod.Oracle(str(f))
+    y0, rtol = 1.0, 1e-9
+    p0 = 300.0 / oracle_mod.kmd_mod.CONV_PPM  # ppm conversion factor 1 at 300 K
+    k = o.kforward(300.0, p0)[0]
+    if case == "linear":  # A -> B: B = y0 - A
+        exact = lambda h: np.array([y0 * np.exp(-k * h), y0 - y0 * np.exp(-k * h)])
+    else:  # 2 A -> B: B = (y0 - A) / 2
+        exact = lambda h: np.array([y0 / (1.0 + 2.0 * k * y0 * h), 0.5 * (y0 - y0 / (1.0 + 2.0 * k * y0 * h))])
+    tn = np.array([1e-30, 1e-15])
+
+    def step(h, rt):
+        st = np.array([[0.0, p0, 300.0, y0, 0.0, 0.0]])
+        ta = _tadv(oracle_mod, 1, 1e30, h, 1e-30, 1e30, 1)
+        t, dt = np.zeros(1), np.zeros(1)
+        o.run_batch(tn, np.tile([0.0, rt], (2, 1)), ta, st, t, dt)
+        return t[0] > 0.0, st[0, 3:5], dt[0]
+
+    def bisect(pred, lo, hi):
+        for _ in range(60):
+            mid = np.sqrt(lo * hi)
+            lo, hi = (mid, hi) if pred(mid) else (lo, mid)
+        return lo
+
+    h_acc = bisect(lambda h: step(h, rtol)[0], 1e-6, 1.0)
+    def true_norm(h):  # the controller's norm applied to the true one-step error
+        y1 = step(h, 1e30)[1]
+        return np.max(np.abs(y1 - exact(h)) / (rtol * np.abs(y1)))
+    h_true = bisect(lambda h: true_norm(h) <= 1.0, 1e-6, 1.0)
+    assert abs(h_acc / h_true - 1.0) < 1e-2, (h_acc, h_true)
+    # growth: the proposal doubles while the estimate is at most 1/8 of the
+    # tolerance, and stays h above
+    h_grow = bisect(lambda h: step(h, rtol)[2] == 2.0 * h, 1e-6, h_acc)
+    h_grow_true = bisect(lambda h: true_norm(h) <= 0.125, 1e-6, 1.0)
+    assert abs(h_grow / h_grow_true - 1.0) < 1e-2, (h_grow, h_grow_true)
+    assert step(1.01 * h_grow, rtol)[2] == 1.01 * h_grow
