@@ -168,6 +168,22 @@ int tchem_atmchem_e3sm_run_device_batch(const tchem_atmchem_options_t* opts, int
                                         double* state_out, tchem_cell_stats_t* stats, tchem_kmcd_t kmcd,
                                         void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): tchem_atmchem_run_device_batch with the same arguments and
+ * results, whose scratch (the batch's rate constants and the work-queue
+ * counter) lives in a file-scope device array of 16 MiB per GPU instead of
+ * library allocations; the call allocates nothing.  Limit: nBatch * nReac *
+ * 8 + 256 bytes <= 16 MiB (11 214 CB05 cells per call); larger batches get
+ * TCHEM_ERR_INVALID.  Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (other streams, other threads) race. */
+int tchem_atmchem_run_device_batch_noalloc(const tchem_atmchem_options_t* opts, int nBatch,
+                                           const double* tol_newton, const double* tol_time, double* fac,
+                                           const tchem_time_advance_t* tadv, double* state,
+                                           const double* photo_rates, double* t_out, double* dt_out,
+                                           double* state_out, tchem_cell_stats_t* stats, tchem_kmcd_t kmcd,
+                                           void* stream);
+
 /* The driver's interval loop (src/examples/TChem_AtmosphericChemistry.cpp:312-352)
  * captured into a HIP graph, bound to the given arrays by pointer: one
  * launch replays nIntervals x (tchem_atmchem_run_device_batch, then

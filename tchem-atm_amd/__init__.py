@@ -62,7 +62,8 @@ REALTIME_HZ = 100e6  # device real-time counter of t_start / t_end
 EXPORTED_SYMBOLS = (
     "tchem_last_error", "tchem_kmd_load_yaml", "tchem_kmd_destroy", "tchem_kmd_info", "tchem_kmd_species_name",
     "tchem_kmd_photolysis_reactions", "tchem_kmd_elimination_order", "tchem_scenario_load", "tchem_kmcd_create", "tchem_kmcd_destroy",
-    "tchem_atmchem_workspace_bytes", "tchem_atmchem_run_device_batch", "tchem_atmchem_update_time_advance",
+    "tchem_atmchem_workspace_bytes", "tchem_atmchem_run_device_batch", "tchem_atmchem_run_device_batch_noalloc",
+    "tchem_atmchem_update_time_advance",
     "tchem_sourceterm_run_device_batch", "tchem_jacobian_run_device_batch", "tchem_adjoint_workspace_bytes",
     "tchem_adjoint_forward", "tchem_adjoint_backward", "tchem_adjoint_steps", "tchem_atmchem_graph_create",
     "tchem_graph_launch", "tchem_graph_status", "tchem_graph_destroy", "tchem_atmchem_autotune",
@@ -112,6 +113,8 @@ def _load():
         "tchem_atmchem_workspace_bytes": [vp, ctypes.POINTER(_Options), ctypes.POINTER(ctypes.c_size_t)],
         "tchem_atmchem_run_device_batch": [ctypes.POINTER(_Options), i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                            vp],
+        "tchem_atmchem_run_device_batch_noalloc": [ctypes.POINTER(_Options), i, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                                   vp, vp, vp],
         "tchem_atmchem_update_time_advance": [i, vp, vp, vp, vp],
         "tchem_sourceterm_run_device_batch": [i, vp, vp, vp, vp, vp, vp],
         "tchem_jacobian_run_device_batch": [i, vp, vp, vp, vp, vp],

@@ -28,6 +28,7 @@ hipError_t integratorMaxBlocksPerCU(int precision, int sparse, int m, int thread
 int sparseMaxCellsPerGroup(int precision, int m);
 size_t integratorStaticLds(int precision, int sparse, int m);
 hipError_t launchResetCounter(int* counter, hipStream_t stream);
+hipError_t noAllocScratch(void** ptr, size_t* bytes);
 hipError_t launchRates(const DevMech& M, int nBatch, const double* state, const double* photo, const double* lscale,
                        int lstride, double* kf, const int* skip, hipStream_t stream, int rowDim = 0);
 int aerosolCellsPerGroup(const DevMech& M, const DevAerosol& X, size_t maxLds);
@@ -744,6 +745,52 @@ int tchem_atmchem_run_device_batch(const tchem_atmchem_options_t* opts, int nBat
   a.stats = stats;
   a.kf = kmcd->kf;
   return runIntegrator(kmcd, o, nullptr, 0, a, s, "run_device_batch");
+}
+
+int tchem_atmchem_run_device_batch_noalloc(const tchem_atmchem_options_t* opts, int nBatch,
+                                           const double* tol_newton, const double* tol_time, double* fac,
+                                           const tchem_time_advance_t* tadv, double* state,
+                                           const double* photo_rates, double* t_out, double* dt_out,
+                                           double* state_out, tchem_cell_stats_t* stats, tchem_kmcd_t kmcd,
+                                           void* stream) {
+  ProfileRange profile_range("TChem::AtmosphericChemistry::runDeviceBatch::noalloc");
+  const char* who = "run_device_batch_noalloc";
+  if (kmcd && !kmcd->cb05Kinetics) return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": " + kmcd->boxError);
+  (void)fac;
+  if (!kmcd) return setError(TCHEM_ERR_INVALID, std::string(who) + ": null kmcd");
+  if (nBatch < 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": negative nBatch");
+  tchem_atmchem_options_t o;
+  const int rc = resolveOptions(opts, kmcd, &o, who);
+  if (rc) return rc;
+  if (nBatch == 0) return TCHEM_OK;
+  if (!tol_newton || !tol_time || !tadv || !state || !t_out || !dt_out || !state_out)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": null device array");
+  if (kmcd->dev.nPhoto == 0) photo_rates = nullptr;
+  void* scr = nullptr;
+  size_t cap = 0;
+  hipError_t e = noAllocScratch(&scr, &cap);
+  if (e != hipSuccess) return hipFail(e, "hipGetSymbolAddress (scratch)");
+  const size_t perCell = sizeof(double) * size_t(kmcd->dev.nReac > 0 ? kmcd->dev.nReac : 1);
+  const size_t kfBytes = (perCell * size_t(nBatch) + 255) & ~size_t(255);
+  if (kfBytes + 256 > cap)
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": " + std::to_string(nBatch) + " cells need " +
+                                           std::to_string(kfBytes + 256) + " bytes of scratch; it holds " +
+                                           std::to_string(cap) + " (at most " + std::to_string((cap - 256) / perCell) +
+                                           " cells of this mechanism per call)");
+  IntegratorArgs a{};
+  a.nBatch = nBatch;
+  a.tol_newton = tol_newton;
+  a.tol_time = tol_time;
+  a.tadv = tadv;
+  a.state = state;
+  a.photo = photo_rates;
+  a.t_out = t_out;
+  a.dt_out = dt_out;
+  a.state_out = state_out;
+  a.stats = stats;
+  a.kf = static_cast<double*>(scr);
+  a.queue = reinterpret_cast<int*>(static_cast<char*>(scr) + kfBytes);
+  return runIntegrator(kmcd, o, nullptr, 0, a, static_cast<hipStream_t>(stream), who);
 }
 
 int tchem_atmchem_e3sm_run_device_batch(const tchem_atmchem_options_t* opts, int nBatch, const double* tol_newton,

@@ -21,6 +21,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "detmath.h"
 #include "device_mech.hpp"
@@ -2578,6 +2579,37 @@ hipError_t launchSourceTerm(const DevMech& M, int nBatch, const double* state, c
   else if (qm == 2) hipLaunchKernelGGL(sourceterm_kernel<2>, dim3(nBatch), dim3(64), lds, stream, M, nBatch, state, photo, ext, omega);
   else hipLaunchKernelGGL(sourceterm_kernel<4>, dim3(nBatch), dim3(64), lds, stream, M, nBatch, state, photo, ext, omega);
   return hipGetLastError();
+}
+
+// ---- scratch of the no-allocation entry point ---------------------------------
+// Device memory the no-allocation variant of runDeviceBatch uses in place of
+// library allocations: the batch's rate constants, then the work-queue
+// counter.  Each GPU holds its own copy of the array, at its own address,
+// for the life of the process; every call on a GPU shares that GPU's copy,
+// so calls that overlap on one GPU race.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// Device address of the current GPU's scratch, looked up once per device
+// (hipGetSymbolAddress with that device current) and kept; host code uses
+// this address, never the array's name.
+hipError_t noAllocScratch(void** ptr, size_t* bytes) {
+  static std::mutex mu;
+  static void* cache[256] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 256) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!cache[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    cache[dev] = p;
+  }
+  *ptr = cache[dev];
+  *bytes = kNoAllocScratchBytes;
+  return hipSuccess;
 }
 
 hipError_t launchJacobian(const DevMech& M, int nBatch, const double* state, const double* photo, double* jac,
