@@ -61,6 +61,8 @@ typedef struct {
 #define TCHEM_LU_AUTO 0   /* sparse when the mechanism has a sparse schedule, dense otherwise */
 #define TCHEM_LU_DENSE 1  /* dense LU with partial pivoting (the oracle's arithmetic) */
 #define TCHEM_LU_SPARSE 2 /* level-scheduled sparse LU, pivots fixed by the fill-reducing order */
+#define TCHEM_LU_MFMA 3   /* dense FP32 LU on the matrix cores, pivots fixed by the fill-reducing order: one cell
+                             per workgroup of 4 wavefronts; mixed precision, m <= 255 */
 #define TCHEM_PROGRAMS_AUTO 0   /* LDS when they leave room for 4 cells, device memory otherwise */
 #define TCHEM_PROGRAMS_LDS 1    /* staged into LDS once per workgroup, shared by its cells */
 #define TCHEM_PROGRAMS_MEMORY 2 /* read from device memory (L2-resident); all LDS goes to cells */
@@ -167,6 +169,14 @@ int tchem_atmchem_e3sm_run_device_batch(const tchem_atmchem_options_t* opts, int
                                         const double* external_sources, double* t_out, double* dt_out,
                                         double* state_out, tchem_cell_stats_t* stats, tchem_kmcd_t kmcd,
                                         void* stream);
+
+/* No counterpart in the reference: the FP32 dense factorisation of the
+ * MFMA form (TCHEM_LU_MFMA) on its own, for dense systems of the caller.
+ * Solves A_i x_i = b_i for nBatch matrices A [nBatch][n][n], b, x [nBatch][n]
+ * (FP64 in and out, [device]) with n <= 255, factored in FP32 on the matrix
+ * cores without row interchanges (diagonal pivots; a zero pivot gives NaN).
+ * One workgroup of 256 threads per system. */
+int tchem_dense_lu_solve_mfma(int n, int nBatch, const double* A, const double* b, double* x, void* stream);
 
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, serving loops, harnesses that forbid

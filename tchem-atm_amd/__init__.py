@@ -41,6 +41,7 @@ SCHEDULE_PERSISTENT = 1
 LU_AUTO = 0     # sparse when the mechanism has a sparse schedule, dense otherwise
 LU_DENSE = 1    # dense LU with partial pivoting (the oracle's arithmetic)
 LU_SPARSE = 2   # level-scheduled sparse LU, pivots fixed by the fill-reducing order
+LU_MFMA = 3     # dense FP32 LU on the matrix cores (mixed precision, m <= 255), one cell per 4 wavefronts
 PROGRAMS_AUTO = 0    # sparse form: programs in LDS when they leave room for 4 cells
 PROGRAMS_LDS = 1     # staged into LDS once per workgroup
 PROGRAMS_MEMORY = 2  # read from device memory (L2-resident), all LDS for cells
@@ -62,7 +63,7 @@ REALTIME_HZ = 100e6  # device real-time counter of t_start / t_end
 EXPORTED_SYMBOLS = (
     "tchem_last_error", "tchem_kmd_load_yaml", "tchem_kmd_destroy", "tchem_kmd_info", "tchem_kmd_species_name",
     "tchem_kmd_photolysis_reactions", "tchem_kmd_elimination_order", "tchem_scenario_load", "tchem_kmcd_create", "tchem_kmcd_destroy",
-    "tchem_atmchem_workspace_bytes", "tchem_atmchem_run_device_batch", "tchem_atmchem_run_device_batch_noalloc",
+    "tchem_atmchem_workspace_bytes", "tchem_atmchem_run_device_batch", "tchem_atmchem_run_device_batch_noalloc", "tchem_dense_lu_solve_mfma",
     "tchem_atmchem_update_time_advance",
     "tchem_sourceterm_run_device_batch", "tchem_jacobian_run_device_batch", "tchem_adjoint_workspace_bytes",
     "tchem_adjoint_forward", "tchem_adjoint_backward", "tchem_adjoint_steps", "tchem_atmchem_graph_create",
@@ -116,6 +117,7 @@ def _load():
         "tchem_atmchem_run_device_batch_noalloc": [ctypes.POINTER(_Options), i, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                                    vp, vp, vp],
         "tchem_atmchem_update_time_advance": [i, vp, vp, vp, vp],
+        "tchem_dense_lu_solve_mfma": [i, i, vp, vp, vp, vp],
         "tchem_sourceterm_run_device_batch": [i, vp, vp, vp, vp, vp, vp],
         "tchem_jacobian_run_device_batch": [i, vp, vp, vp, vp, vp],
         "tchem_adjoint_workspace_bytes": [vp, i, i, ctypes.POINTER(ctypes.c_size_t)],

@@ -28,6 +28,8 @@ hipError_t integratorMaxBlocksPerCU(int precision, int sparse, int m, int thread
 int sparseMaxCellsPerGroup(int precision, int m);
 size_t integratorStaticLds(int precision, int sparse, int m);
 hipError_t launchResetCounter(int* counter, hipStream_t stream);
+size_t mfmaCellLdsBytes(const DevMech& M);
+hipError_t launchDenseMfmaSolve(int n, int nBatch, const double* A, const double* b, double* x, hipStream_t stream);
 hipError_t noAllocScratch(void** ptr, size_t* bytes);
 hipError_t launchRates(const DevMech& M, int nBatch, const double* state, const double* photo, const double* lscale,
                        int lstride, double* kf, const int* skip, hipStream_t stream, int rowDim = 0);
@@ -208,8 +210,20 @@ int resolveOptions(const tchem_atmchem_options_t* opts, const tchem_kmcd_s* k, t
   if (o->schedule != TCHEM_SCHEDULE_GRID && o->schedule != TCHEM_SCHEDULE_PERSISTENT)
     return setError(TCHEM_ERR_INVALID, std::string(who) + ": unknown schedule");
   if (o->blocks_per_cu < 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": negative blocks_per_cu");
-  if (o->lu != TCHEM_LU_AUTO && o->lu != TCHEM_LU_DENSE && o->lu != TCHEM_LU_SPARSE)
+  if (o->lu != TCHEM_LU_AUTO && o->lu != TCHEM_LU_DENSE && o->lu != TCHEM_LU_SPARSE && o->lu != TCHEM_LU_MFMA)
     return setError(TCHEM_ERR_INVALID, std::string(who) + ": unknown lu");
+  if (o->lu == TCHEM_LU_MFMA) {
+    if (o->precision != TCHEM_PRECISION_MIXED)
+      return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": the MFMA LU form is mixed precision (FP32 factors)");
+    if (k->dev.m > 255)
+      return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": the MFMA LU form takes at most 255 species");
+    if (o->cells_per_block > 1)
+      return setError(TCHEM_ERR_INVALID, std::string(who) + ": the MFMA LU form runs one cell per workgroup");
+    o->cells_per_block = 1;
+    if (mfmaCellLdsBytes(k->dev) > k->maxLds)
+      return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": mechanism too large for the MFMA form's LDS");
+    return TCHEM_OK;
+  }
   if (o->cells_per_block < 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": negative cells_per_block");
   if (o->programs < TCHEM_PROGRAMS_AUTO || o->programs > TCHEM_PROGRAMS_MEMORY)
     return setError(TCHEM_ERR_INVALID, std::string(who) + ": unknown programs placement");
@@ -260,6 +274,7 @@ bool sparseStaged(const tchem_atmchem_options_t& o) { return o.programs == TCHEM
 
 // LDS bytes of one workgroup (resolved options)
 size_t ldsBytes(const tchem_kmcd_s* k, const tchem_atmchem_options_t& o) {
+  if (o.lu == TCHEM_LU_MFMA) return mfmaCellLdsBytes(k->dev);
   const size_t bytesA = o.precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float);
   if (o.lu == TCHEM_LU_SPARSE)
     return (sparseStaged(o) ? sparseProgramBytes(k->sparse) : 0) +
@@ -290,7 +305,7 @@ int ensureKf(tchem_kmcd_s* kmcd, int nBatch, hipStream_t s) {
 int resolveBlocksPerCU(const tchem_kmcd_s* kmcd, const tchem_atmchem_options_t& o, bool rec, int* bpc) {
   *bpc = o.blocks_per_cu;
   if (*bpc > 0) return TCHEM_OK;
-  const int sparse = o.lu == TCHEM_LU_SPARSE ? (sparseStaged(o) ? 1 : 2) : 0;
+  const int sparse = o.lu == TCHEM_LU_MFMA ? 3 : (o.lu == TCHEM_LU_SPARSE ? (sparseStaged(o) ? 1 : 2) : 0);
   hipError_t e = integratorMaxBlocksPerCU(o.precision, sparse, kmcd->dev.m, 64 * o.cells_per_block,
                                           ldsBytes(kmcd, o), bpc, rec);
   if (e != hipSuccess) return hipFail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
@@ -312,13 +327,14 @@ int runIntegrator(tchem_kmcd_s* kmcd, const tchem_atmchem_options_t& o, const do
     return setError(TCHEM_ERR_INVALID, std::string(who) + ": kmcd belongs to another device");
   a.M = kmcd->dev;
   a.S = kmcd->sparse;
-  a.sparse = o.lu == TCHEM_LU_SPARSE ? (sparseStaged(o) ? 1 : 2) : 0;
+  a.sparse = o.lu == TCHEM_LU_MFMA ? 3 : (o.lu == TCHEM_LU_SPARSE ? (sparseStaged(o) ? 1 : 2) : 0);
   if (!a.queue) a.queue = kmcd->queue;
   a.persistent = (o.schedule == TCHEM_SCHEDULE_PERSISTENT) ? 1 : 0;
   a.cellsPerGroup = o.cells_per_block;
-  a.cellBytes = a.sparse ? int(sparseCellBytes(kmcd->dev, kmcd->sparse,
-                                               o.precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float)))
-                         : 0;
+  a.cellBytes = (a.sparse == 1 || a.sparse == 2)
+                    ? int(sparseCellBytes(kmcd->dev, kmcd->sparse,
+                                          o.precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float)))
+                    : 0;
   if (!ratesReady) {
     hipError_t e = launchRates(kmcd->dev, a.nBatch, a.state, a.photo, lscale, lstride, const_cast<double*>(a.kf), a.skip,
                                s);
@@ -791,6 +807,16 @@ int tchem_atmchem_run_device_batch_noalloc(const tchem_atmchem_options_t* opts, 
   a.kf = static_cast<double*>(scr);
   a.queue = reinterpret_cast<int*>(static_cast<char*>(scr) + kfBytes);
   return runIntegrator(kmcd, o, nullptr, 0, a, static_cast<hipStream_t>(stream), who);
+}
+
+int tchem_dense_lu_solve_mfma(int n, int nBatch, const double* A, const double* b, double* x, void* stream) {
+  ProfileRange profile_range("TChem::DenseLuSolveMfma");
+  if (n < 1 || n > 255 || nBatch < 0) return setError(TCHEM_ERR_INVALID, "tchem_dense_lu_solve_mfma: bad size");
+  if (nBatch == 0) return TCHEM_OK;
+  if (!A || !b || !x) return setError(TCHEM_ERR_INVALID, "tchem_dense_lu_solve_mfma: null device array");
+  hipError_t e = launchDenseMfmaSolve(n, nBatch, A, b, x, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hipFail(e, "dense_mfma_solve_kernel launch");
+  return TCHEM_OK;
 }
 
 int tchem_atmchem_e3sm_run_device_batch(const tchem_atmchem_options_t* opts, int nBatch, const double* tol_newton,

@@ -175,7 +175,7 @@ inline size_t aerosolLdsBytes(const DevMech& M, const DevAerosol& X, bool withMa
 // rate constants, rate-of-progress / derivative scratch and source-term
 // segment sums and reciprocal pivots, then the row-interchange list of the
 // factorisation.
-inline size_t integratorLdsBytes(const DevMech& M, size_t bytesA) {
+__host__ __device__ inline size_t integratorLdsBytes(const DevMech& M, size_t bytesA) {
   const size_t nScr = size_t(M.nReac > M.nPairs ? M.nReac : M.nPairs);
   size_t a = bytesA * size_t(M.m) * size_t(M.m);
   a = (a + 15) & ~size_t(15);

@@ -25,6 +25,7 @@
 
 #include "detmath.h"
 #include "device_mech.hpp"
+#include "mfma_lu.hpp"
 #include "schedule.hpp"
 
 namespace tchem_amd {
@@ -1333,6 +1334,240 @@ __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L
 }
 
 
+// ---- dense LU on the matrix cores (mfma_lu.hpp) ------------------------------
+// One cell per workgroup of four wavefronts: wavefront 0 runs integrate_cell
+// (FP64 state, residual, error estimate and step control, serial source
+// term); at every factorisation it hands the cell's iteration matrix to the
+// whole workgroup, whose four wavefronts hold it in their accumulator
+// registers for the FP32 MFMA factorisation and solve (mfma::factorAndSolve),
+// then continues alone.  Wavefronts 1-3 wait at the workgroup barrier for the
+// next command (factor, or finish).
+
+// d rop_r / d c_s of every derivative pair into out (the first part of
+// assemble_jacobian_from_lds)
+template <typename TO>
+__device__ __forceinline__ void derivative_pairs(const DevMech& M, const double* c, const double* kf, TO* out,
+                                                 int lane) {
+  for (int r = lane; r < M.nReac; r += kWave) {
+    const int p0 = M.pairPtr()[r], p1 = M.pairPtr()[r + 1];
+    if (p0 == p1) continue;
+    const int* sp = M.reacSpec() + r * M.maxR;
+    const int* pw = M.reacPow() + r * M.maxR;
+    for (int p = p0; p < p1; ++p) {
+      const int js = M.pairSlot()[p];
+      const int ns = pw[js];
+      double d = kf[r] * double(ns) * pow_small(c[sp[js]], ns - 1);
+      for (int j = 0; j < M.maxR; ++j) {
+        const int s = sp[j];
+        if (s < 0) break;
+        if (j == js) continue;
+        d *= pow_small(c[s], pw[j]);
+      }
+      out[p] = TO(d);
+    }
+  }
+}
+
+// LDS bytes of one cell of the MFMA form: the dense cell region without a
+// matrix, the FP32 derivative pairs, then the factorisation's region
+inline size_t mfmaCellBytes(const DevMech& M) {
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  return al(integratorLdsBytes(M, 0)) + al(4 * size_t(M.nPairs + 1)) + mfma::luLdsBytes();
+}
+
+template <int QM>
+struct MfmaBackend {
+  using TA = float;
+  struct Lds : CellLds<float> {
+    float* pairs;  // [nPairs] derivative pairs of the last Jacobian refresh (FP32)
+    mfma::Lu lu;
+  };
+  struct Fact {};
+  static __device__ __forceinline__ int unknowns(const IntegratorArgs& a) { return a.M.m; }
+  static __device__ __forceinline__ int rowDim(const IntegratorArgs& a) { return a.M.nSpec + 3; }
+  static __device__ __forceinline__ int column(const IntegratorArgs& a, int ai) { return 3 + a.M.perm()[ai]; }
+  static __device__ __forceinline__ int tolIndex(const IntegratorArgs& a, int ai) { return a.M.perm()[ai]; }
+  static __device__ __forceinline__ void bind(const IntegratorArgs& a, Lds& L, int cell) {
+    L.src = a.ext ? a.ext + size_t(cell) * a.M.m : nullptr;
+  }
+  static __device__ void stage(const IntegratorArgs&, char*) {}
+  static __device__ Lds carve(char* base, const IntegratorArgs& a, int) {
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    Lds L;
+    static_cast<CellLds<float>&>(L) = tchem_amd::carve<float>(base, a.M, false);
+    size_t off = al(integratorLdsBytes(a.M, 0));
+    L.pairs = reinterpret_cast<float*>(base + off);
+    off += al(4 * size_t(a.M.nPairs + 1));
+    L.lu = mfma::carveLu(base + off);
+    return L;
+  }
+  static __device__ void init(const IntegratorArgs&, const Lds&, int) {}
+  // f (FP64, serial source term) and the Newton residual; on a refresh the
+  // derivative pairs and c h of the iteration matrix (assembled by the
+  // workgroup in factor_solve)
+  static __device__ void evaluate(const IntegratorArgs& a, const Lds& L, int lane, const double (&u)[QM],
+                                  const double (&bc)[QM], double ch, bool newton, bool refresh, double (&f)[QM],
+                                  TA (&x)[QM]) {
+#pragma clang fp contract(off)
+    eval_rhs<QM>(a.M, L, lane, u, f);
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const int i = lane + kWave * q;
+      x[q] = (newton && i < a.M.m) ? TA(-(u[q] - bc[q] - ch * f[q])) : TA(0);
+    }
+    if (refresh) {
+      derivative_pairs(a.M, L.c, L.kf, L.pairs, lane);
+      if (lane == 0) *L.lu.ch = ch;
+      wave_sync();
+    }
+  }
+  static __device__ bool factor_solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM], Fact&);
+  // factors kept between refreshes: the kept derivative pairs and c h are
+  // assembled and factored again (the same matrix, the same factors)
+  static __device__ void solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM], const Fact&) {
+    Fact f;
+    factor_solve(a, L, lane, x, f);
+  }
+};
+
+// Assembly of I - c h J (FP32 sums of the serial Jacobian lists) plus the
+// identity padding and the right-hand side (stashed in z) as column 255,
+// by column quarters through LDS into the tiles of the four wavefronts,
+// then the factorisation and back substitution.
+template <int QM>
+__device__ __forceinline__ void mfma_collective(const IntegratorArgs& a, const typename MfmaBackend<QM>::Lds& L,
+                                                int wave, int lane) {
+#pragma clang fp contract(off)
+  using namespace mfma;
+  const DevMech& M = a.M;
+  const int m = M.m;
+  const float scale = float(-*L.lu.ch);
+  f32x16 acc[16];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {  // unrolled: the tiles of quarter q are known at compile time
+    for (int idx = threadIdx.x; idx < int(kStageWords); idx += blockDim.x) L.lu.stage[idx] = 0.0f;
+    __syncthreads();
+    for (int n = threadIdx.x; n < M.nSerJ; n += blockDim.x) {
+      const int rc = M.serJRowCol()[n];
+      const int row = rc >> 16, col = rc & 0xffff;
+      if ((col >> 6) != q) continue;
+      float sum = 0.0f;
+      const int e1 = M.serJPtr()[n + 1];
+      for (int e = M.serJPtr()[n]; e < e1; ++e) sum = fmaf(float(M.serJCoef()[e]), L.pairs[M.serJPair()[e]], sum);
+      L.lu.stage[row * 65 + (col - 64 * q)] = (row == col ? 1.0f : 0.0f) + scale * sum;
+    }
+    for (int p = m + int(threadIdx.x); p < kN - 1; p += blockDim.x)
+      if ((p >> 6) == q) L.lu.stage[p * 65 + (p - 64 * q)] = 1.0f;
+    if (q == 3) {
+      for (int i = threadIdx.x; i < m; i += blockDim.x) L.lu.stage[i * 65 + 63] = L.lu.z[i];
+      if (threadIdx.x == 0) L.lu.stage[(kN - 1) * 65 + 63] = 1.0f;
+    }
+    __syncthreads();
+    if (q == 0) loadQuarter<0>(acc, L.lu.stage, wave, lane);
+    if (q == 1) loadQuarter<1>(acc, L.lu.stage, wave, lane);
+    if (q == 2) loadQuarter<2>(acc, L.lu.stage, wave, lane);
+    if (q == 3) loadQuarter<3>(acc, L.lu.stage, wave, lane);
+    __syncthreads();
+  }
+  factorAndSolve(acc, L.lu, wave, lane);
+}
+
+template <int QM>
+__device__ bool MfmaBackend<QM>::factor_solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM], Fact&) {
+  const int m = a.M.m;
+#pragma unroll
+  for (int q = 0; q < QM; ++q) {
+    const int i = lane + kWave * q;
+    if (i < m) L.lu.z[i] = x[q];
+  }
+  if (lane == 0) {
+    L.lu.cmd[0] = 1;
+    L.lu.cmd[1] = 0;
+  }
+  __syncthreads();  // hands the matrix to the workgroup
+  mfma_collective<QM>(a, L, 0, lane);
+  // x = -(U^-1 e_255)[0..m)
+#pragma unroll
+  for (int q = 0; q < QM; ++q) {
+    const int i = lane + kWave * q;
+    x[q] = (i < m) ? -L.lu.z[i] : TA(0);
+  }
+  bool ok = L.lu.cmd[1] == 0;
+#pragma unroll
+  for (int q = 0; q < QM; ++q) ok = ok && isfinite(x[q]);
+  return ok;
+}
+
+template <int QM>
+__global__ __launch_bounds__(256, 1) void trbdf2_mfma_kernel(IntegratorArgs a) {
+  extern __shared__ __align__(16) char lds_raw[];
+  if (a.skip && __builtin_amdgcn_readfirstlane(*a.skip)) return;
+  using B = MfmaBackend<QM>;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // uniform: tile choices are scalar branches
+  typename B::Lds L = B::carve(lds_raw, a, 0);
+  if (wave != 0) {
+    // helpers: one collective factorisation per command until told to finish
+    for (;;) {
+      __syncthreads();
+      if (L.lu.cmd[0] == 0) break;
+      mfma_collective<QM>(a, L, wave, lane);
+    }
+    return;
+  }
+  // one call site, so integrate_cell (with the collective's register tiles) is inlined
+  for (int first = 1;; first = 0) {
+    int cell = a.nBatch;
+    if (a.persistent) {
+      if (lane == 0) cell = atomicAdd(a.queue, 1);
+      cell = __builtin_amdgcn_readfirstlane(__shfl(cell, 0, kWave));
+    } else if (first) {
+      cell = blockIdx.x;
+    }
+    if (cell >= a.nBatch) break;
+    integrate_cell<B, QM, false>(a, L, cell, lane);
+    wave_sync();
+  }
+  if (lane == 0) L.lu.cmd[0] = 0;
+  __syncthreads();  // releases the helpers
+}
+
+// Batched dense solve A x = b (n <= 255, FP64 in and out, FP32 MFMA
+// factorisation without interchanges): the LU of the MFMA form on its own.
+__global__ __launch_bounds__(256, 1) void dense_mfma_solve_kernel(int n, int nBatch, const double* A, const double* b,
+                                                                  double* x) {
+  extern __shared__ __align__(16) char lds_raw[];
+  using namespace mfma;
+  const int cell = blockIdx.x;
+  if (cell >= nBatch) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);  // uniform: tile choices are scalar branches
+  const Lu L = carveLu(lds_raw);
+  const double* Ac = A + size_t(cell) * n * n;
+  const double* bc = b + size_t(cell) * n;
+  if (threadIdx.x == 0) L.cmd[1] = 0;
+  f32x16 acc[16];
+  // [A | b] padded with the identity, by column quarters through LDS
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    for (int idx = threadIdx.x; idx < kN * 64; idx += blockDim.x) {
+      const int row = idx >> 6, col = 64 * q + (idx & 63);
+      float v = (row == col) ? 1.0f : 0.0f;
+      if (row < n && col < n) v = float(Ac[size_t(row) * n + col]);
+      else if (col == kN - 1) v = row < n ? float(bc[row]) : (row == kN - 1 ? 1.0f : 0.0f);
+      L.stage[row * 65 + (idx & 63)] = v;
+    }
+    __syncthreads();
+    if (q == 0) loadQuarter<0>(acc, L.stage, wave, lane);
+    if (q == 1) loadQuarter<1>(acc, L.stage, wave, lane);
+    if (q == 2) loadQuarter<2>(acc, L.stage, wave, lane);
+    if (q == 3) loadQuarter<3>(acc, L.stage, wave, lane);
+    __syncthreads();
+  }
+  factorAndSolve(acc, L, wave, lane);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) x[size_t(cell) * n + i] = L.cmd[1] ? NAN : -double(L.z[i]);
+}
+
 // ---- single-particle SIMPOL.1 aerosol integrator ---------------------------
 // AerosolChemistry::runDeviceBatch (src/core/TChem_AerosolChemistry.cpp:31-186):
 // the gas-phase box model extended by gas-particle partitioning
@@ -2391,6 +2626,12 @@ int sparseMaxCellsPerGroup(int precision, int m) {
 // adjoint, FP64 sparse forms only; cells per workgroup); nullptr when there
 // is none
 static const void* integratorKernel(int precision, int sparse, int qm, bool rec, int cells) {
+  if (sparse == 3) {  // dense LU on the matrix cores: mixed precision only
+    if (rec || precision != TCHEM_PRECISION_MIXED) return nullptr;
+    return qm == 1 ? reinterpret_cast<const void*>(&trbdf2_mfma_kernel<1>)
+                   : (qm == 2 ? reinterpret_cast<const void*>(&trbdf2_mfma_kernel<2>)
+                              : reinterpret_cast<const void*>(&trbdf2_mfma_kernel<4>));
+  }
 #define TCHEM_PICK_D(TA) \
   return qm == 1 ? integratorFn<DenseBackend<TA, 1>, 1, 64>() \
                  : (qm == 2 ? integratorFn<DenseBackend<TA, 2>, 2, 64>() : integratorFn<DenseBackend<TA, 4>, 4, 64>())
@@ -2435,10 +2676,11 @@ hipError_t launchIntegrator(const IntegratorArgs& a, int precision, int grid, si
   hipFuncAttributes attr;
   const hipError_t ea = hipFuncGetAttributes(&attr, fn);
   if (ea != hipSuccess) return ea;
-  if (64 * a.cellsPerGroup > attr.maxThreadsPerBlock) return hipErrorInvalidConfiguration;  // instance bound
+  const int threads = a.sparse == 3 ? 256 : 64 * a.cellsPerGroup;
+  if (threads > attr.maxThreadsPerBlock) return hipErrorInvalidConfiguration;  // instance bound
   IntegratorArgs arg = a;
   void* params[] = {&arg};
-  return hipLaunchKernel(fn, dim3(grid), dim3(64 * a.cellsPerGroup), params, lds, stream);
+  return hipLaunchKernel(fn, dim3(grid), dim3(threads), params, lds, stream);
 }
 
 // static LDS of the integrator instance (profiling builds declare some)
@@ -2454,6 +2696,7 @@ hipError_t integratorMaxBlocksPerCU(int precision, int sparse, int m, int thread
   const int qm = qmOf(m);
   if (qm == 0) return hipErrorInvalidValue;
   const void* fn = integratorKernel(precision, sparse, qm, rec, threads / 64);
+  if (sparse == 3) threads = 256;
   if (!fn) return hipErrorInvalidValue;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, fn, threads, lds);
 }
@@ -2610,6 +2853,15 @@ hipError_t noAllocScratch(void** ptr, size_t* bytes) {
   *ptr = cache[dev];
   *bytes = kNoAllocScratchBytes;
   return hipSuccess;
+}
+
+size_t mfmaCellLdsBytes(const DevMech& M) { return mfmaCellBytes(M); }
+
+hipError_t launchDenseMfmaSolve(int n, int nBatch, const double* A, const double* b, double* x, hipStream_t stream) {
+  if (nBatch <= 0) return hipSuccess;
+  hipLaunchKernelGGL(dense_mfma_solve_kernel, dim3(nBatch), dim3(256), mfma::luLdsBytes(), stream, n, nBatch, A, b,
+                     x);
+  return hipGetLastError();
 }
 
 hipError_t launchJacobian(const DevMech& M, int nBatch, const double* state, const double* photo, double* jac,

@@ -186,7 +186,7 @@ def config4(kmd, base_state_row, seed: int = 3, n_batches: int = 48, size_scale:
 
 
 def synthetic_mechanism_yaml(path: str, seed: int = 3, n_species: int = 250, n_reactions: int = 750,
-                             n_radicals: int = 12, window: int = 40) -> str:
+                             n_radicals: int = 12, window: int = 40, coupling: str = "cascade") -> str:
     """Writes a seeded synthetic mechanism in the NCAR YAML format the loader
     reads: n_species active species S000.. plus the constant third body M and
     n_reactions reactions with 1-2 reactants (integer stoichiometry, a
@@ -202,8 +202,11 @@ def synthetic_mechanism_yaml(path: str, seed: int = 3, n_species: int = 250, n_r
     first order, 20 % radical + radical; organic products come from the
     next `window` organics, radical products from the radicals.  The
     Jacobian is then a cascade plus dense radical rows and columns, which a
-    fill-reducing order factors with little fill; uniformly random coupling
-    would instead make every 250 x 250 factorisation nearly dense."""
+    fill-reducing order factors with little fill.
+
+    coupling="uniform": reactants and products drawn uniformly from all
+    species instead (the same draws of counts, yields and rate parameters);
+    the LU factor of such a mechanism is nearly dense."""
     g = np.random.Generator(np.random.Philox(key=seed))
     names = [f"S{i:03d}" for i in range(n_species)]
     nr = n_radicals
@@ -235,7 +238,10 @@ def synthetic_mechanism_yaml(path: str, seed: int = 3, n_species: int = 250, n_r
     for r in range(n_reactions):
         u = g.uniform()
         nprod = 1 + int(g.integers(0, 4))
-        if u < 0.6:    # organic + radical
+        if coupling == "uniform":
+            reac = [int(x) for x in g.integers(0, n_species, 1 + int(u < 0.8))]
+            prods = [int(x) for x in g.choice(n_species, nprod, replace=False)]
+        elif u < 0.6:    # organic + radical
             org = int(g.integers(nr, n_species - 1))
             reac = [org, int(g.integers(0, nr))]
             n_org = max(1, nprod - int(g.integers(0, 2)))
