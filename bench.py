@@ -231,39 +231,56 @@ def precision_study(cells, threads, out_path=None):
             f.write(text + "\n")
 
 
-def config4_run(size_scale, out_path=None):
-    """Uneven batches of the synthetic 250-species mechanism in one
-    persistent launch (mixed form): cell-hours/s and per-batch completion
-    times (last cell of the batch done, from the per-cell end timestamps)."""
+def config4_run(size_scale, out_path=None, forms=("mfma", "sparse"), coupling="uniform"):
+    """Uneven batches of the synthetic 250-species mechanism (48 batches,
+    sizes 10^U[log 256, log 65536] times size_scale) in one persistent
+    launch of the mixed form, once per LU form: cell-hours/s, per-batch
+    completion times (last cell of the batch done, from the per-cell end
+    timestamps), and the forms' end states against each other."""
     import tempfile
 
     import torch
     T = importlib.import_module("tchem-atm_amd")
     S = importlib.import_module("tchem-atm_amd.scenarios")
     D = importlib.import_module("tchem-atm_amd.driver")
-    mech = S.synthetic_mechanism_yaml(os.path.join(tempfile.mkdtemp(), "synthetic250.yaml"))
+    mech = S.synthetic_mechanism_yaml(os.path.join(tempfile.mkdtemp(), "synthetic250.yaml"), coupling=coupling)
     kmd = T.KineticModelData(mech)
     kmcd = kmd.createConstData()
     base = T.setScenarioConditions(mech, kmd)
     w, sizes = S.config4(kmd, base[0], size_scale=size_scale)
+    m = kmd.nActive
+    lu = {"mfma": T.LU_MFMA, "sparse": T.LU_SPARSE, "dense": T.LU_DENSE}
+    res = {"workload": "synthetic250_48_batches", "coupling": coupling, "size_scale": size_scale,
+           "cells": int(sizes.sum()), "batch_sizes_min_max": [int(sizes.min()), int(sizes.max())], "forms": {}}
+    log(f"config 4: {coupling} synthetic mechanism, {int(sizes.sum())} cells in 48 batches (scale {size_scale:g})")
+    ends = {}
     run = D.DeviceRun(kmcd, w)
-    run.reset()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    run.run_to_end(precision=T.PRECISION_MIXED, schedule=T.SCHEDULE_PERSISTENT, collect_stats=True)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    st = run.total_stats
-    start = run.t_start[run.t_start > 0].min()
-    ends = np.split(run.t_end, np.cumsum(sizes)[:-1])
-    done = np.array([(e.max() - start) / T.REALTIME_HZ for e in ends])
-    res = {"workload": "synthetic250_48_batches", "size_scale": size_scale, "cells": int(sizes.sum()),
-           "batch_sizes_min_max": [int(sizes.min()), int(sizes.max())], "cell_hours_per_s": w.cell_hours / wall,
-           "wall_s": wall, "batch_completion_s_p50": float(np.percentile(done, 50)),
-           "batch_completion_s_p99": float(np.percentile(done, 99)),
-           "accepted_steps_per_cell": float(st[:, 0].mean()),
-           "newton_iterations_per_step": float(st[:, 3].sum() / max(1, st[:, 0].sum())),
-           "failed_cells": int(st[:, 5].sum())}
+    for form in forms:
+        run.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run.run_to_end(precision=T.PRECISION_MIXED, schedule=T.SCHEDULE_PERSISTENT, collect_stats=True, lu=lu[form])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        st = run.total_stats
+        start = run.t_start[run.t_start > 0].min()
+        done = np.array([(e.max() - start) / T.REALTIME_HZ for e in np.split(run.t_end, np.cumsum(sizes)[:-1])])
+        ends[form] = run.state.cpu().numpy()[:, 3:3 + m]
+        r = {"cell_hours_per_s": w.cell_hours / wall, "wall_s": wall,
+             "batch_completion_s_p50": float(np.percentile(done, 50)),
+             "batch_completion_s_p99": float(np.percentile(done, 99)),
+             "accepted_steps_per_cell": float(st[:, 0].mean()),
+             "newton_iterations_per_step": float(st[:, 3].sum() / max(1, st[:, 0].sum())),
+             "failed_cells": int(st[:, 5].sum())}
+        res["forms"][form] = r
+        log(f"  {form}: {r['cell_hours_per_s']:.2f} cell-hours/s ({wall:.1f} s), batch completion p50 "
+            f"{r['batch_completion_s_p50']:.1f} s p99 {r['batch_completion_s_p99']:.1f} s, "
+            f"steps/cell {r['accepted_steps_per_cell']:.0f}, failed {r['failed_cells']}")
+    if len(ends) > 1:
+        ref = ends[forms[-1]]
+        for form in forms[:-1]:
+            e8 = rel_err(ends[form], ref, 1e-8)
+            res["forms"][form][f"max_rel_err_vs_{forms[-1]}_floor_1e-8"] = float(np.nanmax(e8))
     text = json.dumps(res)
     print(text, flush=True)
     if out_path:
@@ -599,6 +616,8 @@ def main():
     ap.add_argument("--config4", metavar="OUT", nargs="?", const="",
                     help="run the uneven-batch synthetic-mechanism workload instead of the benchmark")
     ap.add_argument("--config4-scale", type=float, default=1 / 64, help="batch-size scale of --config4")
+    ap.add_argument("--config4-forms", default="mfma,sparse", help="LU forms of --config4 (mfma, sparse, dense)")
+    ap.add_argument("--config4-coupling", choices=["uniform", "cascade"], default="uniform")
     ap.add_argument("--config3", metavar="OUT", nargs="?", const="",
                     help="time the discrete adjoint (forward, recording forward, backward) instead of the benchmark")
     ap.add_argument("--config3-cells", type=int, default=4096)
@@ -625,7 +644,8 @@ def main():
         cpu_study(threads, args.cpu_study or None)
         return
     if args.config4 is not None:
-        config4_run(args.config4_scale, args.config4 or None)
+        config4_run(args.config4_scale, args.config4 or None, tuple(args.config4_forms.split(",")),
+                    args.config4_coupling)
         return
     if args.config3 is not None:
         config3_run(args.config3_cells, args.config3 or None)

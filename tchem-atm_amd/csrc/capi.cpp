@@ -560,6 +560,31 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
   d.oSerJPtr = addI(t.serJPtr);
   d.oSerJPair = addI(t.serJPair);
   d.oSerJCoef = addD(t.serJCoef);
+  if (t.m <= 255) {  // the MFMA form's assembly lists: entries by column quarter, FP32 coefficients
+    std::vector<int> quarter(5, 0), dest, ptr(1, 0), pair, coef;
+    for (int q = 0; q < 4; ++q) {
+      quarter[q] = int(dest.size());
+      for (size_t n = 0; n < t.serJRow.size(); ++n) {
+        const int row = t.serJRow[n], col = t.serJCol[n];
+        if ((col >> 6) != q) continue;
+        dest.push_back(row * 65 + (col & 63) + (row == col ? (1 << 30) : 0));
+        for (int e = t.serJPtr[n]; e < t.serJPtr[n + 1]; ++e) {
+          pair.push_back(t.serJPair[e]);
+          const float c = float(t.serJCoef[e]);
+          int bits;
+          std::memcpy(&bits, &c, sizeof(bits));
+          coef.push_back(bits);
+        }
+        ptr.push_back(int(pair.size()));
+      }
+    }
+    quarter[4] = int(dest.size());
+    d.oMfjQuarter = addI(quarter);
+    d.oMfjDest = addI(dest);
+    d.oMfjPtr = addI(ptr);
+    d.oMfjPair = addI(pair);
+    d.oMfjCoef = addI(coef);
+  }
   // sparse schedules (mechanisms they cannot express run the dense form only)
   auto words = [](const std::vector<unsigned>& v) { return std::vector<int>(v.begin(), v.end()); };
   auto pack = [&](const SparseSchedule& sc, DevSparse& sp) {

@@ -21,6 +21,9 @@ struct DevMech {
       oPairPtr, oPairSlot, oJnzRowCol, oJnzPtr, oJcPair, oSerOmegaPtr, oSerOmegaReac, oSerJRowCol, oSerJPtr,
       oSerJPair;
   int oRatePar, oOmegaCoef, oEmission, oJcCoef, oSerOmegaCoef, oSerJCoef;
+  // the serial Jacobian lists regrouped for the MFMA form's assembly by
+  // column quarters (m <= 255 only; offsets 0 otherwise)
+  int oMfjQuarter, oMfjDest, oMfjPtr, oMfjPair, oMfjCoef;
 
   __host__ __device__ const int* perm() const { return ib + oPerm; }             // [m] internal -> reference species
   __host__ __device__ const int* rateKind() const { return ib + oRateKind; }     // [nReac]
@@ -49,6 +52,13 @@ struct DevMech {
   __host__ __device__ const int* serJPtr() const { return ib + oSerJPtr; }           // [nSerJ+1]
   __host__ __device__ const int* serJPair() const { return ib + oSerJPair; }         // [nSerContrib]
   __host__ __device__ const double* serJCoef() const { return db + oSerJCoef; }
+  __host__ __device__ const int* mfjQuarter() const { return ib + oMfjQuarter; }  // [5] entry ranges by col / 64
+  __host__ __device__ const int* mfjDest() const { return ib + oMfjDest; }        // [nSerJ] row * 65 + col % 64, bit 30: diagonal
+  __host__ __device__ const int* mfjPtr() const { return ib + oMfjPtr; }          // [nSerJ + 1]
+  __host__ __device__ const int* mfjPair() const { return ib + oMfjPair; }        // [nSerContrib]
+  __host__ __device__ const float* mfjCoef() const {                              // [nSerContrib] FP32 bits
+    return reinterpret_cast<const float*>(ib + oMfjCoef);
+  }
 };
 
 // Device view of the compiled sparse programs (schedule.hpp); arrays live

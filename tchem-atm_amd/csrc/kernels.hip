@@ -25,7 +25,6 @@
 
 #include "detmath.h"
 #include "device_mech.hpp"
-#include "mfma_lu.hpp"
 #include "schedule.hpp"
 
 namespace tchem_amd {
@@ -39,7 +38,7 @@ __shared__ unsigned long long tchem_pt_lds[16][16];  // [wavefront][slot], flush
 #define TCHEM_PT_BEGIN(nm) const unsigned long long tchem_pt_##nm = __builtin_readcyclecounter();
 #define TCHEM_PT_END(nm, slot)          \
   if ((threadIdx.x & 63) == 0)          \
-    tchem_pt_lds[threadIdx.x >> 6][slot] += __builtin_readcyclecounter() - tchem_pt_##nm;
+    ::tchem_amd::tchem_pt_lds[threadIdx.x >> 6][slot] += __builtin_readcyclecounter() - tchem_pt_##nm;
 #define TCHEM_PT_RESET() \
   if ((threadIdx.x & 63) < 16) tchem_pt_lds[threadIdx.x >> 6][threadIdx.x & 63] = 0;
 #define TCHEM_PT_FLUSH()                                                                          \
@@ -52,6 +51,12 @@ __shared__ unsigned long long tchem_pt_lds[16][16];  // [wavefront][slot], flush
 #define TCHEM_PT_RESET()
 #define TCHEM_PT_FLUSH()
 #endif
+
+}  // namespace tchem_amd
+
+#include "mfma_lu.hpp"
+
+namespace tchem_amd {
 
 namespace {
 
@@ -896,6 +901,7 @@ __device__ bool run_program(const unsigned* hdr, const unsigned* body, int nRoun
 template <typename TA_, int QM>
 struct DenseBackend {
   using TA = TA_;
+  static constexpr bool kCollective = false;  // factorisation by the whole workgroup (collective_solve)
   using Lds = CellLds<TA>;
   struct Fact { int nswp = 0; };
   // unknowns, state row length, state column and tolerance row of unknown ai
@@ -938,6 +944,7 @@ struct DenseBackend {
 template <typename TA_, int QM, bool STAGED>
 struct SparseBackend {
   using TA = TA_;
+  static constexpr bool kCollective = false;  // factorisation by the whole workgroup (collective_solve)
   using Lds = SpLds<TA>;
   struct Fact {};
   static __device__ __forceinline__ int unknowns(const IntegratorArgs& a) { return a.M.m; }
@@ -1199,7 +1206,10 @@ __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L
     if (nw) {
       TCHEM_PT_BEGIN(lin)
       bool ok = true;
-      if (refresh) {
+      if constexpr (B::kCollective) {  // one call site for the workgroup's factorisation code
+        ok = B::collective_solve(a, L, lane, x, refresh);
+        if (ok && refresh) ++cnt.factorizations;
+      } else if (refresh) {
         ok = B::factor_solve(a, L, lane, x, fs);
         if (ok) ++cnt.factorizations;
       } else {
@@ -1336,19 +1346,20 @@ __device__ void integrate_cell(const IntegratorArgs& a, const typename B::Lds& L
 
 // ---- dense LU on the matrix cores (mfma_lu.hpp) ------------------------------
 // One cell per workgroup of four wavefronts: wavefront 0 runs integrate_cell
-// (FP64 state, residual, error estimate and step control, serial source
-// term); at every factorisation it hands the cell's iteration matrix to the
-// whole workgroup, whose four wavefronts hold it in their accumulator
-// registers for the FP32 MFMA factorisation and solve (mfma::factorAndSolve),
-// then continues alone.  Wavefronts 1-3 wait at the workgroup barrier for the
-// next command (factor, or finish).
+// (FP64 state, residual, error estimate and step control) and hands two
+// jobs to the whole workgroup: the FP64 source term (serial per-species
+// sums), and at every Newton iteration the assembly of the FP32 iteration
+// matrix, which the four wavefronts hold in their registers for the MFMA
+// factorisation and solve (mfma::factorAndSolve).  Wavefronts 1-3 wait at
+// the workgroup barrier for the next command (source term, factor and
+// solve, or finish).
 
 // d rop_r / d c_s of every derivative pair into out (the first part of
 // assemble_jacobian_from_lds)
 template <typename TO>
 __device__ __forceinline__ void derivative_pairs(const DevMech& M, const double* c, const double* kf, TO* out,
-                                                 int lane) {
-  for (int r = lane; r < M.nReac; r += kWave) {
+                                                 int tid, int nthreads) {
+  for (int r = tid; r < M.nReac; r += nthreads) {
     const int p0 = M.pairPtr()[r], p1 = M.pairPtr()[r + 1];
     if (p0 == p1) continue;
     const int* sp = M.reacSpec() + r * M.maxR;
@@ -1369,17 +1380,42 @@ __device__ __forceinline__ void derivative_pairs(const DevMech& M, const double*
 }
 
 // LDS bytes of one cell of the MFMA form: the dense cell region without a
-// matrix, the FP32 derivative pairs, then the factorisation's region
+// matrix, the FP32 derivative pairs, the source term, then the
+// factorisation's region
 inline size_t mfmaCellBytes(const DevMech& M) {
   auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
-  return al(integratorLdsBytes(M, 0)) + al(4 * size_t(M.nPairs + 1)) + mfma::luLdsBytes();
+  return al(integratorLdsBytes(M, 0)) + al(4 * size_t(M.nPairs + 1)) + al(8 * size_t(M.m)) + mfma::luLdsBytes();
+}
+
+// The source term by the whole workgroup (256 threads): rates of progress,
+// then the serial per-species gathers of source_term_from_lds into fbuf.
+template <class Lds>
+__device__ void mfma_rhs_collective(const IntegratorArgs& a, const Lds& L) {
+#pragma clang fp contract(off)
+  const DevMech& M = a.M;
+  const int tid = threadIdx.x;
+  for (int r = tid; r < M.nReac; r += 256) L.scr[r] = rate_of_progress(M, L.c, L.kf, r);
+  __syncthreads();
+  for (int i = tid; i < M.m; i += 256) {
+    double s = M.emission()[i];
+    if (L.src) s += L.src[M.perm()[i]];
+    const int e1 = M.serOmegaPtr()[i + 1];
+    for (int e = M.serOmegaPtr()[i]; e < e1; ++e) {
+      const double v = M.serOmegaCoef()[e] * L.scr[M.serOmegaReac()[e]];
+      s = s + v;
+    }
+    L.fbuf[i] = s;
+  }
+  __syncthreads();
 }
 
 template <int QM>
 struct MfmaBackend {
   using TA = float;
+  static constexpr bool kCollective = true;  // factorisation by the whole workgroup (collective_solve)
   struct Lds : CellLds<float> {
-    float* pairs;  // [nPairs] derivative pairs of the last Jacobian refresh (FP32)
+    float* pairs;   // [nPairs] derivative pairs of the last Jacobian refresh (FP32)
+    double* fbuf;   // [m] source term of the workgroup's evaluation
     mfma::Lu lu;
   };
   struct Fact {};
@@ -1398,25 +1434,35 @@ struct MfmaBackend {
     size_t off = al(integratorLdsBytes(a.M, 0));
     L.pairs = reinterpret_cast<float*>(base + off);
     off += al(4 * size_t(a.M.nPairs + 1));
+    L.fbuf = reinterpret_cast<double*>(base + off);
+    off += al(8 * size_t(a.M.m));
     L.lu = mfma::carveLu(base + off);
     return L;
   }
   static __device__ void init(const IntegratorArgs&, const Lds&, int) {}
-  // f (FP64, serial source term) and the Newton residual; on a refresh the
-  // derivative pairs and c h of the iteration matrix (assembled by the
-  // workgroup in factor_solve)
+  // f (FP64, serial source term) and the Newton residual; on a refresh c h
+  // of the iteration matrix (its derivative pairs and assembly are the
+  // workgroup's, in factor_solve)
   static __device__ void evaluate(const IntegratorArgs& a, const Lds& L, int lane, const double (&u)[QM],
                                   const double (&bc)[QM], double ch, bool newton, bool refresh, double (&f)[QM],
                                   TA (&x)[QM]) {
 #pragma clang fp contract(off)
-    eval_rhs<QM>(a.M, L, lane, u, f);
+    const int m = a.M.m;
 #pragma unroll
     for (int q = 0; q < QM; ++q) {
       const int i = lane + kWave * q;
-      x[q] = (newton && i < a.M.m) ? TA(-(u[q] - bc[q] - ch * f[q])) : TA(0);
+      if (i < m) L.c[i] = u[q];
     }
-    if (refresh) {
-      derivative_pairs(a.M, L.c, L.kf, L.pairs, lane);
+    if (lane == 0) L.lu.cmd[0] = 2;
+    __syncthreads();  // hands the source term to the workgroup
+    mfma_rhs_collective(a, L);
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const int i = lane + kWave * q;
+      f[q] = i < m ? L.fbuf[i] : 0.0;
+      x[q] = (newton && i < m) ? TA(-(u[q] - bc[q] - ch * f[q])) : TA(0);
+    }
+    if (refresh) {  // the workgroup computes the derivative pairs in collective_solve
       if (lane == 0) *L.lu.ch = ch;
       wave_sync();
     }
@@ -1425,9 +1471,10 @@ struct MfmaBackend {
   // factors kept between refreshes: the kept derivative pairs and c h are
   // assembled and factored again (the same matrix, the same factors)
   static __device__ void solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM], const Fact&) {
-    Fact f;
-    factor_solve(a, L, lane, x, f);
+    collective_solve(a, L, lane, x, false);
   }
+  static __device__ bool collective_solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM],
+                                          bool refresh);
 };
 
 // Assembly of I - c h J (FP32 sums of the serial Jacobian lists) plus the
@@ -1441,26 +1488,31 @@ __device__ __forceinline__ void mfma_collective(const IntegratorArgs& a, const t
   using namespace mfma;
   const DevMech& M = a.M;
   const int m = M.m;
+  const int tid = threadIdx.x;
+  TCHEM_PT_BEGIN(dp)
+  if (L.lu.cmd[2]) derivative_pairs(M, L.c, L.kf, L.pairs, tid, 256);  // refresh: pairs of the current iterate
+  TCHEM_PT_END(dp, 1)
   const float scale = float(-*L.lu.ch);
   f32x16 acc[16];
+  TCHEM_PT_BEGIN(asm)
+  float4* stage4 = reinterpret_cast<float4*>(L.lu.stage);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {  // unrolled: the tiles of quarter q are known at compile time
-    for (int idx = threadIdx.x; idx < int(kStageWords); idx += blockDim.x) L.lu.stage[idx] = 0.0f;
+    for (int idx = tid; idx < int(kStageWords / 4); idx += 256) stage4[idx] = float4{0.0f, 0.0f, 0.0f, 0.0f};
     __syncthreads();
-    for (int n = threadIdx.x; n < M.nSerJ; n += blockDim.x) {
-      const int rc = M.serJRowCol()[n];
-      const int row = rc >> 16, col = rc & 0xffff;
-      if ((col >> 6) != q) continue;
+    const int e1 = M.mfjQuarter()[q + 1];
+    for (int n = M.mfjQuarter()[q] + tid; n < e1; n += 256) {
+      const int d = M.mfjDest()[n];
+      const int p1 = M.mfjPtr()[n + 1];
       float sum = 0.0f;
-      const int e1 = M.serJPtr()[n + 1];
-      for (int e = M.serJPtr()[n]; e < e1; ++e) sum = fmaf(float(M.serJCoef()[e]), L.pairs[M.serJPair()[e]], sum);
-      L.lu.stage[row * 65 + (col - 64 * q)] = (row == col ? 1.0f : 0.0f) + scale * sum;
+      for (int e = M.mfjPtr()[n]; e < p1; ++e) sum = fmaf(M.mfjCoef()[e], L.pairs[M.mfjPair()[e]], sum);
+      L.lu.stage[d & 0x3fffffff] = ((d >> 30) ? 1.0f : 0.0f) + scale * sum;
     }
-    for (int p = m + int(threadIdx.x); p < kN - 1; p += blockDim.x)
+    for (int p = m + tid; p < kN - 1; p += 256)
       if ((p >> 6) == q) L.lu.stage[p * 65 + (p - 64 * q)] = 1.0f;
     if (q == 3) {
-      for (int i = threadIdx.x; i < m; i += blockDim.x) L.lu.stage[i * 65 + 63] = L.lu.z[i];
-      if (threadIdx.x == 0) L.lu.stage[(kN - 1) * 65 + 63] = 1.0f;
+      for (int i = tid; i < m; i += 256) L.lu.stage[i * 65 + 63] = L.lu.z[i];
+      if (tid == 0) L.lu.stage[(kN - 1) * 65 + 63] = 1.0f;
     }
     __syncthreads();
     if (q == 0) loadQuarter<0>(acc, L.lu.stage, wave, lane);
@@ -1469,11 +1521,18 @@ __device__ __forceinline__ void mfma_collective(const IntegratorArgs& a, const t
     if (q == 3) loadQuarter<3>(acc, L.lu.stage, wave, lane);
     __syncthreads();
   }
+  TCHEM_PT_END(asm, 4)
   factorAndSolve(acc, L.lu, wave, lane);
 }
 
 template <int QM>
 __device__ bool MfmaBackend<QM>::factor_solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM], Fact&) {
+  return collective_solve(a, L, lane, x, true);
+}
+
+template <int QM>
+__device__ bool MfmaBackend<QM>::collective_solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM],
+                                                  bool refresh) {
   const int m = a.M.m;
 #pragma unroll
   for (int q = 0; q < QM; ++q) {
@@ -1483,6 +1542,7 @@ __device__ bool MfmaBackend<QM>::factor_solve(const IntegratorArgs& a, const Lds
   if (lane == 0) {
     L.lu.cmd[0] = 1;
     L.lu.cmd[1] = 0;
+    L.lu.cmd[2] = refresh ? 1 : 0;
   }
   __syncthreads();  // hands the matrix to the workgroup
   mfma_collective<QM>(a, L, 0, lane);
@@ -1502,6 +1562,7 @@ template <int QM>
 __global__ __launch_bounds__(256, 1) void trbdf2_mfma_kernel(IntegratorArgs a) {
   extern __shared__ __align__(16) char lds_raw[];
   if (a.skip && __builtin_amdgcn_readfirstlane(*a.skip)) return;
+  TCHEM_PT_RESET()
   using B = MfmaBackend<QM>;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // uniform: tile choices are scalar branches
@@ -1510,8 +1571,13 @@ __global__ __launch_bounds__(256, 1) void trbdf2_mfma_kernel(IntegratorArgs a) {
     // helpers: one collective factorisation per command until told to finish
     for (;;) {
       __syncthreads();
-      if (L.lu.cmd[0] == 0) break;
-      mfma_collective<QM>(a, L, wave, lane);
+      const int cmd = __builtin_amdgcn_readfirstlane(L.lu.cmd[0]);
+      if (cmd == 0) break;
+      if (cmd == 2) {
+        mfma_rhs_collective(a, L);
+      } else {
+        mfma_collective<QM>(a, L, wave, lane);
+      }
     }
     return;
   }
@@ -1720,6 +1786,7 @@ struct AerosolLayout {
 template <int QM>
 struct AerosolBackend : AerosolLayout {
   using TA = double;
+  static constexpr bool kCollective = false;  // factorisation by the whole workgroup (collective_solve)
   using Lds = AeroLds;
   struct Fact { int nswp = 0; };
   static __device__ void stage(const IntegratorArgs&, char*) {}
@@ -1906,6 +1973,7 @@ __device__ AeroSpLds carve_aero_sparse(char* base, const IntegratorArgs& a, int 
 template <int QM>
 struct AerosolSparseBackend : AerosolLayout {
   using TA = double;
+  static constexpr bool kCollective = false;  // factorisation by the whole workgroup (collective_solve)
   using Lds = AeroSpLds;
   struct Fact {};
   static __device__ void stage(const IntegratorArgs&, char*) {}

@@ -2,26 +2,31 @@
 // counterpart in the reference, which factors a dense m x m matrix with
 // Tines per Newton iteration, src/core/problems/TChem_Impl_AtmosphericChemistry.hpp:138-155):
 // the FP32 iteration matrix I - c h J of one cell, padded to 256 x 256, lives
-// in the accumulator registers of the four wavefronts of a workgroup (16
-// tiles of 32 x 32 per wavefront, 256 registers per lane), and is factored
-// in place without row interchanges (the pivots of the fill-reducing order,
-// as the sparse form) by 32 x 32 blocks:
-//   for k = 0..7: factor the diagonal tile (one wavefront, rows in
-//   registers), invert its two triangles (L^-1, U^-1 in LDS), panels
-//   L_ik = A_ik U_kk^-1 and U_kj = L_kk^-1 A_kj and the trailing update
-//   A_ij -= L_ik U_kj, each a 32 x 32 x 32 product of 16
-//   v_mfma_f32_32x32x2f32 instructions.
+// in the registers of the four wavefronts of a workgroup (16 tiles of
+// 32 x 32 per wavefront), and is factored in place without interchanges
+// between blocks (the pivots of the fill-reducing order, as in the sparse
+// form) by 32 x 32 blocks in Schur-complement form (factorAndSolve):
+// the diagonal tile is inverted by one wavefront, and the panel products
+// and the trailing update are 32 x 32 x 32 products of 16
+// v_mfma_f32_32x32x2f32 instructions each.
 // The right-hand side is column 255 (rows >= m padded with the identity), so
 // the factorisation also does the forward substitution; the solution is
 // x = -z[0..m), z = U^-1 e_255, by block back substitution with the kept
-// U_kk^-1.  Tile (I, J) belongs to wavefront 2 (I & 1) + (J & 1), so every
-// trailing update spreads over the four wavefronts.
+// inverses of the diagonal blocks.  Tile (I, J) belongs to wavefront
+// 2 (I & 1) + (J & 1), so every trailing update spreads over the four
+// wavefronts.
 //
 // MFMA register layouts (v_mfma_f32_32x32x2f32, D = A B + C):
 //   A (32 x 2): lane l holds A[l % 32][l / 32];  B (2 x 32): lane l holds
 //   B[l / 32][l % 32];  C/D (32 x 32): register r of lane l holds
 //   C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
 #pragma once
+
+// phase timers of profiling builds (kernels.hip defines them first)
+#ifndef TCHEM_PT_BEGIN
+#define TCHEM_PT_BEGIN(nm)
+#define TCHEM_PT_END(nm, slot)
+#endif
 
 namespace mfma {
 
@@ -39,11 +44,10 @@ __device__ __forceinline__ int crow(int r, int lane) { return 8 * (r >> 2) + 4 *
 // integrator's cell region
 struct Lu {
   float* stage;    // [kN][64 + 1] column quarter of the matrix during assembly; reused for the panels
-  float* lpanel;   // [kNb - 1][kTileWords] -L_ik of the current step (aliases stage)
-  float* upanel;   // [kNb - 1][kTileWords] U_kj (aliases stage)
-  float* diag;     // [kTileWords] diagonal tile, factored in place
-  float* linv;     // [kTileWords] L_kk^-1
-  float* uinv;     // [kNb][kTileWords] U_kk^-1 of every block (back substitution)
+  float* lpanel;   // [kNb - 1][kTileWords] -P_ik of the current step (aliases stage)
+  float* upanel;   // [kNb - 1][kTileWords] A_kj of the current step (aliases stage)
+  float* diag;     // [kTileWords] diagonal tile, then D_k^-1 - I of the current step
+  float* uinv;     // [kNb][kTileWords] D_k^-1 of every block (back substitution)
   float* z;        // [kN] back substitution unknowns
   float* ypart;    // [4][kTile] partial block sums of the four wavefronts
   int* cmd;        // [2] command word of the driver wavefront, zero-pivot flag
@@ -54,7 +58,7 @@ constexpr size_t kStageWords = size_t(kN) * 65;
 __host__ __device__ inline size_t luLdsBytes() {
   const size_t panels = size_t(2 * (kNb - 1)) * kTileWords;
   const size_t stage = kStageWords > panels ? kStageWords : panels;
-  return 4 * (stage + kTileWords * (2 + kNb) + kN + 4 * kTile) + 16 + 16;
+  return 4 * (stage + kTileWords * (1 + kNb) + kN + 4 * kTile) + 16 + 16;
 }
 
 __device__ inline Lu carveLu(char* base) {
@@ -66,8 +70,7 @@ __device__ inline Lu carveLu(char* base) {
   L.lpanel = f;
   L.upanel = f + size_t(kNb - 1) * kTileWords;
   L.diag = f + stage;
-  L.linv = L.diag + kTileWords;
-  L.uinv = L.linv + kTileWords;
+  L.uinv = L.diag + kTileWords;
   L.z = L.uinv + size_t(kNb) * kTileWords;
   L.ypart = L.z + kN;
   char* tail = reinterpret_cast<char*>(L.ypart + 4 * kTile);
@@ -116,158 +119,159 @@ __device__ __forceinline__ void wsync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One wavefront factors the diagonal tile in LDS (no interchanges; row i on
-// lane i, 32 lanes) and leaves L^-1 - I in linv, U^-1 in uinvK (kept for the
-// back substitution) and U^-1 - I in diag: with the identity taken out, the
-// panels L_ik = A_ik U^-1 = A_ik + A_ik (U^-1 - I) and U_kj = L^-1 A_kj =
-// A_kj + (L^-1 - I) A_kj become accumulations into the tile registers, the
-// same form as the trailing update.  A zero or non-finite pivot sets *bad.
-__device__ inline void factorDiag(const Lu& L, float* uinvK, int lane, bool* bad) {
+// One wavefront inverts the diagonal tile (Gauss-Jordan on [D | I], no
+// interchanges: the pivots of the fill-reducing order, as in the sparse
+// form).  Lane c holds column c of [D | I] (32 rows), so a pivot step is
+// 32 reads of the pivot column from lane p and 32 FMAs per lane, all in
+// registers.  Leaves D^-1 in dinvK (kept for the back substitution) and
+// D^-1 - I in L.diag: with the identity taken out, the panel
+// P_ik = A_ik D^-1 = A_ik + A_ik (D^-1 - I) is an accumulation into the tile
+// registers.  A zero or non-finite pivot sets *bad.
+// The tile is in L.diag (row-major) on entry.
+__device__ __forceinline__ void invertDiag(const Lu& L, float* dinvK, int lane, bool* bad) {
 #pragma clang fp contract(off)
-  float row[kTile];
-  const int i = lane & 31;
+  const int c = lane & 31;
+  float x[kTile];
 #pragma unroll
-  for (int j = 0; j < kTile; ++j) row[j] = L.diag[i * kLd + j];
+  for (int i = 0; i < kTile; ++i) x[i] = lane < 32 ? L.diag[i * kLd + c] : (i == c ? 1.0f : 0.0f);
+  wsync();  // every lane has read the tile before L.diag is rewritten
   bool ok = true;
 #pragma unroll
   for (int p = 0; p < kTile; ++p) {
-    // row p of the partly factored tile, read by every lane from lane p
-    float up[kTile];
+    float colp[kTile];
 #pragma unroll
-    for (int j = p; j < kTile; ++j) up[j] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(row[j]), p));
-    ok = ok && up[p] != 0.0f && isfinite(up[p]);
-    if (i > p) {
-      const float l = row[p] / up[p];
-      row[p] = l;
+    for (int i = 0; i < kTile; ++i) colp[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x[i]), p));
+    ok = ok && colp[p] != 0.0f && isfinite(colp[p]);
+    const float xp = x[p] * __builtin_amdgcn_rcpf(colp[p]);  // hardware reciprocal: the factors are FP32 anyway
 #pragma unroll
-      for (int j = p + 1; j < kTile; ++j) row[j] = fmaf(-l, up[j], row[j]);
-    }
+    for (int i = 0; i < kTile; ++i)
+      if (i != p) x[i] = fmaf(-colp[i], xp, x[i]);
+    x[p] = xp;
   }
-  if (lane < 32) {
-#pragma unroll
-    for (int j = 0; j < kTile; ++j) L.diag[i * kLd + j] = row[j];
-  }
-  wsync();
-  // lanes 0..31: column c of L^-1 (forward, unit diagonal); lanes 32..63:
-  // column c of U^-1 (backward)
-  const int c = lane & 31;
-  float x[kTile];
-  if (lane < 32) {
-#pragma unroll
-    for (int r = 0; r < kTile; ++r) {
-      float s = (r == c) ? 1.0f : 0.0f;
-#pragma unroll
-      for (int p = 0; p < r; ++p) s = fmaf(-L.diag[r * kLd + p], x[p], s);
-      x[r] = s;
-    }
-#pragma unroll
-    for (int r = 0; r < kTile; ++r) L.linv[r * kLd + c] = (r == c) ? 0.0f : x[r];
-  } else {
-#pragma unroll
-    for (int r = kTile - 1; r >= 0; --r) {
-      float s = (r == c) ? 1.0f : 0.0f;
-#pragma unroll
-      for (int p = r + 1; p < kTile; ++p) s = fmaf(-L.diag[r * kLd + p], x[p], s);
-      x[r] = s / L.diag[r * kLd + r];
-    }
-#pragma unroll
-    for (int r = 0; r < kTile; ++r) uinvK[r * kLd + c] = x[r];
-  }
-  wsync();
   if (lane >= 32) {
 #pragma unroll
-    for (int r = 0; r < kTile; ++r) L.diag[r * kLd + c] = (r == c) ? x[r] - 1.0f : x[r];
+    for (int i = 0; i < kTile; ++i) {
+      dinvK[i * kLd + c] = x[i];
+      L.diag[i * kLd + c] = i == c ? x[i] - 1.0f : x[i];
+    }
   }
   if (!ok) *bad = true;
 }
 
-// Factorisation and back substitution of the assembled tiles.  All four
-// wavefronts of the workgroup call it with their own tiles (wave = 0..3); it
-// contains the workgroup barriers.  Every write to a tile is an MFMA
-// accumulation into its own registers.  On return z[0..256) holds
+// Block LU without interchanges between blocks, Schur-complement form:
+//   step k: P_ik = A_ik D_k^-1 (D_k the diagonal tile, inverted by its
+//   owner), A_ij -= P_ik A_kj for i, j > k.
+// The block row A_kj (j > k) is left as it stands, so the factor is
+// A = L U with L_ik = P_ik (unit diagonal blocks) and U_kj = A_kj, U_kk = D_k;
+// the right-hand side is column 255, so the elimination also applies L^-1
+// to it and the solution needs only the back substitution with U, whose
+// diagonal blocks are inverted already.  Look-ahead: in step k's trailing
+// update the owner of tile (k+1, k+1) updates it first and inverts it while
+// the other wavefronts finish their tiles.
+//
+// All four wavefronts of the workgroup call this with their own tiles (wave
+// = 0..3); it contains the workgroup barriers.  Every write to a tile is an
+// MFMA accumulation into its own registers.  On return z[0..256) holds
 // U^-1 e_255; cmd[1] is set when a pivot was zero or not finite.
 __device__ __forceinline__ void factorAndSolve(f32x16 (&acc)[16], const Lu& L, int wave, int lane) {
-  for (int k = 0; k < kNb; ++k) {
-    bool bad = false;
-    if (wave == 2 * (k & 1) + (k & 1)) {
+  bool bad = false;
+  // step -1 only inverts D_0; step k < 7 inverts D_{k+1} in its trailing update
+  for (int k = -1; k < kNb; ++k) {
+    TCHEM_PT_BEGIN(pn)
+    if (k >= 0) {
 #pragma unroll
-      for (int t = 0; t < 16; ++t)
-        if (tileI(t, wave) == k && tileJ(t, wave) == k) storeTile(acc[t], L.diag, lane);
+      for (int t = 0; t < 16; ++t) {
+        const int I = tileI(t, wave), J = tileJ(t, wave);
+        if (J == k && I > k) {  // P_ik, kept negated for the trailing update
+          float* slot = L.lpanel + size_t(I - k - 1) * kTileWords;
+          storeTile(acc[t], slot, lane);
+          wsync();
+          acc[t] = mma32(acc[t], slot, L.diag, lane);
+          wsync();
+          storeTileNeg(acc[t], slot, lane);
+        }
+        if (I == k && J > k) storeTile(acc[t], L.upanel + size_t(J - k - 1) * kTileWords, lane);  // U_kj
+      }
+      __syncthreads();
+    }
+    TCHEM_PT_END(pn, 7)
+    TCHEM_PT_BEGIN(dg)
+    const int n = k + 1;  // next diagonal tile, updated and inverted first by its owner
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int I = tileI(t, wave), J = tileJ(t, wave);
+      if (I == n && J == n && n < kNb) {
+        if (k >= 0)
+          acc[t] = mma32(acc[t], L.lpanel + size_t(I - k - 1) * kTileWords, L.upanel + size_t(J - k - 1) * kTileWords,
+                         lane);
+        storeTile(acc[t], L.diag, lane);
+      }
+    }
+    if (n < kNb && wave == 3 * (n & 1)) {  // tile (n, n) belongs to wavefront 3 (n & 1)
       wsync();
-      factorDiag(L, L.uinv + size_t(k) * kTileWords, lane, &bad);
+      invertDiag(L, L.uinv + size_t(n) * kTileWords, lane, &bad);
     }
-    if (bad && lane == 0) L.cmd[1] = 1;
-    __syncthreads();
+    TCHEM_PT_END(dg, 5)
+    TCHEM_PT_BEGIN(tr)
+    if (k >= 0) {
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      const int I = tileI(t, wave), J = tileJ(t, wave);
-      if (J == k && I > k) {  // L_ik, kept negated for the trailing update
-        float* slot = L.lpanel + size_t(I - k - 1) * kTileWords;
-        storeTile(acc[t], slot, lane);
-        wsync();
-        acc[t] = mma32(acc[t], slot, L.diag, lane);
-        wsync();
-        storeTileNeg(acc[t], slot, lane);
-      }
-      if (I == k && J > k) {  // U_kj, kept in the registers for the back substitution
-        float* slot = L.upanel + size_t(J - k - 1) * kTileWords;
-        storeTile(acc[t], slot, lane);
-        wsync();
-        acc[t] = mma32(acc[t], L.linv, slot, lane);
-        wsync();
-        storeTile(acc[t], slot, lane);
+      for (int t = 0; t < 16; ++t) {
+        const int I = tileI(t, wave), J = tileJ(t, wave);
+        if (I > k && J > k && !(I == n && J == n))
+          acc[t] = mma32(acc[t], L.lpanel + size_t(I - k - 1) * kTileWords, L.upanel + size_t(J - k - 1) * kTileWords,
+                         lane);
       }
     }
     __syncthreads();
-#pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      const int I = tileI(t, wave), J = tileJ(t, wave);
-      if (I > k && J > k)
-        acc[t] = mma32(acc[t], L.lpanel + size_t(I - k - 1) * kTileWords, L.upanel + size_t(J - k - 1) * kTileWords,
-                       lane);
-    }
-    __syncthreads();
+    TCHEM_PT_END(tr, 8)
   }
-  // z = U^-1 e_255: the last block is column 31 of U_77^-1, then blocks 6..0
+  if (bad && lane == 0) L.cmd[1] = 1;
+  TCHEM_PT_BEGIN(bs)
+  // z = U^-1 e_255: the last block is column 31 of D_7^-1, then blocks 6..0
   if (wave == 0 && lane < kTile) L.z[(kNb - 1) * kTile + lane] = L.uinv[size_t(kNb - 1) * kTileWords + lane * kLd + 31];
   __syncthreads();
   for (int I = kNb - 2; I >= 0; --I) {
-    float part[16];
+    // the two wavefronts holding block row I: U_IJ z_J of their tiles, row
+    // sums by a transposed store (the stage region is free by now)
+    if ((wave >> 1) == (I & 1)) {
+      float part[16];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) part[r] = 0.0f;
+      for (int r = 0; r < 16; ++r) part[r] = 0.0f;
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      const int J = tileJ(t, wave);
-      if (tileI(t, wave) == I && J > I) {
-        const float zj = L.z[J * kTile + (lane & 31)];
+      for (int t = 0; t < 16; ++t) {
+        const int J = tileJ(t, wave);
+        if (tileI(t, wave) == I && J > I) {
+          const float zj = L.z[J * kTile + (lane & 31)];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) part[r] = fmaf(acc[t][r], zj, part[r]);
+          for (int r = 0; r < 16; ++r) part[r] = fmaf(acc[t][r], zj, part[r]);
+        }
       }
-    }
+      float* yt = L.stage + (wave & 1) * kTileWords;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      float v = part[r];
-#pragma unroll
-      for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-      if ((lane & 31) == 0) L.ypart[wave * kTile + crow(r, lane)] = v;
+      for (int r = 0; r < 16; ++r) yt[crow(r, lane) * kLd + (lane & 31)] = part[r];
     }
     __syncthreads();
     if (wave == 0) {
+      // lane i + 32 h sums row i of half h; y_I = -(both halves)
+      const float* yt = L.stage + (lane >> 5) * kTileWords + (lane & 31) * kLd;
       float y = 0.0f;
-      if (lane < kTile) y = -(L.ypart[lane] + L.ypart[kTile + lane] + L.ypart[2 * kTile + lane] + L.ypart[3 * kTile + lane]);
-      wsync();
-      if (lane < kTile) L.ypart[lane] = y;
-      wsync();
-      if (lane < kTile) {
-        const float* ui = L.uinv + size_t(I) * kTileWords + lane * kLd;
-        float zi = 0.0f;
 #pragma unroll
-        for (int c = 0; c < kTile; ++c) zi = fmaf(ui[c], L.ypart[c], zi);
-        L.z[I * kTile + lane] = zi;
-      }
+      for (int c = 0; c < kTile; ++c) y += yt[c];
+      y += __shfl_xor(y, 32, 64);
+      if (lane < kTile) L.ypart[lane] = -y;
+      wsync();
+      // z_I = D_I^-1 y_I: lane i + 32 h takes columns 16 h .. 16 h + 15 of row i
+      const float* ui = L.uinv + size_t(I) * kTileWords + (lane & 31) * kLd + 16 * (lane >> 5);
+      const float* yv = L.ypart + 16 * (lane >> 5);
+      float zi = 0.0f;
+#pragma unroll
+      for (int c = 0; c < 16; ++c) zi = fmaf(ui[c], yv[c], zi);
+      zi += __shfl_xor(zi, 32, 64);
+      if (lane < kTile) L.z[I * kTile + lane] = zi;
     }
     __syncthreads();
   }
+  TCHEM_PT_END(bs, 14)
 }
 
 // Loads the tiles of wavefront `wave` whose column block pair is q (tiles

@@ -1,6 +1,7 @@
-"""The synthetic 250-species mechanism (seeded, NCAR YAML through the same
-loader): its LU / evaluation programs do not fit in LDS, so the sparse form
-reads them from device memory.  GPU against the FP64 oracle on 8 cells of the
+"""The cascade variant of the synthetic 250-species mechanism (seeded, NCAR
+YAML through the same loader; its LU factor stays sparse): its LU /
+evaluation programs do not fit in LDS, so the sparse form reads them from
+device memory.  GPU against the FP64 oracle on 8 cells of the
 uneven-batch workload over 60 s (about 1700 steps per cell)."""
 import dataclasses
 import importlib
@@ -14,7 +15,7 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def synth(gpu, pkg, oracle_mod, tmp_path_factory):
     S = importlib.import_module("tchem-atm_amd.scenarios")
-    mech = S.synthetic_mechanism_yaml(str(tmp_path_factory.mktemp("mech") / "synthetic250.yaml"))
+    mech = S.synthetic_mechanism_yaml(str(tmp_path_factory.mktemp("mech") / "synthetic250.yaml"), coupling="cascade")
     kmd = pkg.KineticModelData(mech)
     kmcd = kmd.createConstData()
     base = pkg.setScenarioConditions(mech, kmd)

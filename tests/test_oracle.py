@@ -298,7 +298,7 @@ def test_elimination_order_matches_the_library(pkg, oracle_mod, mech):
 def test_elimination_order_matches_on_the_synthetic_mechanism(pkg, oracle_mod, tmp_path):
     import importlib
     S = importlib.import_module("tchem-atm_amd.scenarios")
-    path = S.synthetic_mechanism_yaml(str(tmp_path / "synthetic250.yaml"))
+    path = S.synthetic_mechanism_yaml(str(tmp_path / "synthetic250.yaml"), coupling="cascade")
     k = oracle_mod.kmd_mod.load(path)
     np.testing.assert_array_equal(oracle_mod.kmd_mod.elimination_order(k),
                                   pkg.KineticModelData(path).eliminationOrder())

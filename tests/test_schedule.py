@@ -47,7 +47,7 @@ def test_sparse_programs_synthetic_patterns(checker, tmp_path, seed, n_species, 
     scen = importlib.import_module("tchem-atm_amd.scenarios")
     path = str(tmp_path / f"synthetic_{seed}.yaml")
     scen.synthetic_mechanism_yaml(path, seed=seed, n_species=n_species, n_reactions=n_reactions,
-                                  n_radicals=n_radicals, window=window)
+                                  n_radicals=n_radicals, window=window, coupling="cascade")
     r = subprocess.run([checker, path], capture_output=True, text=True)
     print(r.stdout[-1500:])
     assert r.returncode == 0, r.stdout[-3000:]
