@@ -212,6 +212,13 @@ int resolveOptions(const tchem_atmchem_options_t* opts, const tchem_kmcd_s* k, t
   if (o->blocks_per_cu < 0) return setError(TCHEM_ERR_INVALID, std::string(who) + ": negative blocks_per_cu");
   if (o->lu != TCHEM_LU_AUTO && o->lu != TCHEM_LU_DENSE && o->lu != TCHEM_LU_SPARSE && o->lu != TCHEM_LU_MFMA)
     return setError(TCHEM_ERR_INVALID, std::string(who) + ": unknown lu");
+  // AUTO in mixed precision: the MFMA form when the factor is dense (at
+  // least 30 % of the m^2 entries in the fill-reducing order) and large
+  // enough to fill its 256 x 256 tiles (m >= 128)
+  if (o->lu == TCHEM_LU_AUTO && o->precision == TCHEM_PRECISION_MIXED && o->cells_per_block <= 1 &&
+      k->dev.m >= 128 && k->dev.m <= 255 && mfmaCellLdsBytes(k->dev) <= k->maxLds &&
+      (!k->sparse.enabled || double(k->sparse.nnzF) >= 0.3 * double(k->dev.m) * double(k->dev.m)))
+    o->lu = TCHEM_LU_MFMA;
   if (o->lu == TCHEM_LU_MFMA) {
     if (o->precision != TCHEM_PRECISION_MIXED)
       return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": the MFMA LU form is mixed precision (FP32 factors)");
@@ -265,6 +272,8 @@ int resolveOptions(const tchem_atmchem_options_t* opts, const tchem_kmcd_s* k, t
     if (o->cells_per_block > 1)
       return setError(TCHEM_ERR_INVALID, std::string(who) + ": the dense form runs one cell per block");
     o->cells_per_block = 1;
+    if (integratorLdsBytes(k->dev, bytesA) > maxLds)
+      return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": the dense form's matrix does not fit in LDS");
   }
   return TCHEM_OK;
 }
@@ -1209,7 +1218,15 @@ int tchem_atmchem_autotune(tchem_atmchem_options_t* opts, int nBatch, const doub
     if (resolveOptions(&t, kmcd, &r, who) == TCHEM_OK) lus.push_back(TCHEM_LU_SPARSE);
     else if (luIn == TCHEM_LU_SPARSE) return TCHEM_ERR_UNSUPPORTED;
   }
-  if (luIn == TCHEM_LU_AUTO || luIn == TCHEM_LU_DENSE) lus.push_back(TCHEM_LU_DENSE);
+  for (int lu : {TCHEM_LU_DENSE, TCHEM_LU_MFMA}) {
+    if (luIn != TCHEM_LU_AUTO && luIn != lu) continue;
+    if (luIn == TCHEM_LU_AUTO && lu == TCHEM_LU_MFMA && kmcd->dev.m < 128) continue;  // tiles mostly padding
+    tchem_atmchem_options_t t = base;
+    t.lu = lu;
+    tchem_atmchem_options_t r;
+    if (resolveOptions(&t, kmcd, &r, who) == TCHEM_OK) lus.push_back(lu);
+    else if (luIn == lu) return TCHEM_ERR_UNSUPPORTED;
+  }
   {
     tchem_atmchem_options_t r;
     int rc = resolveOptions(&base, kmcd, &r, who);  // validates the remaining fields
