@@ -174,7 +174,8 @@ int tchem_atmchem_e3sm_run_device_batch(const tchem_atmchem_options_t* opts, int
  * MFMA form (TCHEM_LU_MFMA) on its own, for dense systems of the caller.
  * Solves A_i x_i = b_i for nBatch matrices A [nBatch][n][n], b, x [nBatch][n]
  * (FP64 in and out, [device]) with n <= 255, factored in FP32 on the matrix
- * cores without row interchanges (diagonal pivots; a zero pivot gives NaN).
+ * cores without row interchanges (2 x 2 diagonal pivot blocks; a singular
+ * pivot block gives NaN for that system).
  * One workgroup of 256 threads per system. */
 int tchem_dense_lu_solve_mfma(int n, int nBatch, const double* A, const double* b, double* x, void* stream);
 

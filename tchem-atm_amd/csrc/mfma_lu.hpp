@@ -119,43 +119,61 @@ __device__ __forceinline__ void wsync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One wavefront inverts the diagonal tile (Gauss-Jordan on [D | I], no
-// interchanges: the pivots of the fill-reducing order, as in the sparse
-// form).  Lane c holds column c of [D | I] (32 rows), so a pivot step is
-// 32 reads of the pivot column from lane p and 32 FMAs per lane, all in
-// registers.  Leaves D^-1 in dinvK (kept for the back substitution) and
-// D^-1 - I in L.diag: with the identity taken out, the panel
-// P_ik = A_ik D^-1 = A_ik + A_ik (D^-1 - I) is an accumulation into the tile
-// registers.  A zero or non-finite pivot sets *bad.
-// The tile is in L.diag (row-major) on entry.
-__device__ __forceinline__ void invertDiag(const Lu& L, float* dinvK, int lane, bool* bad) {
+// One wavefront inverts a tile in its own registers (C layout), in place,
+// by block Gauss-Jordan with 2 x 2 pivot blocks P = {p, p+1} and no
+// interchanges (the pivots of the fill-reducing order, as in the sparse
+// form).  With S = M_PP^-1, one step is the rank-2 update
+//   M <- M + A B,  A = E_P - M_:P (32 x 2),
+//   B = S M_P: with columns P replaced by S + I (2 x 32),
+// which leaves S M_PQ in the pivot rows, -M_OP S in the pivot columns, S in
+// the pivot block and M_OQ - M_OP S M_PQ elsewhere: after 16 steps the tile
+// holds M^-1.  Each step is one v_mfma_f32_32x32x2f32; its operands are
+// read from a column-major copy of the tile in LDS (buf, [32][36] floats,
+// 16-byte aligned).  A zero or non-finite pivot block sets *bad.
+constexpr int kCm = 36;  // column stride of the column-major copy
+static_assert(kStageWords >= size_t(2 * (kNb - 1)) * kTileWords + size_t(kTile) * kCm,
+              "the inversion buffer sits in the stage region past the panels");
+__device__ __forceinline__ void invertTileInPlace(f32x16& acc, float* buf, int lane, bool* bad) {
 #pragma clang fp contract(off)
-  const int c = lane & 31;
-  float x[kTile];
-#pragma unroll
-  for (int i = 0; i < kTile; ++i) x[i] = lane < 32 ? L.diag[i * kLd + c] : (i == c ? 1.0f : 0.0f);
-  wsync();  // every lane has read the tile before L.diag is rewritten
+  const int c = lane & 31, h = lane >> 5;
   bool ok = true;
+  for (int p = 0; p < kTile; p += 2) {
+    // registers 4j..4j+3 hold rows 8j + 4h .. 8j + 4h + 3 of column c
 #pragma unroll
-  for (int p = 0; p < kTile; ++p) {
-    float colp[kTile];
-#pragma unroll
-    for (int i = 0; i < kTile; ++i) colp[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x[i]), p));
-    ok = ok && colp[p] != 0.0f && isfinite(colp[p]);
-    const float xp = x[p] * __builtin_amdgcn_rcpf(colp[p]);  // hardware reciprocal: the factors are FP32 anyway
-#pragma unroll
-    for (int i = 0; i < kTile; ++i)
-      if (i != p) x[i] = fmaf(-colp[i], xp, x[i]);
-    x[p] = xp;
-  }
-  if (lane >= 32) {
-#pragma unroll
-    for (int i = 0; i < kTile; ++i) {
-      dinvK[i * kLd + c] = x[i];
-      L.diag[i * kLd + c] = i == c ? x[i] - 1.0f : x[i];
-    }
+    for (int j = 0; j < 4; ++j)
+      *reinterpret_cast<float4*>(buf + c * kCm + 8 * j + 4 * h) = float4{acc[4 * j], acc[4 * j + 1], acc[4 * j + 2],
+                                                                           acc[4 * j + 3]};
+    wsync();
+    const float mcol = buf[(p + h) * kCm + c];  // M[c][p + h]
+    const float r0 = buf[c * kCm + p], r1 = buf[c * kCm + p + 1];  // M[p][c], M[p + 1][c]
+    const float m00 = buf[p * kCm + p], m10 = buf[p * kCm + p + 1];
+    const float m01 = buf[(p + 1) * kCm + p], m11 = buf[(p + 1) * kCm + p + 1];
+    wsync();  // every read done before the next step's copy
+    const float det = m00 * m11 - m01 * m10;
+    ok = ok && det != 0.0f && isfinite(det);
+    const float rdet = __builtin_amdgcn_rcpf(det);
+    const float s0 = h ? -m10 * rdet : m11 * rdet;  // row h of S
+    const float s1 = h ? m00 * rdet : -m01 * rdet;
+    const float a = (c == p + h ? 1.0f : 0.0f) - mcol;
+    float b = s0 * r0 + s1 * r1;
+    if (c == p) b = s0 + (h == 0 ? 1.0f : 0.0f);
+    if (c == p + 1) b = s1 + (h == 1 ? 1.0f : 0.0f);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
   }
   if (!ok) *bad = true;
+}
+
+// After invertTileInPlace: D^-1 into dinvK (kept for the back substitution)
+// and D^-1 - I into L.diag (the panel operand: with the identity taken out,
+// P_ik = A_ik D^-1 = A_ik + A_ik (D^-1 - I) is an accumulation into the tile
+// registers), both row-major with stride kLd.
+__device__ __forceinline__ void storeInverse(const f32x16& inv, const Lu& L, float* dinvK, int lane) {
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = crow(r, lane), col = lane & 31;
+    dinvK[row * kLd + col] = inv[r];
+    L.diag[row * kLd + col] = row == col ? inv[r] - 1.0f : inv[r];
+  }
 }
 
 // Block LU without interchanges between blocks, Schur-complement form:
@@ -175,6 +193,7 @@ __device__ __forceinline__ void invertDiag(const Lu& L, float* dinvK, int lane, 
 // U^-1 e_255; cmd[1] is set when a pivot was zero or not finite.
 __device__ __forceinline__ void factorAndSolve(f32x16 (&acc)[16], const Lu& L, int wave, int lane) {
   bool bad = false;
+  float* invBuf = L.stage + size_t(2 * (kNb - 1)) * kTileWords;  // past the panels: free during the factorisation
   // step -1 only inverts D_0; step k < 7 inverts D_{k+1} in its trailing update
   for (int k = -1; k < kNb; ++k) {
     TCHEM_PT_BEGIN(pn)
@@ -200,16 +219,13 @@ __device__ __forceinline__ void factorAndSolve(f32x16 (&acc)[16], const Lu& L, i
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
       const int I = tileI(t, wave), J = tileJ(t, wave);
-      if (I == n && J == n && n < kNb) {
+      if (I == n && J == n && n < kNb) {  // tile (n, n) belongs to wavefront 3 (n & 1)
         if (k >= 0)
           acc[t] = mma32(acc[t], L.lpanel + size_t(I - k - 1) * kTileWords, L.upanel + size_t(J - k - 1) * kTileWords,
                          lane);
-        storeTile(acc[t], L.diag, lane);
+        invertTileInPlace(acc[t], invBuf, lane, &bad);  // the tile is not needed after its inverse
+        storeInverse(acc[t], L, L.uinv + size_t(n) * kTileWords, lane);
       }
-    }
-    if (n < kNb && wave == 3 * (n & 1)) {  // tile (n, n) belongs to wavefront 3 (n & 1)
-      wsync();
-      invertDiag(L, L.uinv + size_t(n) * kTileWords, lane, &bad);
     }
     TCHEM_PT_END(dg, 5)
     TCHEM_PT_BEGIN(tr)

@@ -3,7 +3,7 @@
 - tchem_dense_lu_solve_mfma against numpy's FP64 solve on seeded diagonally
   dominant systems of every size class the 32 x 32 tiling distinguishes (one
   tile, a tile boundary, a partly filled last tile, the full 255), and the
-  zero-pivot convention (NaN for that system only).  A wrong register layout,
+  singular-pivot convention (NaN for that system only).  A wrong register layout,
   tile owner or panel orientation gives O(1) errors; FP32 factors of
   well-conditioned systems give about 1e-6.
 - The TR-BDF2 integrator with lu = TCHEM_LU_MFMA on the uniformly coupled
@@ -56,7 +56,7 @@ def test_dense_lu_solve_mfma_matches_fp64_solve(pkg, gpu, n):
 
 def test_dense_lu_solve_mfma_zero_pivot(pkg, gpu):
     A, b = _systems(40, 3, 7)
-    A[1, 0, 0] = 0.0  # no interchanges: the first pivot of system 1 is zero
+    A[1, 0, :2] = 0.0  # no interchanges: the first 2 x 2 pivot block of system 1 is singular
     x = _solve(pkg, gpu, A, b)
     assert np.isnan(x[1]).all()
     ref = np.linalg.solve(A[[0, 2]], b[[0, 2], :, None])[..., 0]
