@@ -7,7 +7,9 @@
 #include <algorithm>
 #include <cstdlib>
 #include <array>
+#include <chrono>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <memory>
@@ -28,6 +30,10 @@ hipError_t integratorMaxBlocksPerCU(int precision, int sparse, int m, int thread
 int sparseMaxCellsPerGroup(int precision, int m);
 size_t integratorStaticLds(int precision, int sparse, int m);
 hipError_t launchResetCounter(int* counter, hipStream_t stream);
+hipError_t launchAutotuneSample(int n, long long stride, int nb, int np, int iters, const double* state,
+                                const double* photo, const tchem_time_advance_t* tadv, double* st0, double* ph,
+                                tchem_time_advance_t* ta, double* tt0, int* live, hipStream_t stream);
+hipError_t launchCopyWords(void* dst, const void* src, size_t words, hipStream_t stream);
 size_t mfmaCellLdsBytes(const DevMech& M);
 hipError_t launchDenseMfmaSolve(int n, int nBatch, const double* A, const double* b, double* x, hipStream_t stream);
 hipError_t noAllocScratch(void** ptr, size_t* bytes);
@@ -1069,10 +1075,10 @@ int tchem_adjoint_steps(int nBatch, const void* workspace, int maxSteps, int* ns
   const AdjointLayout W = adjointLayout(kmcd->dev, nBatch, maxSteps);
   const char* ws = static_cast<const char*>(workspace);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipError_t e = hipMemcpyAsync(nsteps, ws + W.nsteps, sizeof(int) * size_t(nBatch), hipMemcpyDeviceToDevice, s);
-  if (e == hipSuccess && h)
-    e = hipMemcpyAsync(h, ws + W.h, sizeof(double) * size_t(nBatch) * size_t(maxSteps), hipMemcpyDeviceToDevice, s);
-  if (e != hipSuccess) return hipFail(e, "hipMemcpyAsync");
+  // device copies: both arrays are the caller's
+  hipError_t e = launchCopyWords(nsteps, ws + W.nsteps, size_t(nBatch), s);
+  if (e == hipSuccess && h) e = launchCopyWords(h, ws + W.h, 2 * size_t(nBatch) * size_t(maxSteps), s);
+  if (e != hipSuccess) return hipFail(e, "copy launch");
   return TCHEM_OK;
 }
 
@@ -1253,43 +1259,39 @@ int tchem_atmchem_autotune(tchem_atmchem_options_t* opts, int nBatch, const doub
   };
   double* st0 = static_cast<double*>(alloc(sizeof(double) * size_t(n) * nb));
   double* st = static_cast<double*>(alloc(sizeof(double) * size_t(n) * nb));
-  double* ph = np ? static_cast<double*>(alloc(sizeof(double) * size_t(n) * np)) : nullptr;
+  double* ph = (np && photo_rates) ? static_cast<double*>(alloc(sizeof(double) * size_t(n) * np)) : nullptr;
   tchem_time_advance_t* ta = static_cast<tchem_time_advance_t*>(alloc(sizeof(tchem_time_advance_t) * size_t(n)));
   double* tt = static_cast<double*>(alloc(sizeof(double) * size_t(n) * 2));
-  if (!st0 || !st || (np && !ph) || !ta || !tt) {
+  if (!st0 || !st || (np && photo_rates && !ph) || !ta || !tt) {
     freeAll();
     return setError(TCHEM_ERR_HIP, std::string(who) + ": scratch allocation failed");
   }
-  std::vector<tchem_time_advance_t> hta(n);
-  std::vector<double> ht(size_t(n) * 2, 0.0);
-  hipError_t e = hipMemcpy2DAsync(st0, sizeof(double) * nb, state, sizeof(double) * nb * stride, sizeof(double) * nb,
-                                  n, hipMemcpyDeviceToDevice, s);
-  if (e == hipSuccess && ph)
-    e = hipMemcpy2DAsync(ph, sizeof(double) * np, photo_rates, sizeof(double) * np * stride, sizeof(double) * np, n,
-                         hipMemcpyDeviceToDevice, s);
-  if (e == hipSuccess)
-    e = hipMemcpy2DAsync(hta.data(), sizeof(tchem_time_advance_t), tadv, sizeof(tchem_time_advance_t) * stride,
-                         sizeof(tchem_time_advance_t), n, hipMemcpyDeviceToHost, s);
+  // the sample, extracted on the device (tt0: t_out = tbeg, dt_out = 0)
+  double* tt0 = static_cast<double*>(alloc(sizeof(double) * size_t(n) * 2));
+  int* live = static_cast<int*>(alloc(sizeof(int)));
+  if (!tt0 || !live) {
+    freeAll();
+    return setError(TCHEM_ERR_HIP, std::string(who) + ": scratch allocation failed");
+  }
+  int hlive = 0;
+  hipError_t e = hipMemcpy(live, &hlive, sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess)
+    e = launchAutotuneSample(n, (long long)stride, nb, ph ? np : 0, iters, state, ph ? photo_rates : nullptr, tadv, st0,
+                             ph, ta, tt0, live, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess) e = hipMemcpy(&hlive, live, sizeof(int), hipMemcpyDeviceToHost);
   if (e != hipSuccess) {
     freeAll();
     return hipFail(e, "sample copy");
   }
-  for (int i = 0; i < n; ++i) {
-    hta[i]._num_time_iterations_per_interval = iters;
-    ht[i] = hta[i]._tbeg;  // t_out starts at tbeg: no cell is skipped
-  }
-  e = hipMemcpyAsync(ta, hta.data(), sizeof(tchem_time_advance_t) * n, hipMemcpyHostToDevice, s);
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  if (e == hipSuccess) e = hipEventCreate(&ev0);
-  if (e == hipSuccess) e = hipEventCreate(&ev1);
-  if (e != hipSuccess) {
+  if (hlive == 0) {
     freeAll();
-    return hipFail(e, "autotune setup");
+    return setError(TCHEM_ERR_INVALID, std::string(who) + ": no cell of the sample has time left to integrate");
   }
   auto trial = [&](const tchem_atmchem_options_t& o, float* ms) -> int {
     hipError_t ex = hipMemcpyAsync(st, st0, sizeof(double) * size_t(n) * nb, hipMemcpyDeviceToDevice, s);
-    if (ex == hipSuccess) ex = hipMemcpyAsync(tt, ht.data(), sizeof(double) * size_t(n) * 2, hipMemcpyHostToDevice, s);
+    if (ex == hipSuccess) ex = hipMemcpyAsync(tt, tt0, sizeof(double) * size_t(n) * 2, hipMemcpyDeviceToDevice, s);
     if (ex != hipSuccess) return hipFail(ex, "trial reset");
     IntegratorArgs a{};
     a.nBatch = n;
@@ -1302,16 +1304,25 @@ int tchem_atmchem_autotune(tchem_atmchem_options_t* opts, int nBatch, const doub
     a.dt_out = tt + n;
     a.state_out = st;
     a.kf = kmcd->kf;
-    if ((ex = hipEventRecord(ev0, s)) != hipSuccess) return hipFail(ex, "hipEventRecord");
+    // host clock around a synchronised launch: robust where events are not
+    // (under some profilers they complete early); a trial runs for ms
+    if ((ex = hipStreamSynchronize(s)) != hipSuccess) return hipFail(ex, "hipStreamSynchronize");
+    const auto t0 = std::chrono::steady_clock::now();
     const int r = runIntegrator(kmcd, o, nullptr, 0, a, s, who);
     if (r) return r;
-    if ((ex = hipEventRecord(ev1, s)) != hipSuccess) return hipFail(ex, "hipEventRecord");
-    if ((ex = hipEventSynchronize(ev1)) != hipSuccess) return hipFail(ex, "hipEventSynchronize");
-    if ((ex = hipEventElapsedTime(ms, ev0, ev1)) != hipSuccess) return hipFail(ex, "hipEventElapsedTime");
+    if ((ex = hipStreamSynchronize(s)) != hipSuccess) return hipFail(ex, "hipStreamSynchronize");
+    *ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return TCHEM_OK;
   };
   float best = -1.0f;
   tchem_atmchem_options_t bestO{};
+  const char* venv = std::getenv("TCHEM_AUTOTUNE_VERBOSE");  // trial timings on stderr
+  const bool verbose = venv && *venv && *venv != '0';
+  if (verbose) {
+    std::fprintf(stderr, "autotune: %d sample cells, %d iterations, candidate LU forms:", n, iters);
+    for (int lu : lus) std::fprintf(stderr, " %d", lu);
+    std::fprintf(stderr, "\n");
+  }
   // (lu, programs placement) candidates; placement only matters for the sparse form
   std::vector<std::pair<int, int>> forms;
   for (int lu : lus) {
@@ -1338,7 +1349,10 @@ int tchem_atmchem_autotune(tchem_atmchem_options_t* opts, int nBatch, const doub
         oc.blocks_per_cu = oc.schedule == TCHEM_SCHEDULE_PERSISTENT ? b : 0;
         float ms = 0.0f;
         if ((rc = trial(oc, &ms)) || (rc = trial(oc, &ms))) break;  // first run warms the kernel instance
-        if (best < 0.0f || ms < best) {
+        if (verbose)
+          std::fprintf(stderr, "autotune: lu %d programs %d cells %d blocks/CU %d: %.3f ms\n", oc.lu, oc.programs,
+                       oc.cells_per_block, oc.blocks_per_cu, double(ms));
+        if (best < 0.0f || ms < best) {  // ms >= 0: host clock
           best = ms;
           bestO = oc;
         }
@@ -1347,8 +1361,6 @@ int tchem_atmchem_autotune(tchem_atmchem_options_t* opts, int nBatch, const doub
     }
     if (rc) break;
   }
-  (void)hipEventDestroy(ev0);
-  (void)hipEventDestroy(ev1);
   freeAll();
   if (rc) return rc;
   if (best < 0.0f) return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": no launch configuration fits");

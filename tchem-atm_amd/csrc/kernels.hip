@@ -2604,6 +2604,34 @@ __global__ __launch_bounds__(1024) void stop_check_kernel(int n, const double* t
   if (threadIdx.x == 0 && part[0] / double(n) > fraction * tadv[0]._tend) ctl[0] = 1;
 }
 
+// The autotuner's sample (rows 0, stride, 2 stride, ... of the caller's
+// batch) copied by the device: the caller's arrays are never handed to a
+// runtime copy (the runtime that owns them may not be the library's, e.g.
+// under a profiler that loads its own), and the records get the trial's
+// iteration cap, t_out = tbeg and dt_out = 0.  live counts cells with time
+// left.
+__global__ void autotune_sample_kernel(int n, long long stride, int nb, int np, int iters, const double* state,
+                                       const double* photo, const tchem_time_advance_t* tadv, double* st0,
+                                       double* ph, tchem_time_advance_t* ta, double* tt0, int* live) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long src = i * stride;
+  for (int j = 0; j < nb; ++j) st0[size_t(i) * nb + j] = state[size_t(src) * nb + j];
+  for (int j = 0; j < np; ++j) ph[size_t(i) * np + j] = photo[size_t(src) * np + j];
+  tchem_time_advance_t r = tadv[src];
+  r._num_time_iterations_per_interval = iters;
+  ta[i] = r;
+  tt0[i] = r._tbeg;
+  tt0[n + i] = 0.0;
+  if (r._tend > r._tbeg && r._dt > 0.0) atomicAdd(live, 1);
+}
+
+// dst[i] = src[i] for n 4-byte words (device copies of caller arrays)
+__global__ void copy_words_kernel(unsigned* dst, const unsigned* src, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void reset_counter_kernel(int* counter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
 }
@@ -2858,6 +2886,23 @@ hipError_t launchE3smKinetics(const DevE3sm& E, int mode, int nBatch, const doub
   const size_t lds = sizeof(double) * (size_t(E.nSpec) + 2 * size_t(E.nReac) + size_t(E.nSpec - E.nConst));
   hipLaunchKernelGGL(e3sm_kinetics_kernel, dim3(nBatch), dim3(64), lds, stream, E, mode, nBatch, state, photo, nPhoto,
                      ext, out, ldOut);
+  return hipGetLastError();
+}
+
+hipError_t launchAutotuneSample(int n, long long stride, int nb, int np, int iters, const double* state,
+                                const double* photo, const tchem_time_advance_t* tadv, double* st0, double* ph,
+                                tchem_time_advance_t* ta, double* tt0, int* live, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(autotune_sample_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, stride, nb, np, iters,
+                     state, photo, tadv, st0, ph, ta, tt0, live);
+  return hipGetLastError();
+}
+
+hipError_t launchCopyWords(void* dst, const void* src, size_t words, hipStream_t stream) {
+  if (words == 0) return hipSuccess;
+  const size_t blocks = std::min<size_t>((words + 255) / 256, 4096);
+  hipLaunchKernelGGL(copy_words_kernel, dim3(unsigned(blocks)), dim3(256), 0, stream, static_cast<unsigned*>(dst),
+                     static_cast<const unsigned*>(src), words);
   return hipGetLastError();
 }
 
