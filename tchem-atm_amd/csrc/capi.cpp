@@ -1565,9 +1565,9 @@ int tchem_amcd_create(tchem_amd_t amd, tchem_kmcd_t kmcd, tchem_amcd_t* out) {
     const MechTables ta = aerosolTables(t, am, &ent);
     h->sched = buildSparseSchedule(ta, false);
     const SparseSchedule& sc = h->sched;
-    auto position = [&](int r, int c) {
+    auto position = [&](int r, int c) {  // the LDS slot of factor entry (r, c)
       for (int q = sc.rowPtr[r]; q < sc.rowPtr[r + 1]; ++q)
-        if (sc.colIdx[q] == c) return q;
+        if (sc.colIdx[q] == c) return sc.slotOf[size_t(q)];
       throw std::runtime_error("transfer derivative outside the factor pattern");
     };
     X.oJpos = int(ib.size());

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <array>
 #include <map>
+#include <random>
 #include <stdexcept>
 
 namespace tchem_amd {
@@ -73,6 +74,237 @@ int emitChains(std::vector<Chain> chains, int kind, unsigned padTarget, unsigned
     ++emitted;
   }
   return emitted;
+}
+
+// Bank-aware numbering of the factor positions.  Every matrix access of a
+// round (one operand field read or written by the 64 lanes) is an LDS
+// instruction; on 32 banks of 4 bytes per half wavefront it takes as many
+// passes as the most-used bank holds distinct addresses.  The positions
+// [0, nnzF) are permuted (RHS, zero and trash slots fixed) to minimise the
+// passes summed over the LU, BACK, FWD and Jacobian rounds: local search
+// over swaps from the row-major numbering, deterministic.  The program
+// words are rewritten in place.
+void bankAwareSlots(SparseSchedule& s) {
+  using SS = SparseSchedule;
+  const int P = s.nnzF;
+  constexpr int kBanks = 32, kHalf = 32;
+  s.slotOf.resize(size_t(P));
+  for (int p = 0; p < P; ++p) s.slotOf[size_t(p)] = p;
+  // an access: for one (round, field, half wavefront), the distinct movable
+  // positions and the bank counts of the fixed addresses; weight 2 for a
+  // target that is read and written
+  struct Access {
+    std::vector<int> pos;
+    std::array<int, kBanks> fixedCnt{};
+    int weight = 1;
+  };
+  std::vector<Access> acc;
+  // field extractors over a section: calls f(words of lane l) per round
+  auto addField = [&](const std::vector<std::vector<unsigned>>& vals, int weight) {
+    // vals[l] = addresses of lane l for this field (0 or 1 address)
+    for (int h = 0; h < 2; ++h) {
+      Access a;
+      a.weight = weight;
+      std::vector<int> seen;
+      for (int l = h * kHalf; l < (h + 1) * kHalf; ++l)
+        for (unsigned v : vals[size_t(l)]) seen.push_back(int(v));
+      std::sort(seen.begin(), seen.end());
+      seen.erase(std::unique(seen.begin(), seen.end()), seen.end());
+      for (int v : seen) {
+        if (v < P) a.pos.push_back(v);
+        else a.fixedCnt[size_t(v % kBanks)]++;
+      }
+      if (a.pos.size() + size_t(std::count_if(a.fixedCnt.begin(), a.fixedCnt.end(), [](int c) { return c > 0; })) > 1)
+        acc.push_back(std::move(a));
+    }
+  };
+  // walk a section of the LU program: callback per round with (kind, len, word pointer)
+  auto walkLu = [&](int r0, int nr, size_t body0, auto&& fn) {
+    size_t b = body0;
+    for (int r = r0; r < r0 + nr; ++r) {
+      const int kind = int(s.progHdr[size_t(r)] >> 16), len = int(s.progHdr[size_t(r)] & 0xffff);
+      fn(kind, len, b);
+      b += size_t(kLanes) * (kind == SS::kMul ? 1 : size_t(1 + len));
+    }
+  };
+  const unsigned rhs0 = unsigned(P);
+  auto collect = [&](int kind, int len, size_t b) {
+    std::vector<std::vector<unsigned>> f(kLanes);
+    auto word = [&](int j, int l) { return s.progBody[b + size_t(kLanes) * size_t(j) + size_t(l)]; };
+    auto flush = [&](int weight) {
+      addField(f, weight);
+      for (auto& x : f) x.clear();
+    };
+    if (kind == SS::kMul) {
+      for (int l = 0; l < kLanes; ++l) f[size_t(l)].push_back(word(0, l) >> 16);
+      flush(1);
+      for (int l = 0; l < kLanes; ++l) {
+        const unsigned dst = word(0, l) & 0xffff;
+        f[size_t(l)].push_back((dst & 0x8000) ? unsigned(s.trashSlot) : dst);
+      }
+      flush(2);
+    } else if (kind == SS::kUpd || kind == SS::kBack) {
+      for (int l = 0; l < kLanes; ++l) f[size_t(l)].push_back(word(0, l));
+      flush(2);
+      for (int j = 1; j <= len; ++j) {
+        for (int l = 0; l < kLanes; ++l) f[size_t(l)].push_back(word(j, l) & 0xffff);
+        flush(1);
+        if (kind == SS::kUpd) {
+          for (int l = 0; l < kLanes; ++l) f[size_t(l)].push_back(word(j, l) >> 16);
+          flush(1);
+        } else {
+          for (int l = 0; l < kLanes; ++l) f[size_t(l)].push_back(rhs0 + (word(j, l) >> 16));
+          flush(1);
+        }
+      }
+    } else if (kind == SS::kMulUpd) {
+      for (int l = 0; l < kLanes; ++l) f[size_t(l)].push_back(word(0, l));
+      flush(2);
+      for (int l = 0; l < kLanes; ++l) f[size_t(l)].push_back(word(1, l) & 0xffff);
+      flush(1);
+      for (int l = 0; l < kLanes; ++l) f[size_t(l)].push_back(word(1, l) >> 16);
+      flush(1);
+      for (int l = 0; l < kLanes; ++l) f[size_t(l)].push_back(word(2, l) & 0x7fff);
+      flush(1);
+      for (int j = 3; j <= len; ++j) {
+        for (int l = 0; l < kLanes; ++l) f[size_t(l)].push_back(word(j, l) & 0xffff);
+        flush(1);
+        for (int l = 0; l < kLanes; ++l) f[size_t(l)].push_back(word(j, l) >> 16);
+        flush(1);
+      }
+    }
+  };
+  walkLu(0, s.luRounds, 0, collect);
+  walkLu(s.luRounds, s.backRounds, size_t(s.backBody), collect);
+  walkLu(s.luRounds + s.backRounds, s.fwdRounds, size_t(s.fwdBody), collect);
+  // Jacobian rounds: targets (position in the low 30 bits)
+  {
+    size_t b = size_t(s.jacBody);
+    const int r0 = s.ropRounds + s.pairRounds + s.segRounds;
+    for (int r = r0; r < r0 + s.jacRounds; ++r) {
+      const int len = int(s.evalHdr[size_t(r)] & 0xffff);
+      std::vector<std::vector<unsigned>> f(kLanes);
+      for (int l = 0; l < kLanes; ++l) f[size_t(l)].push_back(s.evalBody[b + size_t(l)] & 0x3fffffffu);
+      addField(f, 2);
+      b += size_t(kLanes) * size_t(1 + len);
+    }
+  }
+  // per position: the accesses it is in
+  std::vector<std::vector<int>> inAcc(static_cast<size_t>(P));
+  for (size_t g = 0; g < acc.size(); ++g)
+    for (int p : acc[g].pos) inAcc[size_t(p)].push_back(int(g));
+  std::vector<std::array<int, kBanks>> cnt(acc.size());
+  auto passes = [&](const std::array<int, kBanks>& c) { return *std::max_element(c.begin(), c.end()); };
+  long long total = 0;
+  for (size_t g = 0; g < acc.size(); ++g) {
+    cnt[g] = acc[g].fixedCnt;
+    for (int p : acc[g].pos) cnt[g][size_t(s.slotOf[size_t(p)] % kBanks)]++;
+    total += (long long)acc[g].weight * (passes(cnt[g]) - 1);
+  }
+  s.bankPassesBefore = total;
+  // local search: swap the slots of two positions in different banks when
+  // that does not add passes
+  std::mt19937_64 rng(0x5eedULL);
+  // budget: 400 tries per position, and at most ~3e8 access updates (a
+  // deterministic work bound that keeps kmcd creation fast on the largest
+  // mechanisms, whose sparse form is not the chosen one)
+  const long long iters = std::min<long long>(400LL * P, 1500000LL);
+  long long work = 0;
+  const long long maxWork = 300000000LL;
+  std::vector<int> mark(acc.size(), 0);
+  int stamp = 0;
+  for (long long it = 0; it < iters && P > 1 && work < maxWork; ++it) {
+    const int a = int(rng() % unsigned(P)), b = int(rng() % unsigned(P));
+    work += (long long)(inAcc[size_t(a)].size() + inAcc[size_t(b)].size()) * 2;
+    const int sa = s.slotOf[size_t(a)], sb = s.slotOf[size_t(b)];
+    const int ra = sa % kBanks, rb = sb % kBanks;
+    if (a == b || ra == rb) continue;
+    ++stamp;
+    long long delta = 0;
+    // accesses holding both keep their counts
+    for (int g : inAcc[size_t(b)]) mark[size_t(g)] = stamp;
+    auto moveCost = [&](int g, int from, int to) {
+      std::array<int, kBanks>& c = cnt[size_t(g)];
+      const int before = passes(c);
+      c[size_t(from)]--;
+      c[size_t(to)]++;
+      const int after = passes(c);
+      c[size_t(to)]--;
+      c[size_t(from)]++;
+      return (long long)acc[size_t(g)].weight * (after - before);
+    };
+    std::vector<int> onlyA, onlyB;
+    for (int g : inAcc[size_t(a)])
+      if (mark[size_t(g)] != stamp) onlyA.push_back(g);
+      else mark[size_t(g)] = -stamp;  // both
+    for (int g : inAcc[size_t(b)])
+      if (mark[size_t(g)] == stamp) onlyB.push_back(g);
+    for (int g : onlyA) delta += moveCost(g, ra, rb);
+    for (int g : onlyB) delta += moveCost(g, rb, ra);
+    if (delta <= 0) {
+      for (int g : onlyA) {
+        cnt[size_t(g)][size_t(ra)]--;
+        cnt[size_t(g)][size_t(rb)]++;
+      }
+      for (int g : onlyB) {
+        cnt[size_t(g)][size_t(rb)]--;
+        cnt[size_t(g)][size_t(ra)]++;
+      }
+      s.slotOf[size_t(a)] = sb;
+      s.slotOf[size_t(b)] = sa;
+      total += delta;
+    }
+  }
+  s.bankPassesAfter = total;
+  // rewrite the program words: every position field below nnzF
+  auto mapPos = [&](unsigned v) { return v < unsigned(P) ? unsigned(s.slotOf[v]) : v; };
+  auto remapLu = [&](int kind, int len, size_t b) {
+    auto wref = [&](int j, int l) -> unsigned& { return s.progBody[b + size_t(kLanes) * size_t(j) + size_t(l)]; };
+    for (int l = 0; l < kLanes; ++l) {
+      if (kind == SS::kMul) {
+        unsigned& w = wref(0, l);
+        const unsigned dst = w & 0xffff, src = w >> 16;
+        w = ((dst & 0x8000) ? dst : mapPos(dst)) | (mapPos(src) << 16);
+      } else if (kind == SS::kUpd) {
+        wref(0, l) = mapPos(wref(0, l));
+        for (int j = 1; j <= len; ++j) {
+          unsigned& w = wref(j, l);
+          w = mapPos(w & 0xffff) | (mapPos(w >> 16) << 16);
+        }
+      } else if (kind == SS::kBack) {
+        wref(0, l) = mapPos(wref(0, l));
+        for (int j = 1; j <= len; ++j) {
+          unsigned& w = wref(j, l);
+          w = mapPos(w & 0xffff) | (w & 0xffff0000u);
+        }
+      } else if (kind == SS::kMulUpd) {
+        wref(0, l) = mapPos(wref(0, l));
+        unsigned& w1 = wref(1, l);
+        w1 = mapPos(w1 & 0xffff) | (mapPos(w1 >> 16) << 16);
+        unsigned& w2 = wref(2, l);
+        w2 = mapPos(w2 & 0x7fff) | (w2 & 0xffff8000u);
+        for (int j = 3; j <= len; ++j) {
+          unsigned& w = wref(j, l);
+          w = mapPos(w & 0xffff) | (mapPos(w >> 16) << 16);
+        }
+      }
+    }
+  };
+  walkLu(0, s.luRounds, 0, remapLu);
+  walkLu(s.luRounds, s.backRounds, size_t(s.backBody), remapLu);
+  walkLu(s.luRounds + s.backRounds, s.fwdRounds, size_t(s.fwdBody), remapLu);
+  {
+    size_t b = size_t(s.jacBody);
+    const int r0 = s.ropRounds + s.pairRounds + s.segRounds;
+    for (int r = r0; r < r0 + s.jacRounds; ++r) {
+      const int len = int(s.evalHdr[size_t(r)] & 0xffff);
+      for (int l = 0; l < kLanes; ++l) {
+        unsigned& w = s.evalBody[b + size_t(l)];
+        w = mapPos(w & 0x3fffffffu) | (w & 0xc0000000u);
+      }
+      b += size_t(kLanes) * size_t(1 + len);
+    }
+  }
 }
 
 }  // namespace
@@ -501,6 +733,7 @@ SparseSchedule buildSparseSchedule(const MechTables& t, bool transposed) {
     s.fwdRounds += emitChains(ch, SparseSchedule::kUpd, T, zz, false, s.progHdr, s.progBody);
   }
   s.progBody.insert(s.progBody.end(), slack, 0u);
+  bankAwareSlots(s);
   return s;
 }
 

@@ -91,6 +91,16 @@ struct SparseSchedule {
   std::vector<unsigned> progHdr, progBody;  // progBody ends with kSlack x 64 words of slack
   int luRounds = 0, backRounds = 0, fwdRounds = 0;
   int backBody = 0, fwdBody = 0;
+
+  // LDS slot of each factor position (CSR order of F): the positions are
+  // renumbered so that the lanes of a round touch distinct LDS banks
+  // (bankAwareSlots); every program word holds slots.  RHS, zero and trash
+  // slots keep their numbers.
+  std::vector<int> slotOf;
+  // extra LDS passes (bank conflicts, 32 banks of 4 bytes per half
+  // wavefront) summed over the matrix accesses of one pass through the LU
+  // program and the Jacobian rounds, before and after the renumbering
+  long long bankPassesBefore = 0, bankPassesAfter = 0;
 };
 
 // Builds both programs.  Requires at most three reactants per reaction,

@@ -156,6 +156,15 @@ static void check_lu_program(const MechTables& t, const SparseSchedule& s) {
   for (size_t r = 0; r < s.progHdr.size(); ++r) slots[r < size_t(s.luRounds) ? 0 : (r < size_t(s.luRounds + s.backRounds) ? 1 : 2)] += 1 + (s.progHdr[r] & 0xffff);
   std::printf("m %d nnzF %d levels %d back levels %d | rounds lu %d back %d fwd %d | sequential slots lu %d back %d fwd %d | program %.1f KB + hdr %.1f KB\n",
               m, s.nnzF, s.nLevels, s.nBackLevels, s.luRounds, s.backRounds, s.fwdRounds, slots[0], slots[1], slots[2], bodyBytes / 1024.0, hdrBytes / 1024.0);
+  std::printf("bank passes (extra LDS passes per program pass) before %lld after %lld\n", s.bankPassesBefore, s.bankPassesAfter);
+  {  // the renumbering is a permutation of [0, nnzF)
+    std::vector<char> seen(size_t(s.nnzF), 0);
+    for (int p = 0; p < s.nnzF; ++p) {
+      const int q = s.slotOf[size_t(p)];
+      CHECK(q >= 0 && q < s.nnzF && !seen[size_t(q)], "slotOf is not a permutation at %d", p);
+      seen[size_t(q)] = 1;
+    }
+  }
   CHECK((int)s.progHdr.size() == s.luRounds + s.backRounds + s.fwdRounds, "round count");
   std::mt19937_64 rng(7);
   std::uniform_real_distribution<double> U(0.0, 1.0);
@@ -174,7 +183,7 @@ static void check_lu_program(const MechTables& t, const SparseSchedule& s) {
       int j = s.colIdx[p]; double v = 0;
       if (isJ[size_t(i) * m + j]) v = scale * (U(rng) - 0.5);
       if (i == j) v += 1.0 + scale * U(rng);
-      e.A[p] = v; D[size_t(i) * m + j] = v;
+      e.A[size_t(s.slotOf[size_t(p)])] = v; D[size_t(i) * m + j] = v;
     }
     e.A[s.trashSlot] = std::nan("");
     std::vector<double> b(m); std::vector<long double> bl(m);
@@ -363,7 +372,7 @@ static void check_eval_program(const MechTables& t, const SparseSchedule& s) {
       }
     }
     double je = 0;
-    for (int i = 0; i < m; ++i) for (int p = s.rowPtr[i]; p < s.rowPtr[i + 1]; ++p) { int j = s.colIdx[p]; long double sc = scl[size_t(i) * m + j] + (i == j); je = std::max(je, (double)(fabsl(D[size_t(i) * m + j] - A[p]) / (sc + 1e-300))); }
+    for (int i = 0; i < m; ++i) for (int p = s.rowPtr[i]; p < s.rowPtr[i + 1]; ++p) { int j = s.colIdx[p]; long double sc = scl[size_t(i) * m + j] + (i == j); je = std::max(je, (double)(fabsl(D[size_t(i) * m + j] - A[size_t(s.slotOf[size_t(p)])]) / (sc + 1e-300))); }
     std::printf("trial %d: omega rel err %.2e, J rel err %.2e\n", trial, maxe, je);
     CHECK(je < 1e-14, "J err");
   }
