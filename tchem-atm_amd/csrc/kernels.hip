@@ -677,6 +677,50 @@ __device__ __forceinline__ void jtv_round(const unsigned (&w)[1 + SparseSchedule
   L.v[t] = acc;
 }
 
+// Round driver for programs read from device memory, where a round's words
+// take longer to arrive than a round takes to run: the words of round r + 2
+// are loaded while round r runs, into three register sets used in rotation
+// (no copies, so no wait until a set's round comes up), and the headers come
+// through scalar loads.  exec(h, w) runs one round with header h and this
+// lane's words w; size(h) is the round's body length in words.  The loads
+// stop at the last round, so nothing past the program is read.
+template <class Size, class Exec>
+__device__ __forceinline__ void pipelined_rounds(const unsigned* hdr, const unsigned* body, int nr, int lane,
+                                                 Size size, Exec exec) {
+  constexpr int kW = 1 + SparseSchedule::kMaxChain;
+  if (nr <= 0) return;
+  using cptr = const __attribute__((address_space(4))) unsigned*;
+  const cptr sh = (cptr)hdr;
+  unsigned wa[kW], wb[kW], wc[kW];
+  auto load = [&](unsigned (&w)[kW], const unsigned* b) {
+#pragma unroll
+    for (int j = 0; j < kW; ++j) w[j] = b[kWave * j + lane];
+  };
+  unsigned h0 = sh[0];
+  unsigned h1 = nr > 1 ? sh[1] : 0u;
+  const unsigned* b1 = body + size(h0);  // body of round r + 1
+  load(wa, body);
+  if (nr > 1) load(wb, b1);
+  int r = 0;
+  // one round: run round r from `cur`, load round r + 2 into `nxt2`
+  auto step = [&](unsigned (&cur)[kW], unsigned (&nxt2)[kW]) -> bool {
+    const unsigned* b2 = b1 + size(h1);
+    unsigned h2 = 0u;
+    if (r + 2 < nr) {
+      h2 = sh[r + 2];
+      load(nxt2, b2);
+    }
+    exec(h0, cur);
+    if (++r >= nr) return false;
+    h0 = h1;
+    h1 = h2;
+    b1 = b2;
+    return true;
+  };
+  while (step(wa, wc) && step(wb, wa) && step(wc, wb)) {
+  }
+}
+
 // Runs nr chained rounds of the evaluation program starting at header
 // index r0 and body word offset body0: kSeg / kJac rounds, and with ADJ
 // also kNuT / kJtv.  Rounds run in order (a chain split over rounds
@@ -734,12 +778,44 @@ __device__ __forceinline__ void eval_rounds(const SpLds<TA>& L, int lane, int r0
   }
 }
 
+// eval_rounds (kSeg / kJac only) for programs in device memory
+template <typename TA>
+__device__ __forceinline__ void eval_rounds_pf(const SpLds<TA>& L, int lane, int r0, int nr, int body0, double diag,
+                                               double scale) {
+  constexpr int kW = 1 + SparseSchedule::kMaxChain;
+  const TA tdiag = TA(diag), tscale = TA(scale);
+  pipelined_rounds(
+      L.ehdr + r0, L.ebody + body0, nr, lane, [](unsigned h) { return kWave * (1 + int(h & 0xffff)); },
+      [&](unsigned h, const unsigned (&w)[kW]) {
+        const int kind = int(h >> 16), len = int(h & 0xffff);
+        if (kind == SparseSchedule::kSeg) {
+          switch (len) {
+            case 1: seg_round<1>(w, L); break;
+            case 2: seg_round<2>(w, L); break;
+            case 3: seg_round<3>(w, L); break;
+            default: seg_round<4>(w, L); break;
+          }
+        } else {
+          switch (len) {
+            case 1: jac_round<1>(w, L, tdiag, tscale); break;
+            case 2: jac_round<2>(w, L, tdiag, tscale); break;
+            case 3: jac_round<3>(w, L, tdiag, tscale); break;
+            default: jac_round<4>(w, L, tdiag, tscale); break;
+          }
+        }
+      });
+}
+
 // Source-term segment sums (kSeg rounds) and, with jac, A = diag * I +
 // scale * J on the factor pattern (kJac rounds; fill positions cleared).
-template <typename TA>
+template <typename TA, bool PF = false>
 __device__ void sp_sums(const DevSparse& S, const SpLds<TA>& L, int lane, bool jac, double diag, double scale) {
-  eval_rounds<false>(L, lane, S.ropRounds + S.pairRounds, S.segRounds + (jac ? S.jacRounds : 0), S.segBody, diag,
-                     scale);
+  if constexpr (PF)
+    eval_rounds_pf(L, lane, S.ropRounds + S.pairRounds, S.segRounds + (jac ? S.jacRounds : 0), S.segBody, diag,
+                   scale);
+  else
+    eval_rounds<false>(L, lane, S.ropRounds + S.pairRounds, S.segRounds + (jac ? S.jacRounds : 0), S.segBody, diag,
+                       scale);
 }
 
 // per-species sums of the segment sums plus emission and external sources
@@ -892,6 +968,65 @@ __device__ bool run_program(const unsigned* hdr, const unsigned* body, int nRoun
   return ballot(bad) == 0;
 }
 
+// run_program for programs in device memory (pipelined_rounds)
+template <typename TA>
+__device__ bool run_program_pf(const unsigned* hdr, const unsigned* body, int nRounds, TA* A, TA* rdiag, int m,
+                               int rhs0, int trash, int lane) {
+  constexpr int kW = 1 + SparseSchedule::kMaxChain;
+  bool bad = false;
+  pipelined_rounds(
+      hdr, body, nRounds, lane,
+      [](unsigned h) { return int(h >> 16) == SparseSchedule::kMul ? kWave : kWave * (1 + int(h & 0xffff)); },
+      [&](unsigned h, const unsigned (&w)[kW]) {
+        const int kind = int(h >> 16), len = int(h & 0xffff);
+        if (kind == SparseSchedule::kMul) {
+          const int dst = int(w[0] & 0xffff), src = int(w[0] >> 16);
+          const bool piv = (dst & 0x8000) != 0;
+          const int ai = piv ? trash : dst;
+          const int ri = piv ? (dst & 0x7fff) : m + 1;
+          const TA p = A[src];
+          const TA a = A[ai];
+          const TA inv = TA(1) / p;
+          A[ai] = a * inv;
+          rdiag[ri] = inv;
+          bad |= piv && !(p != TA(0) && isfinite(p) && isfinite(inv));
+        } else if (kind == SparseSchedule::kUpd) {
+          switch (len) {
+            case 1: upd_round<1>(w, A); break;
+            case 2: upd_round<2>(w, A); break;
+            case 3: upd_round<3>(w, A); break;
+            default: upd_round<4>(w, A); break;
+          }
+        } else if (kind == SparseSchedule::kMulUpd) {
+          switch (len) {
+            case 2: bad |= mulupd_round<0>(w, A, rdiag, m); break;
+            case 3: bad |= mulupd_round<1>(w, A, rdiag, m); break;
+            default: bad |= mulupd_round<2>(w, A, rdiag, m); break;
+          }
+        } else {
+          switch (len) {
+            case 1: back_round<1>(w, A, rdiag, rhs0); break;
+            case 2: back_round<2>(w, A, rdiag, rhs0); break;
+            case 3: back_round<3>(w, A, rdiag, rhs0); break;
+            default: back_round<4>(w, A, rdiag, rhs0); break;
+          }
+        }
+        wave_sync();
+      });
+  return ballot(bad) == 0;
+}
+
+// the LU / substitution interpreter for programs staged in LDS or read from
+// device memory
+template <bool STAGED, typename TA>
+__device__ __forceinline__ bool program(const unsigned* hdr, const unsigned* body, int nRounds, TA* A, TA* rdiag,
+                                        int m, int rhs0, int trash, int lane) {
+  if constexpr (STAGED)
+    return run_program(hdr, body, nRounds, A, rdiag, m, rhs0, trash, lane);
+  else
+    return run_program_pf(hdr, body, nRounds, A, rdiag, m, rhs0, trash, lane);
+}
+
 // ---- backends -----------------------------------------------------------------
 // Both provide: Lds (per-cell LDS view), Fact (state of the stored factors),
 // carve, init, evaluate (f at u; for a Newton iteration also the residual
@@ -1003,7 +1138,7 @@ struct SparseBackend {
     wave_sync();
     TCHEM_PT_END(rates, 11)
     TCHEM_PT_BEGIN(sums)
-    sp_sums(a.S, L, lane, refresh, 1.0, -ch);
+    sp_sums<TA, !STAGED>(a.S, L, lane, refresh, 1.0, -ch);
     wave_sync();
     TCHEM_PT_END(sums, 12)
     TCHEM_PT_BEGIN(rows)
@@ -1027,14 +1162,14 @@ struct SparseBackend {
   }
   static __device__ void back(const IntegratorArgs& a, const Lds& L, int lane) {
     TCHEM_PT_BEGIN(back)
-    run_program(L.hdr + a.S.luRounds, L.body + a.S.backBody, a.S.backRounds, L.A, L.rdiag, a.M.m, a.S.nnzF,
-                a.S.trashSlot, lane);
+    program<STAGED>(L.hdr + a.S.luRounds, L.body + a.S.backBody, a.S.backRounds, L.A, L.rdiag, a.M.m, a.S.nnzF,
+                    a.S.trashSlot, lane);
     TCHEM_PT_END(back, 10)
   }
   static __device__ __forceinline__ bool factor_solve(const IntegratorArgs& a, const Lds& L, int lane, TA (&x)[QM],
                                                       Fact&) {
     TCHEM_PT_BEGIN(lu)
-    const bool ok = run_program(L.hdr, L.body, a.S.luRounds, L.A, L.rdiag, a.M.m, a.S.nnzF, a.S.trashSlot, lane);
+    const bool ok = program<STAGED>(L.hdr, L.body, a.S.luRounds, L.A, L.rdiag, a.M.m, a.S.nnzF, a.S.trashSlot, lane);
     TCHEM_PT_END(lu, 9)
     if (!ok) return false;
     back(a, L, lane);
@@ -1045,7 +1180,7 @@ struct SparseBackend {
                                                const Fact&) {
     const unsigned* fh = reinterpret_cast<const unsigned*>(a.M.ib + a.S.oProgHdr) + a.S.luRounds + a.S.backRounds;
     const unsigned* fb = reinterpret_cast<const unsigned*>(a.M.ib + a.S.oProgBody) + a.S.fwdBody;
-    run_program(fh, fb, a.S.fwdRounds, L.A, L.rdiag, a.M.m, a.S.nnzF, a.S.trashSlot, lane);
+    program<false>(fh, fb, a.S.fwdRounds, L.A, L.rdiag, a.M.m, a.S.nnzF, a.S.trashSlot, lane);  // always in memory
     back(a, L, lane);
     get_x(a, L, lane, x);
   }
