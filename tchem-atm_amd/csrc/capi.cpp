@@ -222,14 +222,15 @@ int resolveOptions(const tchem_atmchem_options_t* opts, const tchem_kmcd_s* k, t
   // least 30 % of the m^2 entries in the fill-reducing order) and large
   // enough to fill its 256 x 256 tiles (m >= 128)
   if (o->lu == TCHEM_LU_AUTO && o->precision == TCHEM_PRECISION_MIXED && o->cells_per_block <= 1 &&
-      k->dev.m >= 128 && k->dev.m <= 255 && mfmaCellLdsBytes(k->dev) <= k->maxLds &&
+      k->dev.m >= 128 && k->dev.m <= 255 && k->dev.oMfjTermW != 0 && mfmaCellLdsBytes(k->dev) <= k->maxLds &&
       (!k->sparse.enabled || double(k->sparse.nnzF) >= 0.3 * double(k->dev.m) * double(k->dev.m)))
     o->lu = TCHEM_LU_MFMA;
   if (o->lu == TCHEM_LU_MFMA) {
     if (o->precision != TCHEM_PRECISION_MIXED)
       return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": the MFMA LU form is mixed precision (FP32 factors)");
-    if (k->dev.m > 255)
-      return setError(TCHEM_ERR_UNSUPPORTED, std::string(who) + ": the MFMA LU form takes at most 255 species");
+    if (k->dev.m > 255 || k->dev.oMfjTermW == 0)
+      return setError(TCHEM_ERR_UNSUPPORTED,
+                      std::string(who) + ": the MFMA LU form takes at most 255 species and 16383 derivative pairs");
     if (o->cells_per_block > 1)
       return setError(TCHEM_ERR_INVALID, std::string(who) + ": the MFMA LU form runs one cell per workgroup");
     o->cells_per_block = 1;
@@ -575,30 +576,56 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out) {
   d.oSerJPtr = addI(t.serJPtr);
   d.oSerJPair = addI(t.serJPair);
   d.oSerJCoef = addD(t.serJCoef);
-  if (t.m <= 255) {  // the MFMA form's assembly lists: entries by column quarter, FP32 coefficients
-    std::vector<int> quarter(5, 0), dest, ptr(1, 0), pair, coef;
+  d.oMfjQuarter = d.oMfjTermW = d.oMfjTermC = 0;
+  if (t.m <= 255 && t.nPairs() < (1 << 14)) {
+    // the MFMA form's assembly lists: the structural entries of each column
+    // quarter dealt to the 256 threads by term count (largest first, to the
+    // least loaded thread), each thread's terms in entry order, padded to the
+    // quarter's longest run, stored [term row][thread]
+    constexpr int kThreads = 256;
+    std::vector<int> quarter(5, 0);
+    std::vector<int> termW, termC;
     for (int q = 0; q < 4; ++q) {
-      quarter[q] = int(dest.size());
-      for (size_t n = 0; n < t.serJRow.size(); ++n) {
-        const int row = t.serJRow[n], col = t.serJCol[n];
-        if ((col >> 6) != q) continue;
-        dest.push_back(row * 65 + (col & 63) + (row == col ? (1 << 30) : 0));
-        for (int e = t.serJPtr[n]; e < t.serJPtr[n + 1]; ++e) {
-          pair.push_back(t.serJPair[e]);
-          const float c = float(t.serJCoef[e]);
-          int bits;
-          std::memcpy(&bits, &c, sizeof(bits));
-          coef.push_back(bits);
+      quarter[size_t(q)] = int(termW.size() / kThreads);
+      std::vector<size_t> ent;
+      for (size_t n = 0; n < t.serJRow.size(); ++n)
+        if ((t.serJCol[n] >> 6) == q) ent.push_back(n);
+      std::stable_sort(ent.begin(), ent.end(), [&](size_t x, size_t y) {
+        return t.serJPtr[x + 1] - t.serJPtr[x] > t.serJPtr[y + 1] - t.serJPtr[y];
+      });
+      std::vector<std::vector<size_t>> own(kThreads);
+      std::vector<int> load(kThreads, 0);
+      for (size_t n : ent) {
+        const int th = int(std::min_element(load.begin(), load.end()) - load.begin());
+        own[size_t(th)].push_back(n);
+        load[size_t(th)] += std::max(1, t.serJPtr[n + 1] - t.serJPtr[n]);
+      }
+      const int rows = *std::max_element(load.begin(), load.end());
+      const size_t base = termW.size();
+      termW.resize(base + size_t(rows) * kThreads, 0);
+      termC.resize(base + size_t(rows) * kThreads, 0);
+      for (int th = 0; th < kThreads; ++th) {
+        int j = 0;
+        for (size_t n : own[size_t(th)]) {
+          const int row = t.serJRow[n], col = t.serJCol[n];
+          const unsigned dest = unsigned(row * 65 + (col & 63));
+          const int e0 = t.serJPtr[n], e1 = t.serJPtr[n + 1];
+          for (int e = e0; e < std::max(e1, e0 + 1); ++e, ++j) {
+            unsigned w = e < e1 ? unsigned(t.serJPair[size_t(e)]) : 0u;
+            float c = e < e1 ? float(t.serJCoef[size_t(e)]) : 0.0f;
+            if (e + 1 >= e1) w |= (dest << 14) | (row == col ? (1u << 29) : 0u) | (1u << 31);
+            int bits;
+            std::memcpy(&bits, &c, sizeof(bits));
+            termW[base + size_t(j) * kThreads + size_t(th)] = int(w);
+            termC[base + size_t(j) * kThreads + size_t(th)] = bits;
+          }
         }
-        ptr.push_back(int(pair.size()));
       }
     }
-    quarter[4] = int(dest.size());
+    quarter[4] = int(termW.size() / kThreads);
     d.oMfjQuarter = addI(quarter);
-    d.oMfjDest = addI(dest);
-    d.oMfjPtr = addI(ptr);
-    d.oMfjPair = addI(pair);
-    d.oMfjCoef = addI(coef);
+    d.oMfjTermW = addI(termW);
+    d.oMfjTermC = addI(termC);
   }
   // sparse schedules (mechanisms they cannot express run the dense form only)
   auto words = [](const std::vector<unsigned>& v) { return std::vector<int>(v.begin(), v.end()); };

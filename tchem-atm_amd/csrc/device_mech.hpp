@@ -21,9 +21,11 @@ struct DevMech {
       oPairPtr, oPairSlot, oJnzRowCol, oJnzPtr, oJcPair, oSerOmegaPtr, oSerOmegaReac, oSerJRowCol, oSerJPtr,
       oSerJPair;
   int oRatePar, oOmegaCoef, oEmission, oJcCoef, oSerOmegaCoef, oSerJCoef;
-  // the serial Jacobian lists regrouped for the MFMA form's assembly by
-  // column quarters (m <= 255 only; offsets 0 otherwise)
-  int oMfjQuarter, oMfjDest, oMfjPtr, oMfjPair, oMfjCoef;
+  // the serial Jacobian lists regrouped for the MFMA form's assembly
+  // (m <= 255 only; offsets 0 otherwise): per column quarter, each of the
+  // 256 threads of the workgroup walks its own run of terms, stored
+  // [term row][thread] so that a row is one coalesced load
+  int oMfjQuarter, oMfjTermW, oMfjTermC;
 
   __host__ __device__ const int* perm() const { return ib + oPerm; }             // [m] internal -> reference species
   __host__ __device__ const int* rateKind() const { return ib + oRateKind; }     // [nReac]
@@ -52,12 +54,13 @@ struct DevMech {
   __host__ __device__ const int* serJPtr() const { return ib + oSerJPtr; }           // [nSerJ+1]
   __host__ __device__ const int* serJPair() const { return ib + oSerJPair; }         // [nSerContrib]
   __host__ __device__ const double* serJCoef() const { return db + oSerJCoef; }
-  __host__ __device__ const int* mfjQuarter() const { return ib + oMfjQuarter; }  // [5] entry ranges by col / 64
-  __host__ __device__ const int* mfjDest() const { return ib + oMfjDest; }        // [nSerJ] row * 65 + col % 64, bit 30: diagonal
-  __host__ __device__ const int* mfjPtr() const { return ib + oMfjPtr; }          // [nSerJ + 1]
-  __host__ __device__ const int* mfjPair() const { return ib + oMfjPair; }        // [nSerContrib]
-  __host__ __device__ const float* mfjCoef() const {                              // [nSerContrib] FP32 bits
-    return reinterpret_cast<const float*>(ib + oMfjCoef);
+  __host__ __device__ const int* mfjQuarter() const { return ib + oMfjQuarter; }  // [5] first term row of each quarter
+  // [rows][256] term words: derivative pair (bits 0-13), stage index of the
+  // entry (bits 14-28, on its last term), diagonal entry (bit 29), last
+  // term of an entry (bit 31); padding terms are 0 with coefficient 0
+  __host__ __device__ const unsigned* mfjTermW() const { return reinterpret_cast<const unsigned*>(ib + oMfjTermW); }
+  __host__ __device__ const float* mfjTermC() const {  // [rows][256] FP32 coefficients
+    return reinterpret_cast<const float*>(ib + oMfjTermC);
   }
 };
 

@@ -1635,13 +1635,18 @@ __device__ __forceinline__ void mfma_collective(const IntegratorArgs& a, const t
   for (int q = 0; q < 4; ++q) {  // unrolled: the tiles of quarter q are known at compile time
     for (int idx = tid; idx < int(kStageWords / 4); idx += 256) stage4[idx] = float4{0.0f, 0.0f, 0.0f, 0.0f};
     __syncthreads();
-    const int e1 = M.mfjQuarter()[q + 1];
-    for (int n = M.mfjQuarter()[q] + tid; n < e1; n += 256) {
-      const int d = M.mfjDest()[n];
-      const int p1 = M.mfjPtr()[n + 1];
+    {  // this thread's run of terms: coalesced, independent loads
+      const int j1 = M.mfjQuarter()[q + 1];
       float sum = 0.0f;
-      for (int e = M.mfjPtr()[n]; e < p1; ++e) sum = fmaf(M.mfjCoef()[e], L.pairs[M.mfjPair()[e]], sum);
-      L.lu.stage[d & 0x3fffffff] = ((d >> 30) ? 1.0f : 0.0f) + scale * sum;
+#pragma unroll 4
+      for (int j = M.mfjQuarter()[q]; j < j1; ++j) {
+        const unsigned w = M.mfjTermW()[size_t(j) * 256 + tid];
+        sum = fmaf(M.mfjTermC()[size_t(j) * 256 + tid], L.pairs[w & 0x3fff], sum);
+        if (w >> 31) {
+          L.lu.stage[(w >> 14) & 0x7fff] = (((w >> 29) & 1u) ? 1.0f : 0.0f) + scale * sum;
+          sum = 0.0f;
+        }
+      }
     }
     for (int p = m + tid; p < kN - 1; p += 256)
       if ((p >> 6) == q) L.lu.stage[p * 65 + (p - 64 * q)] = 1.0f;
