@@ -805,10 +805,11 @@ def main():
         log(f"fp64 same launch: {v64:.1f} cell-hours/s; mixed/fp64 {value / v64:.2f}; mixed vs fp64 max "
             f"{e12.max():.2e} (floor 1e-12) {e8.max():.2e} (floor 1e-8), median {np.median(e12):.1e}")
         state64 = run64.state.cpu().numpy()
+        acc64 = run64.acc_stats.cpu().numpy()
         del run64
         torch.cuda.empty_cache()
     else:
-        state64, fail64 = None, None
+        state64, fail64, acc64 = None, None, None
 
     # ---- plain FP64: dense LU, one wavefront per cell, host interval loop --------
     # The dense FP64 form computes in the oracle's arithmetic (bit for bit,
@@ -830,30 +831,46 @@ def main():
         wallp = time.perf_counter() - t0
         vp = wp.cell_hours / wallp
         oracle_rows = runp.state.cpu().numpy()[:, 3:3 + m]
-        ofail = runp.acc_stats[:, 5].cpu().numpy() > 0
+        pacc = runp.acc_stats.cpu().numpy()
+        ofail = pacc[:, 5] > 0
         config.update({"fp64_plain_value": vp, "fp64_plain_cells": int(idx.size),
                        "fp64_plain_sample": f"every {stride}th cell", "fp64_plain_launches": launches,
                        "mixed_over_fp64_plain": value / vp})
         log(f"fp64-plain on {idx.size} cells: {vp:.1f} cell-hours/s ({launches} interval launches, {wallp:.1f} s)")
 
-        def against_oracle(rows, fail):
+        def against_oracle(rows, fail, stats):
             ok = ~ofail & ~fail[idx]
             a, b = rows[idx][ok][:, 3:3 + m], oracle_rows[ok]
             e12, e8 = rel_err(a, b, 1e-12), rel_err(a, b, 1e-8)
             tn = np.abs(a - b) / (w.atol_time + w.rtol_time * np.abs(b))
+            # the 1e-4 bar of the metric, on every cell and on the cells whose
+            # accept / reject sequence is the oracle's (elsewhere the step
+            # sequences part and the cells differ within the integrator's
+            # global error, as two FP64 forms do)
+            same = (stats[idx, 0] == pacc[:, 0]) & (stats[idx, 1] == pacc[:, 1])
+            e8s = e8.max(axis=1)[same[ok]]
             return {"cells": int(ok.sum()), "failed_sets_equal": bool(np.array_equal(fail[idx], ofail)),
                     "max_floor_1e-8": float(e8.max()), "max_floor_1e-12": float(e12.max()),
                     "median_floor_1e-12": float(np.median(e12)), "q99_floor_1e-12": float(np.quantile(e12, 0.99)),
                     "tolerance_normalised_max": float(tn.max()),
-                    "cells_over_1e-9_floor_1e-12": int(np.sum(e12.max(axis=1) > 1e-9))}
+                    "cells_over_1e-9_floor_1e-12": int(np.sum(e12.max(axis=1) > 1e-9)),
+                    "bar_1e-4_floor_1e-8": {
+                        "met_on_all_cells": bool(e8.max() <= 1e-4),
+                        "cells_over_bar": int(np.sum(e8.max(axis=1) > 1e-4)),
+                        "cells_with_the_oracle_step_sequence": int(e8s.size),
+                        "max_on_those": float(e8s.max()) if e8s.size else None,
+                        "met_on_those": bool(e8s.size == 0 or e8s.max() <= 1e-4)}}
 
-        acc_err = {form: against_oracle(state_main, failed_mask)}
+        acc_err = {form: against_oracle(state_main, failed_mask, acc)}
         if state64 is not None:
-            acc_err["fp64"] = against_oracle(state64, fail64)
+            acc_err["fp64"] = against_oracle(state64, fail64, acc64)
         config["error_vs_fp64_oracle_on_plain_sample"] = acc_err
         log("vs FP64 oracle on the plain sample: " + "; ".join(
             f"{k}: max {v['max_floor_1e-8']:.2e} (floor 1e-8) {v['max_floor_1e-12']:.2e} (floor 1e-12), "
-            f"tolerance-normalised {v['tolerance_normalised_max']:.3f}" for k, v in acc_err.items()))
+            f"tolerance-normalised {v['tolerance_normalised_max']:.3f}; 1e-4 bar met on all cells "
+            f"{v['bar_1e-4_floor_1e-8']['met_on_all_cells']} ({v['bar_1e-4_floor_1e-8']['cells_over_bar']} over), on the "
+            f"{v['bar_1e-4_floor_1e-8']['cells_with_the_oracle_step_sequence']} cells with the oracle's step sequence "
+            f"{v['bar_1e-4_floor_1e-8']['met_on_those']}" for k, v in acc_err.items()))
         del runp
         if args.graph_cells > 0 and left() > 90:
             gidx = np.arange(0, n, max(1, n // args.graph_cells))[: args.graph_cells]
@@ -953,6 +970,11 @@ def main():
     }
     if cpu is not None:
         out["cpu_baseline"] = cpu
+    bar = config.get("error_vs_fp64_oracle_on_plain_sample", {}).get(form, {}).get("bar_1e-4_floor_1e-8")
+    if bar is not None:
+        # the metric's accuracy bar (1e-4 relative, 1e-8 ppm floor) as measured on
+        # the plain sample, stated beside the value rather than assumed
+        out["accuracy_bar_1e-4"] = bar
     out["bench_wall_s"] = time.perf_counter() - T0
     print(json.dumps(out), flush=True)
 
