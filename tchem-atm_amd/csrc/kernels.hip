@@ -852,6 +852,17 @@ __device__ __forceinline__ void upd_round(const unsigned (&w)[1 + SparseSchedule
   for (int j = 0; j < LEN; ++j) acc -= l[j] * u[j];
   A[w[0]] = acc;
 }
+// Reciprocal of a pivot: the correctly rounded quotient in FP64; in FP32 the
+// hardware reciprocal (within 1 ulp: one instruction instead of the eleven of
+// a correctly rounded division, well inside the FP32 factorisation's own
+// rounding).  Zero and non-finite pivots give non-finite reciprocals in both.
+template <typename TA>
+__device__ __forceinline__ TA pivot_recip(TA p) {
+  if constexpr (sizeof(TA) == sizeof(float))
+    return __builtin_amdgcn_rcpf(p);
+  else
+    return TA(1) / p;
+}
 // A kMulUpd round with LEN chain entries per lane (w[3..3+LEN)); returns
 // true for a zero or non-finite pivot on a lane that stores one.
 template <int LEN, typename TA>
@@ -871,7 +882,7 @@ __device__ __forceinline__ bool mulupd_round(const unsigned (&w)[1 + SparseSched
     lo[j] = A[w[3 + j] & 0xffff];
     hi[j] = A[w[3 + j] >> 16];
   }
-  const TA inv = TA(1) / p;
+  const TA inv = pivot_recip(p);
   const TA l = a == TA(0) ? TA(0) : a * inv;
 #pragma unroll
   for (int j = 0; j < LEN; ++j) acc -= lo[j] * hi[j];
@@ -932,7 +943,7 @@ __device__ bool run_program(const unsigned* hdr, const unsigned* body, int nRoun
       const int ri = piv ? (dst & 0x7fff) : m + 1;
       const TA p = A[src];
       const TA a = A[ai];
-      const TA inv = TA(1) / p;
+      const TA inv = pivot_recip(p);
       A[ai] = a * inv;
       rdiag[ri] = inv;
       bad |= piv && !(p != TA(0) && isfinite(p) && isfinite(inv));
@@ -986,7 +997,7 @@ __device__ bool run_program_pf(const unsigned* hdr, const unsigned* body, int nR
           const int ri = piv ? (dst & 0x7fff) : m + 1;
           const TA p = A[src];
           const TA a = A[ai];
-          const TA inv = TA(1) / p;
+          const TA inv = pivot_recip(p);
           A[ai] = a * inv;
           rdiag[ri] = inv;
           bad |= piv && !(p != TA(0) && isfinite(p) && isfinite(inv));
