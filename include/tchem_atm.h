@@ -65,7 +65,8 @@ typedef struct {
                              per workgroup of 4 wavefronts; mixed precision, m <= 255 */
 #define TCHEM_PROGRAMS_AUTO 0   /* LDS when they leave room for 4 cells, device memory otherwise */
 #define TCHEM_PROGRAMS_LDS 1    /* staged into LDS once per workgroup, shared by its cells */
-#define TCHEM_PROGRAMS_MEMORY 2 /* read from device memory (L2-resident); all LDS goes to cells */
+#define TCHEM_PROGRAMS_MEMORY 2 /* read from device memory (L2-resident); the LDS holds the cells and the
+                                   coefficient table */
 typedef struct {
   int precision;         /* TCHEM_PRECISION_* */
   int schedule;          /* TCHEM_SCHEDULE_* */

@@ -251,13 +251,14 @@ int resolveOptions(const tchem_atmchem_options_t* opts, const tchem_kmcd_s* k, t
   if (o->programs == TCHEM_PROGRAMS_AUTO) {
     const int cap = sparseMaxCellsPerGroup(o->precision, k->dev.m);
     const size_t prog = sparseProgramBytes(k->sparse);
-    const size_t inMem = std::min<size_t>(cap, cellB ? maxLds / cellB : 0);
+    const size_t coefB = sparseCoefBytes(k->sparse);
+    const size_t inMem = coefB < maxLds ? std::min<size_t>(cap, cellB ? (maxLds - coefB) / cellB : 0) : 0;
     const size_t inLds = prog < maxLds ? std::min<size_t>(cap, cellB ? (maxLds - prog) / cellB : 0) : 0;
     o->programs = (inLds >= 1 && inLds >= inMem) ? TCHEM_PROGRAMS_LDS : TCHEM_PROGRAMS_MEMORY;
   }
   const bool staged = o->programs == TCHEM_PROGRAMS_LDS;
   auto sparseFits = [&](int cells) {
-    return (staged ? sparseProgramBytes(k->sparse) : 0) + size_t(cells) * cellB <= maxLds;
+    return (staged ? sparseProgramBytes(k->sparse) : sparseCoefBytes(k->sparse)) + size_t(cells) * cellB <= maxLds;
   };
   if (o->lu == TCHEM_LU_AUTO)
     o->lu = (k->sparse.enabled && sparseFits(1)) ? TCHEM_LU_SPARSE : TCHEM_LU_DENSE;
@@ -293,7 +294,7 @@ size_t ldsBytes(const tchem_kmcd_s* k, const tchem_atmchem_options_t& o) {
   if (o.lu == TCHEM_LU_MFMA) return mfmaCellLdsBytes(k->dev);
   const size_t bytesA = o.precision == TCHEM_PRECISION_FP64 ? sizeof(double) : sizeof(float);
   if (o.lu == TCHEM_LU_SPARSE)
-    return (sparseStaged(o) ? sparseProgramBytes(k->sparse) : 0) +
+    return (sparseStaged(o) ? sparseProgramBytes(k->sparse) : sparseCoefBytes(k->sparse)) +
            size_t(o.cells_per_block) * sparseCellBytes(k->dev, k->sparse, bytesA);
   return integratorLdsBytes(k->dev, bytesA);
 }

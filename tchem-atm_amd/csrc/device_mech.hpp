@@ -100,6 +100,12 @@ __host__ __device__ inline size_t sparseCoefOffsetWords(const DevSparse& S) {
 __host__ __device__ inline size_t sparseProgramBytes(const DevSparse& S) {
   return 4 * sparseCoefOffsetWords(S) + ((8 * size_t(S.nCoef) + 15) & ~size_t(15));
 }
+// With the programs in device memory only the coefficient table is staged:
+// it sits at the start of the workgroup's LDS, before the cell regions (its
+// entries are read on the critical path of every segment and Jacobian round).
+__host__ __device__ inline size_t sparseCoefBytes(const DevSparse& S) {
+  return (8 * size_t(S.nCoef) + 15) & ~size_t(15);
+}
 
 // Forward record of the discrete adjoint: accepted step sizes and the state
 // every kAdjointSegment accepted steps (checkpoint j = state after 16 j

@@ -524,8 +524,9 @@ struct SpLds {
 
 // STAGED: programs at the start of the workgroup's LDS, then one cell region
 // per wavefront; otherwise the programs are read from device memory (they
-// stay resident in L2) and the cell regions start at 0.
-template <typename TA, bool STAGED>
+// stay resident in L2) and the cell regions start at 0, or, with COEF, after
+// the coefficient table staged at the start of the LDS (sparseCoefBytes).
+template <typename TA, bool STAGED, bool COEF = false>
 __device__ SpLds<TA> carve_sparse(char* base, const DevMech& M, const DevSparse& S, int cellBytes, int wave) {
   auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
   SpLds<TA> L;
@@ -543,8 +544,8 @@ __device__ SpLds<TA> carve_sparse(char* base, const DevMech& M, const DevSparse&
     L.body = reinterpret_cast<const unsigned*>(M.ib + S.oProgBody);
     L.ehdr = reinterpret_cast<const unsigned*>(M.ib + S.oEvalHdr);
     L.ebody = reinterpret_cast<const unsigned*>(M.ib + S.oEvalBody);
-    L.coef = M.db + S.oCoef;
-    cell = base + size_t(wave) * size_t(cellBytes);
+    L.coef = COEF ? reinterpret_cast<const double*>(base) : M.db + S.oCoef;
+    cell = base + (COEF ? sparseCoefBytes(S) : 0) + size_t(wave) * size_t(cellBytes);
   }
   size_t off = 0;
   L.A = reinterpret_cast<TA*>(cell);
@@ -1101,10 +1102,16 @@ struct SparseBackend {
     L.src = a.ext ? a.ext + size_t(cell) * a.M.m : nullptr;
   }
   // the mechanism program (LU and BACK sections) is copied into the
-  // workgroup's LDS once; every wavefront of the workgroup reads it
+  // workgroup's LDS once; every wavefront of the workgroup reads it.  With
+  // the programs in device memory the coefficient table alone is copied.
   static __device__ void stage(const IntegratorArgs& a, char* base) {
-    if (!STAGED) return;
     const DevSparse& S = a.S;
+    if (!STAGED) {
+      double* cd = reinterpret_cast<double*>(base);
+      for (int i = threadIdx.x; i < S.nCoef; i += blockDim.x) cd[i] = a.M.db[S.oCoef + i];
+      __syncthreads();
+      return;
+    }
     unsigned* dst = reinterpret_cast<unsigned*>(base);
     auto copy = [&](size_t at, const int* src, int n) {
       for (int i = threadIdx.x; i < n; i += blockDim.x) dst[at + size_t(i)] = unsigned(src[i]);
@@ -1119,7 +1126,7 @@ struct SparseBackend {
     __syncthreads();
   }
   static __device__ Lds carve(char* base, const IntegratorArgs& a, int wave) {
-    return carve_sparse<TA, STAGED>(base, a.M, a.S, a.cellBytes, wave);
+    return carve_sparse<TA, STAGED, !STAGED>(base, a.M, a.S, a.cellBytes, wave);
   }
   // zero slots read by padded records
   static __device__ void init(const IntegratorArgs& a, const Lds& L, int lane) {
