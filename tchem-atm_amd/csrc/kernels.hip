@@ -564,15 +564,20 @@ __device__ SpLds<TA> carve_sparse(char* base, const DevMech& M, const DevSparse&
 }
 
 // x^n for small n >= 0 by repeated multiplication (exact for n <= 2; at
-// most n-1 roundings above)
+// most n-1 roundings above); powers up to 3 without a loop (selects, the
+// same products in the same order)
 __device__ __forceinline__ double ipow(double x, int n) {
   double r = n >= 1 ? x : 1.0;
-  for (int i = 1; i < n; ++i) r *= x;
+  r = n >= 2 ? r * x : r;
+  r = n >= 3 ? r * x : r;
+  for (int i = 3; i < n; ++i) r *= x;
   return r;
 }
-// packed reactant field: species | power << 12, kNone for no reactant
+// packed reactant field: species | power << 12, kNone for no reactant (a
+// factor 1: power 0 of an in-range slot, no branch)
 __device__ __forceinline__ double reactant_factor(const double* c, unsigned f) {
-  return f == SparseSchedule::kNone ? 1.0 : ipow(c[f & 0xfff], int(f >> 12));
+  const bool none = f == SparseSchedule::kNone;
+  return ipow(c[none ? 0u : (f & 0xfff)], none ? 0 : int(f >> 12));
 }
 
 // Rates of progress (kRop rounds) and, with pairs, the derivatives
