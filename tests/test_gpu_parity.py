@@ -433,6 +433,29 @@ def test_autotuned_launch_shape_gives_identical_results(fp64_vs_oracle, cb05, gp
     np.testing.assert_array_equal(run.state.cpu().numpy(), g_state)
 
 
+def test_mixed_program_placement_is_bitwise_identical(fp64_vs_oracle, gpu, pkg):
+    """Mixed sparse form: programs and coefficient table staged in LDS, or
+    programs read from device memory with the coefficient table staged
+    alone, at 4 and 16 cells per workgroup: the same per-cell arithmetic
+    (FP32 pivot reciprocals included), identical bits."""
+    w, run, *_ = fp64_vs_oracle
+    if run.lu != pkg.LU_SPARSE:
+        pytest.skip("program placement applies to the sparse form")
+    out = {}
+    for code, cpb in ((pkg.PROGRAMS_LDS, 4), (pkg.PROGRAMS_MEMORY, 4), (pkg.PROGRAMS_MEMORY, 16)):
+        print("case: config0 seed 0, 64 cells, mixed sparse, programs", code, "cells per block", cpb, flush=True)
+        run.reset()
+        run.run_intervals(precision=pkg.PRECISION_MIXED, schedule=pkg.SCHEDULE_GRID, lu=run.lu, cells_per_block=cpb,
+                          programs=code)
+        gpu.cuda.synchronize()
+        out[(code, cpb)] = (run.state.cpu().numpy(), run.t.cpu().numpy())
+    ref = out[(pkg.PROGRAMS_LDS, 4)]
+    assert np.all(ref[1] == w.tend)
+    for k, (s, t) in out.items():
+        assert np.array_equal(s, ref[0]), k
+        assert np.array_equal(t, ref[1]), k
+
+
 def test_mixed_precision_error_against_fp64_oracle(fp64_vs_oracle, cb05, gpu, pkg):
     """FP32 iteration matrix and LU, FP64 state/residual/error/step control,
     against the FP64 oracle (64 config-0 cells, 3600 s): the relative error
