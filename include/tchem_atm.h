@@ -123,7 +123,9 @@ int tchem_kmcd_create(tchem_kmd_t kmd, tchem_kmcd_t* out);
 int tchem_kmcd_destroy(tchem_kmcd_t kmcd);
 
 /* Replaces AtmosphericChemistry::getWorkSpaceSize (src/core/TChem_AtmosphericChemistry.hpp:47-81):
- * LDS bytes one cell uses in the form opts selects (NULL: defaults). */
+ * LDS bytes of one workgroup in the form opts selects (NULL: defaults): its cells_per_block cell regions plus
+ * what the workgroup stages once for its cells (the sparse form's programs, or with the programs in device
+ * memory its coefficient table). */
 int tchem_atmchem_workspace_bytes(tchem_kmcd_t kmcd, const tchem_atmchem_options_t* opts, size_t* bytes);
 
 /* ---- batched kernels ------------------------------------------------------ */
