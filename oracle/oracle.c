@@ -2,7 +2,8 @@
  *
  * Each function cites the reference code it restates.  Parity status of each
  * part is stated in the header comment of the function; pins live in
- * tests/test_oracle_*.py.
+ * tests/test_oracle.py, tests/test_oracle_adjoint.py and
+ * tests/test_oracle_aerosol.py.
  */
 #include "oracle.h"
 
@@ -30,7 +31,9 @@ static double pow_int(double x, int n) { return tchem_ipow(x, n); }
  * Same expression order as the reference; photolysis overrides replace A of
  * the listed Arrhenius reactions (the per-cell kmcd "model variation" form,
  * src/core/TChem_AtmosphericChemistry.cpp:79).
- * Pinned by: closed forms at fixed T, p (tests/test_oracle_rates.py). */
+ * Pinned by: the CAMP analytic solutions' rate constants at fixed T, p and
+ * the reference's CB05 goldens (tests/test_oracle.py:
+ * test_rate_constants_against_camp_analytic_solution, test_cb05cl_ae5_golden). */
 void oracle_kforward(const oracle_kmd* k, double t, double p, const double* photo, double* kfor) {
   const double t_1 = 1.0 / t;
   const double one = 1.0;
@@ -101,8 +104,10 @@ void oracle_omega(const oracle_kmd* k, const double* kfor, const double* c, doub
 /* Analytic Jacobian of oracle_omega with respect to the active species.
  * The reference differentiates the same function by forward differences or
  * Sacado forward AD (src/core/problems/TChem_Impl_AtmosphericChemistry_Problem.hpp:166-241).
- * Pinned by: central differences and complex-step-free brute force in
- * tests/test_oracle_jacobian.py. */
+ * Pinned by: central differences on CB05 and a long-double brute force on
+ * small hand-written mechanisms (tests/test_oracle.py:
+ * test_analytic_jacobian_against_central_differences,
+ * test_source_term_and_jacobian_by_brute_force). */
 void oracle_jacobian(const oracle_kmd* k, const double* kfor, const double* c, double* J) {
   const int m = k->nSpec - k->nConstSpec;
   const int ms = k->maxSpecInReac;
