@@ -175,7 +175,7 @@ def test_fp64_integration_matches_oracle(fp64_vs_oracle, cb05, pkg):
 
     Dense form (the plain FP64 form): the same bits as the oracle in every
     cell — state, t_out, dt_out and every step / Newton counter — so the
-    north-star bar of 1e-9 relative per species holds with room to spare.
+    FP64 bar of 1e-9 relative per species holds with room to spare.
 
     Sparse form (static pivots in the fill-reducing order, other summation
     orders): rounds differently; the accept/reject decisions and Newton
